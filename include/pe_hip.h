@@ -1,0 +1,194 @@
+/*
+ * pe_hip.h -- C-ABI of the MI355X-native HCSITS rigid-body time step.
+ *
+ * One context = one pe world (bodies, materials, collision-system settings) whose
+ * state lives in HBM; pehip_step advances it with hand-written gfx950 kernels:
+ * hash-grid broadphase, sphere/box/plane narrowphase, contact-graph colouring,
+ * coloured non-linear block Gauss-Seidel (NBGS) relaxation with the
+ * InelasticCoulombContactByDecoupling model, and semi-implicit integration.
+ * Arithmetic is FP64 throughout.
+ *
+ * Conventions: every call returns pehip_status; no exception crosses the ABI.
+ * The caller owns host arrays (copied during the call); the library owns all
+ * device memory.  A context is used by one host thread at a time and owns one
+ * HIP stream.  Arrays are structure-of-arrays in the caller's body order, which
+ * is pe's BodyStorage order (creation order, pe/core/rigidbody/BodyStorage.h:487-492).
+ * Reference citations are relative to the pe source tree.
+ */
+#ifndef PE_HIP_H
+#define PE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pehip_ctx pehip_ctx;
+
+typedef enum {
+  PEHIP_OK = 0,
+  PEHIP_INVALID_ARG = 1,
+  PEHIP_OUT_OF_MEMORY = 2,
+  PEHIP_CAPACITY_RETRY = 3,
+  PEHIP_HIP_ERROR = 4,
+  PEHIP_NONFINITE = 5,
+  PEHIP_NOT_SUPPORTED = 6
+} pehip_status;
+
+typedef enum { PEHIP_SPHERE = 1, PEHIP_BOX = 2, PEHIP_PLANE = 3 } pehip_shape;
+
+/* CollisionSystem<HCSITS>::RelaxationModel, same numbering
+ * (pe/core/collisionsystem/HardContactSemiImplicitTimesteppingSolvers.h:177-184). */
+typedef enum {
+  PEHIP_INELASTIC_FRICTIONLESS = 0,
+  PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING = 1,
+  PEHIP_APPROX_INELASTIC_COULOMB_ORTHO = 2,
+  PEHIP_INELASTIC_COULOMB_DECOUPLING = 3,
+  PEHIP_INELASTIC_COULOMB_ORTHO = 4,
+  PEHIP_INELASTIC_GMD = 5
+} pehip_relaxation_model;
+
+typedef enum {
+  /* Contacts are coloured so that no two contacts of one colour share a non-fixed
+   * body; each colour is relaxed in parallel.  The production mode. */
+  PEHIP_SWEEP_COLOURED = 0,
+  /* One wavefront relaxes the contacts strictly sequentially, in the order given by
+   * pehip_set_contact_order (verification mode: pe's sweep order, HCSITS.h:2341). */
+  PEHIP_SWEEP_REFERENCE_ORDER = 1
+} pehip_sweep_mode;
+
+typedef struct {
+  uint32_t max_iterations;     /* setMaxIterations, HCSITS.h:793 (default 100, HCSITS.h:420) */
+  int32_t relaxation_model;    /* setRelaxationModel, HCSITS.h:814 */
+  double erp;                  /* setErrorReductionParameter, HCSITS.h:840 (default 0.7) */
+  double abort_threshold;      /* <= 0: always max_iterations sweeps, as pe (HCSITS.h:1935-1946
+                                  is compiled out); > 0: stop when the L-inf impulse change of
+                                  a sweep drops below it (the reference's 1e-7, HCSITS.h:424) */
+  double gravity[3];           /* World::setGravity, World.h:715 */
+  double damping;              /* World::setDamping, World.h:767 (1 = off) */
+  double relaxation_param;     /* setRelaxationParameter, HCSITS.h:779 (MPI-only, stored) */
+  uint32_t max_sub_iterations; /* golden-section steps, HCSITS.h:423 (default 20) */
+  int32_t sweep_mode;          /* pehip_sweep_mode */
+} pehip_params;
+
+/* Fills p with pe's defaults (HCSITS.h:408-431, Settings.cpp:46-48). */
+void pehip_default_params(pehip_params* p);
+
+/* Replaces the CollisionSystem<HCSITS> singleton constructor (HCSITS.h:408-431) and
+ * World (src/core/World.cpp:65).  device = HIP device ordinal. */
+pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** out);
+pehip_status pehip_destroy(pehip_ctx* ctx);
+pehip_status pehip_set_params(pehip_ctx* ctx, const pehip_params* params);
+pehip_status pehip_get_params(const pehip_ctx* ctx, pehip_params* params);
+
+/* Material pair tables already combined on the host: n x n row-major static and
+ * dynamic friction (Materials.cpp:237-243 combine csf_i + csf_j), replacing
+ * Material::get{Static,Dynamic}Friction (pe/core/Materials.h:500-537). */
+pehip_status pehip_set_materials(pehip_ctx* ctx, uint32_t n, const double* csf_pair,
+                                 const double* cdf_pair);
+
+/* Uploads all bodies, replacing World::add -> CollisionSystem::add (World.cpp:262-277,
+ * HCSITS.h:1305-1325).  Per body i:
+ *   shape[i]           pehip_shape
+ *   sid[i]             system id (UniqueID.h:143); sphere-sphere contacts order their bodies by it
+ *   material[i]        row of the pair tables
+ *   fixed[i]           1 = fixed (planes are always fixed)
+ *   param[3i..]        sphere: radius,0,0; box: side lengths; plane: unused
+ *   pos[3i], q[4i], v[3i], w[3i]   state (R is derived from q as R = q.toRotationMatrix())
+ *   inv_mass[i], inertia_body[9i], inv_inertia_body[9i]   mass properties (body frame)
+ *   plane_normal[3i], plane_d[i]   plane parameters, host-computed (meaningful for planes)
+ * Any previous bodies are replaced. */
+pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shape,
+                                 const uint64_t* sid, const uint32_t* material,
+                                 const uint8_t* fixed, const double* param, const double* pos,
+                                 const double* q, const double* v, const double* w,
+                                 const double* inv_mass, const double* inertia_body,
+                                 const double* inv_inertia_body, const double* plane_normal,
+                                 const double* plane_d);
+
+/* Overwrites the dynamic state of all n bodies (checkpoint restore). */
+pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, const double* q,
+                                const double* v, const double* w, const double* plane_normal,
+                                const double* plane_d);
+
+/* External force and torque for the next step (RigidBody::addForce/addTorque);
+ * reset after the step as Settings::forceReset (Settings.cpp:46). */
+pehip_status pehip_set_forces(pehip_ctx* ctx, uint32_t n, const double* force,
+                              const double* torque);
+
+/* Reference-order mode: the order in which the next step relaxes its contacts, as
+ * body-index pairs (body1, body2) in pe's contact emission order.  Contacts of one
+ * pair keep their narrowphase order.  Used only with PEHIP_SWEEP_REFERENCE_ORDER;
+ * consumed by the next pehip_step. */
+pehip_status pehip_set_contact_order(pehip_ctx* ctx, uint32_t n, const uint32_t* body1,
+                                     const uint32_t* body2);
+
+/* World::run(steps, dt) (src/core/World.cpp:374-378): nsteps x World::simulationStep
+ * (World.cpp:330-357) -> CollisionSystem<HCSITS>::simulationStep (HCSITS.h:1472-1531).
+ * dt <= 0 -> PEHIP_INVALID_ARG (World.cpp:333-334). */
+pehip_status pehip_step(pehip_ctx* ctx, double dt, uint32_t nsteps);
+
+/* Detection only (HashGrids::findContacts, HashGrids.h:1121-1314) on the current state,
+ * without resolving or integrating; the contacts are then read with pehip_get_contacts. */
+pehip_status pehip_detect(pehip_ctx* ctx);
+
+pehip_status pehip_download_state(pehip_ctx* ctx, uint32_t n, double* pos, double* q,
+                                  double* v, double* w, double* plane_normal, double* plane_d);
+
+/* Contacts of the last step or detection in the library's order (deterministic).
+ * cap = capacity of the arrays; *n receives the number of contacts (if *n > cap only
+ * cap entries are written).  Any output pointer may be NULL. */
+pehip_status pehip_get_contacts(pehip_ctx* ctx, size_t cap, size_t* n, uint32_t* body1,
+                                uint32_t* body2, double* pos, double* normal, double* dist,
+                                double* mu);
+/* Sweep order of the last step: slot k of the sweep relaxes contact contact_index[k]
+ * (pehip_get_contacts numbering).  Coloured mode: colour c occupies slots
+ * [colour_offsets[c], colour_offsets[c+1]); *ncolours receives the colour count and
+ * colour_offsets needs room for it + 1 entries (at most 129).  Any pointer may be NULL. */
+pehip_status pehip_get_sweep_order(pehip_ctx* ctx, size_t cap, size_t* n, uint32_t* contact_index,
+                                   uint32_t* ncolours, uint32_t* colour_offsets);
+
+/* Solved world-frame impulses of the last step, in pehip_get_contacts order. */
+pehip_status pehip_get_impulses(pehip_ctx* ctx, size_t cap, size_t* n, double* p);
+
+typedef struct {
+  uint64_t num_contacts;      /* CollisionSystem::getNumberOfContacts, HCSITS.h:737 */
+  double max_penetration;     /* getMaximumPenetration, HCSITS.h:715 */
+  uint32_t iterations;        /* sweeps run in the last step */
+  uint32_t num_colours;       /* colours of the last step's contact graph */
+  double last_delta;          /* L-inf impulse change of the last sweep */
+  double kinetic_energy;      /* 0.5 m v^2 + 0.5 w.(I_w w) over non-fixed bodies, after the step */
+  uint64_t fixed_contacts;    /* contacts of the last step that involve a fixed body */
+  double sweep_bytes;         /* algorithmic HBM bytes of the last step's sweeps:
+                                 sweeps * (296 * contacts + 200 * non-fixed contact ends) */
+  /* profiling (pehip_set_profiling): phase times of the last step, and totals since
+   * profiling was enabled of the sweep-kernel launches, their summed durations
+   * (one HIP event pair per launch on the context stream) and algorithmic bytes */
+  double t_detection_ms, t_colouring_ms, t_setup_ms, t_solve_ms, t_integrate_ms;
+  uint64_t prof_steps;
+  uint64_t prof_sweep_launches;
+  double prof_sweep_kernel_ms;
+  double prof_sweep_bytes;
+} pehip_stats;
+pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);
+
+/* Phase timing and per-sweep-launch timing with HIP events on the context's stream
+ * (off by default); enabling resets the profiling totals. */
+pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable);
+
+/* The context's HIP stream (hipStream_t), for callers that time or order work on it. */
+void* pehip_stream(pehip_ctx* ctx);
+
+/* Human-readable message of the last failing call on this context (or of the last
+ * failing pehip_create when ctx is NULL). */
+const char* pehip_last_error(const pehip_ctx* ctx);
+
+/* Library version string. */
+const char* pehip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,817 @@
+// Host side of the C-ABI: one context = one pe world in HBM.  Orchestrates the time
+// step of CollisionSystem<HCSITS>::simulationStep (HCSITS.h:1472-1531) on the GPU:
+// detection -> body caching -> colouring -> contact caching -> sweeps -> integration.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/pe_hip.h"
+#include "kernels.hpp"
+
+using namespace pe_amd;
+
+namespace {
+
+const char* kVersion = "pe_hip 0.1.0 (gfx950)";
+std::string g_create_error;
+
+template <class T>
+hipError_t dalloc(T*& p, size_t n) {
+  p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void**)&p, n * sizeof(T));
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+uint32_t next_pow2(uint64_t x) {
+  uint32_t p = 1;
+  while (p < x && p < (1u << 30)) p <<= 1;
+  return p;
+}
+
+}  // namespace
+
+struct pehip_ctx {
+  pehip_params prm;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool profiling = false;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> launch_ev;  // profiling: one event before and after every sweep launch
+  size_t n_launch_events = 0;
+
+  // bodies
+  uint32_t n = 0;
+  uint8_t *shape = nullptr, *fixed = nullptr;
+  uint32_t* mat = nullptr;
+  uint64_t* sid = nullptr;
+  double *param = nullptr, *pos = nullptr, *q = nullptr, *R = nullptr, *v = nullptr, *w = nullptr;
+  double *pnormal = nullptr, *pd = nullptr, *aabb = nullptr, *inv_mass = nullptr, *Ib = nullptr, *Ibinv = nullptr;
+  double *force = nullptr, *torque = nullptr, *sb = nullptr, *dvw = nullptr;
+  std::vector<uint8_t> h_fixed, h_shape;
+
+  // materials
+  uint32_t nmat = 0;
+  double *csf = nullptr, *cdf = nullptr;
+
+  // broadphase
+  GridParams grid{};
+  int key_bits = 0;
+  BroadphaseBuffers bp{};
+  uint32_t *planes = nullptr, *scan_scratch = nullptr, *sort_scratch = nullptr;
+  uint32_t *flags = nullptr, *total = nullptr;
+
+  // contacts
+  size_t ccap = 0;
+  uint32_t nc = 0;
+  RawContacts raw{};
+  SolverContacts sol{};
+  ColourBuffers col{};
+  uint32_t *perm = nullptr, *sort_keys = nullptr, *sort_keys_tmp = nullptr, *sort_vals_tmp = nullptr;
+  uint32_t *cscan_scratch = nullptr, *csort_scratch = nullptr;
+  std::vector<uint32_t> colour_off;
+  unsigned long long *deltas = nullptr, *maxpen = nullptr;
+  double* ke_partial = nullptr;
+  uint32_t* hstage = nullptr;  // pinned
+
+  // reference-order request
+  std::vector<uint32_t> order_b1, order_b2;
+  bool order_pending = false;
+
+  pehip_stats stats{};
+
+  Bodies bodies() const {
+    Bodies B;
+    B.n = n; B.shape = shape; B.fixed = fixed; B.mat = mat; B.sid = sid; B.param = param;
+    B.pos = pos; B.q = q; B.R = R; B.v = v; B.w = w; B.pnormal = pnormal; B.pd = pd; B.aabb = aabb;
+    B.inv_mass = inv_mass; B.Ib = Ib; B.Ibinv = Ibinv; B.force = force; B.torque = torque;
+    B.sb = sb; B.dvw = dvw;
+    return B;
+  }
+
+  void free_bodies() {
+    dfree(shape); dfree(fixed); dfree(mat); dfree(sid); dfree(param); dfree(pos); dfree(q); dfree(R);
+    dfree(v); dfree(w); dfree(pnormal); dfree(pd); dfree(aabb); dfree(inv_mass); dfree(Ib); dfree(Ibinv);
+    dfree(force); dfree(torque); dfree(sb); dfree(dvw); dfree(planes);
+    dfree(bp.keys); dfree(bp.vals); dfree(bp.keys_tmp); dfree(bp.vals_tmp); dfree(bp.cell_start);
+    dfree(bp.cell_end); dfree(bp.counts); dfree(bp.offsets); dfree(scan_scratch); dfree(sort_scratch);
+    dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial);
+    n = 0;
+  }
+  void free_contacts() {
+    dfree(raw.b1); dfree(raw.b2); dfree(raw.pos); dfree(raw.nrm); dfree(raw.dist); dfree(raw.mu);
+    dfree(sol.b1); dfree(sol.b2); dfree(sol.f);
+    dfree(col.adj); dfree(col.colour); dfree(col.prio); dfree(col.cround);
+    dfree(perm); dfree(sort_keys); dfree(sort_keys_tmp); dfree(sort_vals_tmp);
+    dfree(cscan_scratch); dfree(csort_scratch);
+    ccap = 0;
+  }
+};
+
+#define CK(call)                                                                 \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);              \
+      return e_ == hipErrorOutOfMemory ? PEHIP_OUT_OF_MEMORY : PEHIP_HIP_ERROR;  \
+    }                                                                            \
+  } while (0)
+
+#define CKS(st)                             \
+  do {                                      \
+    pehip_status s_ = (st);                 \
+    if (s_ != PEHIP_OK) return s_;          \
+  } while (0)
+
+static pehip_status fail(pehip_ctx* ctx, pehip_status st, const std::string& msg) {
+  ctx->err = msg;
+  return st;
+}
+
+static pehip_status sync_check(pehip_ctx* ctx) {
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(ctx->stream));
+  return PEHIP_OK;
+}
+
+static pehip_status ensure_contact_capacity(pehip_ctx* ctx, size_t need) {
+  if (need <= ctx->ccap) return PEHIP_OK;
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->free_contacts();
+  size_t cap = std::max<size_t>(need + need / 2, 4096);
+  CK(dalloc(ctx->raw.b1, cap)); CK(dalloc(ctx->raw.b2, cap)); CK(dalloc(ctx->raw.pos, 3 * cap));
+  CK(dalloc(ctx->raw.nrm, 3 * cap)); CK(dalloc(ctx->raw.dist, cap)); CK(dalloc(ctx->raw.mu, cap));
+  CK(dalloc(ctx->sol.b1, cap)); CK(dalloc(ctx->sol.b2, cap)); CK(dalloc(ctx->sol.f, (size_t)CF_COUNT * cap));
+  CK(dalloc(ctx->col.adj, 2 * cap)); CK(dalloc(ctx->col.colour, cap)); CK(dalloc(ctx->col.prio, cap));
+  CK(dalloc(ctx->col.cround, cap));
+  CK(dalloc(ctx->perm, cap)); CK(dalloc(ctx->sort_keys, cap)); CK(dalloc(ctx->sort_keys_tmp, cap));
+  CK(dalloc(ctx->sort_vals_tmp, cap));
+  CK(dalloc(ctx->cscan_scratch, scan_scratch_words(cap)));
+  CK(dalloc(ctx->csort_scratch, sort_scratch_words(cap)));
+  ctx->raw.cap = cap;
+  ctx->sol.cap = cap;
+  ctx->ccap = cap;
+  return PEHIP_OK;
+}
+
+// Static AABB-size bounds and the grid hierarchy / bucket table of the uploaded scene.
+static void plan_grid(pehip_ctx* ctx, const uint8_t* shape, const double* param, const double* pos) {
+  double mx = 0, mn = 1e300;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  uint32_t ngrid = 0;
+  for (uint32_t i = 0; i < ctx->n; ++i) {
+    if (shape[i] == PEHIP_PLANE) continue;
+    ++ngrid;
+    const double* p = param + 3 * (size_t)i;
+    double smax_b, smin_b;
+    if (shape[i] == PEHIP_SPHERE) {
+      smax_b = 2.0 * (p[0] + kContactThreshold);
+      smin_b = smax_b;
+    } else {
+      smax_b = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2.0 * kContactThreshold;
+      smin_b = std::max(p[0], std::max(p[1], p[2]));
+    }
+    mx = std::max(mx, smax_b);
+    mn = std::min(mn, smin_b);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], pos[3 * (size_t)i + k]);
+      hi[k] = std::max(hi[k], pos[3 * (size_t)i + k]);
+    }
+  }
+  GridParams& G = ctx->grid;
+  if (ngrid == 0) { mx = mn = 1.0; lo[0] = lo[1] = lo[2] = 0; hi[0] = hi[1] = hi[2] = 1; }
+  const double top = mx * 1.001;
+  int nl = 1 + (int)std::floor(std::log2(mx / mn) - 1e-9);
+  nl = std::max(1, std::min(kMaxLevels, nl));
+  G.nlevels = nl;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    const int e = std::max(0, nl - 1 - l);
+    G.span[l] = top / std::ldexp(1.0, e);
+    G.inv[l] = 1.0 / G.span[l];
+  }
+  uint32_t d[3];
+  for (int k = 0; k < 3; ++k) {
+    const double cells = std::ceil((hi[k] - lo[k] + mx) / top) + 2.0;
+    d[k] = std::max<uint32_t>(4, std::min<uint32_t>(512, next_pow2((uint64_t)cells)));
+  }
+  const uint64_t cap = std::max<uint64_t>(4096, 8ull * ngrid);
+  while ((uint64_t)d[0] * d[1] * d[2] > cap) {
+    int k = 0;
+    if (d[1] > d[k]) k = 1;
+    if (d[2] > d[k]) k = 2;
+    if (d[k] <= 4) break;
+    d[k] >>= 1;
+  }
+  G.dimx = d[0]; G.dimy = d[1]; G.dimz = d[2];
+  G.table = d[0] * d[1] * d[2];
+  const uint64_t keys = (uint64_t)nl * G.table + 1;
+  int bits = 1;
+  while ((1ull << bits) < keys) ++bits;
+  ctx->key_bits = bits;
+}
+
+extern "C" {
+
+void pehip_default_params(pehip_params* p) {
+  p->max_iterations = 100;
+  p->relaxation_model = PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING;
+  p->erp = 0.7;
+  p->abort_threshold = 0.0;
+  p->gravity[0] = 0.0; p->gravity[1] = 0.0; p->gravity[2] = 0.0;
+  p->damping = 1.0;
+  p->relaxation_param = 0.9;
+  p->max_sub_iterations = 20;
+  p->sweep_mode = PEHIP_SWEEP_COLOURED;
+}
+
+static pehip_status check_params(const pehip_params* p, std::string& why) {
+  if (p->relaxation_model != PEHIP_INELASTIC_FRICTIONLESS &&
+      p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
+      p->relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING) {
+    why = "Unsupported relaxation model.";  // HCSITS.h:1926
+    return PEHIP_NOT_SUPPORTED;
+  }
+  if (!(p->erp >= 0 && p->erp <= 1)) { why = "Error reduction parameter out of range."; return PEHIP_INVALID_ARG; }
+  if (p->sweep_mode != PEHIP_SWEEP_COLOURED && p->sweep_mode != PEHIP_SWEEP_REFERENCE_ORDER) {
+    why = "Invalid sweep mode.";
+    return PEHIP_INVALID_ARG;
+  }
+  if (p->damping > 1 || p->damping < 0) { why = "Invalid damping."; return PEHIP_INVALID_ARG; }
+  return PEHIP_OK;
+}
+
+pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** out) {
+  if (!out) { g_create_error = "out is NULL"; return PEHIP_INVALID_ARG; }
+  *out = nullptr;
+  pehip_params p;
+  if (params) p = *params; else pehip_default_params(&p);
+  std::string why;
+  pehip_status st = check_params(&p, why);
+  if (st != PEHIP_OK) { g_create_error = why; return st; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_create_error = "no HIP device available";
+    return PEHIP_HIP_ERROR;
+  }
+  if (device < 0 || device >= ndev) { g_create_error = "invalid device ordinal"; return PEHIP_INVALID_ARG; }
+  pehip_ctx* ctx = new pehip_ctx();
+  ctx->prm = p;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->hstage, 4096, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&ctx->flags, 64) != hipSuccess ||
+      hipMalloc((void**)&ctx->total, 64) != hipSuccess || hipMalloc((void**)&ctx->maxpen, 64) != hipSuccess ||
+      hipMalloc((void**)&ctx->col.remaining, 64) != hipSuccess || hipMalloc((void**)&ctx->col.hist, 4 * kMaxColours) != hipSuccess) {
+    g_create_error = "HIP initialisation failed";
+    pehip_destroy(ctx);
+    return PEHIP_HIP_ERROR;
+  }
+  if (hipMalloc((void**)&ctx->deltas, sizeof(unsigned long long) * (p.max_iterations + 1)) != hipSuccess) {
+    g_create_error = "HIP allocation failed";
+    pehip_destroy(ctx);
+    return PEHIP_OUT_OF_MEMORY;
+  }
+  for (auto& e : ctx->ev) hipEventCreate(&e);
+  *out = ctx;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_destroy(pehip_ctx* ctx) {
+  if (!ctx) return PEHIP_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  ctx->free_bodies();
+  ctx->free_contacts();
+  dfree(ctx->csf); dfree(ctx->cdf); dfree(ctx->flags); dfree(ctx->total); dfree(ctx->maxpen);
+  dfree(ctx->col.remaining); dfree(ctx->col.hist); dfree(ctx->deltas);
+  if (ctx->hstage) hipHostFree(ctx->hstage);
+  for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
+  for (auto& e : ctx->launch_ev) hipEventDestroy(e);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_set_params(pehip_ctx* ctx, const pehip_params* params) {
+  if (!ctx || !params) return PEHIP_INVALID_ARG;
+  std::string why;
+  pehip_status st = check_params(params, why);
+  if (st != PEHIP_OK) return fail(ctx, st, why);
+  if (params->max_iterations != ctx->prm.max_iterations) {
+    hipSetDevice(ctx->device);
+    CK(hipStreamSynchronize(ctx->stream));
+    dfree(ctx->deltas);
+    CK(dalloc(ctx->deltas, (size_t)params->max_iterations + 1));
+  }
+  ctx->prm = *params;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_get_params(const pehip_ctx* ctx, pehip_params* params) {
+  if (!ctx || !params) return PEHIP_INVALID_ARG;
+  *params = ctx->prm;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_set_materials(pehip_ctx* ctx, uint32_t n, const double* csf_pair, const double* cdf_pair) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (n == 0 || !csf_pair || !cdf_pair) return fail(ctx, PEHIP_INVALID_ARG, "empty material table");
+  for (size_t i = 0; i < (size_t)n * n; ++i)
+    if (!(csf_pair[i] >= 0) || !(cdf_pair[i] >= 0))
+      return fail(ctx, PEHIP_INVALID_ARG, "Invalid coefficient of friction!");  // Materials.cpp:203-209
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  dfree(ctx->csf); dfree(ctx->cdf);
+  CK(dalloc(ctx->csf, (size_t)n * n));
+  CK(dalloc(ctx->cdf, (size_t)n * n));
+  CK(hipMemcpy(ctx->csf, csf_pair, sizeof(double) * n * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->cdf, cdf_pair, sizeof(double) * n * n, hipMemcpyHostToDevice));
+  ctx->nmat = n;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shape, const uint64_t* sid,
+                                 const uint32_t* material, const uint8_t* fixed, const double* param,
+                                 const double* pos, const double* q, const double* v, const double* w,
+                                 const double* inv_mass, const double* inertia_body,
+                                 const double* inv_inertia_body, const double* plane_normal,
+                                 const double* plane_d) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (n == 0) return fail(ctx, PEHIP_INVALID_ARG, "no bodies");
+  if (n >= kFixedBit) return fail(ctx, PEHIP_INVALID_ARG, "too many bodies");
+  if (!shape || !sid || !material || !fixed || !param || !pos || !q || !v || !w || !inv_mass || !inertia_body ||
+      !inv_inertia_body || !plane_normal || !plane_d)
+    return fail(ctx, PEHIP_INVALID_ARG, "NULL body array");
+  if (ctx->nmat == 0) return fail(ctx, PEHIP_INVALID_ARG, "materials must be set before bodies");
+  std::vector<uint8_t> fx(fixed, fixed + n);
+  std::vector<uint32_t> hplanes;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_PLANE) return fail(ctx, PEHIP_INVALID_ARG, "invalid shape");
+    if (material[i] >= ctx->nmat) return fail(ctx, PEHIP_INVALID_ARG, "Invalid material ID");
+    if (shape[i] == PEHIP_PLANE) {
+      hplanes.push_back(i);
+      fx[i] = 1;
+      if (inv_mass[i] != 0) return fail(ctx, PEHIP_INVALID_ARG, "plane with non-zero inverse mass");
+    } else {
+      const double* p = param + 3 * (size_t)i;
+      if (shape[i] == PEHIP_SPHERE && !(p[0] > 0)) return fail(ctx, PEHIP_INVALID_ARG, "Invalid sphere radius");
+      if (shape[i] == PEHIP_BOX && !(p[0] > 0 && p[1] > 0 && p[2] > 0))
+        return fail(ctx, PEHIP_INVALID_ARG, "Invalid side length");
+      if (fx[i] && inv_mass[i] != 0) return fail(ctx, PEHIP_INVALID_ARG, "fixed body with non-zero inverse mass");
+    }
+  }
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->free_bodies();
+  ctx->n = n;
+  ctx->h_fixed = fx;
+  ctx->h_shape.assign(shape, shape + n);
+  // rotation matrices from the quaternions (Quaternion::toRotationMatrix)
+  std::vector<double> hR(9 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i) {
+    qd qq;
+    for (int k = 0; k < 4; ++k) qq.v[k] = q[4 * (size_t)i + k];
+    const m33 m = quat_to_R(qq);
+    for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
+  }
+  CK(dalloc(ctx->shape, n)); CK(dalloc(ctx->fixed, n)); CK(dalloc(ctx->mat, n)); CK(dalloc(ctx->sid, n));
+  CK(dalloc(ctx->param, 3 * (size_t)n)); CK(dalloc(ctx->pos, 3 * (size_t)n)); CK(dalloc(ctx->q, 4 * (size_t)n));
+  CK(dalloc(ctx->R, 9 * (size_t)n)); CK(dalloc(ctx->v, 3 * (size_t)n)); CK(dalloc(ctx->w, 3 * (size_t)n));
+  CK(dalloc(ctx->pnormal, 3 * (size_t)n)); CK(dalloc(ctx->pd, n)); CK(dalloc(ctx->aabb, 6 * (size_t)n));
+  CK(dalloc(ctx->inv_mass, n)); CK(dalloc(ctx->Ib, 9 * (size_t)n)); CK(dalloc(ctx->Ibinv, 9 * (size_t)n));
+  CK(dalloc(ctx->force, 3 * (size_t)n)); CK(dalloc(ctx->torque, 3 * (size_t)n));
+  CK(dalloc(ctx->sb, (size_t)kSB * n)); CK(dalloc(ctx->dvw, (size_t)kDVW * n));
+  CK(dalloc(ctx->planes, hplanes.size()));
+  const size_t D = sizeof(double);
+  CK(hipMemcpy(ctx->shape, shape, n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->fixed, fx.data(), n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mat, material, 4 * (size_t)n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->sid, sid, 8 * (size_t)n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->param, param, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->q, q, 4 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->v, v, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->w, w, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pnormal, plane_normal, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pd, plane_d, D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->inv_mass, inv_mass, D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->Ib, inertia_body, 9 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->Ibinv, inv_inertia_body, 9 * D * n, hipMemcpyHostToDevice));
+  if (!hplanes.empty()) CK(hipMemcpy(ctx->planes, hplanes.data(), 4 * hplanes.size(), hipMemcpyHostToDevice));
+  CK(hipMemset(ctx->force, 0, 3 * D * n));
+  CK(hipMemset(ctx->torque, 0, 3 * D * n));
+  CK(hipMemset(ctx->dvw, 0, kDVW * D * n));
+  CK(hipMemset(ctx->sb, 0, kSB * D * n));
+
+  plan_grid(ctx, shape, param, pos);
+  const size_t cells = (size_t)ctx->grid.nlevels * ctx->grid.table;
+  BroadphaseBuffers& P = ctx->bp;
+  CK(dalloc(P.keys, n)); CK(dalloc(P.vals, n)); CK(dalloc(P.keys_tmp, n)); CK(dalloc(P.vals_tmp, n));
+  CK(dalloc(P.cell_start, cells)); CK(dalloc(P.cell_end, cells));
+  CK(dalloc(P.counts, (size_t)n + 1)); CK(dalloc(P.offsets, (size_t)n + 1));
+  CK(dalloc(ctx->scan_scratch, scan_scratch_words((size_t)n + 1)));
+  CK(dalloc(ctx->sort_scratch, sort_scratch_words(n)));
+  CK(dalloc(ctx->col.deg, n)); CK(dalloc(ctx->col.adj_start, (size_t)n + 1)); CK(dalloc(ctx->col.adj_cursor, n));
+  CK(dalloc(ctx->ke_partial, (n + 255) / 256));
+  P.planes = ctx->planes;
+  P.nplanes = (uint32_t)hplanes.size();
+  P.flags = ctx->flags;
+  launch_aabb(ctx->bodies(), ctx->stream);
+  ctx->nc = 0;
+  ctx->order_pending = false;
+  return sync_check(ctx);
+}
+
+pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, const double* q, const double* v,
+                                const double* w, const double* plane_normal, const double* plane_d) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (n != ctx->n) return fail(ctx, PEHIP_INVALID_ARG, "body count mismatch");
+  if (!pos || !q || !v || !w || !plane_normal || !plane_d) return fail(ctx, PEHIP_INVALID_ARG, "NULL state array");
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  std::vector<double> hR(9 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i) {
+    qd qq;
+    for (int k = 0; k < 4; ++k) qq.v[k] = q[4 * (size_t)i + k];
+    const m33 m = quat_to_R(qq);
+    for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
+  }
+  const size_t D = sizeof(double);
+  CK(hipMemcpy(ctx->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->q, q, 4 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->v, v, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->w, w, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pnormal, plane_normal, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pd, plane_d, D * n, hipMemcpyHostToDevice));
+  launch_aabb(ctx->bodies(), ctx->stream);
+  return sync_check(ctx);
+}
+
+pehip_status pehip_set_forces(pehip_ctx* ctx, uint32_t n, const double* force, const double* torque) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (n != ctx->n || !force || !torque) return fail(ctx, PEHIP_INVALID_ARG, "force arrays do not match the bodies");
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(ctx->force, force, 3 * sizeof(double) * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->torque, torque, 3 * sizeof(double) * n, hipMemcpyHostToDevice));
+  return PEHIP_OK;
+}
+
+pehip_status pehip_set_contact_order(pehip_ctx* ctx, uint32_t n, const uint32_t* body1, const uint32_t* body2) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (n && (!body1 || !body2)) return fail(ctx, PEHIP_INVALID_ARG, "NULL order arrays");
+  ctx->order_b1.assign(body1, body1 + n);
+  ctx->order_b2.assign(body2, body2 + n);
+  ctx->order_pending = true;
+  return PEHIP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// The step.
+// ---------------------------------------------------------------------------------------------
+static void mark(pehip_ctx* ctx, int k) {
+  if (ctx->profiling) hipEventRecord(ctx->ev[k], ctx->stream);
+}
+
+static pehip_status record_launch_event(pehip_ctx* ctx, size_t k) {
+  while (ctx->launch_ev.size() <= k) {
+    hipEvent_t e;
+    CK(hipEventCreate(&e));
+    ctx->launch_ev.push_back(e);
+  }
+  CK(hipEventRecord(ctx->launch_ev[k], ctx->stream));
+  return PEHIP_OK;
+}
+
+// HashGrids::findContacts equivalent: keys, sort, cell ranges, count, scan, write.
+static pehip_status detect(pehip_ctx* ctx) {
+  const hipStream_t s = ctx->stream;
+  Bodies B = ctx->bodies();
+  fill_u32(ctx->flags, 0u, 1, s);
+  launch_cell_keys(B, ctx->grid, ctx->bp, s);
+  radix_sort_pairs(ctx->bp.keys, ctx->bp.vals, ctx->bp.keys_tmp, ctx->bp.vals_tmp, ctx->n, ctx->key_bits,
+                   ctx->sort_scratch, s);
+  launch_cell_ranges(ctx->n, ctx->grid, ctx->bp, s);
+  launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, false, s);
+  const uint32_t ngrid = ctx->n - ctx->bp.nplanes;
+  exclusive_scan_u32(ctx->bp.counts, ctx->bp.offsets, ngrid, ctx->total, ctx->scan_scratch, s);
+  CK(hipMemcpyAsync(ctx->hstage, ctx->total, 4, hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(ctx->hstage + 1, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+  CKS(sync_check(ctx));
+  const uint32_t nc = ctx->hstage[0], fl = ctx->hstage[1];
+  if (fl & ERR_NONFINITE) return fail(ctx, PEHIP_NONFINITE, "non-finite body position or bounding box");
+  if (fl & ERR_LEVEL_OVERFLOW) return fail(ctx, PEHIP_INVALID_ARG, "body larger than the coarsest grid cell");
+  if (fl & ERR_BOX_BOX) return fail(ctx, PEHIP_NOT_SUPPORTED, "box-box contacts are not supported");
+  CKS(ensure_contact_capacity(ctx, nc));
+  if (nc) launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, true, s);
+  ctx->nc = nc;
+  return PEHIP_OK;
+}
+
+// Contact-graph colouring -> ctx->perm (colour-major) and ctx->colour_off.
+static pehip_status colour(pehip_ctx* ctx) {
+  const hipStream_t s = ctx->stream;
+  const uint32_t nc = ctx->nc;
+  Bodies B = ctx->bodies();
+  launch_adjacency(B, ctx->raw, nc, ctx->col, ctx->scan_scratch, s);
+  launch_colour_init(ctx->raw, B, nc, ctx->col, s);
+  uint32_t round = 0;
+  for (;;) {
+    for (int k = 0; k < 4; ++k) launch_colour_round(ctx->raw, B, nc, ctx->col, round++, ctx->flags, s);
+    CK(hipMemcpyAsync(ctx->hstage, ctx->col.remaining, 4, hipMemcpyDeviceToHost, s));
+    CKS(sync_check(ctx));
+    if (ctx->hstage[0] == 0) break;
+    if (round > 4u * 4096u) return fail(ctx, PEHIP_HIP_ERROR, "colouring did not terminate");
+  }
+  launch_colour_hist(nc, ctx->col, s);
+  CK(hipMemcpyAsync(ctx->hstage, ctx->col.hist, 4 * kMaxColours, hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(ctx->hstage + kMaxColours, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+  CKS(sync_check(ctx));
+  if (ctx->hstage[kMaxColours] & ERR_COLOURS) return fail(ctx, PEHIP_NOT_SUPPORTED, "more than 128 colours needed");
+  uint32_t ncol = 0;
+  for (uint32_t k = 0; k < kMaxColours; ++k)
+    if (ctx->hstage[k]) ncol = k + 1;
+  ctx->colour_off.assign(ncol + 1, 0);
+  for (uint32_t k = 0; k < ncol; ++k) ctx->colour_off[k + 1] = ctx->colour_off[k] + ctx->hstage[k];
+  // stable sort of contact indices by colour (8-bit keys: one pass)
+  CK(hipMemcpyAsync(ctx->sort_keys, ctx->col.colour, 4 * (size_t)nc, hipMemcpyDeviceToDevice, s));
+  iota_u32(ctx->perm, nc, s);
+  radix_sort_pairs(ctx->sort_keys, ctx->perm, ctx->sort_keys_tmp, ctx->sort_vals_tmp, nc, 8, ctx->csort_scratch, s);
+  ctx->stats.num_colours = ncol;
+  return PEHIP_OK;
+}
+
+// Reference order: map the requested (body1, body2) sequence onto the detected contacts.
+static pehip_status reference_perm(pehip_ctx* ctx) {
+  const uint32_t nc = ctx->nc;
+  if (!ctx->order_pending) return fail(ctx, PEHIP_INVALID_ARG, "reference-order mode needs pehip_set_contact_order");
+  ctx->order_pending = false;
+  if (ctx->order_b1.size() != nc)
+    return fail(ctx, PEHIP_INVALID_ARG, "contact order does not match the detected contact set (count " +
+                                            std::to_string(ctx->order_b1.size()) + " vs " + std::to_string(nc) + ")");
+  std::vector<uint32_t> b1(nc), b2(nc);
+  if (nc) {
+    CK(hipMemcpy(b1.data(), ctx->raw.b1, 4 * (size_t)nc, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(b2.data(), ctx->raw.b2, 4 * (size_t)nc, hipMemcpyDeviceToHost));
+  }
+  std::unordered_map<uint64_t, std::vector<uint32_t>> byPair;
+  for (uint32_t c = 0; c < nc; ++c) byPair[((uint64_t)b1[c] << 32) | b2[c]].push_back(c);
+  std::unordered_map<uint64_t, size_t> used;
+  std::vector<uint32_t> perm(nc);
+  for (uint32_t k = 0; k < nc; ++k) {
+    const uint64_t key = ((uint64_t)ctx->order_b1[k] << 32) | ctx->order_b2[k];
+    auto it = byPair.find(key);
+    size_t& u = used[key];
+    if (it == byPair.end() || u >= it->second.size())
+      return fail(ctx, PEHIP_INVALID_ARG, "contact order does not match the detected contact set (pair " +
+                                              std::to_string(ctx->order_b1[k]) + "," + std::to_string(ctx->order_b2[k]) + ")");
+    perm[k] = it->second[u++];
+  }
+  if (nc) CK(hipMemcpy(ctx->perm, perm.data(), 4 * (size_t)nc, hipMemcpyHostToDevice));
+  ctx->colour_off.assign(2, 0);
+  ctx->colour_off[1] = nc;
+  ctx->stats.num_colours = 0;
+  return PEHIP_OK;
+}
+
+static pehip_status one_step(pehip_ctx* ctx, double dt) {
+  const hipStream_t s = ctx->stream;
+  const double dtinv = 1.0 / dt;
+  const pehip_params& P = ctx->prm;
+  Bodies B = ctx->bodies();
+  mark(ctx, 0);
+  CKS(detect(ctx));
+  mark(ctx, 1);
+  const d3 g = mk3(P.gravity[0], P.gravity[1], P.gravity[2]);
+  launch_predict(B, g, dt, s);
+  const uint32_t nc = ctx->nc;
+  uint32_t iters = 0;
+  double last_delta = 0;
+  fill_u64(ctx->maxpen, 0ull, 2, s);
+  if (nc) {
+    if (P.sweep_mode == PEHIP_SWEEP_REFERENCE_ORDER) CKS(reference_perm(ctx));
+    else CKS(colour(ctx));
+  } else {
+    ctx->order_pending = false;
+    ctx->stats.num_colours = 0;
+    ctx->colour_off.clear();
+  }
+  mark(ctx, 2);
+  if (nc) {
+    launch_setup(B, ctx->raw, ctx->perm, nc, ctx->sol, P.erp, dtinv, ctx->maxpen, s);
+    fill_u64(ctx->deltas, 0ull, (size_t)P.max_iterations + 1, s);
+  }
+  mark(ctx, 3);
+  if (nc && P.max_iterations) {
+    if (P.sweep_mode == PEHIP_SWEEP_REFERENCE_ORDER) {
+      launch_sweep_serial(P.relaxation_model, ctx->sol, nc, P.max_iterations, B, dtinv, (int)P.max_sub_iterations,
+                          P.abort_threshold, ctx->deltas, s);
+      CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + P.max_iterations, 8, hipMemcpyDeviceToHost, s));
+      CKS(sync_check(ctx));
+      iters = (uint32_t)(*(unsigned long long*)ctx->hstage);
+    } else {
+      const uint32_t ncol = (uint32_t)ctx->colour_off.size() - 1;
+      size_t nev = 0;
+      for (uint32_t it = 0; it < P.max_iterations; ++it) {
+        for (uint32_t k = 0; k < ncol; ++k) {
+          if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+          launch_sweep(P.relaxation_model, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
+                       (int)P.max_sub_iterations, ctx->deltas + it, s);
+          if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+        }
+        iters = it + 1;
+        if (P.abort_threshold > 0) {
+          CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + it, 8, hipMemcpyDeviceToHost, s));
+          CKS(sync_check(ctx));
+          double d;
+          std::memcpy(&d, ctx->hstage, 8);
+          if (d < P.abort_threshold) break;
+        }
+      }
+      ctx->n_launch_events = nev;
+    }
+  }
+  mark(ctx, 4);
+  launch_integrate(B, dt, P.damping, s);
+  mark(ctx, 5);
+  // stats of this step
+  CK(hipMemcpyAsync(ctx->hstage, ctx->maxpen, 16, hipMemcpyDeviceToHost, s));
+  if (iters) CK(hipMemcpyAsync(ctx->hstage + 4, ctx->deltas + (iters - 1), 8, hipMemcpyDeviceToHost, s));
+  CKS(sync_check(ctx));
+  double mp;
+  std::memcpy(&mp, ctx->hstage, 8);
+  unsigned long long nfx;
+  std::memcpy(&nfx, ctx->hstage + 2, 8);
+  if (iters) std::memcpy(&last_delta, ctx->hstage + 4, 8);
+  ctx->stats.num_contacts = nc;
+  ctx->stats.max_penetration = mp;
+  ctx->stats.iterations = iters;
+  ctx->stats.last_delta = last_delta;
+  ctx->stats.fixed_contacts = nc ? nfx : 0;
+  // Compulsory bytes of one sweep: per contact its record (r1, r2, n, t, o, dist, mu, Dn^-1,
+  // D^-1 sym, Dto^-1 sym, Dto, p read+write: 36 doubles + 2 indices = 296 B) and per
+  // non-fixed contact end the body's v, w, m^-1, Iw^-1 sym and dv, dw read+write (200 B).
+  const double sweep_bytes = nc ? (double)iters * (296.0 * nc + 200.0 * (2.0 * nc - (double)nfx)) : 0.0;
+  ctx->stats.sweep_bytes = sweep_bytes;
+  if (ctx->profiling && P.sweep_mode == PEHIP_SWEEP_COLOURED && nc && iters) {
+    double ms = 0;
+    for (size_t k = 0; k + 1 < ctx->n_launch_events; k += 2) {
+      float t = 0;
+      hipEventElapsedTime(&t, ctx->launch_ev[k], ctx->launch_ev[k + 1]);
+      ms += t;
+    }
+    ctx->stats.prof_sweep_launches += ctx->n_launch_events / 2;
+    ctx->stats.prof_sweep_kernel_ms += ms;
+    ctx->stats.prof_sweep_bytes += sweep_bytes;
+  }
+  if (ctx->profiling) ++ctx->stats.prof_steps;
+  if (ctx->profiling) {
+    float t[5];
+    for (int k = 0; k < 5; ++k) hipEventElapsedTime(&t[k], ctx->ev[k], ctx->ev[k + 1]);
+    ctx->stats.t_detection_ms = t[0];
+    ctx->stats.t_colouring_ms = t[1];
+    ctx->stats.t_setup_ms = t[2];
+    ctx->stats.t_solve_ms = t[3];
+    ctx->stats.t_integrate_ms = t[4];
+  }
+  return PEHIP_OK;
+}
+
+extern "C" {
+
+pehip_status pehip_step(pehip_ctx* ctx, double dt, uint32_t nsteps) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (!(dt > 0)) return fail(ctx, PEHIP_INVALID_ARG, "Invalid time step size");  // World.cpp:333-334
+  if (ctx->n == 0) return fail(ctx, PEHIP_INVALID_ARG, "no bodies uploaded");
+  hipSetDevice(ctx->device);
+  for (uint32_t k = 0; k < nsteps; ++k) CKS(one_step(ctx, dt));
+  return PEHIP_OK;
+}
+
+pehip_status pehip_detect(pehip_ctx* ctx) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (ctx->n == 0) return fail(ctx, PEHIP_INVALID_ARG, "no bodies uploaded");
+  hipSetDevice(ctx->device);
+  CKS(detect(ctx));
+  return sync_check(ctx);
+}
+
+pehip_status pehip_download_state(pehip_ctx* ctx, uint32_t n, double* pos, double* q, double* v, double* w,
+                                  double* plane_normal, double* plane_d) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (n != ctx->n) return fail(ctx, PEHIP_INVALID_ARG, "body count mismatch");
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  const size_t D = sizeof(double);
+  if (pos) CK(hipMemcpy(pos, ctx->pos, 3 * D * n, hipMemcpyDeviceToHost));
+  if (q) CK(hipMemcpy(q, ctx->q, 4 * D * n, hipMemcpyDeviceToHost));
+  if (v) CK(hipMemcpy(v, ctx->v, 3 * D * n, hipMemcpyDeviceToHost));
+  if (w) CK(hipMemcpy(w, ctx->w, 3 * D * n, hipMemcpyDeviceToHost));
+  if (plane_normal) CK(hipMemcpy(plane_normal, ctx->pnormal, 3 * D * n, hipMemcpyDeviceToHost));
+  if (plane_d) CK(hipMemcpy(plane_d, ctx->pd, D * n, hipMemcpyDeviceToHost));
+  return PEHIP_OK;
+}
+
+pehip_status pehip_get_contacts(pehip_ctx* ctx, size_t cap, size_t* n, uint32_t* body1, uint32_t* body2,
+                                double* pos, double* normal, double* dist, double* mu) {
+  if (!ctx || !n) return PEHIP_INVALID_ARG;
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  *n = ctx->nc;
+  const size_t m = std::min<size_t>(cap, ctx->nc);
+  if (!m) return PEHIP_OK;
+  if (body1) CK(hipMemcpy(body1, ctx->raw.b1, 4 * m, hipMemcpyDeviceToHost));
+  if (body2) CK(hipMemcpy(body2, ctx->raw.b2, 4 * m, hipMemcpyDeviceToHost));
+  if (pos) CK(hipMemcpy(pos, ctx->raw.pos, 24 * m, hipMemcpyDeviceToHost));
+  if (normal) CK(hipMemcpy(normal, ctx->raw.nrm, 24 * m, hipMemcpyDeviceToHost));
+  if (dist) CK(hipMemcpy(dist, ctx->raw.dist, 8 * m, hipMemcpyDeviceToHost));
+  if (mu) CK(hipMemcpy(mu, ctx->raw.mu, 8 * m, hipMemcpyDeviceToHost));
+  return PEHIP_OK;
+}
+
+pehip_status pehip_get_sweep_order(pehip_ctx* ctx, size_t cap, size_t* n, uint32_t* contact_index,
+                                   uint32_t* ncolours, uint32_t* colour_offsets) {
+  if (!ctx || !n) return PEHIP_INVALID_ARG;
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  *n = ctx->nc;
+  const size_t m = std::min<size_t>(cap, ctx->nc);
+  if (m && contact_index) CK(hipMemcpy(contact_index, ctx->perm, 4 * m, hipMemcpyDeviceToHost));
+  const uint32_t ncol = ctx->colour_off.empty() ? 0u : (uint32_t)ctx->colour_off.size() - 1;
+  if (ncolours) *ncolours = ncol;
+  if (colour_offsets)
+    for (uint32_t k = 0; k <= ncol && !ctx->colour_off.empty(); ++k) colour_offsets[k] = ctx->colour_off[k];
+  return PEHIP_OK;
+}
+
+pehip_status pehip_get_impulses(pehip_ctx* ctx, size_t cap, size_t* n, double* p) {
+  if (!ctx || !n) return PEHIP_INVALID_ARG;
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  const uint32_t nc = ctx->nc;
+  *n = nc;
+  if (!nc || !p) return PEHIP_OK;
+  std::vector<uint32_t> perm(nc);
+  std::vector<double> ps(3 * (size_t)nc);
+  CK(hipMemcpy(perm.data(), ctx->perm, 4 * (size_t)nc, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 3; ++k)
+    CK(hipMemcpy(ps.data() + (size_t)k * nc, ctx->sol.f + (size_t)(CF_P + k) * ctx->sol.cap, 8 * (size_t)nc,
+                 hipMemcpyDeviceToHost));
+  for (uint32_t sidx = 0; sidx < nc; ++sidx) {
+    const uint32_t c = perm[sidx];
+    if (c < cap)
+      for (int k = 0; k < 3; ++k) p[3 * (size_t)c + k] = ps[(size_t)k * nc + sidx];
+  }
+  return PEHIP_OK;
+}
+
+pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out) {
+  if (!ctx || !out) return PEHIP_INVALID_ARG;
+  hipSetDevice(ctx->device);
+  if (ctx->n) {
+    const uint32_t nb = launch_kinetic(ctx->bodies(), ctx->ke_partial, ctx->stream);
+    std::vector<double> part(nb);
+    CK(hipStreamSynchronize(ctx->stream));
+    CK(hipMemcpy(part.data(), ctx->ke_partial, 8 * (size_t)nb, hipMemcpyDeviceToHost));
+    double e = 0;
+    for (double x : part) e += x;
+    ctx->stats.kinetic_energy = e;
+  }
+  *out = ctx->stats;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  ctx->profiling = enable != 0;
+  if (ctx->profiling) {
+    ctx->stats.prof_steps = 0;
+    ctx->stats.prof_sweep_launches = 0;
+    ctx->stats.prof_sweep_kernel_ms = 0;
+    ctx->stats.prof_sweep_bytes = 0;
+  }
+  return PEHIP_OK;
+}
+
+void* pehip_stream(pehip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+const char* pehip_last_error(const pehip_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+const char* pehip_version(void) { return kVersion; }
+
+}  // extern "C"

@@ -1,0 +1,420 @@
+// Device-side data layout of one pe world in HBM, and the per-pair / per-contact
+// routines shared by the kernels.  Citations are pe source-tree file:line.
+#pragma once
+
+#include <stdint.h>
+
+#include "dmath.hpp"
+
+namespace pe_amd {
+
+constexpr double kContactThreshold = 1e-8;   // pe/core/Thresholds.h:151
+constexpr double kFrictionThreshold = 1e-8;  // pe/core/Thresholds.h:161
+constexpr uint32_t kFixedBit = 0x80000000u;  // set in solver body indices of fixed bodies
+constexpr int kShapeSphere = 1, kShapeBox = 2, kShapePlane = 3;
+
+// Solver body record (128 B, one L2 line): predicted v, w; inverse mass; world-frame
+// inverse inertia R*Iinv*R^T as 9 doubles (pe recomputes the full product on every
+// access, RigidBody.h:866-869, so it is not symmetric to the last bit).
+constexpr int kSB = 16;
+constexpr int SB_V = 0, SB_W = 3, SB_IM = 6, SB_IW = 7;
+// Velocity-correction record (64 B): dv, dw, 2 pad.
+constexpr int kDVW = 8;
+
+// Bodies: per-body arrays in the caller's body order.
+struct Bodies {
+  uint32_t n;
+  const uint8_t* shape;
+  const uint8_t* fixed;
+  const uint32_t* mat;
+  const uint64_t* sid;
+  const double* param;     // 3n: radius | side lengths
+  double* pos;             // 3n
+  double* q;               // 4n
+  double* R;               // 9n
+  double* v;               // 3n
+  double* w;               // 3n
+  double* pnormal;         // 3n (planes)
+  double* pd;              // n  (planes)
+  double* aabb;            // 6n
+  const double* inv_mass;  // n
+  const double* Ib;        // 9n body-frame inertia
+  const double* Ibinv;     // 9n body-frame inverse inertia
+  double* force;           // 3n
+  double* torque;          // 3n
+  double* sb;              // kSB * n
+  double* dvw;             // kDVW * n
+};
+
+PE_HD d3 ld3(const double* p, uint32_t i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+PE_HD void st3(double* p, uint32_t i, d3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+PE_HD m33 ld9(const double* p, uint32_t i) {
+  m33 m;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m.a[k] = p[9 * (size_t)i + k];
+  return m;
+}
+PE_HD qd ld4(const double* p, uint32_t i) {
+  qd q;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q.v[k] = p[4 * (size_t)i + k];
+  return q;
+}
+
+// Closed-interval AABB overlap (BoundingBox::overlaps, BoundingBox.h:487-497).
+PE_HD bool aabb_overlap(const double* a, const double* b) {
+  return !(b[0] > a[3] || b[1] > a[4] || b[2] > a[5] || b[3] < a[0] || b[4] < a[1] || b[5] < a[2]);
+}
+
+// Largest AABB extent (BodyTrait<HashGrids>::getAABBSize, coarse/BodyTrait.h:209-218).
+PE_HD double aabb_size(const double* a) {
+  double s = a[3] - a[0];
+  const double y = a[4] - a[1], z = a[5] - a[2];
+  if (y > s) s = y;
+  if (z > s) s = z;
+  return s;
+}
+
+// AABB of one body (SphereBase.h:208-221, BoxBase.cpp:127-141, PlaneBase.cpp:132-148).
+PE_HD void body_aabb(int shape, d3 gp, const m33& R, const double* prm, d3 nrm, double d, double* o) {
+  if (shape == kShapeSphere) {
+    const double l = prm[0] + kContactThreshold;
+    o[0] = gp.x - l; o[1] = gp.y - l; o[2] = gp.z - l;
+    o[3] = gp.x + l; o[4] = gp.y + l; o[5] = gp.z + l;
+  } else if (shape == kShapeBox) {
+    const double* r = R.a;
+    const double xl = 0.5 * (fabs(r[0] * prm[0]) + fabs(r[1] * prm[1]) + fabs(r[2] * prm[2])) + kContactThreshold;
+    const double yl = 0.5 * (fabs(r[3] * prm[0]) + fabs(r[4] * prm[1]) + fabs(r[5] * prm[2])) + kContactThreshold;
+    const double zl = 0.5 * (fabs(r[6] * prm[0]) + fabs(r[7] * prm[1]) + fabs(r[8] * prm[2])) + kContactThreshold;
+    o[0] = gp.x - xl; o[1] = gp.y - yl; o[2] = gp.z - zl;
+    o[3] = gp.x + xl; o[4] = gp.y + yl; o[5] = gp.z + zl;
+  } else {
+    const double inf = __builtin_huge_val();
+    o[0] = (nrm.x < 0 && nrm.y == 0 && nrm.z == 0) ? (-d - kContactThreshold) : -inf;
+    o[1] = (nrm.x == 0 && nrm.y < 0 && nrm.z == 0) ? (-d - kContactThreshold) : -inf;
+    o[2] = (nrm.x == 0 && nrm.y == 0 && nrm.z < 0) ? (-d - kContactThreshold) : -inf;
+    o[3] = (nrm.x > 0 && nrm.y == 0 && nrm.z == 0) ? (d + kContactThreshold) : inf;
+    o[4] = (nrm.x == 0 && nrm.y > 0 && nrm.z == 0) ? (d + kContactThreshold) : inf;
+    o[5] = (nrm.x == 0 && nrm.y == 0 && nrm.z > 0) ? (d + kContactThreshold) : inf;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Narrowphase (pe/core/detection/fine/MaxContacts.h).  Each routine calls
+// emit(b1, b2, gpos, normal, dist) for every contact it creates, in pe's order.
+// ---------------------------------------------------------------------------------------------
+
+// collideSphereSphere, MaxContacts.h:547-568: body1 is the one with the smaller SID.
+template <class Emit>
+__device__ void np_sphere_sphere(const Bodies& B, uint32_t s1, uint32_t s2, Emit& emit) {
+  if (B.sid[s2] < B.sid[s1]) { uint32_t t = s1; s1 = s2; s2 = t; }
+  const d3 p1 = ld3(B.pos, s1), p2 = ld3(B.pos, s2);
+  const double r1 = B.param[3 * s1], r2 = B.param[3 * s2];
+  d3 normal = p1 - p2;
+  const double dist = len3(normal) - r1 - r2;
+  if (dist < kContactThreshold) {
+    normal = nrm3(normal);
+    const double k = r2 + 0.5 * dist;
+    emit(s1, s2, p2 + normal * k, normal, dist);
+  }
+}
+
+// collideSphereBox, MaxContacts.h:589-667: the sphere is body1.
+template <class Emit>
+__device__ void np_sphere_box(const Bodies& B, uint32_t s, uint32_t b, Emit& emit) {
+  const double* L = B.param + 3 * (size_t)b;
+  const double l0 = 0.5 * L[0], l1 = 0.5 * L[1], l2 = 0.5 * L[2];
+  const d3 spos = ld3(B.pos, s), bpos = ld3(B.pos, b);
+  const m33 R = ld9(B.R, b);
+  const double* r = R.a;
+  bool outside = false;
+  const d3 d = spos - bpos;
+  double p[3] = {d.x * r[0] + d.y * r[3] + d.z * r[6], d.x * r[1] + d.y * r[4] + d.z * r[7],
+                 d.x * r[2] + d.y * r[5] + d.z * r[8]};
+  const double l[3] = {l0, l1, l2};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (p[i] < -l[i]) { p[i] = -l[i]; outside = true; }
+    else if (p[i] > l[i]) { p[i] = l[i]; outside = true; }
+  }
+  const double radius = B.param[3 * s];
+  if (!outside) {
+    double dist = fabs(p[0]) - l[0];
+    int face = 0;
+    for (int i = 1; i < 3; ++i) {
+      const double tmp = fabs(p[i]) - l[i];
+      if (dist < tmp) { dist = tmp; face = i; }
+    }
+    double nn[3] = {0.0, 0.0, 0.0};
+    nn[face] = (p[face] > 0.0) ? 1.0 : -1.0;
+    const d3 n = mat_vec(R, mk3(nn[0], nn[1], nn[2]));
+    dist -= radius;
+    emit(s, b, spos, n, dist);
+    return;
+  }
+  const d3 qv = mat_vec(R, mk3(p[0], p[1], p[2]));
+  const d3 n = d - qv;
+  const double dist = len3(n) - radius;
+  if (dist < kContactThreshold) emit(s, b, bpos + qv, nrm3(n), dist);
+}
+
+// collideSpherePlane, MaxContacts.h:777-792: the sphere is body1.
+template <class Emit>
+__device__ void np_sphere_plane(const Bodies& B, uint32_t s, uint32_t p, Emit& emit) {
+  const d3 n = ld3(B.pnormal, p);
+  const d3 x = ld3(B.pos, s);
+  const double r = B.param[3 * s];
+  const double k = dot3(n, x);
+  const double dist = k - r - B.pd[p];
+  if (dist < kContactThreshold) emit(s, p, x - (r + dist) * n, n, dist);
+}
+
+// collideBoxPlane, MaxContacts.h:2380-2528: corners in pe's order, each pre-filtered.
+template <class Emit>
+__device__ void np_box_plane(const Bodies& B, uint32_t b, uint32_t p, Emit& emit) {
+  const d3 bpos = ld3(B.pos, b);
+  const double* L = B.param + 3 * (size_t)b;
+  const m33 R = ld9(B.R, b);
+  const double xr = 0.5 * L[0], yr = 0.5 * L[1], zr = 0.5 * L[2];
+  const d3 pn = ld3(B.pnormal, p);
+  const double pdist = B.pd[p];
+  const d3 nrel = mat_tvec(R, pn);
+  const double mn = fmin(fmin(xr, yr), zr);
+  const double minlength = -0.99 * mn;
+  // corner sign patterns (-,-,-) (+,-,-) (+,+,-) (-,+,-) (-,-,+) (+,-,+) (+,+,+) (-,+,+)
+  const unsigned sx = 0x66u, sy = 0xCCu, sz = 0xF0u;  // bit c set = '+' for corner c
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const bool px = (sx >> c) & 1u, py = (sy >> c) & 1u, pz = (sz >> c) & 1u;
+    const double cx = px ? xr : -xr, cy = py ? yr : -yr, cz = pz ? zr : -zr;
+    double t = cx * nrel.x;
+    t = py ? t + yr * nrel.y : t - yr * nrel.y;
+    t = pz ? t + zr * nrel.z : t - zr * nrel.z;
+    if (t < minlength) {
+      const d3 pos = bpos + mat_vec(R, mk3(cx, cy, cz));
+      const double k = dot3(pos, pn);
+      const double dist = k - pdist;
+      if (dist < kContactThreshold) emit(b, p, pos, pn, dist);
+    }
+  }
+}
+
+// MaxContacts::collide dispatch for (a, b) as handed over by the broadphase
+// (MaxContacts.h:242-526).  Box-box pairs are handled by their own kernel.
+template <class Emit>
+__device__ void narrowphase(const Bodies& B, uint32_t a, uint32_t b, Emit& emit) {
+  const int ta = B.shape[a], tb = B.shape[b];
+  if (ta == kShapeSphere) {
+    if (tb == kShapeSphere) np_sphere_sphere(B, a, b, emit);
+    else if (tb == kShapeBox) np_sphere_box(B, a, b, emit);
+    else np_sphere_plane(B, a, b, emit);
+  } else if (ta == kShapeBox) {
+    if (tb == kShapeSphere) np_sphere_box(B, b, a, emit);
+    else if (tb == kShapePlane) np_box_plane(B, a, b, emit);
+  } else {
+    if (tb == kShapeSphere) np_sphere_plane(B, b, a, emit);
+    else if (tb == kShapeBox) np_box_plane(B, b, a, emit);
+  }
+}
+
+// Contact friction as chosen by Contact::Contact (src/core/contact/Contact.cpp:83-105):
+// dynamic if the tangential relative speed exceeds frictionThreshold, else static.
+__device__ inline double contact_friction(const Bodies& B, const double* csf, const double* cdf,
+                                          uint32_t nmat, uint32_t b1, uint32_t b2, d3 gpos, d3 n) {
+  const d3 x1 = ld3(B.pos, b1), x2 = ld3(B.pos, b2);
+  const d3 v1 = ld3(B.v, b1) + cross3(ld3(B.w, b1), gpos - x1);  // RigidBody::velFromWF, RigidBody.h:1080-1085
+  const d3 v2 = ld3(B.v, b2) + cross3(ld3(B.w, b2), gpos - x2);
+  const d3 rvel = v1 - v2;
+  const double nvel = dot3(n, rvel);
+  const double tvel = dot3(n, rvel - n * nvel);
+  const size_t idx = (size_t)B.mat[b1] * nmat + B.mat[b2];
+  return (fabs(tvel) > kFrictionThreshold) ? cdf[idx] : csf[idx];
+}
+
+// std::max(a, b) as the reference evaluates it: (a < b) ? b : a.
+PE_HD double smax(double a, double b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------------------------------------
+// Solver contacts: field-major arrays (field f of contact i at base[f * cap + i]).
+// ---------------------------------------------------------------------------------------------
+enum ContactField {
+  CF_R1 = 0, CF_R2 = 3, CF_N = 6, CF_T = 9, CF_O = 12,
+  CF_DDT = 15,   // dist * (1/dt), dist already scaled by erp if negative
+  CF_MU = 16, CF_DNI = 17,
+  CF_DINV = 18,  // 9: inverse of the 3x3 block in the contact frame
+  CF_DTOI = 27,  // 4: inverse of its tangential 2x2 block
+  CF_DTO = 31,   // 4: the tangential 2x2 block (golden-section objective)
+  CF_P = 35,     // 3: impulse, world frame
+  CF_COUNT = 38
+};
+
+struct SolverContacts {
+  size_t cap;
+  uint32_t* b1;  // body index | kFixedBit
+  uint32_t* b2;
+  double* f;     // CF_COUNT * cap
+};
+
+__device__ __forceinline__ d3 cf_ld3(const double* f, size_t cap, int fld, size_t i) {
+  return mk3(f[fld * cap + i], f[(fld + 1) * cap + i], f[(fld + 2) * cap + i]);
+}
+
+// One contact's NBGS update: HCSITS.h:2341-2536 (InelasticCoulombContactByDecoupling),
+// 2141-2290 (ApproximateInelasticCoulombContactByDecoupling), 2048-2093
+// (InelasticFrictionlessContact).  Returns |p_old - p_new|_inf (or |p_old|_inf for a
+// separating contact).  The caller guarantees that no other thread touches the
+// non-fixed bodies of contact i concurrently.
+template <int MODEL>
+__device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
+                                       double* __restrict__ dvw, double dtinv, int maxSub) {
+  const size_t cap = C.cap;
+  double* f = C.f;
+  const uint32_t e1 = C.b1[i], e2 = C.b2[i];
+  const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
+  const bool fx1 = (e1 & kFixedBit) != 0, fx2 = (e2 & kFixedBit) != 0;
+
+  const double* s1 = sb + (size_t)kSB * i1;
+  const double* s2 = sb + (size_t)kSB * i2;
+  const d3 v1 = mk3(s1[SB_V], s1[SB_V + 1], s1[SB_V + 2]);
+  const d3 w1 = mk3(s1[SB_W], s1[SB_W + 1], s1[SB_W + 2]);
+  const d3 v2 = mk3(s2[SB_V], s2[SB_V + 1], s2[SB_V + 2]);
+  const d3 w2 = mk3(s2[SB_W], s2[SB_W + 1], s2[SB_W + 2]);
+  const double im1 = s1[SB_IM], im2 = s2[SB_IM];
+  m33 I1, I2;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) { I1.a[k] = s1[SB_IW + k]; I2.a[k] = s2[SB_IW + k]; }
+  double* q1 = dvw + (size_t)kDVW * i1;
+  double* q2 = dvw + (size_t)kDVW * i2;
+  d3 dv1 = mk3(0, 0, 0), dw1 = mk3(0, 0, 0), dv2 = mk3(0, 0, 0), dw2 = mk3(0, 0, 0);
+  if (!fx1) { dv1 = mk3(q1[0], q1[1], q1[2]); dw1 = mk3(q1[3], q1[4], q1[5]); }
+  if (!fx2) { dv2 = mk3(q2[0], q2[1], q2[2]); dw2 = mk3(q2[3], q2[4], q2[5]); }
+
+  const d3 r1 = cf_ld3(f, cap, CF_R1, i), r2 = cf_ld3(f, cap, CF_R2, i);
+  const d3 n = cf_ld3(f, cap, CF_N, i), t = cf_ld3(f, cap, CF_T, i), o = cf_ld3(f, cap, CF_O, i);
+  const double ddt = f[CF_DDT * cap + i];
+  d3 p = cf_ld3(f, cap, CF_P, i);
+
+  // Remove this contact's reaction (HCSITS.h:2343-2346).
+  dv1 = dv1 - im1 * p;
+  dw1 = dw1 - mat_vec(I1, cross3(r1, p));
+  dv2 = dv2 + im2 * p;
+  dw2 = dw2 + mat_vec(I2, cross3(r2, p));
+
+  // Relative contact velocity in the contact frame (HCSITS.h:2349-2360).
+  const d3 gdot = (v1 + dv1) - (v2 + dv2) + cross3(w1 + dw1, r1) - cross3(w2 + dw2, r2);
+  const m33 cfm = mat_cols(n, t, o);
+  d3 g = mat_tvec(cfm, gdot);
+  g.x += ddt;
+
+  double delta;
+  if (g.x >= 0) {
+    delta = fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
+    p = mk3(0, 0, 0);
+  } else {
+    d3 pwf;
+    if (MODEL == 0) {
+      pwf = n * (-f[CF_DNI * cap + i] * g.x);
+    } else {
+      m33 Dinv;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Dinv.a[k] = f[(CF_DINV + k) * cap + i];
+      const double mu = f[CF_MU * cap + i];
+      double pc[3];
+      {
+        const d3 s = -mat_vec(Dinv, g);
+        pc[0] = s.x; pc[1] = s.y; pc[2] = s.z;
+      }
+      double flimit = mu * pc[0];
+      double fsq = pc[1] * pc[1] + pc[2] * pc[2];
+      if (fsq > flimit * flimit || pc[0] < 0) {
+        const double dni = f[CF_DNI * cap + i];
+        const double T00 = f[(CF_DTOI + 0) * cap + i], T01 = f[(CF_DTOI + 1) * cap + i];
+        const double T10 = f[(CF_DTOI + 2) * cap + i], T11 = f[(CF_DTOI + 3) * cap + i];
+        if (MODEL == 1) {
+          // ApproximateInelasticCoulombContactByDecoupling, HCSITS.h:2186-2226
+          d3 ptmp = dot3(t, p) * t + dot3(o, p) * o;
+          const double gn = g.x + dot3(n, cross3(mat_vec(I1, cross3(r1, ptmp)), r1) +
+                                             cross3(mat_vec(I2, cross3(r2, ptmp)), r2));
+          pc[0] = -(dni * gn);
+          pc[0] = smax(0.0, pc[0]);
+          ptmp = n * pc[0];
+          const d3 g2 = gdot + cross3(mat_vec(I1, cross3(r1, ptmp)), r1) + cross3(mat_vec(I2, cross3(r2, ptmp)), r2);
+          const double gt = dot3(t, g2), go = dot3(o, g2);
+          pc[1] = -(T00 * gt + T01 * go);
+          pc[2] = -(T10 * gt + T11 * go);
+          flimit = mu * pc[0];
+          fsq = pc[1] * pc[1] + pc[2] * pc[2];
+          if (fsq > flimit * flimit) {
+            const double fr = flimit / sqrt(fsq);
+            pc[1] *= fr;
+            pc[2] *= fr;
+          }
+        } else {
+          // InelasticCoulombContactByDecoupling: 20 decoupled passes, HCSITS.h:2395-2520
+          const double A00 = f[(CF_DTO + 0) * cap + i], A01 = f[(CF_DTO + 1) * cap + i];
+          const double A10 = f[(CF_DTO + 2) * cap + i], A11 = f[(CF_DTO + 3) * cap + i];
+          for (int j = 0; j < 20; ++j) {
+            pc[0] = 0;
+            const d3 tp = t * pc[1] + o * pc[2];
+            d3 gc = gdot + cross3(mat_vec(I1, cross3(r1, tp)), r1) + cross3(mat_vec(I2, cross3(r2, tp)), r2);
+            const double gcn = dot3(n, gc) + ddt;
+            pc[0] = smax(0.0, -(dni * gcn));
+            pc[1] = pc[2] = 0.0;
+            const d3 np0 = n * pc[0];
+            gc = gdot + cross3(mat_vec(I1, cross3(r1, np0)), r1) + cross3(mat_vec(I2, cross3(r2, np0)), r2);
+            const double gt = dot3(t, gc), go = dot3(o, gc);
+            pc[1] = -(T00 * gt + T01 * go);
+            pc[2] = -(T10 * gt + T11 * go);
+            flimit = mu * pc[0];
+            fsq = pc[1] * pc[1] + pc[2] * pc[2];
+            if (fsq > flimit * flimit) {
+              const double fr = flimit / sqrt(fsq);
+              const double phi = 0.5 * (1.0 + sqrt(5.0));
+              const double shift = atan2(-pc[2], pc[1]);
+              const double acos_f = acos(fr);
+              double al = -acos_f - shift;
+              double ar = acos_f - shift;
+              double am = (ar + al * phi) / (1 + phi);
+              double xm0 = flimit * cos(am), xm1 = flimit * sin(am);
+              double fm = 0.5 * xm0 * (A00 * xm0 + A01 * xm1) + 0.5 * xm1 * (A10 * xm0 + A11 * xm1) -
+                          (xm0 * (-gt) + xm1 * (-go));
+              bool leftlarger = false;
+              for (int k = 0; k < maxSub; ++k) {
+                const double an = al + (ar - am);
+                const double xn0 = flimit * cos(an), xn1 = flimit * sin(an);
+                const double fn = 0.5 * xn0 * (A00 * xn0 + A01 * xn1) + 0.5 * xn1 * (A10 * xn0 + A11 * xn1) -
+                                  (xn0 * (-gt) + xn1 * (-go));
+                if (leftlarger) {
+                  if (fn < fm) { ar = am; am = an; xm0 = xn0; xm1 = xn1; fm = fn; leftlarger = true; }
+                  else { al = an; leftlarger = false; }
+                } else {
+                  if (fn < fm) { al = am; am = an; xm0 = xn0; xm1 = xn1; fm = fn; leftlarger = false; }
+                  else { ar = an; leftlarger = true; }
+                }
+              }
+              pc[1] = xm0;
+              pc[2] = xm1;
+            }
+          }
+        }
+      }
+      pwf = mat_vec(cfm, mk3(pc[0], pc[1], pc[2]));
+    }
+    const d3 dp = p - pwf;
+    delta = fmax(fabs(dp.x), fmax(fabs(dp.y), fabs(dp.z)));
+    p = pwf;
+    // Apply the new reaction (HCSITS.h:2532-2535).
+    dv1 = dv1 + im1 * p;
+    dw1 = dw1 + mat_vec(I1, cross3(r1, p));
+    dv2 = dv2 - im2 * p;
+    dw2 = dw2 - mat_vec(I2, cross3(r2, p));
+  }
+  f[CF_P * cap + i] = p.x;
+  f[(CF_P + 1) * cap + i] = p.y;
+  f[(CF_P + 2) * cap + i] = p.z;
+  if (!fx1) { q1[0] = dv1.x; q1[1] = dv1.y; q1[2] = dv1.z; q1[3] = dw1.x; q1[4] = dw1.y; q1[5] = dw1.z; }
+  if (!fx2) { q2[0] = dv2.x; q2[1] = dv2.y; q2[2] = dv2.z; q2[3] = dw2.x; q2[4] = dw2.y; q2[5] = dw2.z; }
+  return delta;
+}
+
+}  // namespace pe_amd

@@ -1,0 +1,278 @@
+// Solver kernels: contact-graph colouring, contact caching and the NBGS sweeps.
+#include "kernels.hpp"
+
+namespace pe_amd {
+
+// ---------------------------------------------------------------------------------------------
+// Conflict graph.  Two contacts conflict when they share a non-fixed body (fixed bodies have
+// zero inverse mass and inertia, so their velocity corrections never change).  The graph is
+// held as body -> contacts adjacency lists.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_adj_count(RawContacts R, const uint8_t* fixed, uint32_t nc, uint32_t* deg) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const uint32_t b1 = R.b1[c], b2 = R.b2[c];
+  if (!fixed[b1]) atomicAdd(&deg[b1], 1u);
+  if (!fixed[b2]) atomicAdd(&deg[b2], 1u);
+}
+
+__global__ void k_adj_fill(RawContacts R, const uint8_t* fixed, uint32_t nc, const uint32_t* start,
+                           uint32_t* cursor, uint32_t* adj) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const uint32_t b1 = R.b1[c], b2 = R.b2[c];
+  if (!fixed[b1]) adj[start[b1] + atomicAdd(&cursor[b1], 1u)] = c;
+  if (!fixed[b2]) adj[start[b2] + atomicAdd(&cursor[b2], 1u)] = c;
+}
+
+void launch_adjacency(const Bodies& B, const RawContacts& R, uint32_t nc, ColourBuffers& C,
+                      uint32_t* scan_scratch, hipStream_t s) {
+  fill_u32(C.deg, 0u, B.n, s);
+  fill_u32(C.adj_cursor, 0u, B.n, s);
+  if (nc) hipLaunchKernelGGL(k_adj_count, dim3((nc + 255) / 256), dim3(256), 0, s, R, B.fixed, nc, C.deg);
+  exclusive_scan_u32(C.deg, C.adj_start, B.n, C.adj_start + B.n, scan_scratch, s);
+  if (nc) hipLaunchKernelGGL(k_adj_fill, dim3((nc + 255) / 256), dim3(256), 0, s, R, B.fixed, nc, C.adj_start,
+                             C.adj_cursor, C.adj);
+}
+
+// Priority of a contact: conflict degree first (largest-degree-first keeps the colour count
+// low), then a hash of the index; ties are impossible because the index is unique.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ uint64_t prio_key(uint32_t deg, uint32_t c) {
+  return ((uint64_t)deg << 32) | (uint64_t)mix32(c);
+}
+
+__global__ void k_colour_init(RawContacts R, const uint8_t* fixed, uint32_t nc, const uint32_t* start,
+                              uint32_t* prio, uint32_t* colour, uint32_t* cround, uint32_t* remaining) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *remaining = nc;
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const uint32_t b1 = R.b1[c], b2 = R.b2[c];
+  uint32_t d = 0;
+  if (!fixed[b1]) d += start[b1 + 1] - start[b1] - 1;
+  if (!fixed[b2]) d += start[b2 + 1] - start[b2] - 1;
+  prio[c] = d;
+  colour[c] = kUncoloured;
+  cround[c] = kUncoloured;
+}
+
+// One Jones-Plassmann round r: an uncoloured contact whose priority beats every uncoloured
+// neighbour takes the smallest colour that no neighbour coloured in an earlier round has.
+// A neighbour coloured in this same round counts as uncoloured (it was a local maximum, so
+// it outranks this contact), hence two neighbours are never coloured in one round and the
+// result depends neither on thread timing nor on the order of the adjacency lists.
+__global__ void k_colour_round(RawContacts R, const uint8_t* fixed, uint32_t nc, const uint32_t* start,
+                               const uint32_t* adj, const uint32_t* prio, uint32_t* colour, uint32_t* cround,
+                               uint32_t round, uint32_t* remaining, uint32_t* flags) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  if (cround[c] < round) return;
+  const uint64_t my = prio_key(prio[c], c);
+  uint64_t used[kMaxColours / 64] = {0, 0};
+  const uint32_t ends[2] = {R.b1[c], R.b2[c]};
+  for (int e = 0; e < 2; ++e) {
+    const uint32_t b = ends[e];
+    if (fixed[b]) continue;
+    for (uint32_t k = start[b]; k < start[b + 1]; ++k) {
+      const uint32_t o = adj[k];
+      if (o == c) continue;
+      if (cround[o] < round) {
+        const uint32_t oc = colour[o];
+        used[oc >> 6] |= 1ull << (oc & 63);
+      } else if (prio_key(prio[o], o) > my) {
+        return;  // a neighbour goes first
+      }
+    }
+  }
+  uint32_t pick = kMaxColours;
+  for (uint32_t w = 0; w < kMaxColours / 64; ++w) {
+    if (~used[w]) { pick = w * 64 + (uint32_t)__builtin_ctzll(~used[w]); break; }
+  }
+  if (pick >= kMaxColours) { atomicOr(flags, ERR_COLOURS); pick = kMaxColours - 1; }
+  colour[c] = pick;
+  cround[c] = round;
+  atomicSub(remaining, 1u);
+}
+
+__global__ void k_colour_hist(const uint32_t* colour, uint32_t nc, uint32_t* hist) {
+  __shared__ uint32_t h[kMaxColours];
+  if (threadIdx.x < kMaxColours) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    atomicAdd(&h[colour[c]], 1u);
+  __syncthreads();
+  if (threadIdx.x < kMaxColours && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+void launch_colour_init(const RawContacts& R, const Bodies& B, uint32_t nc, ColourBuffers& C, hipStream_t s) {
+  hipLaunchKernelGGL(k_colour_init, dim3((nc + 255) / 256 + 1), dim3(256), 0, s, R, B.fixed, nc, C.adj_start,
+                     C.prio, C.colour, C.cround, C.remaining);
+}
+
+void launch_colour_round(const RawContacts& R, const Bodies& B, uint32_t nc, ColourBuffers& C, uint32_t round,
+                         uint32_t* flags, hipStream_t s) {
+  if (nc) hipLaunchKernelGGL(k_colour_round, dim3((nc + 255) / 256), dim3(256), 0, s, R, B.fixed, nc, C.adj_start,
+                             C.adj, C.prio, C.colour, C.cround, round, C.remaining, flags);
+}
+
+void launch_colour_hist(uint32_t nc, ColourBuffers& C, hipStream_t s) {
+  fill_u32(C.hist, 0u, kMaxColours, s);
+  if (nc) {
+    uint32_t nb = (nc + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_colour_hist, dim3(nb), dim3(256), 0, s, C.colour, nc, C.hist);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Contact caching, HCSITS.h:1741-1818: slot s of the solver arrays receives raw contact
+// perm[s] -- lever arms, contact frame, error-reduced distance, friction and the Delassus
+// block in the contact frame with its inverses.  Impulses start at zero (no warm start).
+// ---------------------------------------------------------------------------------------------
+// counters[0]: max penetration (bits of a non-negative double); counters[1]: contacts that
+// involve a fixed body.
+__global__ void k_setup(Bodies B, RawContacts R, const uint32_t* perm, uint32_t nc, SolverContacts C,
+                        double erp, double dtinv, unsigned long long* counters) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nc) return;
+  unsigned long long* maxpen = counters;
+  const uint32_t c = perm[s];
+  const uint32_t i1 = R.b1[c], i2 = R.b2[c];
+  if (B.fixed[i1] || B.fixed[i2]) atomicAdd(&counters[1], 1ull);
+  const d3 pos = ld3(R.pos, c);
+  const d3 r1 = pos - ld3(B.pos, i1);
+  const d3 r2 = pos - ld3(B.pos, i2);
+  const d3 n = ld3(R.nrm, c);
+  d3 t;
+  if (fabs(n.x) < fabs(n.y)) {
+    if (fabs(n.x) < fabs(n.z)) t = mk3(0, n.z, -n.y);
+    else t = mk3(n.y, -n.x, 0);
+  } else {
+    if (fabs(n.y) < fabs(n.z)) t = mk3(-n.z, 0, n.x);
+    else t = mk3(n.y, -n.x, 0);
+  }
+  t = nrm3(t);
+  const d3 o = cross3(n, t);
+  const m33 cf = mat_cols(n, t, o);
+  double dist = R.dist[c];
+  if (dist < 0) {
+    atomicMax(maxpen, (unsigned long long)__double_as_longlong(-dist));
+    dist *= erp;
+  }
+  const double* s1 = B.sb + (size_t)kSB * i1;
+  const double* s2 = B.sb + (size_t)kSB * i2;
+  m33 I1, I2;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) { I1.a[k] = s1[SB_IW + k]; I2.a[k] = s2[SB_IW + k]; }
+  const double im1 = s1[SB_IM], im2 = s2[SB_IM];
+  const m33 A = mul_skew(skew_mul(r1, I1), r1);
+  const m33 Bm = mul_skew(skew_mul(r2, I2), r2);
+  m33 D;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) D.a[k] = -(A.a[k] + Bm.a[k]);
+  D.a[0] += im1 + im2;
+  D.a[4] += im1 + im2;
+  D.a[8] += im1 + im2;
+  D = mat_mul(mat_mul(mat_trans(cf), D), cf);
+  const m33 Di = mat_inv(D);
+  double det = D.a[4] * D.a[8] - D.a[5] * D.a[7];  // Matrix2x2::getInverse, Matrix2x2.h:977-988
+  det = 1.0 / det;
+  const size_t cap = C.cap;
+  double* f = C.f;
+  const d3 vals[5] = {r1, r2, n, t, o};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    f[(3 * k) * cap + s] = vals[k].x;
+    f[(3 * k + 1) * cap + s] = vals[k].y;
+    f[(3 * k + 2) * cap + s] = vals[k].z;
+  }
+  f[CF_DDT * cap + s] = dist * dtinv;
+  f[CF_MU * cap + s] = R.mu[c];
+  f[CF_DNI * cap + s] = 1.0 / D.a[0];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) f[(CF_DINV + k) * cap + s] = Di.a[k];
+  f[(CF_DTOI + 0) * cap + s] = det * D.a[8];
+  f[(CF_DTOI + 1) * cap + s] = -det * D.a[5];
+  f[(CF_DTOI + 2) * cap + s] = -det * D.a[7];
+  f[(CF_DTOI + 3) * cap + s] = det * D.a[4];
+  f[(CF_DTO + 0) * cap + s] = D.a[4];
+  f[(CF_DTO + 1) * cap + s] = D.a[5];
+  f[(CF_DTO + 2) * cap + s] = D.a[7];
+  f[(CF_DTO + 3) * cap + s] = D.a[8];
+  f[CF_P * cap + s] = 0.0;
+  f[(CF_P + 1) * cap + s] = 0.0;
+  f[(CF_P + 2) * cap + s] = 0.0;
+  C.b1[s] = i1 | (B.fixed[i1] ? kFixedBit : 0u);
+  C.b2[s] = i2 | (B.fixed[i2] ? kFixedBit : 0u);
+}
+
+void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, uint32_t nc,
+                  SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen, hipStream_t s) {
+  if (nc) hipLaunchKernelGGL(k_setup, dim3((nc + 255) / 256), dim3(256), 0, s, B, R, perm, nc, C, erp, dtinv, maxpen);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Sweeps.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void wave_max_atomic(double v, unsigned long long* out) {
+  // non-negative doubles order like their bit patterns
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(u, d, 64);
+    u = o > u ? o : u;
+  }
+  if ((threadIdx.x & 63) == 0 && u) atomicMax(out, u);
+}
+
+// One colour: every contact in [begin, end) is independent of the others.
+template <int MODEL>
+__global__ void __launch_bounds__(256) k_sweep(SolverContacts C, uint32_t begin, uint32_t end, const double* sb,
+                                               double* dvw, double dtinv, int maxSub, unsigned long long* delta) {
+  const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0;
+  if (i < end) d = relax_contact<MODEL>(C, i, sb, dvw, dtinv, maxSub);
+  wave_max_atomic(d, delta);
+}
+
+// All sweeps in the given contact order, one contact at a time (lane 0 of one wavefront).
+template <int MODEL>
+__global__ void k_sweep_serial(SolverContacts C, uint32_t nc, uint32_t iters, const double* sb, double* dvw,
+                               double dtinv, int maxSub, double thr, unsigned long long* deltas) {
+  if (threadIdx.x != 0) return;
+  uint32_t it = 0;
+  while (it < iters) {
+    double dmax = 0;
+    for (uint32_t i = 0; i < nc; ++i) dmax = smax(dmax, relax_contact<MODEL>(C, i, sb, dvw, dtinv, maxSub));
+    deltas[it] = (unsigned long long)__double_as_longlong(dmax);
+    ++it;
+    if (thr > 0 && dmax < thr) break;
+  }
+  deltas[iters] = it;
+}
+
+void launch_sweep(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
+                  double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
+  const uint32_t n = end - begin;
+  if (!n) return;
+  const dim3 g((n + 255) / 256), b(256);
+  switch (model) {
+    case 0: hipLaunchKernelGGL(k_sweep<0>, g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    case 1: hipLaunchKernelGGL(k_sweep<1>, g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    default: hipLaunchKernelGGL(k_sweep<3>, g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+  }
+}
+
+void launch_sweep_serial(int model, const SolverContacts& C, uint32_t nc, uint32_t iters, const Bodies& B,
+                         double dtinv, int maxSub, double thr, unsigned long long* deltas, hipStream_t s) {
+  switch (model) {
+    case 0: hipLaunchKernelGGL(k_sweep_serial<0>, dim3(1), dim3(64), 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 1: hipLaunchKernelGGL(k_sweep_serial<1>, dim3(1), dim3(64), 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    default: hipLaunchKernelGGL(k_sweep_serial<3>, dim3(1), dim3(64), 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+  }
+}
+
+}  // namespace pe_amd

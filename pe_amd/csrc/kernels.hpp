@@ -1,0 +1,116 @@
+// Launchers of the HIP kernels of the time step (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.hpp"
+
+namespace pe_amd {
+
+// Hash-grid hierarchy: level l has cubic cells of edge span[l] (ascending); every
+// level uses the same power-of-two periodic bucket table of dimx*dimy*dimz cells.
+constexpr int kMaxLevels = 8;
+struct GridParams {
+  int nlevels;
+  double span[kMaxLevels];
+  double inv[kMaxLevels];
+  uint32_t dimx, dimy, dimz;
+  uint32_t table;  // dimx*dimy*dimz
+};
+
+// Raw contacts as emitted by the narrowphase (detection order).
+struct RawContacts {
+  size_t cap;
+  uint32_t* b1;
+  uint32_t* b2;
+  double* pos;   // 3*cap
+  double* nrm;   // 3*cap
+  double* dist;
+  double* mu;
+};
+
+// Error flags raised by kernels (read back once per step).
+enum : uint32_t {
+  ERR_NONFINITE = 1u,
+  ERR_LEVEL_OVERFLOW = 2u,
+  ERR_BOX_BOX = 4u,
+  ERR_COLOURS = 8u
+};
+
+struct BroadphaseBuffers {
+  uint32_t* keys;      // n
+  uint32_t* vals;      // n
+  uint32_t* keys_tmp;  // n
+  uint32_t* vals_tmp;  // n
+  uint32_t* cell_start;  // nlevels*table
+  uint32_t* cell_end;    // nlevels*table
+  const uint32_t* planes;  // plane body indices
+  uint32_t nplanes;
+  uint32_t* counts;    // n+1 per sorted slot
+  uint32_t* offsets;   // n+1
+  uint32_t* flags;     // error flags
+};
+
+// Scratch of the device-wide primitives: sized by scan_scratch_words / sort_scratch_words.
+size_t scan_scratch_words(size_t n);
+size_t sort_scratch_words(size_t n);
+// Exclusive prefix sum of n u32 values; writes the total to *total (device pointer).
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total,
+                        uint32_t* scratch, hipStream_t s);
+// Stable LSD radix sort of (key, value) pairs on the low `bits` key bits; the result
+// ends in keys/vals (tmp arrays are ping-pong space).
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
+                      size_t n, int bits, uint32_t* scratch, hipStream_t s);
+
+void launch_aabb(const Bodies& B, hipStream_t s);
+void launch_predict(const Bodies& B, d3 g, double dt, hipStream_t s);
+void launch_integrate(const Bodies& B, double dt, double damping, hipStream_t s);
+uint32_t launch_kinetic(const Bodies& B, double* partial, hipStream_t s);
+
+// Broadphase: cell keys + sort + cell ranges.
+void launch_cell_keys(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, hipStream_t s);
+void launch_cell_ranges(uint32_t n, const GridParams& G, BroadphaseBuffers& P, hipStream_t s);
+// Pair enumeration + narrowphase; count pass (write = false) fills P.counts,
+// write pass emits into R at P.offsets.
+void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, const RawContacts& R,
+                  const double* csf, const double* cdf, uint32_t nmat, bool write, hipStream_t s);
+
+// Colouring of the contact conflict graph.
+struct ColourBuffers {
+  uint32_t* deg;        // n bodies
+  uint32_t* adj_start;  // n+1
+  uint32_t* adj_cursor; // n
+  uint32_t* adj;        // 2*nc
+  uint32_t* colour;     // nc
+  uint32_t* prio;       // nc
+  uint32_t* cround;     // nc: round in which the contact was coloured
+  uint32_t* remaining;  // 1
+  uint32_t* hist;       // 128
+};
+constexpr uint32_t kMaxColours = 128;
+constexpr uint32_t kUncoloured = 0xFFFFFFFFu;
+void launch_adjacency(const Bodies& B, const RawContacts& R, uint32_t nc, ColourBuffers& C,
+                      uint32_t* scan_scratch, hipStream_t s);
+void launch_colour_init(const RawContacts& R, const Bodies& B, uint32_t nc, ColourBuffers& C, hipStream_t s);
+void launch_colour_round(const RawContacts& R, const Bodies& B, uint32_t nc, ColourBuffers& C, uint32_t round,
+                         uint32_t* flags, hipStream_t s);
+void fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t s);
+void fill_u64(unsigned long long* p, unsigned long long value, size_t n, hipStream_t s);
+void iota_u32(uint32_t* p, size_t n, hipStream_t s);
+void launch_colour_hist(uint32_t nc, ColourBuffers& C, hipStream_t s);
+
+// Contact caching (HCSITS.h:1741-1818) in solver order perm[0..nc).
+void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, uint32_t nc,
+                  SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen,
+                  hipStream_t s);
+// One colour of one sweep.
+void launch_sweep(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
+                  double dtinv, int maxSub, unsigned long long* delta, hipStream_t s);
+// All sweeps, strictly sequential, one wavefront.  deltas[it] receives the L-inf impulse
+// change of sweep it; deltas[iters] the number of sweeps run (stops early below thr > 0).
+void launch_sweep_serial(int model, const SolverContacts& C, uint32_t nc, uint32_t iters,
+                         const Bodies& B, double dtinv, int maxSub, double thr,
+                         unsigned long long* deltas, hipStream_t s);
+
+}  // namespace pe_amd

@@ -1,0 +1,197 @@
+"""Deterministic synthetic scenes for the time step (the benchmark workloads).
+
+Each scene is a creation sequence in pe's terms: materials, then planes (as in
+examples/lattice/Lattice.cpp:204-210, +z -z +y -y +x -x), then spheres and boxes.
+Random numbers come from numpy's PCG64 seeded with 12345 + config id; lattice
+sites are generated z-major, then y, then x.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._abi import BOX, PLANE, SPHERE
+from .world import mat3_inverse, normalize, quat_axis_angle, ACCURACY
+
+
+@dataclass
+class Scene:
+    name: str
+    materials: list                      # dicts: name, density, cor, csf, cdf
+    planes: list                         # (normal, d, material)
+    sphere_pos: np.ndarray = None        # (ns, 3)
+    sphere_r: np.ndarray = None          # (ns,)
+    sphere_mat: np.ndarray = None        # (ns,)
+    box_pos: np.ndarray = None           # (nb, 3)
+    box_len: np.ndarray = None           # (nb, 3)
+    box_q: np.ndarray = None             # (nb, 4) orientation
+    box_mat: np.ndarray = None
+    v0: np.ndarray = None                # (n_total, 3) initial linear velocities
+    gravity: tuple = (0.0, 0.0, -9.81)
+    dt: float = 1e-3
+    steps: int = 1
+    max_iterations: int = 100
+    model: int = 3
+    pair_friction: list = field(default_factory=list)
+
+    @property
+    def n_spheres(self):
+        return 0 if self.sphere_pos is None else len(self.sphere_pos)
+
+    @property
+    def n_boxes(self):
+        return 0 if self.box_pos is None else len(self.box_pos)
+
+    @property
+    def n_bodies(self):
+        return len(self.planes) + self.n_spheres + self.n_boxes
+
+    def arrays(self):
+        """Body arrays (SoA) in creation order with pe's mass properties."""
+        dens = np.array([m["density"] for m in self.materials])
+        npl, ns, nb = len(self.planes), self.n_spheres, self.n_boxes
+        n = npl + ns + nb
+        a = {"shape": np.zeros(n, np.uint8), "mat": np.zeros(n, np.uint32), "fixed": np.zeros(n, np.uint8),
+             "param": np.zeros((n, 3)), "pos": np.zeros((n, 3)), "q": np.zeros((n, 4)), "v": np.zeros((n, 3)),
+             "w": np.zeros((n, 3)), "inv_mass": np.zeros(n), "I": np.zeros((n, 9)), "Iinv": np.zeros((n, 9)),
+             "normal": np.zeros((n, 3)), "d": np.zeros(n)}
+        a["q"][:, 0] = 1.0
+        for k, (nrm, d, mat) in enumerate(self.planes):
+            nv = normalize(nrm)
+            if nv[0] * nv[0] + nv[1] * nv[1] < ACCURACY:
+                q = quat_axis_angle((1.0, 0.0, 0.0), math.acos(nv[2]))
+            else:
+                q = quat_axis_angle((-nv[1], nv[0], 0.0), math.acos(nv[2]))
+            a["shape"][k] = PLANE
+            a["mat"][k] = mat
+            a["fixed"][k] = 1
+            a["pos"][k] = nv * float(d)
+            a["q"][k] = q
+            a["normal"][k] = nv
+            a["d"][k] = float(d)
+        if ns:
+            s = slice(npl, npl + ns)
+            r = np.asarray(self.sphere_r, float)
+            rho = dens[np.asarray(self.sphere_mat)]
+            m = 4.0 / 3.0 * math.pi * r * r * r * rho          # SphereBase.h:169-172
+            I = np.zeros((ns, 9))
+            I[:, 0] = I[:, 4] = I[:, 8] = 0.4 * m * r * r      # SphereBase.h:230-234
+            a["shape"][s] = SPHERE
+            a["mat"][s] = self.sphere_mat
+            a["param"][s, 0] = r
+            a["pos"][s] = self.sphere_pos
+            a["inv_mass"][s] = 1.0 / m
+            a["I"][s] = I
+            a["Iinv"][s] = mat3_inverse(I)
+        if nb:
+            s = slice(npl + ns, n)
+            L = np.asarray(self.box_len, float)
+            rho = dens[np.asarray(self.box_mat)]
+            m = L[:, 0] * L[:, 1] * L[:, 2] * rho               # BoxBase.h:200-203
+            I = np.zeros((nb, 9))
+            I[:, 0] = m / 12.0 * (L[:, 1] * L[:, 1] + L[:, 2] * L[:, 2])   # BoxBase.cpp:152-158
+            I[:, 4] = m / 12.0 * (L[:, 0] * L[:, 0] + L[:, 2] * L[:, 2])
+            I[:, 8] = m / 12.0 * (L[:, 0] * L[:, 0] + L[:, 1] * L[:, 1])
+            a["shape"][s] = BOX
+            a["mat"][s] = self.box_mat
+            a["param"][s] = L
+            a["pos"][s] = self.box_pos
+            a["q"][s] = self.box_q
+            a["inv_mass"][s] = 1.0 / m
+            a["I"][s] = I
+            a["Iinv"][s] = mat3_inverse(I)
+        if self.v0 is not None:
+            a["v"][:] = self.v0
+        return a
+
+    def to_world(self, world):
+        """Creates this scene in a pe_amd.World (bulk) and applies its solver settings."""
+        for m in self.materials:
+            world.createMaterial(m["name"], m["density"], m.get("cor", 0.0), m["csf"], m["cdf"])
+        for (m1, m2, csf, cdf) in self.pair_friction:
+            world.setPairFriction(m1, m2, csf, cdf)
+        world.setGravity(*self.gravity)
+        world.setMaxIterations(self.max_iterations)
+        world.setRelaxationModel(self.model)
+        world.add_bodies(self.arrays())
+        return world
+
+    def to_oracle(self, ow):
+        """Creates this scene in the CPU oracle (one create call per body, in order)."""
+        for m in self.materials:
+            ow.create_material(m["density"], m.get("cor", 0.0), m["csf"], m["cdf"])
+        for (m1, m2, csf, cdf) in self.pair_friction:
+            ow.set_pair_friction(m1, m2, csf, cdf)
+        for (nrm, d, mat) in self.planes:
+            ow.create_plane(nrm, d, mat)
+        for k in range(self.n_spheres):
+            ow.create_sphere(self.sphere_pos[k], self.sphere_r[k], int(self.sphere_mat[k]))
+        for k in range(self.n_boxes):
+            ow.create_box(self.box_pos[k], self.box_len[k], int(self.box_mat[k]))
+        if self.v0 is not None or self.n_boxes:
+            st = ow.get_state()
+            if self.v0 is not None:
+                st["v"][:] = self.v0
+            if self.n_boxes:
+                st["q"][len(self.planes) + self.n_spheres:] = self.box_q
+            ow.set_state(st)
+        return ow
+
+
+def box_planes(lx, ly, lz, mat=0):
+    """Six inward planes bounding [0,lx]x[0,ly]x[0,lz] (Lattice.cpp:204-210 order)."""
+    return [((0, 0, 1), 0.0, mat), ((0, 0, -1), -lz, mat), ((0, 1, 0), 0.0, mat), ((0, -1, 0), -ly, mat),
+            ((1, 0, 0), 0.0, mat), ((-1, 0, 0), -lx, mat)]
+
+
+def lattice(nx, ny, nz, spacing, origin, jitter, rng, count=None):
+    """Jittered lattice sites, z-major then y then x; the first `count` sites."""
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = np.stack([origin[0] + spacing * i, origin[1] + spacing * j, origin[2] + spacing * k], axis=-1).reshape(-1, 3)
+    if count is not None:
+        base = base[:count]
+    return base + rng.uniform(-jitter, jitter, size=base.shape)
+
+
+def config1():
+    """1,000 spheres r=0.5 on a jittered 10x10x10 lattice (spacing 1.2) in a 15x15x30 box;
+    mu=0.4 (csf=cdf=0.2 per material), 100 sweeps, dt=1e-3, 500 steps."""
+    rng = np.random.default_rng(12345 + 1)
+    pos = lattice(10, 10, 10, 1.2, (1.5, 1.5, 1.0), 0.05, rng)
+    n = len(pos)
+    return Scene("C1 1k spheres", [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.2, "cdf": 0.2}],
+                 box_planes(15.0, 15.0, 30.0), pos, np.full(n, 0.5), np.zeros(n, np.uint32),
+                 dt=1e-3, steps=500, max_iterations=100)
+
+
+def config2(n_spheres=200_000):
+    """Polydisperse pile: r ~ U[0.25,0.5], jittered lattice (spacing 1.05, jitter 0.02, 57x57 per
+    layer) in a 60x60x200 box, v0 ~ U[-0.5,0.5]^3, mu=0.5, 50 sweeps, dt=5e-4, 2,000 steps."""
+    rng = np.random.default_rng(12345 + 2)
+    per = 57
+    nz = (n_spheres + per * per - 1) // (per * per)
+    pos = lattice(per, per, nz, 1.05, (0.525, 0.525, 0.525), 0.02, rng, count=n_spheres)
+    r = rng.uniform(0.25, 0.5, size=n_spheres)
+    v = rng.uniform(-0.5, 0.5, size=(n_spheres, 3))
+    v0 = np.concatenate([np.zeros((6, 3)), v])
+    return Scene("C2 %d spheres" % n_spheres,
+                 [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.25, "cdf": 0.25}],
+                 box_planes(60.0, 60.0, 200.0), pos, r, np.zeros(n_spheres, np.uint32), v0=v0,
+                 dt=5e-4, steps=2000, max_iterations=50)
+
+
+def drop_scene(nx, ny, nz, r=0.5, spacing=1.2, jitter=0.05, box=(10.0, 10.0, 40.0), mu=0.4, seed=7,
+               v_amp=0.0, dt=1e-3, max_iterations=100, z0=None):
+    """Small jittered-lattice drop into a plane box (test scenes)."""
+    rng = np.random.default_rng(seed)
+    z0 = r + 0.5 * (spacing - 2 * r) if z0 is None else z0
+    pos = lattice(nx, ny, nz, spacing, (r + 0.5 * (spacing - 2 * r) + 0.3, r + 0.5 * (spacing - 2 * r) + 0.3, z0),
+                  jitter, rng)
+    n = len(pos)
+    v0 = None
+    if v_amp:
+        v0 = np.concatenate([np.zeros((6, 3)), rng.uniform(-v_amp, v_amp, size=(n, 3))])
+    return Scene("drop %dx%dx%d" % (nx, ny, nz),
+                 [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": mu / 2, "cdf": mu / 2}],
+                 box_planes(*box), pos, np.full(n, r), np.zeros(n, np.uint32), v0=v0, dt=dt,
+                 max_iterations=max_iterations)

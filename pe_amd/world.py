@@ -1,0 +1,461 @@
+"""Host-side mirror of pe's World / CollisionSystem API over the HIP library.
+
+Names, argument meanings and error behaviour follow the reference:
+``createMaterial`` (src/core/Materials.cpp:175-247), ``createSphere``
+(src/core/rigidbody/Sphere.cpp:1187), ``createBox`` (src/core/rigidbody/Box.cpp:1312),
+``createPlane`` (src/core/rigidbody/Plane.cpp:1202-1240), ``World::run`` /
+``World::simulationStep`` (src/core/World.cpp:330-378) and the HCSITS collision
+system setters (pe/core/collisionsystem/HardContactSemiImplicitTimesteppingSolvers.h:793-840).
+Mass properties are computed on the host exactly as pe computes them; the step
+itself always runs on the GPU.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _abi
+from ._abi import BOX, PLANE, SPHERE, check, ptr
+
+CONTACT_THRESHOLD = 1e-8   # pe/core/Thresholds.h:151
+ACCURACY = 1e-8            # pe/util/Limits.h:385
+
+# CollisionSystem<HCSITS>::RelaxationModel (HCSITS.h:177-184)
+InelasticFrictionlessContact = 0
+ApproximateInelasticCoulombContactByDecoupling = 1
+ApproximateInelasticCoulombContactByOrthogonalProjections = 2
+InelasticCoulombContactByDecoupling = 3
+InelasticCoulombContactByOrthogonalProjections = 4
+InelasticGeneralizedMaximumDissipationContact = 5
+
+
+def mat3_inverse(m):
+    """Cofactor inverse of (..., 9) row-major matrices in pe's evaluation order
+    (Matrix3x3::getInverse, pe/math/Matrix3x3.h:1084-1104)."""
+    v = [m[..., k] for k in range(9)]
+    det = v[0] * ((v[4] * v[8]) - (v[7] * v[5])) + v[1] * ((v[5] * v[6]) - (v[8] * v[3])) + \
+        v[2] * ((v[3] * v[7]) - (v[4] * v[6]))
+    det = 1.0 / det
+    out = np.stack([det * ((v[4] * v[8]) - (v[5] * v[7])), det * ((v[7] * v[2]) - (v[8] * v[1])),
+                    det * ((v[1] * v[5]) - (v[2] * v[4])), det * ((v[5] * v[6]) - (v[3] * v[8])),
+                    det * ((v[8] * v[0]) - (v[6] * v[2])), det * ((v[2] * v[3]) - (v[0] * v[5])),
+                    det * ((v[3] * v[7]) - (v[4] * v[6])), det * ((v[6] * v[1]) - (v[7] * v[0])),
+                    det * ((v[0] * v[4]) - (v[1] * v[3]))], axis=-1)
+    return out
+
+
+def quat_axis_angle(axis, angle):
+    """Quaternion(axis, angle), pe/math/Quaternion.h:274-294 (host libm)."""
+    if abs(angle) < 1e-15:
+        return np.array([1.0, 0.0, 0.0, 0.0])
+    sina = math.sin(angle * 0.5)
+    cosa = math.cos(angle * 0.5)
+    a = normalize(axis)
+    return np.array([cosa, sina * a[0], sina * a[1], sina * a[2]])
+
+
+def normalize(v):
+    """Vector3::normalize (pe/math/Vector3.h:994-1010): reciprocal length, then scale."""
+    x, y, z = float(v[0]), float(v[1]), float(v[2])
+    ln = math.sqrt(x * x + y * y + z * z)
+    if ln == 0.0:
+        return np.array([x, y, z])
+    il = 1.0 / ln
+    return np.array([x * il, y * il, z * il])
+
+
+class Material:
+    def __init__(self, name, density, cor, csf, cdf, poisson, young, stiffness, dampingN, dampingT):
+        self.name, self.density, self.cor, self.csf, self.cdf = name, density, cor, csf, cdf
+        self.poisson, self.young, self.stiffness, self.dampingN, self.dampingT = poisson, young, stiffness, dampingN, dampingT
+
+
+class World:
+    """One pe world (``theWorld()`` + ``theCollisionSystem()`` of the reference)."""
+
+    def __init__(self, device=0):
+        self.device = device
+        self.materials = []
+        self._csf = np.zeros((0, 0))
+        self._cdf = np.zeros((0, 0))
+        self._gravity = (0.0, 0.0, 0.0)
+        self._damping = 1.0
+        self._max_iterations = 100
+        self._model = ApproximateInelasticCoulombContactByDecoupling
+        self._erp = 0.7
+        self._relax_param = 0.9
+        self._abort = 0.0
+        self._max_sub = 20
+        self._sweep_mode = _abi.SWEEP_COLOURED
+        self._next_sid = 1
+        self._shape, self._sid, self._mat, self._fixed = [], [], [], []
+        self._param, self._pos, self._q, self._v, self._w = [], [], [], [], []
+        self._inv_mass, self._I, self._Iinv, self._normal, self._d = [], [], [], [], []
+        self._arrays = None      # host SoA snapshot (numpy) when built
+        self._ctx = None
+        self._bodies_dirty = True
+        self._params_dirty = True
+        self._state_on_device = False
+        self.timestep = 0
+
+    # ------------------------------------------------------------------ settings
+    def setGravity(self, gx, gy=None, gz=None):
+        if gy is None:
+            gx, gy, gz = gx
+        self._gravity = (float(gx), float(gy), float(gz))
+        self._params_dirty = True
+
+    def getGravity(self):
+        return self._gravity
+
+    def setDamping(self, damping):
+        if damping < 0 or damping > 1:
+            raise ValueError("Invalid damping value")
+        self._damping = float(damping)
+        self._params_dirty = True
+
+    # collision-system setters (HCSITS.h:779-840)
+    def setMaxIterations(self, n):
+        self._max_iterations = int(n)
+        self._params_dirty = True
+
+    def setRelaxationModel(self, model):
+        self._model = int(model)
+        self._params_dirty = True
+
+    def setErrorReductionParameter(self, erp):
+        if erp < 0 or erp > 1:
+            raise ValueError("Error reduction parameter out of range.")
+        self._erp = float(erp)
+        self._params_dirty = True
+
+    def setRelaxationParameter(self, f):
+        if f <= 0:
+            raise ValueError("Relaxation parameter must be positive.")
+        self._relax_param = float(f)
+        self._params_dirty = True
+
+    def setAbortThreshold(self, thr):
+        """Convergence stop on the L-inf impulse change of a sweep (the reference's check at
+        HCSITS.h:1935-1946, compiled out there; 0 = always run max iterations)."""
+        self._abort = float(thr)
+        self._params_dirty = True
+
+    def setSweepMode(self, mode):
+        self._sweep_mode = int(mode)
+        self._params_dirty = True
+
+    # ------------------------------------------------------------------ materials
+    def createMaterial(self, name, density, cor, csf, cdf, poisson=0.5, young=1.0, stiffness=1.0,
+                       dampingN=0.0, dampingT=0.0):
+        """createMaterial, Materials.cpp:175-247 (pair friction = csf_i + csf_j)."""
+        if any(m.name == name for m in self.materials):
+            raise ValueError("Material of that name already exists!")
+        if density <= 0:
+            raise ValueError("Invalid material density!")
+        if cor < 0 or cor > 1:
+            raise ValueError("Invalid coefficient of restitution!")
+        if csf < 0:
+            raise ValueError("Invalid coefficient of static friction!")
+        if cdf < 0:
+            raise ValueError("Invalid coefficient of dynamic friction!")
+        if poisson < -1 or poisson > 0.5:
+            raise ValueError("Invalid Poisson's ratio")
+        if young <= 0:
+            raise ValueError("Invalid Young's modulus")
+        if stiffness <= 0:
+            raise ValueError("Invalid stiffness")
+        if dampingN < 0 or dampingT < 0:
+            raise ValueError("Invalid damping coefficients")
+        self.materials.append(Material(name, float(density), float(cor), float(csf), float(cdf), poisson, young,
+                                       stiffness, dampingN, dampingT))
+        n = len(self.materials)
+        s = np.zeros((n, n))
+        d = np.zeros((n, n))
+        s[:n - 1, :n - 1] = self._csf
+        d[:n - 1, :n - 1] = self._cdf
+        k = n - 1
+        s[k, k] = csf + csf
+        d[k, k] = cdf + cdf
+        for i in range(k):
+            s[k, i] = s[i, k] = self.materials[i].csf + csf
+            d[k, i] = d[i, k] = self.materials[i].cdf + cdf
+        self._csf, self._cdf = s, d
+        self._bodies_dirty = True
+        return k
+
+    def setPairFriction(self, m1, m2, csf, cdf):
+        """Explicit pair coefficients (Materials.h:746)."""
+        self._csf[m1, m2] = self._csf[m2, m1] = csf
+        self._cdf[m1, m2] = self._cdf[m2, m1] = cdf
+        self._bodies_dirty = True
+
+    # ------------------------------------------------------------------ bodies
+    def _sync_from_device(self):
+        if self._state_on_device and self._ctx is not None:
+            st = self.getState()
+            self._arrays.update(st)
+            self._state_on_device = False
+
+    def _materialise_lists(self):
+        if self._arrays is not None:
+            self._sync_from_device()
+            a = self._arrays
+            self._shape, self._sid, self._mat, self._fixed = list(a["shape"]), list(a["sid"]), list(a["mat"]), list(a["fixed"])
+            self._param, self._pos, self._q = list(a["param"]), list(a["pos"]), list(a["q"])
+            self._v, self._w, self._inv_mass = list(a["v"]), list(a["w"]), list(a["inv_mass"])
+            self._I, self._Iinv, self._normal, self._d = list(a["I"]), list(a["Iinv"]), list(a["normal"]), list(a["d"])
+            self._arrays = None
+
+    def _add(self, shape, mat, fixed, param, pos, q, inv_mass, I, Iinv, normal, d):
+        self._materialise_lists()
+        sid = self._next_sid
+        self._next_sid += 1
+        self._shape.append(shape); self._sid.append(sid); self._mat.append(mat); self._fixed.append(1 if fixed else 0)
+        self._param.append(np.asarray(param, float)); self._pos.append(np.asarray(pos, float))
+        self._q.append(np.asarray(q, float)); self._v.append(np.zeros(3)); self._w.append(np.zeros(3))
+        self._inv_mass.append(float(inv_mass)); self._I.append(np.asarray(I, float)); self._Iinv.append(np.asarray(Iinv, float))
+        self._normal.append(np.asarray(normal, float)); self._d.append(float(d))
+        self._bodies_dirty = True
+        return len(self._shape) - 1
+
+    def createSphere(self, uid, gpos, radius, material):
+        """createSphere (Sphere.cpp:1187) with SphereBase's mass properties (SphereBase.cpp:64-87)."""
+        if radius <= 0:
+            raise ValueError("Invalid sphere radius")
+        r = float(radius)
+        m = 4.0 / 3.0 * math.pi * r * r * r * self.materials[material].density
+        I = np.zeros(9)
+        I[0] = I[4] = I[8] = 0.4 * m * r * r
+        return self._add(SPHERE, material, False, (r, 0.0, 0.0), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
+                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0)
+
+    def createBox(self, uid, gpos, lengths, material):
+        """createBox (Box.cpp:1312) with BoxBase's mass properties (BoxBase.cpp:60-160)."""
+        lx, ly, lz = (float(x) for x in lengths)
+        if lx <= 0 or ly <= 0 or lz <= 0:
+            raise ValueError("Invalid side length")
+        m = lx * ly * lz * self.materials[material].density
+        I = np.zeros(9)
+        I[0] = m / 12.0 * (ly * ly + lz * lz)
+        I[4] = m / 12.0 * (lx * lx + lz * lz)
+        I[8] = m / 12.0 * (lx * lx + ly * ly)
+        return self._add(BOX, material, False, (lx, ly, lz), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
+                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0)
+
+    def createPlane(self, uid, normal, d, material):
+        """createPlane (Plane.cpp:1202-1240) and the Plane ctor (Plane.cpp:77-102)."""
+        nv = np.asarray(normal, float)
+        if float(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) == 0.0:
+            raise ValueError("Invalid plane normal!")
+        n = normalize(nv)
+        d = float(d)
+        if n[0] * n[0] + n[1] * n[1] < ACCURACY:
+            q = quat_axis_angle((1.0, 0.0, 0.0), math.acos(n[2]))
+        else:
+            q = quat_axis_angle((-n[1], n[0], 0.0), math.acos(n[2]))
+        return self._add(PLANE, material, True, (0.0, 0.0, 0.0), n * d, q, 0.0, np.zeros(9), np.zeros(9), n, d)
+
+    def setFixed(self, body, fixed=True):
+        """RigidBody::fix (RigidBody.cpp:402-418): zero inverse mass and inertia, zero velocity."""
+        self._materialise_lists()
+        if self._shape[body] == PLANE or not fixed:
+            return
+        self._fixed[body] = 1
+        self._inv_mass[body] = 0.0
+        self._Iinv[body] = np.zeros(9)
+        self._v[body] = np.zeros(3)
+        self._w[body] = np.zeros(3)
+        self._bodies_dirty = True
+
+    def add_bodies(self, arrays):
+        """Bulk creation from SoA arrays in body order (as a sequence of create* calls)."""
+        self._materialise_lists()
+        base = len(self._shape)
+        n = len(arrays["shape"])
+        if base:
+            raise ValueError("bulk creation only into an empty world")
+        self._arrays = dict(arrays)
+        self._arrays["sid"] = np.arange(self._next_sid, self._next_sid + n, dtype=np.uint64)
+        self._next_sid += n
+        self._bodies_dirty = True
+        return np.arange(n)
+
+    @property
+    def size(self):
+        return len(self._arrays["shape"]) if self._arrays is not None else len(self._shape)
+
+    def host_arrays(self):
+        """Body arrays (numpy SoA) in body order."""
+        if self._arrays is None:
+            n = len(self._shape)
+            self._arrays = {
+                "shape": np.asarray(self._shape, np.uint8), "sid": np.asarray(self._sid, np.uint64),
+                "mat": np.asarray(self._mat, np.uint32), "fixed": np.asarray(self._fixed, np.uint8),
+                "param": np.asarray(self._param, float).reshape(n, 3), "pos": np.asarray(self._pos, float).reshape(n, 3),
+                "q": np.asarray(self._q, float).reshape(n, 4), "v": np.asarray(self._v, float).reshape(n, 3),
+                "w": np.asarray(self._w, float).reshape(n, 3), "inv_mass": np.asarray(self._inv_mass, float),
+                "I": np.asarray(self._I, float).reshape(n, 9), "Iinv": np.asarray(self._Iinv, float).reshape(n, 9),
+                "normal": np.asarray(self._normal, float).reshape(n, 3), "d": np.asarray(self._d, float),
+            }
+        else:
+            self._sync_from_device()
+        return self._arrays
+
+    # ------------------------------------------------------------------ device
+    def _params(self):
+        p = _abi.Params()
+        p.max_iterations = self._max_iterations
+        p.relaxation_model = self._model
+        p.erp = self._erp
+        p.abort_threshold = self._abort
+        p.gravity[0], p.gravity[1], p.gravity[2] = self._gravity
+        p.damping = self._damping
+        p.relaxation_param = self._relax_param
+        p.max_sub_iterations = self._max_sub
+        p.sweep_mode = self._sweep_mode
+        return p
+
+    def _ensure_device(self):
+        L = _abi.load()
+        if self._ctx is None:
+            h = ctypes.c_void_p()
+            p = self._params()
+            check(L.pehip_create(ctypes.byref(p), self.device, ctypes.byref(h)), None)
+            self._ctx = h
+            self._params_dirty = False
+        if self._params_dirty:
+            p = self._params()
+            check(L.pehip_set_params(self._ctx, ctypes.byref(p)), self._ctx)
+            self._params_dirty = False
+        if self._bodies_dirty:
+            a = self.host_arrays()
+            n = len(a["shape"])
+            if n == 0:
+                raise ValueError("no bodies")
+            nm = len(self.materials)
+            csf = np.ascontiguousarray(self._csf, float)
+            cdf = np.ascontiguousarray(self._cdf, float)
+            check(L.pehip_set_materials(self._ctx, nm, ptr(csf), ptr(cdf)), self._ctx)
+            c = {k: np.ascontiguousarray(v) for k, v in a.items()}
+            check(L.pehip_upload_bodies(self._ctx, n, ptr(c["shape"], _abi._U8), ptr(c["sid"], _abi._U64),
+                                        ptr(c["mat"], _abi._U32), ptr(c["fixed"], _abi._U8), ptr(c["param"]),
+                                        ptr(c["pos"]), ptr(c["q"]), ptr(c["v"]), ptr(c["w"]), ptr(c["inv_mass"]),
+                                        ptr(c["I"]), ptr(c["Iinv"]), ptr(c["normal"]), ptr(c["d"])), self._ctx)
+            self._bodies_dirty = False
+            self._state_on_device = False
+
+    def close(self):
+        if self._ctx is not None:
+            _abi.load().pehip_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ stepping
+    def simulationStep(self, dt):
+        """World::simulationStep (World.cpp:330-357)."""
+        self.run(1, dt)
+
+    def run(self, steps, dt):
+        """World::run (World.cpp:374-378)."""
+        if dt <= 0:
+            raise ValueError("Invalid time step size")
+        self._ensure_device()
+        check(_abi.load().pehip_step(self._ctx, float(dt), int(steps)), self._ctx)
+        self._state_on_device = True
+        self.timestep += int(steps)
+
+    def detect(self):
+        """Detection only on the current state; returns the contacts."""
+        self._ensure_device()
+        check(_abi.load().pehip_detect(self._ctx), self._ctx)
+        return self.contacts()
+
+    def setContactOrder(self, body1, body2):
+        self._ensure_device()
+        b1 = np.ascontiguousarray(body1, np.uint32)
+        b2 = np.ascontiguousarray(body2, np.uint32)
+        check(_abi.load().pehip_set_contact_order(self._ctx, len(b1), ptr(b1, _abi._U32), ptr(b2, _abi._U32)), self._ctx)
+
+    def setForces(self, force, torque):
+        self._ensure_device()
+        f = np.ascontiguousarray(force, float)
+        t = np.ascontiguousarray(torque, float)
+        check(_abi.load().pehip_set_forces(self._ctx, self.size, ptr(f), ptr(t)), self._ctx)
+
+    # ------------------------------------------------------------------ state
+    def getState(self):
+        self._ensure_device()
+        n = self.size
+        st = {"pos": np.zeros((n, 3)), "q": np.zeros((n, 4)), "v": np.zeros((n, 3)), "w": np.zeros((n, 3)),
+              "normal": np.zeros((n, 3)), "d": np.zeros(n)}
+        check(_abi.load().pehip_download_state(self._ctx, n, ptr(st["pos"]), ptr(st["q"]), ptr(st["v"]),
+                                               ptr(st["w"]), ptr(st["normal"]), ptr(st["d"])), self._ctx)
+        return st
+
+    def setState(self, st):
+        self._ensure_device()
+        n = self.size
+        c = {k: np.ascontiguousarray(st[k], float) for k in ("pos", "q", "v", "w", "normal", "d")}
+        check(_abi.load().pehip_upload_state(self._ctx, n, ptr(c["pos"]), ptr(c["q"]), ptr(c["v"]), ptr(c["w"]),
+                                             ptr(c["normal"]), ptr(c["d"])), self._ctx)
+        if self._arrays is not None:
+            for k in c:
+                self._arrays[k] = c[k].copy()
+        self._state_on_device = False
+
+    def contacts(self):
+        L = _abi.load()
+        n = ctypes.c_size_t()
+        check(L.pehip_get_contacts(self._ctx, 0, ctypes.byref(n), None, None, None, None, None, None), self._ctx)
+        nc = n.value
+        c = {"b1": np.zeros(nc, np.uint32), "b2": np.zeros(nc, np.uint32), "pos": np.zeros((nc, 3)),
+             "normal": np.zeros((nc, 3)), "dist": np.zeros(nc), "mu": np.zeros(nc)}
+        check(L.pehip_get_contacts(self._ctx, nc, ctypes.byref(n), ptr(c["b1"], _abi._U32), ptr(c["b2"], _abi._U32),
+                                   ptr(c["pos"]), ptr(c["normal"]), ptr(c["dist"]), ptr(c["mu"])), self._ctx)
+        return c
+
+    def sweepOrder(self):
+        """(contact index per sweep slot, colour offsets) of the last step."""
+        L = _abi.load()
+        n = ctypes.c_size_t()
+        check(L.pehip_get_sweep_order(self._ctx, 0, ctypes.byref(n), None, None, None), self._ctx)
+        idx = np.zeros(n.value, np.uint32)
+        ncol = ctypes.c_uint32()
+        offs = np.zeros(130, np.uint32)
+        check(L.pehip_get_sweep_order(self._ctx, n.value, ctypes.byref(n), ptr(idx, _abi._U32), ctypes.byref(ncol),
+                                      ptr(offs, _abi._U32)), self._ctx)
+        return idx, offs[:ncol.value + 1].copy()
+
+    def impulses(self):
+        L = _abi.load()
+        n = ctypes.c_size_t()
+        check(L.pehip_get_impulses(self._ctx, 0, ctypes.byref(n), None), self._ctx)
+        p = np.zeros((n.value, 3))
+        check(L.pehip_get_impulses(self._ctx, n.value, ctypes.byref(n), ptr(p)), self._ctx)
+        return p
+
+    def stats(self):
+        s = _abi.Stats()
+        check(_abi.load().pehip_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
+        return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
+
+    def setProfiling(self, on=True):
+        self._ensure_device()
+        check(_abi.load().pehip_set_profiling(self._ctx, 1 if on else 0), self._ctx)
+
+    def stream(self):
+        self._ensure_device()
+        return _abi.load().pehip_stream(self._ctx)
+
+    # CollisionSystem getters (HCSITS.h:715-737)
+    def getMaximumPenetration(self):
+        return self.stats()["max_penetration"]
+
+    def getNumberOfContacts(self):
+        return int(self.stats()["num_contacts"])

@@ -1,0 +1,91 @@
+"""CPU checks of the C-ABI library: it loads, exports every function declared in
+include/pe_hip.h, and its host-only entry points behave (no kernel launches here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from pe_amd import _abi
+from pe_amd.world import World, mat3_inverse
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "pe_hip.h")).read()
+    return sorted(set(re.findall(r"\b(pehip_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree():
+    assert header_functions() == sorted(_abi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("library not built")
+    L = ctypes.CDLL(_abi.LIB_PATH)
+    for name in header_functions():
+        assert hasattr(L, name), name
+
+
+def test_default_params_match_pe():
+    L = _abi.load()
+    p = _abi.Params()
+    L.pehip_default_params(ctypes.byref(p))
+    # HCSITS.h:408-431 and Settings.cpp:46-48
+    assert p.max_iterations == 100 and p.erp == 0.7 and p.max_sub_iterations == 20
+    assert p.relaxation_model == 1 and p.relaxation_param == 0.9 and p.damping == 1.0
+    assert p.abort_threshold == 0.0 and p.sweep_mode == 0
+    assert L.pehip_version().decode().startswith("pe_hip")
+
+
+def test_invalid_create_arguments():
+    L = _abi.load()
+    p = _abi.Params()
+    L.pehip_default_params(ctypes.byref(p))
+    p.relaxation_model = 5
+    h = ctypes.c_void_p()
+    st = L.pehip_create(ctypes.byref(p), 0, ctypes.byref(h))
+    assert st in (_abi.NOT_SUPPORTED, _abi.HIP_ERROR)
+    assert not h.value
+    assert len(L.pehip_last_error(None)) > 0
+
+
+def test_world_host_mass_properties_match_oracle(oracle_mod):
+    # The host-side factories compute the same bits as the oracle's restatement of pe.
+    from pe_amd import scenes
+    sc = scenes.drop_scene(4, 4, 3, r=0.37)
+    ow = oracle_mod.OracleWorld()
+    sc.to_oracle(ow)
+    info = ow.body_info()
+    a = sc.arrays()
+    assert np.array_equal(a["inv_mass"], info["inv_mass"])
+    assert np.array_equal(a["Iinv"], info["inv_inertia"])
+    st = ow.get_state()
+    assert np.array_equal(a["q"], st["q"]) and np.array_equal(a["normal"], st["normal"])
+    assert np.array_equal(a["pos"], st["pos"]) and np.array_equal(a["d"], st["d"])
+
+
+def test_world_api_errors():
+    w = World()
+    with pytest.raises(ValueError):
+        w.createMaterial("m", -1.0, 0, 0.1, 0.1)
+    m = w.createMaterial("m", 1000.0, 0, 0.1, 0.1)
+    with pytest.raises(ValueError):
+        w.createMaterial("m", 1000.0, 0, 0.1, 0.1)
+    with pytest.raises(ValueError):
+        w.createSphere(0, (0, 0, 0), -1.0, m)
+    with pytest.raises(ValueError):
+        w.createPlane(0, (0, 0, 0), 1.0, m)
+    with pytest.raises(ValueError):
+        w.run(1, 0.0)
+    assert w._csf[0, 0] == 0.2
+
+
+def test_mat3_inverse_host():
+    I = np.zeros(9)
+    I[0], I[4], I[8] = 2.0, 3.0, 5.0
+    inv = mat3_inverse(I)
+    assert np.allclose(inv[[0, 4, 8]], [0.5, 1 / 3, 0.2], rtol=1e-15)

@@ -1,0 +1,291 @@
+"""GPU parity of the HIP time step against the CPU oracle (restatement of pe's HCSITS step).
+
+Bars: the broadphase contact set (pairs, points, normals, distances, friction) is bit-exact;
+a step in pe's sweep order (reference-order mode) matches to 1e-12; a coloured step equals
+sequential Gauss-Seidel in colour order to 1e-12; converged coloured and reference-order
+solves agree to 1e-8; trajectory observables stay within 1%.  Every comparison starts from
+an identical state (granular dynamics amplify 1e-16 differences over a trajectory).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pe_amd import World, scenes, SWEEP_REFERENCE_ORDER, SWEEP_COLOURED, PeHipError
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b, floor=1e-30):
+    """Norm-wise relative difference max|a-b| / max|b|."""
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), floor))
+
+
+def contact_map(c):
+    out = {}
+    for i in range(len(c["b1"])):
+        key = (int(c["b1"][i]), int(c["b2"][i]))
+        out.setdefault(key, []).append((tuple(c["pos"][i]), tuple(c["normal"][i]), float(c["dist"][i]), float(c["mu"][i])))
+    return out
+
+
+def oracle_for(sc, iters=None, model=3, abort=0.0):
+    ow = O.OracleWorld(gravity=sc.gravity, max_iterations=sc.max_iterations if iters is None else iters,
+                       model=model, abort_threshold=abort)
+    sc.to_oracle(ow)
+    return ow
+
+
+def gpu_for(sc, iters=None, model=3, mode=SWEEP_COLOURED, abort=0.0):
+    w = World()
+    sc.to_world(w)
+    w.setMaxIterations(sc.max_iterations if iters is None else iters)
+    w.setRelaxationModel(model)
+    w.setSweepMode(mode)
+    w.setAbortThreshold(abort)
+    return w
+
+
+def dense_scene():
+    # 216 spheres at spacing 1.0 (touching) with jitter and random velocities: many contacts at once
+    return scenes.drop_scene(6, 6, 6, r=0.5, spacing=1.0, jitter=0.01, box=(7.5, 7.5, 20.0), mu=0.4, seed=11,
+                             v_amp=0.3, max_iterations=30)
+
+
+def poly_scene():
+    # polydisperse: radius ratio 3 -> two grid levels on the GPU, several HashGrids levels in pe
+    sc = scenes.drop_scene(7, 7, 5, r=0.5, spacing=1.05, jitter=0.01, box=(8.0, 8.0, 20.0), mu=0.5, seed=5,
+                           v_amp=0.5, max_iterations=30)
+    rng = np.random.default_rng(3)
+    sc.sphere_r = rng.uniform(0.18, 0.52, size=sc.n_spheres)
+    return sc
+
+
+def box_scene():
+    # boxes resting on the floor with spheres on and around them (sphere-box, box-plane)
+    rng = np.random.default_rng(9)
+    sc = scenes.drop_scene(5, 5, 2, r=0.3, spacing=0.75, jitter=0.01, box=(6.0, 6.0, 20.0), mu=0.6, seed=9,
+                           v_amp=0.2, max_iterations=30, z0=1.095)
+    nb = 6
+    sc.box_pos = np.array([[1.0 + 1.5 * (k % 3), 1.0 + 2.0 * (k // 3), 0.4 - 0.002] for k in range(nb)])
+    sc.box_len = np.array([[1.0, 0.9, 0.8]] * nb)
+    ang = rng.uniform(-0.3, 0.3, size=nb)
+    sc.box_q = np.array([[np.cos(a / 2), 0, 0, np.sin(a / 2)] for a in ang])
+    sc.box_mat = np.zeros(nb, np.uint32)
+    if sc.v0 is not None:
+        sc.v0 = np.concatenate([sc.v0, np.zeros((nb, 3))])
+    return sc
+
+
+SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene}
+
+
+def oracle_state_after(sc, steps, iters=30):
+    ow = oracle_for(sc, iters=iters)
+    if steps:
+        ow.step(sc.dt, steps)
+    return ow, ow.get_state()
+
+
+# --------------------------------------------------------------------------- broadphase
+@pytest.mark.parametrize("name", sorted(SCENES))
+@pytest.mark.parametrize("steps", [0, 7, 25])
+def test_contact_set_bitexact(name, steps):
+    sc = SCENES[name]()
+    ow, st = oracle_state_after(sc, steps)
+    w = gpu_for(sc)
+    w.setState(st)
+    g = contact_map(w.detect())
+    ex = contact_map(ow.detect(exhaustive=True))
+    hg = contact_map(ow.detect(exhaustive=False))
+    assert len(ex) > 10
+    assert g == ex, "GPU contact set differs from pe's ExhaustiveSearch"
+    assert g == hg, "GPU contact set differs from pe's HashGrids"
+
+
+def test_contact_set_bitexact_c1_late_state():
+    sc = scenes.config1()
+    ow = oracle_for(sc, iters=100)
+    ow.step(sc.dt, 470)
+    st = ow.get_state()
+    w = gpu_for(sc)
+    w.setState(st)
+    g = contact_map(w.detect())
+    ex = contact_map(ow.detect(exhaustive=True))
+    assert len(ex) > 50 and g == ex
+
+
+# --------------------------------------------------------------------------- one step, pe's order
+@pytest.mark.parametrize("name", sorted(SCENES))
+@pytest.mark.parametrize("model", [3, 1, 0])
+def test_reference_order_step_matches_oracle(name, model):
+    sc = SCENES[name]()
+    ow, st0 = oracle_state_after(sc, 12)
+    ow.L.orc_set_relaxation_model(ow.h, model)
+    ow.step(sc.dt)                                  # pe's step from st0, HashGrids order
+    c = ow.contacts()
+    st1 = ow.get_state()
+    w = gpu_for(sc, model=model, mode=SWEEP_REFERENCE_ORDER)
+    w.setState(st0)
+    w.setContactOrder(c["b1"], c["b2"])
+    w.simulationStep(sc.dt)
+    g1 = w.getState()
+    assert len(c["b1"]) > 10
+    for k in ("v", "w", "pos", "q"):
+        assert rel_err(g1[k], st1[k]) <= 1e-12, k
+    assert rel_err(w.impulses()[np.argsort(order_index(w.contacts(), c))], ow.impulses()) <= 1e-12
+
+
+def order_index(gc, oc):
+    """Position of each GPU contact in the oracle's order (pairs matched in sequence)."""
+    pos = {}
+    for i, (a, b) in enumerate(zip(oc["b1"], oc["b2"])):
+        pos.setdefault((int(a), int(b)), []).append(i)
+    used = {}
+    out = np.zeros(len(gc["b1"]), int)
+    for i, (a, b) in enumerate(zip(gc["b1"], gc["b2"])):
+        k = (int(a), int(b))
+        j = used.get(k, 0)
+        out[i] = pos[k][j]
+        used[k] = j + 1
+    return out
+
+
+def test_reference_order_c1_window():
+    # C1 (1k spheres, K = 100) stepped on the GPU in pe's order from pe's states, steps 440..444
+    sc = scenes.config1()
+    ow = oracle_for(sc, iters=100)
+    ow.step(sc.dt, 440)
+    w = gpu_for(sc, iters=100, mode=SWEEP_REFERENCE_ORDER)
+    for _ in range(4):
+        st0 = ow.get_state()
+        ow.step(sc.dt)
+        c = ow.contacts()
+        w.setState(st0)
+        w.setContactOrder(c["b1"], c["b2"])
+        w.simulationStep(sc.dt)
+        g, o = w.getState(), ow.get_state()
+        assert len(c["b1"]) > 30
+        for k in ("v", "w", "pos"):
+            assert rel_err(g[k], o[k]) <= 1e-12, k
+
+
+# --------------------------------------------------------------------------- coloured mode
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_coloured_step_equals_gs_in_colour_order(name):
+    # Contacts of one colour share no non-fixed body, so a coloured sweep is exactly a
+    # sequential Gauss-Seidel sweep in colour-major order; replay that order on the oracle.
+    sc = SCENES[name]()
+    ow, st0 = oracle_state_after(sc, 10)
+    w = gpu_for(sc, iters=30)
+    w.setState(st0)
+    w.simulationStep(sc.dt)
+    g1 = w.getState()
+    idx, offs = w.sweepOrder()
+    gc = w.contacts()
+    assert len(offs) >= 3 and offs[-1] == len(idx) == len(gc["b1"])
+    ow.set_sweep_order(gc["b1"][idx], gc["b2"][idx])
+    ow.step(sc.dt)
+    o1 = ow.get_state()
+    for k in ("v", "w", "pos", "q"):
+        assert rel_err(g1[k], o1[k]) <= 1e-12, k
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_colouring_is_conflict_free(name):
+    sc = SCENES[name]()
+    _, st0 = oracle_state_after(sc, 10)
+    w = gpu_for(sc, iters=2)
+    w.setState(st0)
+    w.simulationStep(sc.dt)
+    idx, offs = w.sweepOrder()
+    gc = w.contacts()
+    fixed = sc.arrays()["fixed"]
+    assert sorted(idx.tolist()) == list(range(len(gc["b1"])))
+    for k in range(len(offs) - 1):
+        seg = idx[offs[k]:offs[k + 1]]
+        bodies = [int(b) for b in np.concatenate([gc["b1"][seg], gc["b2"][seg]]) if not fixed[b]]
+        assert len(bodies) == len(set(bodies)), "colour %d shares a body" % k
+    assert w.stats()["num_colours"] == len(offs) - 1
+
+
+def test_coloured_converged_matches_reference_converged():
+    # Both solvers run to a tight impulse-change threshold from the same state: the
+    # velocities agree to 1e-8 even though the sweep orders differ.
+    sc = scenes.drop_scene(4, 4, 1, r=0.5, spacing=1.0, jitter=0.01, box=(6.0, 6.0, 10.0), mu=0.4, seed=2,
+                           v_amp=0.4, max_iterations=4000, z0=0.49)
+    ow = oracle_for(sc, iters=4000, abort=1e-13)
+    st0 = ow.get_state()
+    ow.step(sc.dt)
+    o1 = ow.get_state()
+    assert ow.stats()["num_contacts"] >= 16 and ow.stats()["iterations"] < 4000
+    w = gpu_for(sc, iters=4000, abort=1e-13)
+    w.setState(st0)
+    w.simulationStep(sc.dt)
+    g1 = w.getState()
+    assert w.stats()["iterations"] < 4000
+    for k in ("v", "w"):
+        assert rel_err(g1[k], o1[k]) <= 1e-8, k
+
+
+def test_trajectory_observables_c1():
+    # C1 over 500 steps: kinetic energy and maximum penetration of the coloured GPU run track
+    # pe's sequential run within 1% (once contacts exist).
+    sc = scenes.config1()
+    ow = oracle_for(sc, iters=100)
+    w = gpu_for(sc, iters=100)
+    ke_o, ke_g, mp_o, mp_g = [], [], [], []
+    m = 1.0 / sc.arrays()["inv_mass"][6:]
+    for k in range(50):
+        ow.step(sc.dt, 10)
+        w.run(10, sc.dt)
+        vo = ow.get_state()["v"][6:]
+        ke_o.append(0.5 * np.sum(m * np.sum(vo * vo, axis=1)))
+        s = w.stats()
+        ke_g.append(s["kinetic_energy"])
+        mp_o.append(ow.stats()["max_penetration"])
+        mp_g.append(s["max_penetration"])
+    ke_o, ke_g = np.array(ke_o), np.array(ke_g)
+    rel = np.abs(ke_g - ke_o) / ke_o
+    assert np.all(rel <= 0.01), "kinetic energy deviates: %s" % rel.max()
+    assert abs(max(mp_g) - max(mp_o)) <= 0.01 * max(mp_o) + 1e-5, (max(mp_g), max(mp_o))
+
+
+def test_coloured_run_is_deterministic():
+    sc = dense_scene()
+    runs = []
+    for _ in range(2):
+        w = gpu_for(sc, iters=20)
+        w.run(15, sc.dt)
+        runs.append(w.getState())
+    for k in ("pos", "v", "w", "q"):
+        assert np.array_equal(runs[0][k], runs[1][k])
+
+
+# --------------------------------------------------------------------------- closed forms / errors
+def test_free_fall_on_gpu():
+    g, dt, n = 9.81, 1e-3, 40
+    sc = scenes.drop_scene(1, 1, 1, r=0.5, box=(10, 10, 200), z0=100.0)
+    w = gpu_for(sc)
+    w.run(n, dt)
+    st = w.getState()
+    z0 = sc.sphere_pos[0][2]
+    assert st["v"][6][2] == pytest.approx(-n * g * dt, rel=1e-13)
+    assert st["pos"][6][2] == pytest.approx(z0 - g * dt * dt * n * (n + 1) / 2, rel=1e-13)
+
+
+def test_resting_sphere_exact_on_gpu():
+    sc = scenes.drop_scene(1, 1, 1, r=0.5, jitter=0.0, box=(10, 10, 20), z0=0.5)
+    w = gpu_for(sc)
+    w.simulationStep(1e-3)
+    assert np.all(np.abs(w.getState()["v"][6]) < 1e-15)
+
+
+def test_invalid_time_step_and_order():
+    sc = dense_scene()
+    w = gpu_for(sc, mode=SWEEP_REFERENCE_ORDER)
+    with pytest.raises(ValueError):
+        w.run(1, -1e-3)
+    w.setContactOrder([0], [1])
+    with pytest.raises(PeHipError):
+        w.simulationStep(sc.dt)

@@ -1,0 +1,287 @@
+"""Pins the CPU oracle (restatement of pe's time step) with closed forms, brute force and
+invariants.  The reference ships no tests or golden vectors, so these are the anchors."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def make_world(gravity=(0, 0, 0), model=O.INELASTIC_COULOMB_DECOUPLING, iters=100, mu=0.4, density=2500.0):
+    w = O.OracleWorld(gravity=gravity, max_iterations=iters, model=model)
+    w.create_material(density, 0.0, mu / 2, mu / 2)
+    return w
+
+
+# --------------------------------------------------------------------------- math
+def test_quaternion_closed_forms():
+    th = 0.7
+    q = O.quat_from_axis_angle([0, 0, 2.0], th)      # axis normalised inside
+    assert np.allclose(q, [math.cos(th / 2), 0, 0, math.sin(th / 2)], atol=1e-16)
+    R = O.quat_to_rotation(q)
+    Rz = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    assert np.allclose(R, Rz, atol=1e-15)
+    v = np.array([0.3, -1.2, 2.5])
+    assert np.allclose(O.quat_rotate(q, v), Rz @ v, atol=1e-15)
+    # product composes rotations: q(a) * q(b) = q(a+b) about the same axis
+    qa, qb = O.quat_from_axis_angle([1, 1, 0], 0.3), O.quat_from_axis_angle([1, 1, 0], 0.5)
+    assert np.allclose(O.quat_mul(qa, qb), O.quat_from_axis_angle([1, 1, 0], 0.8), atol=1e-15)
+    # tiny angles give exactly the identity (Quaternion.h:281)
+    assert np.array_equal(O.quat_from_axis_angle([1, 0, 0], 1e-16), [1, 0, 0, 0])
+
+
+def test_plane_normal_libm_artefact():
+    # The -z plane's quaternion is (cos(pi/2), 1, 0, 0); rotating e_z gives (0, -2 q0, q0^2 - 1):
+    # a 1e-16 y component from libm, which makes that plane's AABB infinite from step 2 on.
+    q = O.quat_from_axis_angle([1, 0, 0], math.acos(-1.0))
+    q0 = math.cos(math.pi / 2)
+    assert q[0] == q0
+    n = O.quat_rotate(q, [0, 0, 1])
+    assert n[0] == 0.0 and n[1] == -2 * q0 and n[2] == -1.0
+
+
+def test_mat3_inverse():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        A = rng.normal(size=(3, 3)) + 3 * np.eye(3)
+        Ai = O.mat3_inverse(A)
+        assert np.allclose(Ai @ A, np.eye(3), atol=1e-13)
+        assert np.allclose(Ai, np.linalg.inv(A), rtol=1e-12)
+
+
+def test_sphere_mass_properties():
+    w = make_world()
+    w.create_sphere([0, 0, 0], 0.5, 0)
+    info = w.body_info()
+    m = 4.0 / 3.0 * math.pi * 0.5 ** 3 * 2500.0
+    assert info["inv_mass"][0] == pytest.approx(1 / m, rel=1e-15)
+    assert info["inv_inertia"][0][0] == pytest.approx(1 / (0.4 * m * 0.25), rel=1e-14)
+    assert np.allclose(w.aabb()[0], [-0.5 - 1e-8] * 3 + [0.5 + 1e-8] * 3, atol=1e-16)
+
+
+def test_box_mass_properties():
+    w = make_world()
+    w.create_box([0, 0, 0], [1.0, 2.0, 3.0], 0)
+    info = w.body_info()
+    m = 6.0 * 2500.0
+    Iinv = info["inv_inertia"][0].reshape(3, 3)
+    assert np.allclose(np.diag(Iinv), [12 / (m * 13), 12 / (m * 10), 12 / (m * 5)], rtol=1e-14)
+
+
+# --------------------------------------------------------------------------- narrowphase
+def test_sphere_sphere_contact():
+    w = make_world()
+    w.create_sphere([0, 0, 0], 0.5, 0)
+    w.create_sphere([0.9, 0.1, 0], 0.45, 0)
+    c = w.detect()
+    assert len(c["b1"]) == 1
+    assert c["b1"][0] == 0 and c["b2"][0] == 1        # body1 = smaller SID
+    x1, x2 = np.array([0, 0, 0.]), np.array([0.9, 0.1, 0])
+    d = np.linalg.norm(x1 - x2) - 0.5 - 0.45
+    n = (x1 - x2) / np.linalg.norm(x1 - x2)           # from body 2 to body 1
+    assert c["dist"][0] == pytest.approx(d, abs=1e-15)
+    assert np.allclose(c["normal"][0], n, atol=1e-15)
+    assert np.allclose(c["pos"][0], x2 + n * (0.45 + 0.5 * d), atol=1e-15)
+    assert c["mu"][0] == 0.4                           # csf + csf
+
+
+def test_touching_spheres_are_in_contact():
+    # dist = 0 exactly (< contactThreshold) is a contact; a gap of 2e-8 is not
+    w = make_world()
+    w.create_sphere([0, 0, 0], 0.5, 0)
+    w.create_sphere([1.0, 0, 0], 0.5, 0)
+    w.create_sphere([0, 0, 5], 0.5, 0)
+    w.create_sphere([0, 0, 6 + 2e-8], 0.5, 0)
+    c = w.detect()
+    assert list(zip(c["b1"], c["b2"])) == [(0, 1)]
+    assert c["dist"][0] == 0.0
+
+
+def test_sphere_plane_contact():
+    w = make_world()
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_sphere([1, 2, 0.49], 0.5, 0)
+    c = w.detect()
+    assert len(c["b1"]) == 1 and c["b1"][0] == 1 and c["b2"][0] == 0   # sphere is body1
+    assert c["dist"][0] == pytest.approx(-0.01, abs=1e-15)
+    assert np.allclose(c["pos"][0], [1, 2, 0.49 - (0.5 - 0.01)], atol=1e-15)
+    assert np.array_equal(c["normal"][0], [0, 0, 1])
+
+
+def test_sphere_box_contacts():
+    w = make_world()
+    w.create_box([0, 0, 0], [2.0, 2.0, 2.0], 0)
+    w.create_sphere([0, 0, 1.4], 0.5, 0)           # above the top face, outside: dist = -0.1
+    w.create_sphere([1.3, 1.3, -5.0], 0.5, 0)      # far away
+    c = w.detect()
+    assert len(c["b1"]) == 1
+    assert c["b1"][0] == 1 and c["b2"][0] == 0     # sphere is body1
+    assert c["dist"][0] == pytest.approx(-0.1, abs=1e-15)
+    assert np.allclose(c["normal"][0], [0, 0, 1], atol=1e-15)
+    assert np.allclose(c["pos"][0], [0, 0, 1.0], atol=1e-15)
+    # centre inside the box: nearest face (+x), contact at the sphere centre
+    w2 = make_world()
+    w2.create_box([0, 0, 0], [2.0, 2.0, 2.0], 0)
+    w2.create_sphere([0.8, 0.1, 0.0], 0.5, 0)
+    c2 = w2.detect()
+    assert np.allclose(c2["normal"][0], [1, 0, 0]) and c2["dist"][0] == pytest.approx(-0.2 - 0.5, abs=1e-15)
+    assert np.allclose(c2["pos"][0], [0.8, 0.1, 0.0])
+
+
+def test_box_plane_four_corners():
+    w = make_world()
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_box([0, 0, 0.499], [1.0, 2.0, 1.0], 0)
+    c = w.detect()
+    assert len(c["b1"]) == 4                       # only the bottom corners pass the pre-filter
+    assert np.all(c["b1"] == 1) and np.all(c["b2"] == 0)
+    assert np.allclose(c["dist"], -0.001, atol=1e-15)
+    corners = {(round(p[0], 6), round(p[1], 6)) for p in c["pos"]}
+    assert corners == {(-0.5, -1.0), (0.5, -1.0), (0.5, 1.0), (-0.5, 1.0)}
+
+
+# --------------------------------------------------------------------------- broadphase
+@pytest.mark.parametrize("n,rmin,rmax,seed", [(300, 0.5, 0.5, 1), (700, 0.1, 0.9, 2), (1200, 0.2, 0.5, 3)])
+def test_hashgrids_pair_set_equals_brute_force(n, rmin, rmax, seed):
+    # Dense random states: multiple grid levels (radius ratio up to 9) and grid enlargement (> 512).
+    rng = np.random.default_rng(seed)
+    L = (n * 4.0) ** (1 / 3) * rmax
+    w = make_world()
+    for p in [((0, 0, 1), 0.0), ((0, 1, 0), 0.0), ((-1, 0, 0), -L)]:
+        w.create_plane(p[0], p[1], 0)
+    pos = rng.uniform(0, L, size=(n, 3))
+    r = rng.uniform(rmin, rmax, size=n)
+    for k in range(n):
+        w.create_sphere(pos[k], r[k], 0)
+    hg = w.detect(exhaustive=False)
+    ex = w.detect(exhaustive=True)
+    pairs_hg = list(zip(hg["b1"].tolist(), hg["b2"].tolist()))
+    assert len(pairs_hg) == len(set(pairs_hg))     # no duplicate pair
+    assert set(pairs_hg) == set(zip(ex["b1"].tolist(), ex["b2"].tolist()))
+    # every sphere pair closer than contactThreshold is present (independent numpy distances)
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=-1) - r[:, None] - r[None]
+    i, j = np.where(np.triu(d < 1e-8 - 1e-12, 1))
+    expect = {(a + 3, b + 3) for a, b in zip(i, j)}
+    assert expect <= set(pairs_hg)
+    # plane contacts: n.x - r - d < 1e-8
+    nz = int(np.sum(pos[:, 2] - r < 1e-8 - 1e-12))
+    assert sum(1 for (a, b) in pairs_hg if b == 0) >= nz
+    # contact records are identical between the two detectors (same narrowphase)
+    key = lambda c: {(int(a), int(b)): (tuple(p), tuple(nn), dd) for a, b, p, nn, dd in
+                     zip(c["b1"], c["b2"], c["pos"], c["normal"], c["dist"])}
+    assert key(hg) == key(ex)
+
+
+def test_hashgrids_small_world_uses_list_order():
+    # <= 32 bodies: all in the non-grid list, pairs in i<j order like ExhaustiveSearch
+    w = make_world()
+    for k in range(10):
+        w.create_sphere([0.9 * k, 0, 0], 0.5, 0)
+    hg = w.detect(False)
+    assert list(zip(hg["b1"], hg["b2"])) == [(k, k + 1) for k in range(9)]
+
+
+# --------------------------------------------------------------------------- solver
+def test_resting_sphere_on_plane_exact():
+    # v = 0, dist = 0: the normal impulse cancels gravity exactly, p = m g dt
+    g, dt = 9.81, 1e-3
+    w = make_world(gravity=(0, 0, -g))
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_sphere([0, 0, 0.5], 0.5, 0)
+    w.step(dt)
+    m = 4.0 / 3.0 * math.pi * 0.125 * 2500.0
+    p = w.impulses()
+    assert p.shape == (1, 3)
+    assert p[0][2] == pytest.approx(m * g * dt, rel=1e-13) and abs(p[0][0]) < 1e-18 and abs(p[0][1]) < 1e-18
+    st = w.get_state()
+    assert np.all(np.abs(st["v"][1]) < 1e-15)
+
+
+def test_separating_contact_has_no_impulse():
+    w = make_world()
+    w.create_sphere([0, 0, 0], 0.5, 0)
+    w.create_sphere([0.999, 0, 0], 0.5, 0)
+    st = w.get_state()
+    st["v"][0] = [-1.0, 0, 0]
+    st["v"][1] = [1.0, 0, 0]
+    w.set_state(st)
+    w.step(1e-3)
+    assert np.all(w.impulses() == 0)
+
+
+def test_head_on_inelastic_collision():
+    # Two spheres approaching: relative normal velocity vanishes; momentum is conserved.
+    w = make_world()
+    w.create_sphere([0, 0, 0], 0.5, 0)
+    w.create_sphere([1.0, 0, 0], 0.5, 0)
+    st = w.get_state()
+    st["v"][0] = [2.0, 0, 0]
+    st["v"][1] = [-1.0, 0, 0]
+    w.set_state(st)
+    w.step(1e-3)
+    v = w.get_state()["v"]
+    assert abs(v[0][0] - v[1][0]) < 1e-12
+    assert v[0][0] + v[1][0] == pytest.approx(1.0, rel=1e-14)
+
+
+@pytest.mark.parametrize("model", [O.INELASTIC_COULOMB_DECOUPLING, O.APPROX_INELASTIC_COULOMB_DECOUPLING])
+def test_sliding_sphere_friction_on_cone(model):
+    # A fast-sliding sphere: the friction impulse sits on the Coulomb cone boundary and
+    # opposes the slip (decoupled normal/tangential relaxation).
+    mu, dt = 0.4, 1e-3
+    w = make_world(gravity=(0, 0, -9.81), model=model, mu=mu)
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_sphere([0, 0, 0.5], 0.5, 0)
+    st = w.get_state()
+    st["v"][1] = [3.0, 4.0, 0.0]
+    w.set_state(st)
+    w.step(dt)
+    p = w.impulses()[0]
+    pt = np.hypot(p[0], p[1])
+    assert pt == pytest.approx(mu * p[2], rel=1e-12)
+    slip = np.array([3.0, 4.0]) / 5.0
+    assert np.dot(p[:2] / pt, -slip) > 1 - 1e-6
+
+
+def test_free_fall_closed_form():
+    g, dt, n = 9.81, 1e-3, 50
+    w = make_world(gravity=(0, 0, -g))
+    w.create_sphere([0, 0, 100.0], 0.5, 0)
+    st = w.get_state()
+    st["v"][0] = [0.5, 0, 1.0]
+    w.set_state(st)
+    w.step(dt, n)
+    s = w.get_state()
+    assert s["v"][0][2] == pytest.approx(1.0 - n * g * dt, rel=1e-13)
+    # symplectic Euler: x_n = x_0 + n v0 dt - g dt^2 n (n+1) / 2
+    assert s["pos"][0][2] == pytest.approx(100.0 + n * 1.0 * dt - g * dt * dt * n * (n + 1) / 2, rel=1e-13)
+    assert s["pos"][0][0] == pytest.approx(0.5 * n * dt, rel=1e-13)
+
+
+def test_torque_free_spin_closed_form():
+    dt, n = 1e-3, 100
+    w = make_world()
+    w.create_sphere([0, 0, 0], 0.5, 0)
+    st = w.get_state()
+    wv = np.array([0.3, -0.4, 1.2])
+    st["w"][0] = wv
+    w.set_state(st)
+    w.step(dt, n)
+    s = w.get_state()
+    assert np.allclose(s["w"][0], wv, rtol=1e-14)      # isotropic inertia: no gyroscopic drift
+    q = O.quat_from_axis_angle(wv, np.linalg.norm(wv) * dt * n)
+    assert np.allclose(s["q"][0], q, atol=1e-12)
+
+
+def test_c1_trajectory_observables():
+    from pe_amd import scenes
+    sc = scenes.config1()
+    w = O.OracleWorld(gravity=sc.gravity, max_iterations=20, model=3)
+    sc.to_oracle(w)
+    w.step(sc.dt, 420)
+    st = w.stats()
+    assert st["num_contacts"] > 20
+    assert st["max_penetration"] < 0.01
+    pos = w.get_state()["pos"][6:]
+    assert pos[:, 2].min() > 0.45      # nothing fell through the floor
