@@ -71,6 +71,10 @@ typedef struct {
   double relaxation_param;     /* setRelaxationParameter, HCSITS.h:779 (MPI-only, stored) */
   uint32_t max_sub_iterations; /* golden-section steps, HCSITS.h:423 (default 20) */
   int32_t sweep_mode;          /* pehip_sweep_mode */
+  int32_t exact_trig;          /* 1: correctly rounded sin/cos/atan2/acos in the golden-section
+                                  search and the orientation update, reproducing the host libm
+                                  (glibc) wherever it is correctly rounded; 0: device libm.  Always
+                                  on in PEHIP_SWEEP_REFERENCE_ORDER. */
 } pehip_params;
 
 /* Fills p with pe's defaults (HCSITS.h:408-431, Settings.cpp:46-48). */
@@ -177,6 +181,12 @@ pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);
 /* Phase timing and per-sweep-launch timing with HIP events on the context's stream
  * (off by default); enabling resets the profiling totals. */
 pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable);
+
+/* Evaluates the device trig of the solver on n arguments (checks against a host libm):
+ * fn 0 = sin(x), 1 = cos(x), 2 = atan2(y, x), 3 = acos(x); exact selects the correctly
+ * rounded implementation (see pehip_params.exact_trig).  y may be NULL unless fn = 2. */
+pehip_status pehip_eval_trig(pehip_ctx* ctx, int fn, int exact, size_t n, const double* x,
+                             const double* y, double* out);
 
 /* The context's HIP stream (hipStream_t), for callers that time or order work on it. */
 void* pehip_stream(pehip_ctx* ctx);

@@ -56,6 +56,8 @@ def lib():
             "orc_set_relaxation_model": (None, [vp, ctypes.c_int]),
             "orc_set_erp": (None, [vp, ctypes.c_double]),
             "orc_set_abort_threshold": (None, [vp, ctypes.c_double]),
+            "orc_set_trig_mode": (None, [vp, ctypes.c_int]),
+            "orc_trig": (ctypes.c_double, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
             "orc_create_material": (ctypes.c_int, [vp] + [ctypes.c_double] * 4),
             "orc_set_pair_friction": (None, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             "orc_create_sphere": (ctypes.c_int, [vp] + [ctypes.c_double] * 4 + [ctypes.c_int]),
@@ -96,7 +98,7 @@ class OracleWorld:
     """One pe world restated on the CPU (HashGrids + MaxContacts + HCSITS)."""
 
     def __init__(self, gravity=(0.0, 0.0, -9.81), max_iterations=100,
-                 model=INELASTIC_COULOMB_DECOUPLING, erp=0.7, abort_threshold=0.0):
+                 model=INELASTIC_COULOMB_DECOUPLING, erp=0.7, abort_threshold=0.0, cr_trig=False):
         self.L = lib()
         self.h = ctypes.c_void_p(self.L.orc_create())
         self.L.orc_set_gravity(self.h, *gravity)
@@ -104,6 +106,7 @@ class OracleWorld:
         self.L.orc_set_relaxation_model(self.h, model)
         self.L.orc_set_erp(self.h, erp)
         self.L.orc_set_abort_threshold(self.h, abort_threshold)
+        self.L.orc_set_trig_mode(self.h, 1 if cr_trig else 0)
 
     def __del__(self):
         if getattr(self, "h", None) is not None:
@@ -203,6 +206,11 @@ class OracleWorld:
         s = OrcStats()
         self.L.orc_get_stats(self.h, ctypes.byref(s))
         return {f: getattr(s, f) for f, _ in OrcStats._fields_}
+
+
+def trig(fn, x, y=0.0, correctly_rounded=False):
+    """sin (0), cos (1), atan2(y, x) (2), acos (3): C library or correctly rounded."""
+    return lib().orc_trig(int(fn), float(x), float(y), 1 if correctly_rounded else 0)
 
 
 def quat_from_axis_angle(axis, angle):

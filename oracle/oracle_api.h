@@ -49,6 +49,11 @@ void orc_set_max_iterations(orc_world* w, size_t n);
 void orc_set_relaxation_model(orc_world* w, int model);
 void orc_set_erp(orc_world* w, double erp);
 void orc_set_abort_threshold(orc_world* w, double thr);
+/* Trig of the golden-section search and of the orientation update: 0 = the C library's
+ * (pe's behaviour, default), 1 = correctly rounded (113-bit quad precision rounded once). */
+void orc_set_trig_mode(orc_world* w, int correctly_rounded);
+/* fn 0 sin(x), 1 cos(x), 2 atan2(y, x), 3 acos(x) in the given trig mode. */
+double orc_trig(int fn, double x, double y, int correctly_rounded);
 
 /* Materials.cpp:175-247: returns the material id. */
 int orc_create_material(orc_world* w, double density, double cor, double csf, double cdf);

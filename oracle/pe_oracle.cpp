@@ -19,6 +19,8 @@
 #include <stdexcept>
 #include <vector>
 
+#include <quadmath.h>
+
 namespace {
 
 // ---------------------------------------------------------------------------------------------
@@ -137,10 +139,20 @@ inline M2 inverse2(const M2& v) {
 struct Q { double v[4]; };
 inline Q quat_identity() { Q q; q.v[0] = 1; q.v[1] = 0; q.v[2] = 0; q.v[3] = 0; return q; }
 // Quaternion(axis, angle), Quaternion.h:274-294
-inline Q quat_axis_angle(V3 axis, double angle) {
+// Trig of the step: the C library's (as pe), or correctly rounded (evaluated in 113-bit quad
+// precision and rounded once to double) when cr is set -- an ideal libm that the device's
+// correctly rounded trig is checked against.
+inline double t_sin(double x, bool cr) { return cr ? (double)sinq((__float128)x) : std::sin(x); }
+inline double t_cos(double x, bool cr) { return cr ? (double)cosq((__float128)x) : std::cos(x); }
+inline double t_atan2(double y, double x, bool cr) {
+  return cr ? (double)atan2q((__float128)y, (__float128)x) : std::atan2(y, x);
+}
+inline double t_acos(double x, bool cr) { return cr ? (double)acosq((__float128)x) : std::acos(x); }
+
+inline Q quat_axis_angle(V3 axis, double angle, bool cr = false) {
   if (std::fabs(angle) < 1e-15) return quat_identity();
-  const double sina = std::sin(angle * 0.5);
-  const double cosa = std::cos(angle * 0.5);
+  const double sina = t_sin(angle * 0.5, cr);
+  const double cosa = t_cos(angle * 0.5, cr);
   axis = normalized(axis);
   Q q; q.v[0] = cosa; q.v[1] = sina * axis[0]; q.v[2] = sina * axis[1]; q.v[3] = sina * axis[2];
   return q;
@@ -431,6 +443,7 @@ struct orc_world {
   size_t maxSubIterations = 20;
   double abortThreshold = 0.0;
   int relaxationModel = ORC_APPROX_INELASTIC_COULOMB_DECOUPLING;
+  bool crTrig = false;
 
   // HashGrids state
   std::vector<int> bodiesToAdd, nonGridBodies;
@@ -779,7 +792,7 @@ void orc_world::integratePositions(Body& b, V3 v, V3 w, double dt) {
   if (b.awake) {
     b.gpos += v * dt;
     const V3 phi = w * dt;
-    const Q dq = quat_axis_angle(phi, length(phi));
+    const Q dq = quat_axis_angle(phi, length(phi), crTrig);
     b.q = quat_mul(dq, b.q);
     b.R = quat_to_R(b.q);
     if (damping < 1.0) {
@@ -944,19 +957,19 @@ double orc_world::relaxInelasticCoulombContactsByDecoupling(double dtinv) {
             const double A00 = D[4], A01 = D[5], A10 = D[7], A11 = D[8];
             const double f = flimit / std::sqrt(fsq);
             const double phi = 0.5 * (1.0 + std::sqrt(5.0));
-            const double shift = std::atan2(-p_cf[2], p_cf[1]);
-            const double acos_f = std::acos(f);
+            const double shift = t_atan2(-p_cf[2], p_cf[1], crTrig);
+            const double acos_f = t_acos(f, crTrig);
             double alpha_left = -acos_f - shift;
             double alpha_right = acos_f - shift;
             double alpha_mid = (alpha_right + alpha_left * phi) / (1 + phi);
-            double xm0 = flimit * std::cos(alpha_mid), xm1 = flimit * std::sin(alpha_mid);
+            double xm0 = flimit * t_cos(alpha_mid, crTrig), xm1 = flimit * t_sin(alpha_mid, crTrig);
             // 0.5 * x^T (A x) - x^T (-g)
             double f_mid = 0.5 * xm0 * (A00 * xm0 + A01 * xm1) + 0.5 * xm1 * (A10 * xm0 + A11 * xm1) -
                            (xm0 * (-g_t) + xm1 * (-g_o));
             bool leftlarger = false;
             for (size_t k = 0; k < maxSubIterations; ++k) {
               const double alpha_next = alpha_left + (alpha_right - alpha_mid);
-              const double xn0 = flimit * std::cos(alpha_next), xn1 = flimit * std::sin(alpha_next);
+              const double xn0 = flimit * t_cos(alpha_next, crTrig), xn1 = flimit * t_sin(alpha_next, crTrig);
               const double f_next = 0.5 * xn0 * (A00 * xn0 + A01 * xn1) + 0.5 * xn1 * (A10 * xn0 + A11 * xn1) -
                                     (xn0 * (-g_t) + xn1 * (-g_o));
               if (leftlarger) {
@@ -1099,6 +1112,16 @@ void orc_set_max_iterations(orc_world* w, size_t n) { w->maxIterations = n; }
 void orc_set_relaxation_model(orc_world* w, int model) { w->relaxationModel = model; }
 void orc_set_erp(orc_world* w, double erp) { w->erp = erp; }
 void orc_set_abort_threshold(orc_world* w, double thr) { w->abortThreshold = thr; }
+void orc_set_trig_mode(orc_world* w, int correctly_rounded) { w->crTrig = correctly_rounded != 0; }
+double orc_trig(int fn, double x, double y, int correctly_rounded) {
+  const bool cr = correctly_rounded != 0;
+  switch (fn) {
+    case 0: return t_sin(x, cr);
+    case 1: return t_cos(x, cr);
+    case 2: return t_atan2(y, x, cr);
+    default: return t_acos(x, cr);
+  }
+}
 
 // createMaterial, Materials.cpp:175-247 (pair tables: csf(i,j) = csf_i + csf_j)
 int orc_create_material(orc_world* w, double density, double cor, double csf, double cdf) {

@@ -21,8 +21,8 @@ EXPORTED = [
     "pehip_default_params", "pehip_create", "pehip_destroy", "pehip_set_params", "pehip_get_params",
     "pehip_set_materials", "pehip_upload_bodies", "pehip_upload_state", "pehip_set_forces",
     "pehip_set_contact_order", "pehip_step", "pehip_detect", "pehip_download_state", "pehip_get_contacts",
-    "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_set_profiling", "pehip_stream", "pehip_last_error",
-    "pehip_version",
+    "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_set_profiling", "pehip_eval_trig",
+    "pehip_stream", "pehip_last_error", "pehip_version",
 ]
 
 
@@ -31,7 +31,7 @@ class Params(ctypes.Structure):
                 ("erp", ctypes.c_double), ("abort_threshold", ctypes.c_double),
                 ("gravity", ctypes.c_double * 3), ("damping", ctypes.c_double),
                 ("relaxation_param", ctypes.c_double), ("max_sub_iterations", ctypes.c_uint32),
-                ("sweep_mode", ctypes.c_int32)]
+                ("sweep_mode", ctypes.c_int32), ("exact_trig", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -91,6 +91,7 @@ def load():
         "pehip_get_sweep_order": (st, [_VP, _SZ, ctypes.POINTER(_SZ), _U32, _U32, _U32]),
         "pehip_get_stats": (st, [_VP, ctypes.POINTER(Stats)]),
         "pehip_set_profiling": (st, [_VP, ctypes.c_int]),
+        "pehip_eval_trig": (st, [_VP, ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D]),
         "pehip_stream": (_VP, [_VP]),
         "pehip_last_error": (ctypes.c_char_p, [_VP]),
         "pehip_version": (ctypes.c_char_p, []),

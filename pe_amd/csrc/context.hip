@@ -230,6 +230,7 @@ void pehip_default_params(pehip_params* p) {
   p->relaxation_param = 0.9;
   p->max_sub_iterations = 20;
   p->sweep_mode = PEHIP_SWEEP_COLOURED;
+  p->exact_trig = 0;
 }
 
 static pehip_status check_params(const pehip_params* p, std::string& why) {
@@ -563,6 +564,8 @@ static pehip_status reference_perm(pehip_ctx* ctx) {
                                             std::to_string(ctx->order_b1.size()) + " vs " + std::to_string(nc) + ")");
   std::vector<uint32_t> b1(nc), b2(nc);
   if (nc) {
+    // the stream is non-blocking: wait for the narrowphase write pass before reading
+    CK(hipStreamSynchronize(ctx->stream));
     CK(hipMemcpy(b1.data(), ctx->raw.b1, 4 * (size_t)nc, hipMemcpyDeviceToHost));
     CK(hipMemcpy(b2.data(), ctx->raw.b2, 4 * (size_t)nc, hipMemcpyDeviceToHost));
   }
@@ -590,6 +593,7 @@ static pehip_status one_step(pehip_ctx* ctx, double dt) {
   const hipStream_t s = ctx->stream;
   const double dtinv = 1.0 / dt;
   const pehip_params& P = ctx->prm;
+  const bool exact = P.exact_trig != 0 || P.sweep_mode == PEHIP_SWEEP_REFERENCE_ORDER;
   Bodies B = ctx->bodies();
   mark(ctx, 0);
   CKS(detect(ctx));
@@ -616,7 +620,7 @@ static pehip_status one_step(pehip_ctx* ctx, double dt) {
   mark(ctx, 3);
   if (nc && P.max_iterations) {
     if (P.sweep_mode == PEHIP_SWEEP_REFERENCE_ORDER) {
-      launch_sweep_serial(P.relaxation_model, ctx->sol, nc, P.max_iterations, B, dtinv, (int)P.max_sub_iterations,
+      launch_sweep_serial(P.relaxation_model, exact, ctx->sol, nc, P.max_iterations, B, dtinv, (int)P.max_sub_iterations,
                           P.abort_threshold, ctx->deltas, s);
       CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + P.max_iterations, 8, hipMemcpyDeviceToHost, s));
       CKS(sync_check(ctx));
@@ -627,7 +631,7 @@ static pehip_status one_step(pehip_ctx* ctx, double dt) {
       for (uint32_t it = 0; it < P.max_iterations; ++it) {
         for (uint32_t k = 0; k < ncol; ++k) {
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
-          launch_sweep(P.relaxation_model, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
+          launch_sweep(P.relaxation_model, exact, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
                        (int)P.max_sub_iterations, ctx->deltas + it, s);
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
         }
@@ -644,7 +648,7 @@ static pehip_status one_step(pehip_ctx* ctx, double dt) {
     }
   }
   mark(ctx, 4);
-  launch_integrate(B, dt, P.damping, s);
+  launch_integrate(B, dt, P.damping, exact, s);
   mark(ctx, 5);
   // stats of this step
   CK(hipMemcpyAsync(ctx->hstage, ctx->maxpen, 16, hipMemcpyDeviceToHost, s));
@@ -803,6 +807,25 @@ pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable) {
     ctx->stats.prof_sweep_bytes = 0;
   }
   return PEHIP_OK;
+}
+
+pehip_status pehip_eval_trig(pehip_ctx* ctx, int fn, int exact, size_t n, const double* x, const double* y,
+                             double* out) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (fn < 0 || fn > 3 || !x || !out || (fn == 2 && !y)) return fail(ctx, PEHIP_INVALID_ARG, "invalid trig request");
+  if (!n) return PEHIP_OK;
+  hipSetDevice(ctx->device);
+  double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+  CK(dalloc(dx, n));
+  CK(dalloc(dy, n));
+  CK(dalloc(dout, n));
+  CK(hipMemcpy(dx, x, 8 * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(dy, y ? y : x, 8 * n, hipMemcpyHostToDevice));
+  launch_eval_trig(fn, exact != 0, n, dx, dy, dout, ctx->stream);
+  CK(hipStreamSynchronize(ctx->stream));
+  CK(hipMemcpy(out, dout, 8 * n, hipMemcpyDeviceToHost));
+  dfree(dx); dfree(dy); dfree(dout);
+  return sync_check(ctx);
 }
 
 void* pehip_stream(pehip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }

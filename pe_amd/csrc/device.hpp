@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "dmath.hpp"
+#include "trig_dd.hpp"
 
 namespace pe_amd {
 
@@ -263,8 +264,9 @@ __device__ __forceinline__ d3 cf_ld3(const double* f, size_t cap, int fld, size_
 // 2141-2290 (ApproximateInelasticCoulombContactByDecoupling), 2048-2093
 // (InelasticFrictionlessContact).  Returns |p_old - p_new|_inf (or |p_old|_inf for a
 // separating contact).  The caller guarantees that no other thread touches the
-// non-fixed bodies of contact i concurrently.
-template <int MODEL>
+// non-fixed bodies of contact i concurrently.  Trig = TrigFast (device libm) or TrigExact
+// (correctly rounded, reproduces the host libm's golden-section decisions).
+template <int MODEL, class Trig>
 __device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
                                        double* __restrict__ dvw, double dtinv, int maxSub) {
   const size_t cap = C.cap;
@@ -370,18 +372,18 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
             if (fsq > flimit * flimit) {
               const double fr = flimit / sqrt(fsq);
               const double phi = 0.5 * (1.0 + sqrt(5.0));
-              const double shift = atan2(-pc[2], pc[1]);
-              const double acos_f = acos(fr);
+              const double shift = Trig::atan2(-pc[2], pc[1]);
+              const double acos_f = Trig::acos(fr);
               double al = -acos_f - shift;
               double ar = acos_f - shift;
               double am = (ar + al * phi) / (1 + phi);
-              double xm0 = flimit * cos(am), xm1 = flimit * sin(am);
+              double xm0 = flimit * Trig::cos(am), xm1 = flimit * Trig::sin(am);
               double fm = 0.5 * xm0 * (A00 * xm0 + A01 * xm1) + 0.5 * xm1 * (A10 * xm0 + A11 * xm1) -
                           (xm0 * (-gt) + xm1 * (-go));
               bool leftlarger = false;
               for (int k = 0; k < maxSub; ++k) {
                 const double an = al + (ar - am);
-                const double xn0 = flimit * cos(an), xn1 = flimit * sin(an);
+                const double xn0 = flimit * Trig::cos(an), xn1 = flimit * Trig::sin(an);
                 const double fn = 0.5 * xn0 * (A00 * xn0 + A01 * xn1) + 0.5 * xn1 * (A10 * xn0 + A11 * xn1) -
                                   (xn0 * (-gt) + xn1 * (-go));
                 if (leftlarger) {

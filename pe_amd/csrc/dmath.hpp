@@ -113,12 +113,13 @@ PE_HD m33 mat_inv(const m33& m) {
 struct qd {
   double v[4];
 };
-// Quaternion(axis, angle) (Quaternion.h:274-294)
+// Quaternion(axis, angle) (Quaternion.h:274-294); Trig supplies sin and cos.
+template <class Trig>
 PE_HD qd quat_axis_angle(d3 axis, double angle) {
   qd q;
   if (fabs(angle) < 1e-15) { q.v[0] = 1; q.v[1] = 0; q.v[2] = 0; q.v[3] = 0; return q; }
-  const double sina = sin(angle * 0.5);
-  const double cosa = cos(angle * 0.5);
+  const double sina = Trig::sin(angle * 0.5);
+  const double cosa = Trig::cos(angle * 0.5);
   axis = nrm3(axis);
   q.v[0] = cosa; q.v[1] = sina * axis.x; q.v[2] = sina * axis.y; q.v[3] = sina * axis.z;
   return q;

@@ -49,6 +49,7 @@ __global__ void k_predict(Bodies B, d3 gravity, double dt) {
 
 // Apply the velocity corrections and integrate positions: resolveContacts' final
 // loop (HCSITS.h:1954-2004) and integratePositions (HCSITS.h:4135-4184).
+template <class Trig>
 __global__ void k_integrate(Bodies B, double dt, double damping) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= B.n) return;
@@ -58,7 +59,7 @@ __global__ void k_integrate(Bodies B, double dt, double damping) {
   d3 w = mk3(s[SB_W], s[SB_W + 1], s[SB_W + 2]) + mk3(q[3], q[4], q[5]);
   d3 gp = ld3(B.pos, i) + v * dt;
   const d3 phi = w * dt;
-  const qd dq = quat_axis_angle(phi, len3(phi));
+  const qd dq = quat_axis_angle<Trig>(phi, len3(phi));
   const qd qn = quat_mul(dq, ld4(B.q, i));
   const m33 R = quat_to_R(qn);
   if (damping < 1.0) {
@@ -115,8 +116,35 @@ void launch_aabb(const Bodies& B, hipStream_t s) {
 void launch_predict(const Bodies& B, d3 g, double dt, hipStream_t s) {
   if (B.n) hipLaunchKernelGGL(k_predict, dim3((B.n + 255) / 256), dim3(256), 0, s, B, g, dt);
 }
-void launch_integrate(const Bodies& B, double dt, double damping, hipStream_t s) {
-  if (B.n) hipLaunchKernelGGL(k_integrate, dim3((B.n + 255) / 256), dim3(256), 0, s, B, dt, damping);
+void launch_integrate(const Bodies& B, double dt, double damping, bool exact_trig, hipStream_t s) {
+  if (!B.n) return;
+  if (exact_trig)
+    hipLaunchKernelGGL(k_integrate<TrigExact>, dim3((B.n + 255) / 256), dim3(256), 0, s, B, dt, damping);
+  else
+    hipLaunchKernelGGL(k_integrate<TrigFast>, dim3((B.n + 255) / 256), dim3(256), 0, s, B, dt, damping);
+}
+
+// Device evaluation of one trig function over an array (checks of the device trig against
+// the host libm): fn 0 sin, 1 cos, 2 atan2(y, x), 3 acos.
+template <class Trig>
+__global__ void k_eval_trig(int fn, size_t n, const double* x, const double* y, double* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r;
+  switch (fn) {
+    case 0: r = Trig::sin(x[i]); break;
+    case 1: r = Trig::cos(x[i]); break;
+    case 2: r = Trig::atan2(y[i], x[i]); break;
+    default: r = Trig::acos(x[i]); break;
+  }
+  out[i] = r;
+}
+
+void launch_eval_trig(int fn, bool exact, size_t n, const double* x, const double* y, double* out, hipStream_t s) {
+  if (!n) return;
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+  if (exact) hipLaunchKernelGGL(k_eval_trig<TrigExact>, g, b, 0, s, fn, n, x, y, out);
+  else hipLaunchKernelGGL(k_eval_trig<TrigFast>, g, b, 0, s, fn, n, x, y, out);
 }
 uint32_t launch_kinetic(const Bodies& B, double* partial, hipStream_t s) {
   const uint32_t nb = (B.n + 255) / 256;

@@ -229,24 +229,24 @@ __device__ __forceinline__ void wave_max_atomic(double v, unsigned long long* ou
 }
 
 // One colour: every contact in [begin, end) is independent of the others.
-template <int MODEL>
+template <int MODEL, class Trig>
 __global__ void __launch_bounds__(256) k_sweep(SolverContacts C, uint32_t begin, uint32_t end, const double* sb,
                                                double* dvw, double dtinv, int maxSub, unsigned long long* delta) {
   const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0;
-  if (i < end) d = relax_contact<MODEL>(C, i, sb, dvw, dtinv, maxSub);
+  if (i < end) d = relax_contact<MODEL, Trig>(C, i, sb, dvw, dtinv, maxSub);
   wave_max_atomic(d, delta);
 }
 
 // All sweeps in the given contact order, one contact at a time (lane 0 of one wavefront).
-template <int MODEL>
+template <int MODEL, class Trig>
 __global__ void k_sweep_serial(SolverContacts C, uint32_t nc, uint32_t iters, const double* sb, double* dvw,
                                double dtinv, int maxSub, double thr, unsigned long long* deltas) {
   if (threadIdx.x != 0) return;
   uint32_t it = 0;
   while (it < iters) {
     double dmax = 0;
-    for (uint32_t i = 0; i < nc; ++i) dmax = smax(dmax, relax_contact<MODEL>(C, i, sb, dvw, dtinv, maxSub));
+    for (uint32_t i = 0; i < nc; ++i) dmax = smax(dmax, relax_contact<MODEL, Trig>(C, i, sb, dvw, dtinv, maxSub));
     deltas[it] = (unsigned long long)__double_as_longlong(dmax);
     ++it;
     if (thr > 0 && dmax < thr) break;
@@ -254,25 +254,41 @@ __global__ void k_sweep_serial(SolverContacts C, uint32_t nc, uint32_t iters, co
   deltas[iters] = it;
 }
 
-void launch_sweep(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
-                  double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
+template <class Trig>
+static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
+                           double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
   const uint32_t n = end - begin;
-  if (!n) return;
   const dim3 g((n + 255) / 256), b(256);
   switch (model) {
-    case 0: hipLaunchKernelGGL(k_sweep<0>, g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
-    case 1: hipLaunchKernelGGL(k_sweep<1>, g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
-    default: hipLaunchKernelGGL(k_sweep<3>, g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    case 0: hipLaunchKernelGGL((k_sweep<0, Trig>), g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    case 1: hipLaunchKernelGGL((k_sweep<1, Trig>), g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    default: hipLaunchKernelGGL((k_sweep<3, Trig>), g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
   }
 }
 
-void launch_sweep_serial(int model, const SolverContacts& C, uint32_t nc, uint32_t iters, const Bodies& B,
-                         double dtinv, int maxSub, double thr, unsigned long long* deltas, hipStream_t s) {
+void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
+  if (end <= begin) return;
+  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, s);
+  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, s);
+}
+
+template <class Trig>
+static void serial_dispatch(int model, const SolverContacts& C, uint32_t nc, uint32_t iters, const Bodies& B,
+                            double dtinv, int maxSub, double thr, unsigned long long* deltas, hipStream_t s) {
+  const dim3 g(1), b(64);
   switch (model) {
-    case 0: hipLaunchKernelGGL(k_sweep_serial<0>, dim3(1), dim3(64), 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
-    case 1: hipLaunchKernelGGL(k_sweep_serial<1>, dim3(1), dim3(64), 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
-    default: hipLaunchKernelGGL(k_sweep_serial<3>, dim3(1), dim3(64), 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 0: hipLaunchKernelGGL((k_sweep_serial<0, Trig>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 1: hipLaunchKernelGGL((k_sweep_serial<1, Trig>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    default: hipLaunchKernelGGL((k_sweep_serial<3, Trig>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
+}
+
+void launch_sweep_serial(int model, bool exact_trig, const SolverContacts& C, uint32_t nc, uint32_t iters,
+                         const Bodies& B, double dtinv, int maxSub, double thr, unsigned long long* deltas,
+                         hipStream_t s) {
+  if (exact_trig) serial_dispatch<TrigExact>(model, C, nc, iters, B, dtinv, maxSub, thr, deltas, s);
+  else serial_dispatch<TrigFast>(model, C, nc, iters, B, dtinv, maxSub, thr, deltas, s);
 }
 
 }  // namespace pe_amd

@@ -65,7 +65,8 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32
 
 void launch_aabb(const Bodies& B, hipStream_t s);
 void launch_predict(const Bodies& B, d3 g, double dt, hipStream_t s);
-void launch_integrate(const Bodies& B, double dt, double damping, hipStream_t s);
+void launch_integrate(const Bodies& B, double dt, double damping, bool exact_trig, hipStream_t s);
+void launch_eval_trig(int fn, bool exact, size_t n, const double* x, const double* y, double* out, hipStream_t s);
 uint32_t launch_kinetic(const Bodies& B, double* partial, hipStream_t s);
 
 // Broadphase: cell keys + sort + cell ranges.
@@ -105,11 +106,11 @@ void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, u
                   SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen,
                   hipStream_t s);
 // One colour of one sweep.
-void launch_sweep(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
-                  double dtinv, int maxSub, unsigned long long* delta, hipStream_t s);
+void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, hipStream_t s);
 // All sweeps, strictly sequential, one wavefront.  deltas[it] receives the L-inf impulse
 // change of sweep it; deltas[iters] the number of sweeps run (stops early below thr > 0).
-void launch_sweep_serial(int model, const SolverContacts& C, uint32_t nc, uint32_t iters,
+void launch_sweep_serial(int model, bool exact_trig, const SolverContacts& C, uint32_t nc, uint32_t iters,
                          const Bodies& B, double dtinv, int maxSub, double thr,
                          unsigned long long* deltas, hipStream_t s);
 

@@ -87,6 +87,7 @@ class World:
         self._abort = 0.0
         self._max_sub = 20
         self._sweep_mode = _abi.SWEEP_COLOURED
+        self._exact_trig = 0
         self._next_sid = 1
         self._shape, self._sid, self._mat, self._fixed = [], [], [], []
         self._param, self._pos, self._q, self._v, self._w = [], [], [], [], []
@@ -143,6 +144,11 @@ class World:
 
     def setSweepMode(self, mode):
         self._sweep_mode = int(mode)
+        self._params_dirty = True
+
+    def setExactTrig(self, on=True):
+        """Correctly rounded sin/cos/atan2/acos in the solver (reproduces the host libm)."""
+        self._exact_trig = 1 if on else 0
         self._params_dirty = True
 
     # ------------------------------------------------------------------ materials
@@ -314,6 +320,7 @@ class World:
         p.relaxation_param = self._relax_param
         p.max_sub_iterations = self._max_sub
         p.sweep_mode = self._sweep_mode
+        p.exact_trig = self._exact_trig
         return p
 
     def _ensure_device(self):
@@ -448,6 +455,16 @@ class World:
     def setProfiling(self, on=True):
         self._ensure_device()
         check(_abi.load().pehip_set_profiling(self._ctx, 1 if on else 0), self._ctx)
+
+    def evalTrig(self, fn, x, y=None, exact=True):
+        """Device sin (0), cos (1), atan2(y, x) (2) or acos (3) over an array."""
+        self._ensure_device()
+        xa = np.ascontiguousarray(x, float)
+        ya = np.ascontiguousarray(y, float) if y is not None else None
+        out = np.zeros_like(xa)
+        check(_abi.load().pehip_eval_trig(self._ctx, int(fn), 1 if exact else 0, len(xa), ptr(xa), ptr(ya), ptr(out)),
+              self._ctx)
+        return out
 
     def stream(self):
         self._ensure_device()

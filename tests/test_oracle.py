@@ -285,3 +285,36 @@ def test_c1_trajectory_observables():
     assert st["max_penetration"] < 0.01
     pos = w.get_state()["pos"][6:]
     assert pos[:, 2].min() > 0.45      # nothing fell through the floor
+
+
+def _dec_sin_cos(x):
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    x = Decimal(x)
+    s, c, term_s, term_c, k = x, Decimal(1), x, Decimal(1), 1
+    while abs(term_s) > Decimal(10) ** -58 or abs(term_c) > Decimal(10) ** -58:
+        term_c = -term_c * x * x / ((2 * k - 1) * (2 * k))
+        term_s = -term_s * x * x / ((2 * k) * (2 * k + 1))
+        c += term_c
+        s += term_s
+        k += 1
+    return s, c
+
+
+def test_oracle_correctly_rounded_trig():
+    # Quad-precision trig rounded once equals the exactly rounded value (60-digit Taylor series);
+    # the C library's agrees except on a small fraction of arguments.
+    rng = np.random.default_rng(4)
+    xs = rng.uniform(-6.5, 6.5, 300)
+    libm_diff = 0
+    for x in xs:
+        s, c = _dec_sin_cos(float(x))
+        assert O.trig(0, x, correctly_rounded=True) == float(s)
+        assert O.trig(1, x, correctly_rounded=True) == float(c)
+        libm_diff += O.trig(0, x) != float(s)
+    assert libm_diff <= 6
+    # atan2 / acos: round trips through the exactly rounded sin/cos
+    for y, x in [(0.3, 0.7), (-0.2, 0.9), (0.5, -0.4), (1e-3, 2.0)]:
+        a = O.trig(2, x, y, correctly_rounded=True)
+        assert a == pytest.approx(math.atan2(y, x), abs=4e-16)
+    assert O.trig(3, 0.25, correctly_rounded=True) == pytest.approx(math.acos(0.25), abs=4e-16)
