@@ -1,0 +1,501 @@
+// pe_amd/pe.hpp -- C++ drop-in for pe's World / CollisionSystem API on the HIP time step.
+//
+// A pe program that uses the hard-contact solver (pe_CONSTRAINT_SOLVER = HCSITS) switches
+// by replacing `#include <pe/engine.h>` with `#include <pe_amd/pe.hpp>` and linking
+// libpehip.so.  Names, argument order and error behaviour follow the reference:
+//   theWorld(), World::run/simulationStep/setGravity/setDamping   (pe/core/World.h, src/core/World.cpp:330-378)
+//   theCollisionSystem() and its HCSITS setters/getters            (HCSITS.h:198-260, 715-840)
+//   createMaterial                                                  (src/core/Materials.cpp:175-247)
+//   createSphere / createBox / createPlane                          (Sphere.cpp:1187, Box.cpp:1312, Plane.cpp:1202)
+// Body handles (SphereID, BoxID, PlaneID) read and write their state through the device;
+// the state is downloaded lazily after a step and uploaded before the next one.
+#ifndef PE_AMD_PE_HPP
+#define PE_AMD_PE_HPP
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../pe_hip.h"
+
+namespace pe {
+
+typedef double real;
+typedef uint64_t id_t;
+
+struct Vec3 {
+  real v[3];
+  Vec3() : v{0, 0, 0} {}
+  Vec3(real x, real y, real z) : v{x, y, z} {}
+  real& operator[](size_t i) { return v[i]; }
+  real operator[](size_t i) const { return v[i]; }
+  Vec3 operator*(real s) const { return Vec3(v[0] * s, v[1] * s, v[2] * s); }
+  real sqrLength() const { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+  real length() const { return std::sqrt(sqrLength()); }
+  // Vector3::normalize (pe/math/Vector3.h:994-1010): reciprocal length, then scale
+  Vec3& normalize() {
+    const real len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len == real(0)) return *this;
+    const real il = real(1) / len;
+    v[0] *= il; v[1] *= il; v[2] *= il;
+    return *this;
+  }
+};
+
+struct Quat {
+  real v[4];  // (w, x, y, z)
+  Quat() : v{1, 0, 0, 0} {}
+  Quat(real w, real x, real y, real z) : v{w, x, y, z} {}
+  // Quaternion(axis, angle), pe/math/Quaternion.h:274-294
+  Quat(Vec3 axis, real angle) : v{1, 0, 0, 0} {
+    if (std::fabs(angle) < real(1e-15)) return;
+    const real s = std::sin(angle * real(0.5)), c = std::cos(angle * real(0.5));
+    axis.normalize();
+    v[0] = c; v[1] = s * axis[0]; v[2] = s * axis[1]; v[3] = s * axis[2];
+  }
+  real operator[](size_t i) const { return v[i]; }
+};
+
+typedef size_t MaterialID;
+
+enum BodyType { sphereType = PEHIP_SPHERE, boxType = PEHIP_BOX, planeType = PEHIP_PLANE };
+
+namespace detail {
+
+// cofactor inverse in pe's evaluation order (Matrix3x3::getInverse, Matrix3x3.h:1084-1104)
+inline void inverse3(const real* v, real* o) {
+  real det = v[0] * ((v[4] * v[8]) - (v[7] * v[5])) + v[1] * ((v[5] * v[6]) - (v[8] * v[3])) +
+             v[2] * ((v[3] * v[7]) - (v[4] * v[6]));
+  det = real(1) / det;
+  o[0] = det * ((v[4] * v[8]) - (v[5] * v[7])); o[1] = det * ((v[7] * v[2]) - (v[8] * v[1]));
+  o[2] = det * ((v[1] * v[5]) - (v[2] * v[4])); o[3] = det * ((v[5] * v[6]) - (v[3] * v[8]));
+  o[4] = det * ((v[8] * v[0]) - (v[6] * v[2])); o[5] = det * ((v[2] * v[3]) - (v[0] * v[5]));
+  o[6] = det * ((v[3] * v[7]) - (v[4] * v[6])); o[7] = det * ((v[6] * v[1]) - (v[7] * v[0]));
+  o[8] = det * ((v[0] * v[4]) - (v[1] * v[3]));
+}
+
+inline void check(pehip_status st, const pehip_ctx* ctx) {
+  if (st == PEHIP_OK) return;
+  const std::string msg = pehip_last_error(ctx);
+  if (st == PEHIP_INVALID_ARG) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+struct Material {
+  std::string name;
+  real density, cor, csf, cdf;
+};
+
+class Engine;
+
+}  // namespace detail
+
+class RigidBody {
+ public:
+  BodyType getType() const;
+  id_t getSystemID() const;
+  id_t getID() const;
+  bool isFixed() const;
+  const Vec3& getPosition() const;
+  const Vec3& getLinearVel() const;
+  const Vec3& getAngularVel() const;
+  const Quat& getQuaternion() const;
+  real getMass() const;
+  real getInvMass() const;
+  void setPosition(const Vec3& p);
+  void setLinearVel(const Vec3& v);
+  void setAngularVel(const Vec3& w);
+  void setOrientation(const Quat& q);
+  void setFixed(bool fixed);
+  void addForce(const Vec3& f);
+  void addTorque(const Vec3& t);
+  size_t index() const { return index_; }
+
+ protected:
+  friend class detail::Engine;
+  explicit RigidBody(size_t index) : index_(index) {}
+  size_t index_;
+};
+
+class Sphere : public RigidBody {
+ public:
+  real getRadius() const;
+ private:
+  friend class detail::Engine;
+  using RigidBody::RigidBody;
+};
+class Box : public RigidBody {
+ public:
+  Vec3 getLengths() const;
+ private:
+  friend class detail::Engine;
+  using RigidBody::RigidBody;
+};
+class Plane : public RigidBody {
+ public:
+  Vec3 getNormal() const;
+  real getDisplacement() const;
+ private:
+  friend class detail::Engine;
+  using RigidBody::RigidBody;
+};
+
+typedef RigidBody* BodyID;
+typedef Sphere* SphereID;
+typedef Box* BoxID;
+typedef Plane* PlaneID;
+
+namespace detail {
+
+// Host mirror of the bodies (pe's BodyStorage order) and the device context.
+class Engine {
+ public:
+  static Engine& instance() {
+    static Engine e;
+    return e;
+  }
+  ~Engine() {
+    if (ctx_) pehip_destroy(ctx_);
+  }
+
+  pehip_params params;
+  std::vector<Material> materials;
+  std::vector<real> csf, cdf;  // pair tables
+  std::vector<std::unique_ptr<RigidBody>> handles;
+  // SoA host state in body order
+  std::vector<uint8_t> shape, fixed;
+  std::vector<uint64_t> sid, uid;
+  std::vector<uint32_t> mat;
+  std::vector<real> param, pos, q, v, w, inv_mass, mass, I, Iinv, normal, d, force, torque;
+  std::vector<Vec3> vpos, vv, vw;
+  std::vector<Quat> vq;
+  bool forces_pending = false;
+  bool bodies_dirty = true, state_dirty = false, state_stale = false, params_dirty = true;
+  uint64_t next_sid = 1;
+
+  pehip_ctx* ctx() {
+    if (!ctx_) {
+      check(pehip_create(&params, 0, &ctx_), nullptr);
+      params_dirty = false;
+    }
+    return ctx_;
+  }
+
+  MaterialID createMaterial(const std::string& name, real density, real cor, real csf_, real cdf_, real poisson,
+                            real young, real stiffness, real dampingN, real dampingT) {
+    for (const Material& m : materials)
+      if (m.name == name) throw std::invalid_argument("Material of that name already exists!");
+    if (density <= real(0)) throw std::invalid_argument("Invalid material density!");
+    if (cor < real(0) || cor > real(1)) throw std::invalid_argument("Invalid coefficient of restitution!");
+    if (csf_ < real(0)) throw std::invalid_argument("Invalid coefficient of static friction!");
+    if (cdf_ < real(0)) throw std::invalid_argument("Invalid coefficient of dynamic friction!");
+    if (poisson < real(-1) || poisson > real(0.5)) throw std::invalid_argument("Invalid Poisson's ratio");
+    if (young <= real(0)) throw std::invalid_argument("Invalid Young's modulus");
+    if (stiffness <= real(0)) throw std::invalid_argument("Invalid stiffness");
+    if (dampingN < real(0) || dampingT < real(0)) throw std::invalid_argument("Invalid damping coefficients");
+    const size_t n0 = materials.size(), n = n0 + 1;
+    materials.push_back(Material{name, density, cor, csf_, cdf_});
+    std::vector<real> s(n * n, 0), dd(n * n, 0);
+    for (size_t i = 0; i < n0; ++i)
+      for (size_t j = 0; j < n0; ++j) { s[i * n + j] = csf[i * n0 + j]; dd[i * n + j] = cdf[i * n0 + j]; }
+    s[n0 * n + n0] = csf_ + csf_;
+    dd[n0 * n + n0] = cdf_ + cdf_;
+    for (size_t i = 0; i < n0; ++i) {
+      s[n0 * n + i] = s[i * n + n0] = materials[i].csf + csf_;
+      dd[n0 * n + i] = dd[i * n + n0] = materials[i].cdf + cdf_;
+    }
+    csf.swap(s);
+    cdf.swap(dd);
+    bodies_dirty = true;
+    return n0;
+  }
+
+  template <class T>
+  T* add(int type, id_t user_id, MaterialID m, bool fx, const real* prm, const Vec3& gpos, const Quat& qq,
+         real massv, real im, const real* Ib, const Vec3& nrm, real dd) {
+    sync_host();
+    const size_t i = shape.size();
+    shape.push_back((uint8_t)type);
+    fixed.push_back(fx ? 1 : 0);
+    sid.push_back(next_sid++);
+    uid.push_back(user_id);
+    mat.push_back((uint32_t)m);
+    for (int k = 0; k < 3; ++k) { param.push_back(prm[k]); pos.push_back(gpos[k]); v.push_back(0); w.push_back(0);
+                                  normal.push_back(nrm[k]); force.push_back(0); torque.push_back(0); }
+    for (int k = 0; k < 4; ++k) q.push_back(qq[k]);
+    mass.push_back(massv);
+    inv_mass.push_back(im);
+    real inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (im != real(0)) inverse3(Ib, inv);
+    for (int k = 0; k < 9; ++k) { I.push_back(Ib[k]); Iinv.push_back(inv[k]); }
+    d.push_back(dd);
+    vpos.emplace_back(); vv.emplace_back(); vw.emplace_back(); vq.emplace_back();
+    T* h = new T(i);
+    handles.emplace_back(h);
+    bodies_dirty = true;
+    return h;
+  }
+
+  // download the device state after a step (lazy)
+  void sync_host() {
+    if (!state_stale || !ctx_) return;
+    const uint32_t n = (uint32_t)shape.size();
+    check(pehip_download_state(ctx_, n, pos.data(), q.data(), v.data(), w.data(), normal.data(), d.data()), ctx_);
+    state_stale = false;
+  }
+
+  void upload() {
+    pehip_ctx* c = ctx();
+    if (params_dirty) { check(pehip_set_params(c, &params), c); params_dirty = false; }
+    const uint32_t n = (uint32_t)shape.size();
+    if (bodies_dirty) {
+      if (materials.empty()) throw std::invalid_argument("no material");
+      check(pehip_set_materials(c, (uint32_t)materials.size(), csf.data(), cdf.data()), c);
+      check(pehip_upload_bodies(c, n, shape.data(), sid.data(), mat.data(), fixed.data(), param.data(), pos.data(),
+                                q.data(), v.data(), w.data(), inv_mass.data(), I.data(), Iinv.data(), normal.data(),
+                                d.data()), c);
+      bodies_dirty = state_dirty = false;
+    } else if (state_dirty) {
+      check(pehip_upload_state(c, n, pos.data(), q.data(), v.data(), w.data(), normal.data(), d.data()), c);
+      state_dirty = false;
+    }
+    if (forces_pending) {
+      check(pehip_set_forces(c, n, force.data(), torque.data()), c);
+      std::fill(force.begin(), force.end(), real(0));  // forces reset after the step (Settings.cpp:46)
+      std::fill(torque.begin(), torque.end(), real(0));
+      forces_pending = false;
+    }
+  }
+
+  void step(unsigned steps, real dt) {
+    if (dt <= real(0)) throw std::invalid_argument("Invalid time step size");  // World.cpp:333-334
+    if (shape.empty()) return;
+    sync_host();
+    upload();
+    check(pehip_step(ctx_, dt, steps), ctx_);
+    state_stale = true;
+  }
+
+  pehip_stats stats() {
+    pehip_stats s{};
+    if (ctx_) check(pehip_get_stats(ctx_, &s), ctx_);
+    return s;
+  }
+
+ private:
+  Engine() { pehip_default_params(&params); }
+  pehip_ctx* ctx_ = nullptr;
+};
+
+inline Engine& E() { return Engine::instance(); }
+
+}  // namespace detail
+
+// ---------------------------------------------------------------- body accessors
+inline BodyType RigidBody::getType() const { return (BodyType)detail::E().shape[index_]; }
+inline id_t RigidBody::getSystemID() const { return detail::E().sid[index_]; }
+inline id_t RigidBody::getID() const { return detail::E().uid[index_]; }
+inline bool RigidBody::isFixed() const { return detail::E().fixed[index_] != 0; }
+inline const Vec3& RigidBody::getPosition() const {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  e.vpos[index_] = Vec3(e.pos[3 * index_], e.pos[3 * index_ + 1], e.pos[3 * index_ + 2]);
+  return e.vpos[index_];
+}
+inline const Vec3& RigidBody::getLinearVel() const {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  e.vv[index_] = Vec3(e.v[3 * index_], e.v[3 * index_ + 1], e.v[3 * index_ + 2]);
+  return e.vv[index_];
+}
+inline const Vec3& RigidBody::getAngularVel() const {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  e.vw[index_] = Vec3(e.w[3 * index_], e.w[3 * index_ + 1], e.w[3 * index_ + 2]);
+  return e.vw[index_];
+}
+inline const Quat& RigidBody::getQuaternion() const {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  e.vq[index_] = Quat(e.q[4 * index_], e.q[4 * index_ + 1], e.q[4 * index_ + 2], e.q[4 * index_ + 3]);
+  return e.vq[index_];
+}
+inline real RigidBody::getMass() const { return detail::E().mass[index_]; }
+inline real RigidBody::getInvMass() const { return detail::E().inv_mass[index_]; }
+inline void RigidBody::setPosition(const Vec3& p) {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  for (int k = 0; k < 3; ++k) e.pos[3 * index_ + k] = p[k];
+  e.state_dirty = true;
+}
+inline void RigidBody::setLinearVel(const Vec3& vel) {
+  detail::Engine& e = detail::E();
+  if (isFixed()) return;
+  e.sync_host();
+  for (int k = 0; k < 3; ++k) e.v[3 * index_ + k] = vel[k];
+  e.state_dirty = true;
+}
+inline void RigidBody::setAngularVel(const Vec3& om) {
+  detail::Engine& e = detail::E();
+  if (isFixed()) return;
+  e.sync_host();
+  for (int k = 0; k < 3; ++k) e.w[3 * index_ + k] = om[k];
+  e.state_dirty = true;
+}
+inline void RigidBody::setOrientation(const Quat& qq) {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  for (int k = 0; k < 4; ++k) e.q[4 * index_ + k] = qq[k];
+  e.state_dirty = true;
+}
+// RigidBody::fix (src/core/rigidbody/RigidBody.cpp:402-418)
+inline void RigidBody::setFixed(bool fx) {
+  detail::Engine& e = detail::E();
+  if (getType() == planeType || !fx || isFixed()) return;
+  e.sync_host();
+  e.fixed[index_] = 1;
+  e.inv_mass[index_] = 0;
+  for (int k = 0; k < 9; ++k) e.Iinv[9 * index_ + k] = 0;
+  for (int k = 0; k < 3; ++k) { e.v[3 * index_ + k] = 0; e.w[3 * index_ + k] = 0; }
+  e.bodies_dirty = true;
+}
+inline void RigidBody::addForce(const Vec3& f) {
+  detail::Engine& e = detail::E();
+  for (int k = 0; k < 3; ++k) e.force[3 * index_ + k] += f[k];
+  e.forces_pending = true;
+}
+inline void RigidBody::addTorque(const Vec3& t) {
+  detail::Engine& e = detail::E();
+  for (int k = 0; k < 3; ++k) e.torque[3 * index_ + k] += t[k];
+  e.forces_pending = true;
+}
+inline real Sphere::getRadius() const { return detail::E().param[3 * index_]; }
+inline Vec3 Box::getLengths() const {
+  const detail::Engine& e = detail::E();
+  return Vec3(e.param[3 * index_], e.param[3 * index_ + 1], e.param[3 * index_ + 2]);
+}
+inline Vec3 Plane::getNormal() const {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  return Vec3(e.normal[3 * index_], e.normal[3 * index_ + 1], e.normal[3 * index_ + 2]);
+}
+inline real Plane::getDisplacement() const {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  return e.d[index_];
+}
+
+// ---------------------------------------------------------------- factories
+// createMaterial, Materials.cpp:175-247
+inline MaterialID createMaterial(const std::string& name, real density, real cor, real csf, real cdf, real poisson,
+                                 real young, real stiffness, real dampingN, real dampingT) {
+  return detail::E().createMaterial(name, density, cor, csf, cdf, poisson, young, stiffness, dampingN, dampingT);
+}
+
+// createSphere (Sphere.cpp:1187) with SphereBase's mass properties (SphereBase.cpp:64-87)
+inline SphereID createSphere(id_t uid, const Vec3& gpos, real radius, MaterialID material, bool visible = true) {
+  (void)visible;
+  if (radius <= real(0)) throw std::invalid_argument("Invalid sphere radius");
+  detail::Engine& e = detail::E();
+  const real m = real(4.0) / real(3.0) * M_PI * radius * radius * radius * e.materials.at(material).density;
+  real I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  I[0] = I[4] = I[8] = real(0.4) * m * radius * radius;
+  const real prm[3] = {radius, 0, 0};
+  return e.add<Sphere>(sphereType, uid, material, false, prm, gpos, Quat(), m, real(1) / m, I, Vec3(), 0);
+}
+
+// createBox (Box.cpp:1312) with BoxBase's mass properties (BoxBase.cpp:60-160)
+inline BoxID createBox(id_t uid, const Vec3& gpos, const Vec3& lengths, MaterialID material, bool visible = true) {
+  (void)visible;
+  if (lengths[0] <= real(0) || lengths[1] <= real(0) || lengths[2] <= real(0))
+    throw std::invalid_argument("Invalid side length");
+  detail::Engine& e = detail::E();
+  const real m = lengths[0] * lengths[1] * lengths[2] * e.materials.at(material).density;
+  real I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  I[0] = m / static_cast<real>(12) * (lengths[1] * lengths[1] + lengths[2] * lengths[2]);
+  I[4] = m / static_cast<real>(12) * (lengths[0] * lengths[0] + lengths[2] * lengths[2]);
+  I[8] = m / static_cast<real>(12) * (lengths[0] * lengths[0] + lengths[1] * lengths[1]);
+  const real prm[3] = {lengths[0], lengths[1], lengths[2]};
+  return e.add<Box>(boxType, uid, material, false, prm, gpos, Quat(), m, real(1) / m, I, Vec3(), 0);
+}
+
+// createPlane (Plane.cpp:1202-1240) and the Plane ctor (Plane.cpp:77-102)
+inline PlaneID createPlane(id_t uid, Vec3 normal, real d, MaterialID material, bool visible = true) {
+  (void)visible;
+  if (normal.sqrLength() == real(0)) throw std::invalid_argument("Invalid plane normal!");
+  normal.normalize();
+  Quat q = (normal[0] * normal[0] + normal[1] * normal[1] < real(1e-8))
+               ? Quat(Vec3(1, 0, 0), std::acos(normal[2]))
+               : Quat(Vec3(-normal[1], normal[0], real(0)), std::acos(normal[2]));
+  const real zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  return detail::E().add<Plane>(planeType, uid, material, true, zero, normal * d, q, 0, 0, zero, normal, d);
+}
+
+// ---------------------------------------------------------------- World / CollisionSystem
+class World {
+ public:
+  void setGravity(real gx, real gy, real gz) {
+    detail::Engine& e = detail::E();
+    e.params.gravity[0] = gx; e.params.gravity[1] = gy; e.params.gravity[2] = gz;
+    e.params_dirty = true;
+  }
+  void setGravity(const Vec3& g) { setGravity(g[0], g[1], g[2]); }
+  void setDamping(real damping) {
+    if (damping < real(0) || damping > real(1)) throw std::invalid_argument("Invalid damping value");
+    detail::E().params.damping = damping;
+    detail::E().params_dirty = true;
+  }
+  size_t size() const { return detail::E().shape.size(); }
+  BodyID operator[](size_t i) { return detail::E().handles.at(i).get(); }
+  void simulationStep(real timestep) { run(1, timestep); }
+  void run(unsigned int steps, real timestep) { detail::E().step(steps, timestep); }
+};
+typedef World* WorldID;
+inline WorldID theWorld() {
+  static World w;
+  return &w;
+}
+
+class CollisionSystem {
+ public:
+  enum RelaxationModel {
+    InelasticFrictionlessContact = PEHIP_INELASTIC_FRICTIONLESS,
+    ApproximateInelasticCoulombContactByDecoupling = PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING,
+    ApproximateInelasticCoulombContactByOrthogonalProjections = PEHIP_APPROX_INELASTIC_COULOMB_ORTHO,
+    InelasticCoulombContactByDecoupling = PEHIP_INELASTIC_COULOMB_DECOUPLING,
+    InelasticCoulombContactByOrthogonalProjections = PEHIP_INELASTIC_COULOMB_ORTHO,
+    InelasticGeneralizedMaximumDissipationContact = PEHIP_INELASTIC_GMD
+  };
+  void setMaxIterations(size_t n) { p().max_iterations = (uint32_t)n; dirty(); }
+  void setRelaxationModel(RelaxationModel m) { p().relaxation_model = (int32_t)m; dirty(); }
+  void setErrorReductionParameter(real erp) {
+    if (erp < real(0) || erp > real(1)) throw std::invalid_argument("Error reduction parameter out of range.");
+    p().erp = erp;
+    dirty();
+  }
+  void setRelaxationParameter(real f) {
+    if (f <= real(0)) throw std::invalid_argument("Relaxation parameter must be positive.");
+    p().relaxation_param = f;
+    dirty();
+  }
+  real getMaximumPenetration() const { return detail::E().stats().max_penetration; }
+  size_t getNumberOfContacts() const { return (size_t)detail::E().stats().num_contacts; }
+  pehip_stats getStats() const { return detail::E().stats(); }
+
+ private:
+  pehip_params& p() { return detail::E().params; }
+  void dirty() { detail::E().params_dirty = true; }
+};
+typedef CollisionSystem* CollisionSystemID;
+inline CollisionSystemID theCollisionSystem() {
+  static CollisionSystem cs;
+  return &cs;
+}
+
+}  // namespace pe
+
+#endif

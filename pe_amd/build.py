@@ -55,7 +55,24 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_examples(verbose)
     return LIB
+
+
+def build_examples(verbose=False):
+    """C++ programs on the drop-in API (include/pe_amd/pe.hpp), linked against the library."""
+    exdir = os.path.join(HERE, "examples")
+    for src in sorted(os.listdir(exdir)) if os.path.isdir(exdir) else []:
+        if not src.endswith(".cpp"):
+            continue
+        exe = os.path.join(OUT, src[:-4])
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), os.path.join(exdir, src),
+               "-o", exe, "-L", OUT, "-lpehip", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("example build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+        if verbose:
+            print("built", exe)
 
 
 def clean():

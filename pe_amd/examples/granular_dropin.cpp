@@ -1,0 +1,50 @@
+// A pe-style program on the drop-in API: spheres on a lattice fall into a box of planes
+// (the setup of examples/lattice/Lattice.cpp:197-217 with a deterministic jitter).
+// Usage: granular_dropin <nx> <ny> <nz> <steps>; prints the final state as hex floats.
+#include <cstdio>
+#include <cstdlib>
+
+#include <pe_amd/pe.hpp>
+
+using namespace pe;
+
+int main(int argc, char** argv) {
+  const int nx = argc > 1 ? std::atoi(argv[1]) : 5;
+  const int ny = argc > 2 ? std::atoi(argv[2]) : 5;
+  const int nz = argc > 3 ? std::atoi(argv[3]) : 5;
+  const unsigned steps = argc > 4 ? (unsigned)std::atoi(argv[4]) : 100;
+  const real r = 0.5, spacing = 1.2, dt = 1e-3;
+  const real lx = 2.0 + nx * spacing, ly = 2.0 + ny * spacing, lz = 40.0;
+
+  WorldID world = theWorld();
+  world->setGravity(0, 0, -9.81);
+  world->setDamping(1);
+  CollisionSystemID cs = theCollisionSystem();
+  cs->setMaxIterations(50);
+  cs->setRelaxationModel(CollisionSystem::InelasticCoulombContactByDecoupling);
+
+  MaterialID granular = createMaterial("granular", 2500, 0, 0.2, 0.2, 0.5, 1, 1, 0, 0);
+  createPlane(0, Vec3(0, 0, +1), 0, granular);
+  createPlane(1, Vec3(0, 0, -1), -lz, granular);
+  createPlane(2, Vec3(0, +1, 0), 0, granular);
+  createPlane(3, Vec3(0, -1, 0), -ly, granular);
+  createPlane(4, Vec3(+1, 0, 0), 0, granular);
+  createPlane(5, Vec3(-1, 0, 0), -lx, granular);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const real jit = 0.001 * (real)((i * 7 + j * 13 + k * 17) % 11 - 5);
+        createSphere(6, Vec3(1.5 + spacing * i + jit, 1.5 + spacing * j - jit, 0.6 + spacing * k + jit), r, granular);
+      }
+
+  world->run(steps, dt);
+
+  std::printf("contacts %zu maxpen %a\n", cs->getNumberOfContacts(), cs->getMaximumPenetration());
+  for (size_t b = 0; b < world->size(); ++b) {
+    BodyID body = (*world)[b];
+    const Vec3& p = body->getPosition();
+    const Vec3& v = body->getLinearVel();
+    std::printf("%zu %a %a %a %a %a %a\n", b, p[0], p[1], p[2], v[0], v[1], v[2]);
+  }
+  return 0;
+}

@@ -1,0 +1,50 @@
+"""The C++ drop-in API (include/pe_amd/pe.hpp) driven like a pe program: its result equals
+the Python World's on the same scene bit for bit (same kernels, same inputs)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from pe_amd import World
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "pe_amd", "_lib", "granular_dropin")
+
+
+def python_run(nx, ny, nz, steps):
+    w = World()
+    w.setGravity(0, 0, -9.81)
+    w.setMaxIterations(50)
+    w.setRelaxationModel(3)
+    m = w.createMaterial("granular", 2500, 0, 0.2, 0.2)
+    lx, ly, lz = 2.0 + nx * 1.2, 2.0 + ny * 1.2, 40.0
+    for n, d in [((0, 0, 1), 0.0), ((0, 0, -1), -lz), ((0, 1, 0), 0.0), ((0, -1, 0), -ly), ((1, 0, 0), 0.0),
+                 ((-1, 0, 0), -lx)]:
+        w.createPlane(0, n, d, m)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                jit = 0.001 * float((i * 7 + j * 13 + k * 17) % 11 - 5)
+                w.createSphere(6, (1.5 + 1.2 * i + jit, 1.5 + 1.2 * j - jit, 0.6 + 1.2 * k + jit), 0.5, m)
+    w.run(steps, 1e-3)
+    return w
+
+
+def test_cpp_dropin_matches_python_world():
+    if not os.path.exists(EXE):
+        pytest.fail("drop-in example not built")
+    nx, ny, nz, steps = 5, 4, 3, 220
+    out = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps)], capture_output=True, text=True, check=True,
+                         timeout=120).stdout.split("\n")
+    head = out[0].split()
+    rows = [l.split() for l in out[1:] if l.strip()]
+    pos = np.array([[float.fromhex(x) for x in r[1:4]] for r in rows])
+    vel = np.array([[float.fromhex(x) for x in r[4:7]] for r in rows])
+    w = python_run(nx, ny, nz, steps)
+    st = w.getState()
+    assert len(rows) == w.size
+    assert int(head[1]) == w.getNumberOfContacts() and int(head[1]) > 0
+    assert np.array_equal(pos, st["pos"]) and np.array_equal(vel, st["v"])
