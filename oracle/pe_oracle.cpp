@@ -6,8 +6,7 @@
  * floating-point operation order of the reference is preserved everywhere
  * (it is compiled with -ffp-contract=off and without fast-math), because the
  * HIP path is checked against it to 1e-12.  Citations are file:line in the pe
- * source tree.  Box-box contacts (MaxContacts.h:867-2268) are not restated
- * here; scenes with two boxes that touch are rejected by orc_step.
+ * source tree.
  */
 #include "oracle_api.h"
 
@@ -285,6 +284,7 @@ struct Fine {
   void sphereBox(int s, int b);
   void spherePlane(int s, int p);
   void boxPlane(int b, int p);
+  void boxBox(int b1, int b2);
   void collide(int a, int b);
 };
 
@@ -726,6 +726,214 @@ void Fine::boxPlane(int b, int p) {
   }
 }
 
+// intersectLines, src/core/GeomTools.cpp:387-408
+void intersectLines(const V3& o1, const V3& d1, const V3& o2, const V3& d2, double& s, double& t) {
+  const double sqrlen = dot(cross(d1, d2), cross(d1, d2));
+  if (sqrlen < 1e-8) {  // parallelThreshold, Thresholds.h:170
+    s = 0.0;
+    t = 0.0;
+  } else {
+    const double isqrlen = 1.0 / sqrlen;
+    const V3 p = o2 - o1;
+    const double dt = dot(d1, d2);
+    const double a = dot(d1, p);
+    const double b = -dot(d2, p);
+    s = (a * dot(d2, d2) + dt * b) * isqrlen;
+    t = (b * dot(d1, d1) + dt * a) * isqrlen;
+  }
+}
+
+// MaxContacts::collideBoxBox, MaxContacts.h:867-2268: separating-axis test over 3 + 3 face
+// axes and 9 edge axes, then one edge-edge contact or a clipped face contact (up to 8).
+void Fine::boxBox(int i1, int i2) {
+  std::vector<Body>& bs = world->bodies;
+  if (bs[i2].sid < bs[i1].sid) std::swap(i1, i2);
+  const Body* B1 = &bs[i1];
+  const Body* B2 = &bs[i2];
+  double maxDepth = -kInf;
+  bool invertNormal = false;
+  unsigned contactCase = 0;
+  V3 contactNormal, normal_c;
+  const M3& R1 = B1->R;
+  const M3& R2 = B2->R;
+  const M3 R = trans(R1) * R2;  // b2_rR
+  M3 Q;                          // b2_rQ = fabs(b2_rR)
+  for (int k = 0; k < 9; ++k) Q[k] = std::fabs(R[k]);
+  const V3 P = trans(R1) * (B2->gpos - B1->gpos);  // b2_rPos
+  const double hl1[3] = {0.5 * B1->lengths[0], 0.5 * B1->lengths[1], 0.5 * B1->lengths[2]};
+  const double hl2[3] = {0.5 * B2->lengths[0], 0.5 * B2->lengths[1], 0.5 * B2->lengths[2]};
+  // face axes of box 1 (cases 1-3)
+  for (int a = 0; a < 3; ++a) {
+    const double term1 = std::fabs(P[a]) - (hl1[a] + hl2[0] * Q[3 * a] + hl2[1] * Q[3 * a + 1] + hl2[2] * Q[3 * a + 2]);
+    if (term1 > kContactThreshold) return;
+    if (a == 0 || term1 > maxDepth) {
+      maxDepth = term1;
+      contactNormal = V3(R1[a], R1[3 + a], R1[6 + a]);
+      contactCase = 1 + a;
+      invertNormal = (P[a] < 0.0);
+    }
+  }
+  // face axes of box 2 (cases 4-6)
+  for (int a = 0; a < 3; ++a) {
+    const double term1 = P[0] * R[a] + P[1] * R[3 + a] + P[2] * R[6 + a];
+    const double term2 = std::fabs(term1) - (hl1[0] * Q[a] + hl1[1] * Q[3 + a] + hl1[2] * Q[6 + a] + hl2[a]);
+    if (term2 > kContactThreshold) return;
+    if (term2 > maxDepth) {
+      maxDepth = term2;
+      contactNormal = V3(R2[a], R2[3 + a], R2[6 + a]);
+      contactCase = 4 + a;
+      invertNormal = (term1 < 0.0);
+    }
+  }
+  // edge axes (cases 7-15): term1 = P[pa]*R[ra] - P[pb]*R[rb]; term2 = hl1[h]*Q[q] + hl1[h']*Q[q'];
+  // term3 = hl2[h]*Q[q] + hl2[h']*Q[q']; length = sqrt(R[la]^2 + R[lb]^2); normal_c components
+  // as (sign, index) with sign 0 for a zero component.
+  static const int E[9][14] = {
+      {2, 3, 1, 6, 1, 6, 2, 3, 2, 1, 1, 2, 6, 3},  // case 7
+      {2, 4, 1, 7, 1, 7, 2, 4, 0, 2, 2, 0, 7, 4},  // case 8
+      {2, 5, 1, 8, 1, 8, 2, 5, 1, 0, 0, 1, 8, 5},  // case 9
+      {0, 6, 2, 0, 2, 0, 0, 6, 2, 4, 1, 5, 6, 0},  // case 10
+      {0, 7, 2, 1, 2, 1, 0, 7, 0, 5, 2, 3, 7, 1},  // case 11
+      {0, 8, 2, 2, 2, 2, 0, 8, 1, 3, 0, 4, 8, 2},  // case 12
+      {1, 0, 0, 3, 0, 3, 1, 0, 2, 7, 1, 8, 3, 0},  // case 13
+      {1, 1, 0, 4, 0, 4, 1, 1, 0, 8, 2, 6, 4, 1},  // case 14
+      {1, 2, 0, 5, 0, 5, 1, 2, 1, 6, 0, 7, 5, 2}}; // case 15
+  static const int NC[9][6] = {{0, 0, -1, 6, 1, 3}, {0, 0, -1, 7, 1, 4}, {0, 0, -1, 8, 1, 5},
+                               {1, 6, 0, 0, -1, 0}, {1, 7, 0, 0, -1, 1}, {1, 8, 0, 0, -1, 2},
+                               {-1, 3, 1, 0, 0, 0}, {-1, 4, 1, 1, 0, 0}, {-1, 5, 1, 2, 0, 0}};
+  for (int e = 0; e < 9; ++e) {
+    const int* t = E[e];
+    const double term1 = P[t[0]] * R[t[1]] - P[t[2]] * R[t[3]];
+    const double term2 = hl1[t[4]] * Q[t[5]] + hl1[t[6]] * Q[t[7]];
+    const double term3 = hl2[t[8]] * Q[t[9]] + hl2[t[10]] * Q[t[11]];
+    double sum = std::fabs(term1) - (term2 + term3);
+    if (std::fabs(term1) > term2 + term3 + kContactThreshold) return;
+    const double length = std::sqrt(R[t[12]] * R[t[12]] + R[t[13]] * R[t[13]]);
+    if (length > std::numeric_limits<double>::epsilon()) {
+      sum /= length;
+      if (sum > maxDepth && std::fabs(sum - maxDepth) > kAccuracy) {
+        maxDepth = sum;
+        double c[3];
+        for (int k = 0; k < 3; ++k) {
+          const int sg = NC[e][2 * k], ix = NC[e][2 * k + 1];
+          c[k] = sg == 0 ? 0.0 : (sg > 0 ? R[ix] / length : -R[ix] / length);
+        }
+        normal_c = V3(c[0], c[1], c[2]);
+        contactCase = 7 + e;
+        invertNormal = (term1 < 0.0);
+      }
+    }
+  }
+  if (contactCase == 0) return;
+  if (contactCase > 6) contactNormal = R1 * normal_c;
+  if (invertNormal) contactNormal = -contactNormal;
+
+  if (contactCase > 6) {  // edge-edge
+    V3 pB1 = B1->gpos;
+    V3 sign = trans(R1) * contactNormal;
+    for (int i = 0; i < 3; ++i) sign[i] = (sign[i] > 0) ? hl1[i] : -hl1[i];
+    pB1 += R1 * sign;
+    V3 pB2 = B2->gpos;
+    sign = trans(R2) * contactNormal;
+    for (int i = 0; i < 3; ++i) sign[i] = (sign[i] > 0) ? -hl2[i] : hl2[i];
+    pB2 += R2 * sign;
+    V3 ua, ub;
+    for (int i = 0; i < 3; ++i) {
+      ua[i] = R1[(contactCase - 7) / 3 + i * 3];
+      ub[i] = R2[(contactCase - 7) % 3 + i * 3];
+    }
+    double s, t;
+    intersectLines(pB1, ua, pB2, ub, s, t);
+    pB1 += s * ua;
+    pB2 += t * ub;
+    const V3 gpos = 0.5 * (pB1 + pB2);
+    add(i2, i1, gpos, contactNormal, maxDepth);  // addEdgeEdgeContact(b2, b1, ...); edges unused by HCSITS
+    return;
+  }
+
+  // face case: the reference face belongs to box "a", the incident face to box "b"
+  const double* hla = hl1;
+  const double* hlb = hl2;
+  if (contactCase > 3) {
+    std::swap(i1, i2);
+    std::swap(B1, B2);
+    std::swap(hla, hlb);
+    contactNormal = -contactNormal;
+  }
+  const M3& Ra = B1->R;
+  const M3& Rb = B2->R;
+  const V3 normala = trans(Ra) * contactNormal;
+  const V3 normalb = trans(Rb) * contactNormal;
+  unsigned xb, yb, zb;
+  if (std::fabs(normalb[0]) > std::fabs(normalb[1])) {
+    if (std::fabs(normalb[0]) > std::fabs(normalb[2])) { xb = 0; yb = 1; zb = 2; }
+    else { xb = 2; yb = 0; zb = 1; }
+  } else if (std::fabs(normalb[1]) > std::fabs(normalb[2])) {
+    xb = 1; yb = 2; zb = 0;
+  } else {
+    xb = 2; yb = 0; zb = 1;
+  }
+  const unsigned xa = (contactCase < 4) ? contactCase - 1 : contactCase - 4;
+  const unsigned ya = (xa == 0) ? 1 : (xa == 1 ? 2 : 0);
+  const unsigned za = (xa == 0) ? 2 : (xa == 1 ? 0 : 1);
+  V3 vertexb[4];
+  for (int k = 0; k < 4; ++k) {
+    vertexb[k][xb] = (normalb[xb] > 0.0) ? -hlb[xb] : hlb[xb];
+    vertexb[k][yb] = (k & 1) ? hlb[yb] : -hlb[yb];
+    vertexb[k][zb] = (k & 2) ? hlb[zb] : -hlb[zb];
+  }
+  const V3 ab = B2->gpos - B1->gpos;
+  V3 vba[4];
+  for (int k = 0; k < 4; ++k) vba[k] = trans(Ra) * (ab + Rb * vertexb[k]);
+  const double offseta = (normala[xa] > 0.0) ? hla[xa] : -hla[xa];
+  // incident-face edges (0,1) (0,2) (3,1) (3,2), each clipped at ya = +-hla[ya], za = +-hla[za]
+  static const int edges[4][2] = {{0, 1}, {0, 2}, {3, 1}, {3, 2}};
+  for (int e = 0; e < 4; ++e) {
+    const V3& v0 = vba[edges[e][0]];
+    const V3& v1 = vba[edges[e][1]];
+    for (int side = 0; side < 4; ++side) {
+      const unsigned d = (side < 2) ? ya : za;
+      const unsigned o = (side < 2) ? za : ya;
+      const double bound = (side & 1) ? -hla[d] : hla[d];
+      if ((v0[d] > bound) ^ (v1[d] > bound)) {
+        const double s = (bound - v0[d]) / (v1[d] - v0[d]);
+        double tmp;
+        if (s > 0.0 && s < 1.0 && std::fabs(tmp = v0[o] + s * (v1[o] - v0[o])) < hla[o]) {
+          const double dist = std::fabs(v0[xa] + s * (v1[xa] - v0[xa])) - hla[xa];
+          if (dist < kContactThreshold) {
+            V3 posa;
+            posa[xa] = offseta;
+            posa[d] = bound;
+            posa[o] = tmp;
+            add(i2, i1, B1->gpos + Ra * posa, contactNormal, dist);
+          }
+        }
+      }
+    }
+  }
+  for (int k = 0; k < 4; ++k) {  // incident vertices inside the reference face
+    double dist;
+    if (std::fabs(vba[k][ya]) <= hla[ya] && std::fabs(vba[k][za]) <= hla[za] &&
+        (dist = std::fabs(vba[k][xa]) - hla[xa]) < kContactThreshold)
+      add(i2, i1, B1->gpos + Ra * vba[k], contactNormal, dist);
+  }
+  V3 vertexa[4];
+  for (int k = 0; k < 4; ++k) {
+    vertexa[k][xa] = (normala[xa] > 0.0) ? hla[xa] : -hla[xa];
+    vertexa[k][ya] = (k & 1) ? hla[ya] : -hla[ya];
+    vertexa[k][za] = (k & 2) ? hla[za] : -hla[za];
+  }
+  const V3 ba = B1->gpos - B2->gpos;
+  for (int k = 0; k < 4; ++k) {  // reference corners inside the incident face
+    const V3 vab = trans(Rb) * (ba + Ra * vertexa[k]);
+    double dist;
+    if (((std::fabs(vab[yb]) <= hlb[yb] && std::fabs(vab[zb]) < hlb[zb]) ||
+         (std::fabs(vab[yb]) < hlb[yb] && std::fabs(vab[zb]) <= hlb[zb])) &&
+        (dist = std::fabs(vab[xb]) - hlb[xb]) < kContactThreshold)
+      add(i2, i1, B2->gpos + Rb * vab, contactNormal, dist);
+  }
+}
+
 // MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/plane subset)
 void Fine::collide(int a, int b) {
   const int ta = world->bodies[a].shape, tb = world->bodies[b].shape;
@@ -735,7 +943,7 @@ void Fine::collide(int a, int b) {
     else spherePlane(a, b);
   } else if (ta == ORC_BOX) {
     if (tb == ORC_SPHERE) sphereBox(b, a);
-    else if (tb == ORC_BOX) throw std::runtime_error("box-box contacts are not restated");
+    else if (tb == ORC_BOX) boxBox(a, b);
     else boxPlane(a, b);
   } else {
     if (tb == ORC_SPHERE) spherePlane(b, a);

@@ -514,7 +514,6 @@ static pehip_status detect(pehip_ctx* ctx) {
   const uint32_t nc = ctx->hstage[0], fl = ctx->hstage[1];
   if (fl & ERR_NONFINITE) return fail(ctx, PEHIP_NONFINITE, "non-finite body position or bounding box");
   if (fl & ERR_LEVEL_OVERFLOW) return fail(ctx, PEHIP_INVALID_ARG, "body larger than the coarsest grid cell");
-  if (fl & ERR_BOX_BOX) return fail(ctx, PEHIP_NOT_SUPPORTED, "box-box contacts are not supported");
   CKS(ensure_contact_capacity(ctx, nc));
   if (nc) launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, true, s);
   ctx->nc = nc;

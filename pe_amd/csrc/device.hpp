@@ -200,8 +200,239 @@ __device__ void np_box_plane(const Bodies& B, uint32_t b, uint32_t p, Emit& emit
   }
 }
 
+// Row-major 3x3 * 3-vector and its transpose on plain arrays.
+PE_HD void mv3(const double* m, const double* v, double* o) {
+  o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  o[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+  o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+PE_HD void mtv3(const double* m, const double* v, double* o) {
+  o[0] = m[0] * v[0] + m[3] * v[1] + m[6] * v[2];
+  o[1] = m[1] * v[0] + m[4] * v[1] + m[7] * v[2];
+  o[2] = m[2] * v[0] + m[5] * v[1] + m[8] * v[2];
+}
+
+// collideBoxBox, MaxContacts.h:867-2268: separating axes (3 + 3 face axes, 9 edge axes), then
+// one edge-edge contact (at the closest points of the two edges, intersectLines,
+// src/core/GeomTools.cpp:387-408) or the clipped face contacts: the incident face's edges
+// against the reference face's sides, incident vertices inside the reference face, and
+// reference corners inside the incident face.
+template <class Emit>
+__device__ void np_box_box(const Bodies& B, uint32_t i1, uint32_t i2, Emit& emit) {
+  if (B.sid[i2] < B.sid[i1]) { uint32_t t = i1; i1 = i2; i2 = t; }
+  double R1[9], R2[9], R[9], Q[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) { R1[k] = B.R[9 * (size_t)i1 + k]; R2[k] = B.R[9 * (size_t)i2 + k]; }
+  {
+    m33 a, b;
+    for (int k = 0; k < 9; ++k) { a.a[k] = R1[k]; b.a[k] = R2[k]; }
+    const m33 r = mat_mul(mat_trans(a), b);
+    for (int k = 0; k < 9; ++k) { R[k] = r.a[k]; Q[k] = fabs(r.a[k]); }
+  }
+  const double* p1 = B.pos + 3 * (size_t)i1;
+  const double* p2 = B.pos + 3 * (size_t)i2;
+  double d12[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  double P[3];
+  mtv3(R1, d12, P);
+  const double* L1 = B.param + 3 * (size_t)i1;
+  const double* L2 = B.param + 3 * (size_t)i2;
+  const double hl1[3] = {0.5 * L1[0], 0.5 * L1[1], 0.5 * L1[2]};
+  const double hl2[3] = {0.5 * L2[0], 0.5 * L2[1], 0.5 * L2[2]};
+  double maxDepth = -__builtin_huge_val();
+  bool invertNormal = false;
+  unsigned contactCase = 0;
+  double cn[3] = {0, 0, 0}, nc[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) {  // face axes of box 1
+    const double term1 = fabs(P[a]) - (hl1[a] + hl2[0] * Q[3 * a] + hl2[1] * Q[3 * a + 1] + hl2[2] * Q[3 * a + 2]);
+    if (term1 > kContactThreshold) return;
+    if (a == 0 || term1 > maxDepth) {
+      maxDepth = term1;
+      cn[0] = R1[a]; cn[1] = R1[3 + a]; cn[2] = R1[6 + a];
+      contactCase = 1 + a;
+      invertNormal = (P[a] < 0.0);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {  // face axes of box 2
+    const double term1 = P[0] * R[a] + P[1] * R[3 + a] + P[2] * R[6 + a];
+    const double term2 = fabs(term1) - (hl1[0] * Q[a] + hl1[1] * Q[3 + a] + hl1[2] * Q[6 + a] + hl2[a]);
+    if (term2 > kContactThreshold) return;
+    if (term2 > maxDepth) {
+      maxDepth = term2;
+      cn[0] = R2[a]; cn[1] = R2[3 + a]; cn[2] = R2[6 + a];
+      contactCase = 4 + a;
+      invertNormal = (term1 < 0.0);
+    }
+  }
+  // edge axes (cases 7-15), table of index tuples (see the column meaning below)
+  //   term1 = P[0]*R[1] - P[2]*R[3];  term2 = hl1[4]*Q[5] + hl1[6]*Q[7];
+  //   term3 = hl2[8]*Q[9] + hl2[10]*Q[11];  length = sqrt(R[12]^2 + R[13]^2)
+  constexpr int8_t E[9][14] = {
+      {2, 3, 1, 6, 1, 6, 2, 3, 2, 1, 1, 2, 6, 3}, {2, 4, 1, 7, 1, 7, 2, 4, 0, 2, 2, 0, 7, 4},
+      {2, 5, 1, 8, 1, 8, 2, 5, 1, 0, 0, 1, 8, 5}, {0, 6, 2, 0, 2, 0, 0, 6, 2, 4, 1, 5, 6, 0},
+      {0, 7, 2, 1, 2, 1, 0, 7, 0, 5, 2, 3, 7, 1}, {0, 8, 2, 2, 2, 2, 0, 8, 1, 3, 0, 4, 8, 2},
+      {1, 0, 0, 3, 0, 3, 1, 0, 2, 7, 1, 8, 3, 0}, {1, 1, 0, 4, 0, 4, 1, 1, 0, 8, 2, 6, 4, 1},
+      {1, 2, 0, 5, 0, 5, 1, 2, 1, 6, 0, 7, 5, 2}};
+  // normal_c per case: (sign, index of R) per component, sign 0 = zero component
+  constexpr int8_t NC[9][6] = {{0, 0, -1, 6, 1, 3}, {0, 0, -1, 7, 1, 4}, {0, 0, -1, 8, 1, 5},
+                               {1, 6, 0, 0, -1, 0}, {1, 7, 0, 0, -1, 1}, {1, 8, 0, 0, -1, 2},
+                               {-1, 3, 1, 0, 0, 0}, {-1, 4, 1, 1, 0, 0}, {-1, 5, 1, 2, 0, 0}};
+  for (int e = 0; e < 9; ++e) {
+    const int8_t* t = E[e];
+    const double term1 = P[t[0]] * R[t[1]] - P[t[2]] * R[t[3]];
+    const double term2 = hl1[t[4]] * Q[t[5]] + hl1[t[6]] * Q[t[7]];
+    const double term3 = hl2[t[8]] * Q[t[9]] + hl2[t[10]] * Q[t[11]];
+    double sum = fabs(term1) - (term2 + term3);
+    if (fabs(term1) > term2 + term3 + kContactThreshold) return;
+    const double length = sqrt(R[t[12]] * R[t[12]] + R[t[13]] * R[t[13]]);
+    if (length > 2.220446049250313e-16) {  // machine epsilon (pe's `epsilon`)
+      sum /= length;
+      if (sum > maxDepth && fabs(sum - maxDepth) > 1e-8) {  // pe's `accuracy`
+        maxDepth = sum;
+        for (int k = 0; k < 3; ++k) {
+          const int sg = NC[e][2 * k], ix = NC[e][2 * k + 1];
+          nc[k] = sg == 0 ? 0.0 : (sg > 0 ? R[ix] / length : -R[ix] / length);
+        }
+        contactCase = 7 + e;
+        invertNormal = (term1 < 0.0);
+      }
+    }
+  }
+  if (contactCase == 0) return;
+  if (contactCase > 6) mv3(R1, nc, cn);
+  if (invertNormal) { cn[0] = -cn[0]; cn[1] = -cn[1]; cn[2] = -cn[2]; }
+  const d3 ncontact = mk3(cn[0], cn[1], cn[2]);
+
+  if (contactCase > 6) {  // edge-edge
+    double sg[3], off[3];
+    mtv3(R1, cn, sg);
+    for (int i = 0; i < 3; ++i) sg[i] = (sg[i] > 0) ? hl1[i] : -hl1[i];
+    mv3(R1, sg, off);
+    d3 pB1 = mk3(p1[0], p1[1], p1[2]) + mk3(off[0], off[1], off[2]);
+    mtv3(R2, cn, sg);
+    for (int i = 0; i < 3; ++i) sg[i] = (sg[i] > 0) ? -hl2[i] : hl2[i];
+    mv3(R2, sg, off);
+    d3 pB2 = mk3(p2[0], p2[1], p2[2]) + mk3(off[0], off[1], off[2]);
+    const d3 ua = mk3(R1[(contactCase - 7) / 3], R1[(contactCase - 7) / 3 + 3], R1[(contactCase - 7) / 3 + 6]);
+    const d3 ub = mk3(R2[(contactCase - 7) % 3], R2[(contactCase - 7) % 3 + 3], R2[(contactCase - 7) % 3 + 6]);
+    double s = 0.0, t = 0.0;
+    const d3 cr = cross3(ua, ub);
+    const double sqrlen = dot3(cr, cr);
+    if (!(sqrlen < 1e-8)) {  // parallelThreshold
+      const double isqrlen = 1.0 / sqrlen;
+      const d3 pp = pB2 - pB1;
+      const double dt = dot3(ua, ub);
+      const double a = dot3(ua, pp);
+      const double b = -dot3(ub, pp);
+      s = (a * dot3(ub, ub) + dt * b) * isqrlen;
+      t = (b * dot3(ua, ua) + dt * a) * isqrlen;
+    }
+    pB1 = pB1 + s * ua;
+    pB2 = pB2 + t * ub;
+    emit(i2, i1, 0.5 * (pB1 + pB2), ncontact, maxDepth);
+    return;
+  }
+
+  // face case: reference face on box "a", incident face on box "b"
+  const double* hla = hl1;
+  const double* hlb = hl2;
+  const double* Ra = R1;
+  const double* Rb = R2;
+  const double* pa = p1;
+  const double* pb = p2;
+  double n[3] = {cn[0], cn[1], cn[2]};
+  if (contactCase > 3) {
+    uint32_t tt = i1; i1 = i2; i2 = tt;
+    hla = hl2; hlb = hl1; Ra = R2; Rb = R1; pa = p2; pb = p1;
+    n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+  }
+  const d3 nn = mk3(n[0], n[1], n[2]);
+  double na[3], nb[3];
+  mtv3(Ra, n, na);
+  mtv3(Rb, n, nb);
+  unsigned xb, yb, zb;
+  if (fabs(nb[0]) > fabs(nb[1])) {
+    if (fabs(nb[0]) > fabs(nb[2])) { xb = 0; yb = 1; zb = 2; }
+    else { xb = 2; yb = 0; zb = 1; }
+  } else if (fabs(nb[1]) > fabs(nb[2])) {
+    xb = 1; yb = 2; zb = 0;
+  } else {
+    xb = 2; yb = 0; zb = 1;
+  }
+  const unsigned xa = (contactCase < 4) ? contactCase - 1 : contactCase - 4;
+  const unsigned ya = (xa == 0) ? 1 : (xa == 1 ? 2 : 0);
+  const unsigned za = (xa == 0) ? 2 : (xa == 1 ? 0 : 1);
+  double ab[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+  double vba[4][3];
+  for (int k = 0; k < 4; ++k) {
+    double vb[3], w[3], u[3];
+    vb[xb] = (nb[xb] > 0.0) ? -hlb[xb] : hlb[xb];
+    vb[yb] = (k & 1) ? hlb[yb] : -hlb[yb];
+    vb[zb] = (k & 2) ? hlb[zb] : -hlb[zb];
+    mv3(Rb, vb, w);
+    u[0] = ab[0] + w[0]; u[1] = ab[1] + w[1]; u[2] = ab[2] + w[2];
+    mtv3(Ra, u, vba[k]);
+  }
+  const double offseta = (na[xa] > 0.0) ? hla[xa] : -hla[xa];
+  constexpr int8_t edges[4][2] = {{0, 1}, {0, 2}, {3, 1}, {3, 2}};
+  for (int e = 0; e < 4; ++e) {
+    const double* v0 = vba[edges[e][0]];
+    const double* v1 = vba[edges[e][1]];
+    for (int side = 0; side < 4; ++side) {
+      const unsigned d = (side < 2) ? ya : za;
+      const unsigned o = (side < 2) ? za : ya;
+      const double bound = (side & 1) ? -hla[d] : hla[d];
+      if ((v0[d] > bound) ^ (v1[d] > bound)) {
+        const double s = (bound - v0[d]) / (v1[d] - v0[d]);
+        if (s > 0.0 && s < 1.0) {
+          const double tmp = v0[o] + s * (v1[o] - v0[o]);
+          if (fabs(tmp) < hla[o]) {
+            const double dist = fabs(v0[xa] + s * (v1[xa] - v0[xa])) - hla[xa];
+            if (dist < kContactThreshold) {
+              double posa[3], w[3];
+              posa[xa] = offseta;
+              posa[d] = bound;
+              posa[o] = tmp;
+              mv3(Ra, posa, w);
+              emit(i2, i1, mk3(pa[0], pa[1], pa[2]) + mk3(w[0], w[1], w[2]), nn, dist);
+            }
+          }
+        }
+      }
+    }
+  }
+  for (int k = 0; k < 4; ++k) {  // incident vertices inside the reference face
+    if (fabs(vba[k][ya]) <= hla[ya] && fabs(vba[k][za]) <= hla[za]) {
+      const double dist = fabs(vba[k][xa]) - hla[xa];
+      if (dist < kContactThreshold) {
+        double w[3];
+        mv3(Ra, vba[k], w);
+        emit(i2, i1, mk3(pa[0], pa[1], pa[2]) + mk3(w[0], w[1], w[2]), nn, dist);
+      }
+    }
+  }
+  double ba[3] = {pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]};
+  for (int k = 0; k < 4; ++k) {  // reference corners inside the incident face
+    double va[3], w[3], u[3], vab[3];
+    va[xa] = (na[xa] > 0.0) ? hla[xa] : -hla[xa];
+    va[ya] = (k & 1) ? hla[ya] : -hla[ya];
+    va[za] = (k & 2) ? hla[za] : -hla[za];
+    mv3(Ra, va, w);
+    u[0] = ba[0] + w[0]; u[1] = ba[1] + w[1]; u[2] = ba[2] + w[2];
+    mtv3(Rb, u, vab);
+    if (((fabs(vab[yb]) <= hlb[yb] && fabs(vab[zb]) < hlb[zb]) ||
+         (fabs(vab[yb]) < hlb[yb] && fabs(vab[zb]) <= hlb[zb]))) {
+      const double dist = fabs(vab[xb]) - hlb[xb];
+      if (dist < kContactThreshold) {
+        double w2[3];
+        mv3(Rb, vab, w2);
+        emit(i2, i1, mk3(pb[0], pb[1], pb[2]) + mk3(w2[0], w2[1], w2[2]), nn, dist);
+      }
+    }
+  }
+}
+
 // MaxContacts::collide dispatch for (a, b) as handed over by the broadphase
-// (MaxContacts.h:242-526).  Box-box pairs are handled by their own kernel.
+// (MaxContacts.h:242-526).
 template <class Emit>
 __device__ void narrowphase(const Bodies& B, uint32_t a, uint32_t b, Emit& emit) {
   const int ta = B.shape[a], tb = B.shape[b];
@@ -211,6 +442,7 @@ __device__ void narrowphase(const Bodies& B, uint32_t a, uint32_t b, Emit& emit)
     else np_sphere_plane(B, a, b, emit);
   } else if (ta == kShapeBox) {
     if (tb == kShapeSphere) np_sphere_box(B, b, a, emit);
+    else if (tb == kShapeBox) np_box_box(B, a, b, emit);
     else if (tb == kShapePlane) np_box_plane(B, a, b, emit);
   } else {
     if (tb == kShapeSphere) np_sphere_plane(B, b, a, emit);

@@ -331,7 +331,6 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
 #pragma unroll
   for (int k = 0; k < 6; ++k) abox[k] = aa[k];
   const bool fa = B.fixed[a] != 0;
-  const bool boxa = B.shape[a] == kShapeBox;
   for (int l = (int)la; l < G.nlevels; ++l) {
     long long c[3];
     bucket_of(G, l, abox, c);
@@ -350,7 +349,6 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
             if (l == (int)la && b <= a) continue;
             if (fa && B.fixed[b]) continue;
             if (!aabb_overlap(abox, B.aabb + 6 * (size_t)b)) continue;
-            if (boxa && B.shape[b] == kShapeBox) { atomicOr(flags, ERR_BOX_BOX); continue; }
             narrowphase(B, a, b, emit);
           }
         }

@@ -96,7 +96,44 @@ def box_scene():
                         box_mat=np.zeros(nb, np.uint32), v0=v0, dt=1e-3, max_iterations=30)
 
 
-SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene}
+def _qaxis(axis, ang):
+    ax = np.asarray(axis, float)
+    return np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * ax / np.linalg.norm(ax)])
+
+
+def stack_scene():
+    # box-box contacts of every kind collideBoxBox distinguishes (MaxContacts.h:867-2268):
+    # face-face between boxes turned about z (incident-face clipping), an edge resting on a
+    # face, two crossed edges (edge-edge), and two boxes side by side (parallel faces); plus
+    # box-plane corners and spheres on the top box of a tower
+    rng = np.random.default_rng(21)
+    e, s45 = 1e-6, np.sqrt(0.5)
+    pos, lens, qs = [], [], []
+    for k in range(3):                                          # tower, turned about z
+        pos.append([1.5, 1.5, 0.25 + 0.5 * k - (k + 1) * e])
+        lens.append([1.0, 1.0, 0.5])
+        qs.append(_qaxis((0, 0, 1), rng.uniform(-0.25, 0.25)))
+    pos.append([4.5, 1.5, 0.3 - e]); lens.append([1.2, 1.2, 0.6]); qs.append(_qaxis((0, 0, 1), 0.1))
+    h = 0.25 * s45 + 0.25 * s45                                 # edge resting on a face
+    pos.append([4.5, 1.5, 0.6 - e + h - e]); lens.append([1.0, 0.5, 0.5]); qs.append(_qaxis((1, 0, 0), np.pi / 4))
+    h = 0.3 * s45 + 0.3 * s45                                   # crossed edges
+    pos.append([5.5, 5.0, h - e]); lens.append([0.6, 1.0, 0.6]); qs.append(_qaxis((0, 1, 0), np.pi / 4))
+    pos.append([5.5, 5.0, 3 * h - 2 * e]); lens.append([1.0, 0.6, 0.6]); qs.append(_qaxis((1, 0, 0), np.pi / 4))
+    pos.append([2.0, 5.0, 0.5 - e]); lens.append([1.0, 1.0, 1.0]); qs.append([1.0, 0, 0, 0])  # side by side
+    pos.append([3.0 - e, 5.0, 0.5 - e]); lens.append([1.0, 1.0, 1.0]); qs.append([1.0, 0, 0, 0])
+    nb = len(pos)
+    sph = np.array([[1.3, 1.5, 1.5 - 3 * e + 0.2 - e], [1.7, 1.6, 1.5 - 3 * e + 0.2 - e]])
+    sph += rng.uniform(-1e-7, 1e-7, size=sph.shape)
+    ns = len(sph)
+    v0 = np.concatenate([np.zeros((6, 3)), rng.uniform(-0.2, 0.2, size=(ns, 3)) * [1, 1, 0],
+                         rng.uniform(-0.05, 0.05, size=(nb, 3)) * [1, 1, 0]])
+    return scenes.Scene("stack", [{"name": "m", "density": 2000.0, "cor": 0.0, "csf": 0.4, "cdf": 0.4}],
+                        scenes.box_planes(8.0, 8.0, 20.0), sph, np.full(ns, 0.2), np.zeros(ns, np.uint32),
+                        box_pos=np.array(pos), box_len=np.array(lens), box_q=np.array(qs),
+                        box_mat=np.zeros(nb, np.uint32), v0=v0, dt=1e-3, max_iterations=30)
+
+
+SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene, "stack": stack_scene}
 
 
 def oracle_state_after(sc, steps, iters=30, cr_trig=False):

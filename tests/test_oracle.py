@@ -318,3 +318,61 @@ def test_oracle_correctly_rounded_trig():
         a = O.trig(2, x, y, correctly_rounded=True)
         assert a == pytest.approx(math.atan2(y, x), abs=4e-16)
     assert O.trig(3, 0.25, correctly_rounded=True) == pytest.approx(math.acos(0.25), abs=4e-16)
+
+
+# --------------------------------------------------------------------------- box-box (SAT)
+def _rot(axis, ang):
+    return O.quat_from_axis_angle(axis, ang)
+
+
+def test_box_box_stacked_face_contacts():
+    # Two axis-aligned unit boxes, the upper one shifted by (0.2, 0.1) and sunk 0.01:
+    # four contacts at the corners of the overlap rectangle, normal +z from the lower box
+    # (body2) to the upper one (body1), distance -0.01.
+    w = make_world()
+    w.create_box([0, 0, 0], [1, 1, 1], 0)
+    w.create_box([0.2, 0.1, 0.99], [1, 1, 1], 0)
+    c = w.detect(True)
+    assert len(c["b1"]) == 4 and np.all(c["b1"] == 1) and np.all(c["b2"] == 0)
+    assert np.allclose(c["dist"], -0.01, atol=1e-15) and np.allclose(c["normal"], [0, 0, 1], atol=1e-15)
+    xy = {(round(p[0], 9), round(p[1], 9)) for p in c["pos"]}
+    assert xy == {(-0.3, -0.4), (0.5, -0.4), (-0.3, 0.5), (0.5, 0.5)}
+    assert np.all((c["pos"][:, 2] >= 0.49 - 1e-12) & (c["pos"][:, 2] <= 0.5 + 1e-12))
+
+
+def test_box_box_crossed_edges_single_contact():
+    # Box 0 rotated 45 deg about x (top is an edge along x), box 1 rotated 45 deg about y
+    # (bottom is an edge along y), gap -1e-3 between the edges: one edge-edge contact midway.
+    h = math.sqrt(0.5)
+    w = make_world()
+    w.create_box([0, 0, 0], [1, 1, 1], 0)
+    w.create_box([0.05, -0.03, 2 * h - 1e-3], [1, 1, 1], 0)
+    st = w.get_state()
+    st["q"][0] = _rot([1, 0, 0], math.pi / 4)
+    st["q"][1] = _rot([0, 1, 0], math.pi / 4)
+    w.set_state(st)
+    c = w.detect(True)
+    assert len(c["b1"]) == 1 and c["b1"][0] == 1 and c["b2"][0] == 0
+    assert c["dist"][0] == pytest.approx(-1e-3, abs=1e-12)
+    assert np.allclose(c["normal"][0], [0, 0, 1], atol=1e-12)
+    # the edges cross at x = 0.05 (box 1's edge) and y = 0 (box 0's edge); midpoint of the gap
+    assert np.allclose(c["pos"][0], [0.05, 0.0, h - 0.5e-3], atol=1e-12)
+
+
+def test_box_box_separated_and_swapped_order():
+    w = make_world()
+    w.create_box([0, 0, 0], [1, 1, 1], 0)
+    w.create_box([0, 0, 1.01], [1, 1, 1], 0)
+    assert len(w.detect(True)["b1"]) == 0
+    # the lower SID body is the reference in face cases of box 2's axes: swap gives body1 = 0
+    w2 = make_world()
+    w2.create_box([0, 0, 0.99], [1, 1, 1], 0)
+    w2.create_box([0, 0, 0], [2, 2, 1], 0)
+    c = w2.detect(True)
+    assert len(c["b1"]) == 4
+    assert np.all(c["b2"] != c["b1"])
+    n = c["normal"][0]
+    # the normal points from body2 to body1
+    x1 = np.array([0, 0, 0.99]) if c["b1"][0] == 0 else np.array([0, 0, 0.0])
+    x2 = np.array([0, 0, 0.0]) if c["b1"][0] == 0 else np.array([0, 0, 0.99])
+    assert np.dot(n, x1 - x2) > 0
