@@ -198,6 +198,91 @@ const char* pehip_last_error(const pehip_ctx* ctx);
 /* Library version string. */
 const char* pehip_version(void);
 
+/* ------------------------------------------------------------------------------------------
+ * Batched scenes.  Many independent pe worlds (scenes) of uneven body counts advance together:
+ * one workgroup runs one scene's whole time step (detection on a per-scene hash grid in LDS,
+ * colouring, contact caching, the NBGS sweeps with workgroup barriers between colours,
+ * integration), and a persistent grid of workgroups pulls scenes from a work queue, largest
+ * scene first.  Each scene follows the same semantics as a pehip_ctx in coloured mode.  pe
+ * has no batched mode: a scene is what one pe process simulates (World + CollisionSystem
+ * singletons, World.cpp:65, HCSITS.h:118).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct pehip_batch pehip_batch;
+
+/* Launch configuration of the batched step (chosen by pehip_batch_autotune or set by hand). */
+typedef struct {
+  uint32_t threads;          /* workgroup size per scene: 64, 128, 256 or 512 */
+  uint32_t blocks_per_cu;    /* persistent workgroups per compute unit (1..16) */
+  uint32_t steps_per_launch; /* time steps a workgroup advances a scene before taking the next */
+  uint32_t use_graph;        /* 1: replay the launches of pehip_batch_step from a captured hipGraph */
+} pehip_batch_tuning;
+
+typedef struct {
+  uint32_t nscenes;
+  uint64_t num_bodies;
+  uint64_t contact_capacity;   /* total contact slots over all scenes */
+  uint64_t num_contacts;       /* contacts of the last step, summed over scenes */
+  uint32_t max_colours;        /* largest colour count of a scene in the last step */
+  uint64_t steps_done_min;     /* fewest steps any scene has completed since upload */
+  uint32_t capacity_retries;   /* contact-capacity growths since upload */
+  uint32_t launches;           /* kernel launches of the last pehip_batch_step */
+  double sweep_bytes;          /* algorithmic sweep bytes of the last step over all scenes */
+} pehip_batch_stats;
+
+pehip_status pehip_batch_create(const pehip_params* params, int device, pehip_batch** out);
+pehip_status pehip_batch_destroy(pehip_batch* b);
+pehip_status pehip_batch_set_params(pehip_batch* b, const pehip_params* params);
+/* Shared material pair tables (as pehip_set_materials). */
+pehip_status pehip_batch_set_materials(pehip_batch* b, uint32_t n, const double* csf_pair,
+                                       const double* cdf_pair);
+/* Uploads all scenes: scene s owns bodies [scene_offset[s], scene_offset[s+1]) of the body
+ * arrays, which have the layout of pehip_upload_bodies.  A scene holds at most 4096 bodies and
+ * 16 planes; its grid bodies must be of similar size (one hash-grid level per scene, cell edge
+ * = the largest grid-body bounding box).  contacts_per_body sizes each scene's contact slots
+ * (capacity = contacts_per_body * bodies + 64; grown on demand, see pehip_batch_step). */
+pehip_status pehip_batch_upload(pehip_batch* b, uint32_t nscenes, const uint32_t* scene_offset,
+                                const uint8_t* shape, const uint64_t* sid, const uint32_t* material,
+                                const uint8_t* fixed, const double* param, const double* pos,
+                                const double* q, const double* v, const double* w,
+                                const double* inv_mass, const double* inertia_body,
+                                const double* inv_inertia_body, const double* plane_normal,
+                                const double* plane_d, double contacts_per_body);
+/* Per-scene friction: every contact of scene s gets static_mu[s] (or dynamic_mu[s] when its
+ * tangential speed exceeds the friction threshold) instead of the material tables.  NULL
+ * pointers restore the material tables. */
+pehip_status pehip_batch_set_scene_friction(pehip_batch* b, uint32_t nscenes, const double* static_mu,
+                                            const double* dynamic_mu);
+/* Overwrites the dynamic state of all bodies (layout of pehip_upload_state). */
+pehip_status pehip_batch_upload_state(pehip_batch* b, uint64_t n, const double* pos, const double* q,
+                                      const double* v, const double* w, const double* plane_normal,
+                                      const double* plane_d);
+/* Advances every scene by nsteps time steps of dt.  A scene whose contacts outgrow its slots
+ * stops where it is; the library then grows that scene's capacity and finishes its steps, so
+ * the call returns with every scene advanced (PEHIP_CAPACITY_RETRY only if the growth fails). */
+pehip_status pehip_batch_step(pehip_batch* b, double dt, uint32_t nsteps);
+pehip_status pehip_batch_download_state(pehip_batch* b, uint64_t n, double* pos, double* q, double* v,
+                                        double* w, double* plane_normal, double* plane_d);
+/* Per-scene results of the last step: contacts, colours, sweeps run, max penetration, steps
+ * completed since upload.  Any pointer may be NULL. */
+pehip_status pehip_batch_scene_stats(pehip_batch* b, uint32_t nscenes, uint32_t* contacts, uint32_t* colours,
+                                     uint32_t* iterations, double* max_penetration, uint32_t* steps_done);
+/* Contacts of scene s from its last step (body indices are global, detection order). */
+pehip_status pehip_batch_get_contacts(pehip_batch* b, uint32_t scene, size_t cap, size_t* n, uint32_t* body1,
+                                      uint32_t* body2, double* pos, double* normal, double* dist, double* mu);
+/* Sweep order of scene s in its last step (as pehip_get_sweep_order). */
+pehip_status pehip_batch_get_sweep_order(pehip_batch* b, uint32_t scene, size_t cap, size_t* n,
+                                         uint32_t* contact_index, uint32_t* ncolours, uint32_t* colour_offsets);
+pehip_status pehip_batch_get_stats(pehip_batch* b, pehip_batch_stats* out);
+pehip_status pehip_batch_get_tuning(pehip_batch* b, pehip_batch_tuning* out);
+pehip_status pehip_batch_set_tuning(pehip_batch* b, const pehip_batch_tuning* t);
+/* Autotuner: times trial_steps steps of dt for each launch configuration from the current state
+ * (restored after every trial, so the scenes do not advance), keeps the fastest and returns it.
+ * Results do not depend on the configuration. */
+pehip_status pehip_batch_autotune(pehip_batch* b, double dt, uint32_t trial_steps, pehip_batch_tuning* best);
+/* The batch's HIP stream (hipStream_t). */
+void* pehip_batch_stream(pehip_batch* b);
+const char* pehip_batch_last_error(const pehip_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

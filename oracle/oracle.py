@@ -71,6 +71,7 @@ def lib():
             "orc_step": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
             "orc_num_contacts": (ctypes.c_size_t, [vp]),
             "orc_set_sweep_order": (None, [vp, ctypes.c_size_t, _U32, _U32]),
+            "orc_set_sweep_order_ranked": (None, [vp, ctypes.c_size_t, _U32, _U32, _U32]),
             "orc_get_contacts": (None, [vp, _U32, _U32, _D, _D, _D, _D]),
             "orc_get_impulses": (None, [vp, _D]),
             "orc_detect": (ctypes.c_size_t, [vp, ctypes.c_int]),
@@ -174,6 +175,14 @@ class OracleWorld:
         a = np.ascontiguousarray(b1, np.uint32)
         b = np.ascontiguousarray(b2, np.uint32)
         self.L.orc_set_sweep_order(self.h, len(a), _p(a, _U32), _p(b, _U32))
+
+    def set_sweep_order_ranked(self, b1, b2, rank):
+        """Relax the next step's contacts in this order; entry k is the rank[k]-th contact (in
+        narrowphase order) of the pair (b1[k], b2[k])."""
+        a = np.ascontiguousarray(b1, np.uint32)
+        b = np.ascontiguousarray(b2, np.uint32)
+        r = np.ascontiguousarray(rank, np.uint32)
+        self.L.orc_set_sweep_order_ranked(self.h, len(a), _p(a, _U32), _p(b, _U32), _p(r, _U32))
 
     def contacts(self):
         nc = int(self.L.orc_num_contacts(self.h))

@@ -87,6 +87,10 @@ int orc_step(orc_world* w, double dt, int nsteps);
  * instead of the emission order (contacts of one pair keep their narrowphase order).
  * Lets the oracle replay a sweep order chosen elsewhere, e.g. colour by colour. */
 void orc_set_sweep_order(orc_world* w, size_t n, const uint32_t* b1, const uint32_t* b2);
+/* As orc_set_sweep_order, but entry k names the rank[k]-th contact (in narrowphase order) of
+ * the pair (b1[k], b2[k]) -- for orders that interleave the contacts of one pair. */
+void orc_set_sweep_order_ranked(orc_world* w, size_t n, const uint32_t* b1, const uint32_t* b2,
+                                const uint32_t* rank);
 
 /* Contacts of the last step, in the reference's emission order. */
 size_t orc_num_contacts(const orc_world* w);

@@ -452,7 +452,7 @@ struct orc_world {
   bool gridActive = false;
 
   std::vector<Contact> contacts;
-  std::vector<uint32_t> orderB1, orderB2;
+  std::vector<uint32_t> orderB1, orderB2, orderRank;  // orderRank empty: k-th occurrence order
   bool orderPending = false;
   // HCSITS caches, HCSITS.h:314-327
   std::vector<V3> v_, w_, dv_, dw_;
@@ -972,8 +972,12 @@ void orc_world::simulationStep(double dt) {
     std::vector<bool> taken(contacts.size(), false);
     for (size_t k = 0; k < orderB1.size(); ++k) {
       size_t found = contacts.size();
-      for (size_t c = 0; c < contacts.size(); ++c)
-        if (!taken[c] && (uint32_t)contacts[c].b1 == orderB1[k] && (uint32_t)contacts[c].b2 == orderB2[k]) { found = c; break; }
+      uint32_t seen = 0;
+      for (size_t c = 0; c < contacts.size(); ++c) {
+        if ((uint32_t)contacts[c].b1 != orderB1[k] || (uint32_t)contacts[c].b2 != orderB2[k]) continue;
+        if (orderRank.empty() ? !taken[c] : seen++ == orderRank[k]) { found = c; break; }
+      }
+      if (found < contacts.size() && taken[found]) throw std::runtime_error("sweep order repeats a contact");
       if (found == contacts.size()) throw std::runtime_error("sweep order does not match the contacts");
       taken[found] = true;
       sorted.push_back(contacts[found]);
@@ -1492,7 +1496,14 @@ int orc_step(orc_world* w, double dt, int nsteps) {
 void orc_set_sweep_order(orc_world* w, size_t n, const uint32_t* b1, const uint32_t* b2) {
   w->orderB1.assign(b1, b1 + n);
   w->orderB2.assign(b2, b2 + n);
+  w->orderRank.clear();
   w->orderPending = true;
+}
+
+void orc_set_sweep_order_ranked(orc_world* w, size_t n, const uint32_t* b1, const uint32_t* b2,
+                                const uint32_t* rank) {
+  orc_set_sweep_order(w, n, b1, b2);
+  w->orderRank.assign(rank, rank + n);
 }
 
 size_t orc_num_contacts(const orc_world* w) { return w->contacts.size(); }

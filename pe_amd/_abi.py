@@ -23,6 +23,11 @@ EXPORTED = [
     "pehip_set_contact_order", "pehip_step", "pehip_detect", "pehip_download_state", "pehip_get_contacts",
     "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_set_profiling", "pehip_eval_trig",
     "pehip_stream", "pehip_last_error", "pehip_version",
+    "pehip_batch_create", "pehip_batch_destroy", "pehip_batch_set_params", "pehip_batch_set_materials",
+    "pehip_batch_upload", "pehip_batch_set_scene_friction", "pehip_batch_upload_state", "pehip_batch_step",
+    "pehip_batch_download_state", "pehip_batch_scene_stats", "pehip_batch_get_contacts",
+    "pehip_batch_get_sweep_order", "pehip_batch_get_stats", "pehip_batch_get_tuning", "pehip_batch_set_tuning",
+    "pehip_batch_autotune", "pehip_batch_stream", "pehip_batch_last_error",
 ]
 
 
@@ -44,6 +49,19 @@ class Stats(ctypes.Structure):
                 ("t_integrate_ms", ctypes.c_double), ("prof_steps", ctypes.c_uint64),
                 ("prof_sweep_launches", ctypes.c_uint64), ("prof_sweep_kernel_ms", ctypes.c_double),
                 ("prof_sweep_bytes", ctypes.c_double)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("steps_per_launch", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("nscenes", ctypes.c_uint32), ("num_bodies", ctypes.c_uint64),
+                ("contact_capacity", ctypes.c_uint64), ("num_contacts", ctypes.c_uint64),
+                ("max_colours", ctypes.c_uint32), ("steps_done_min", ctypes.c_uint64),
+                ("capacity_retries", ctypes.c_uint32), ("launches", ctypes.c_uint32),
+                ("sweep_bytes", ctypes.c_double)]
 
 
 class PeHipError(RuntimeError):
@@ -95,6 +113,26 @@ def load():
         "pehip_stream": (_VP, [_VP]),
         "pehip_last_error": (ctypes.c_char_p, [_VP]),
         "pehip_version": (ctypes.c_char_p, []),
+        "pehip_batch_create": (st, [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_VP)]),
+        "pehip_batch_destroy": (st, [_VP]),
+        "pehip_batch_set_params": (st, [_VP, ctypes.POINTER(Params)]),
+        "pehip_batch_set_materials": (st, [_VP, ctypes.c_uint32, _D, _D]),
+        "pehip_batch_upload": (st, [_VP, ctypes.c_uint32, _U32, _U8, _U64, _U32, _U8, _D, _D, _D, _D, _D, _D, _D,
+                                    _D, _D, _D, ctypes.c_double]),
+        "pehip_batch_set_scene_friction": (st, [_VP, ctypes.c_uint32, _D, _D]),
+        "pehip_batch_upload_state": (st, [_VP, ctypes.c_uint64, _D, _D, _D, _D, _D, _D]),
+        "pehip_batch_step": (st, [_VP, ctypes.c_double, ctypes.c_uint32]),
+        "pehip_batch_download_state": (st, [_VP, ctypes.c_uint64, _D, _D, _D, _D, _D, _D]),
+        "pehip_batch_scene_stats": (st, [_VP, ctypes.c_uint32, _U32, _U32, _U32, _D, _U32]),
+        "pehip_batch_get_contacts": (st, [_VP, ctypes.c_uint32, _SZ, ctypes.POINTER(_SZ), _U32, _U32, _D, _D, _D,
+                                          _D]),
+        "pehip_batch_get_sweep_order": (st, [_VP, ctypes.c_uint32, _SZ, ctypes.POINTER(_SZ), _U32, _U32, _U32]),
+        "pehip_batch_get_stats": (st, [_VP, ctypes.POINTER(BatchStats)]),
+        "pehip_batch_get_tuning": (st, [_VP, ctypes.POINTER(BatchTuning)]),
+        "pehip_batch_set_tuning": (st, [_VP, ctypes.POINTER(BatchTuning)]),
+        "pehip_batch_autotune": (st, [_VP, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(BatchTuning)]),
+        "pehip_batch_stream": (_VP, [_VP]),
+        "pehip_batch_last_error": (ctypes.c_char_p, [_VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -107,6 +145,12 @@ def load():
 def check(status, ctx=None):
     if status != OK:
         msg = load().pehip_last_error(ctx).decode()
+        raise PeHipError(status, msg)
+
+
+def check_batch(status, b=None):
+    if status != OK:
+        msg = load().pehip_batch_last_error(b).decode()
         raise PeHipError(status, msg)
 
 

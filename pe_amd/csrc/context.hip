@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "../../include/pe_hip.h"
-#include "kernels.hpp"
+#include "batch.hpp"
 
 using namespace pe_amd;
 
@@ -218,6 +218,22 @@ static void plan_grid(pehip_ctx* ctx, const uint8_t* shape, const double* param,
   ctx->key_bits = bits;
 }
 
+pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
+  if (p->relaxation_model != PEHIP_INELASTIC_FRICTIONLESS &&
+      p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
+      p->relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING) {
+    why = "Unsupported relaxation model.";  // HCSITS.h:1926
+    return PEHIP_NOT_SUPPORTED;
+  }
+  if (!(p->erp >= 0 && p->erp <= 1)) { why = "Error reduction parameter out of range."; return PEHIP_INVALID_ARG; }
+  if (p->sweep_mode != PEHIP_SWEEP_COLOURED && p->sweep_mode != PEHIP_SWEEP_REFERENCE_ORDER) {
+    why = "Invalid sweep mode.";
+    return PEHIP_INVALID_ARG;
+  }
+  if (p->damping > 1 || p->damping < 0) { why = "Invalid damping."; return PEHIP_INVALID_ARG; }
+  return PEHIP_OK;
+}
+
 extern "C" {
 
 void pehip_default_params(pehip_params* p) {
@@ -233,21 +249,6 @@ void pehip_default_params(pehip_params* p) {
   p->exact_trig = 0;
 }
 
-static pehip_status check_params(const pehip_params* p, std::string& why) {
-  if (p->relaxation_model != PEHIP_INELASTIC_FRICTIONLESS &&
-      p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
-      p->relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING) {
-    why = "Unsupported relaxation model.";  // HCSITS.h:1926
-    return PEHIP_NOT_SUPPORTED;
-  }
-  if (!(p->erp >= 0 && p->erp <= 1)) { why = "Error reduction parameter out of range."; return PEHIP_INVALID_ARG; }
-  if (p->sweep_mode != PEHIP_SWEEP_COLOURED && p->sweep_mode != PEHIP_SWEEP_REFERENCE_ORDER) {
-    why = "Invalid sweep mode.";
-    return PEHIP_INVALID_ARG;
-  }
-  if (p->damping > 1 || p->damping < 0) { why = "Invalid damping."; return PEHIP_INVALID_ARG; }
-  return PEHIP_OK;
-}
 
 pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** out) {
   if (!out) { g_create_error = "out is NULL"; return PEHIP_INVALID_ARG; }

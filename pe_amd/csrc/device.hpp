@@ -450,18 +450,23 @@ __device__ void narrowphase(const Bodies& B, uint32_t a, uint32_t b, Emit& emit)
   }
 }
 
-// Contact friction as chosen by Contact::Contact (src/core/contact/Contact.cpp:83-105):
-// dynamic if the tangential relative speed exceeds frictionThreshold, else static.
-__device__ inline double contact_friction(const Bodies& B, const double* csf, const double* cdf,
-                                          uint32_t nmat, uint32_t b1, uint32_t b2, d3 gpos, d3 n) {
+// Friction choice of Contact::Contact (src/core/contact/Contact.cpp:83-105): true (dynamic)
+// if the tangential relative speed exceeds frictionThreshold, false (static) otherwise.
+__device__ inline bool contact_is_dynamic(const Bodies& B, uint32_t b1, uint32_t b2, d3 gpos, d3 n) {
   const d3 x1 = ld3(B.pos, b1), x2 = ld3(B.pos, b2);
   const d3 v1 = ld3(B.v, b1) + cross3(ld3(B.w, b1), gpos - x1);  // RigidBody::velFromWF, RigidBody.h:1080-1085
   const d3 v2 = ld3(B.v, b2) + cross3(ld3(B.w, b2), gpos - x2);
   const d3 rvel = v1 - v2;
   const double nvel = dot3(n, rvel);
   const double tvel = dot3(n, rvel - n * nvel);
+  return fabs(tvel) > kFrictionThreshold;
+}
+
+// Contact friction from the material pair tables (Materials.h:500-537).
+__device__ inline double contact_friction(const Bodies& B, const double* csf, const double* cdf,
+                                          uint32_t nmat, uint32_t b1, uint32_t b2, d3 gpos, d3 n) {
   const size_t idx = (size_t)B.mat[b1] * nmat + B.mat[b2];
-  return (fabs(tvel) > kFrictionThreshold) ? cdf[idx] : csf[idx];
+  return contact_is_dynamic(B, b1, b2, gpos, n) ? cdf[idx] : csf[idx];
 }
 
 // std::max(a, b) as the reference evaluates it: (a < b) ? b : a.

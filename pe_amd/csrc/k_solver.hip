@@ -1,5 +1,5 @@
 // Solver kernels: contact-graph colouring, contact caching and the NBGS sweeps.
-#include "kernels.hpp"
+#include "step_ops.hpp"
 
 namespace pe_amd {
 
@@ -127,86 +127,11 @@ void launch_colour_hist(uint32_t nc, ColourBuffers& C, hipStream_t s) {
   }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Contact caching, HCSITS.h:1741-1818: slot s of the solver arrays receives raw contact
-// perm[s] -- lever arms, contact frame, error-reduced distance, friction and the Delassus
-// block in the contact frame with its inverses.  Impulses start at zero (no warm start).
-// ---------------------------------------------------------------------------------------------
-// counters[0]: max penetration (bits of a non-negative double); counters[1]: contacts that
-// involve a fixed body.
+// Contact caching (HCSITS.h:1741-1818) in solver order: slot s <- raw contact perm[s].
 __global__ void k_setup(Bodies B, RawContacts R, const uint32_t* perm, uint32_t nc, SolverContacts C,
                         double erp, double dtinv, unsigned long long* counters) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nc) return;
-  unsigned long long* maxpen = counters;
-  const uint32_t c = perm[s];
-  const uint32_t i1 = R.b1[c], i2 = R.b2[c];
-  if (B.fixed[i1] || B.fixed[i2]) atomicAdd(&counters[1], 1ull);
-  const d3 pos = ld3(R.pos, c);
-  const d3 r1 = pos - ld3(B.pos, i1);
-  const d3 r2 = pos - ld3(B.pos, i2);
-  const d3 n = ld3(R.nrm, c);
-  d3 t;
-  if (fabs(n.x) < fabs(n.y)) {
-    if (fabs(n.x) < fabs(n.z)) t = mk3(0, n.z, -n.y);
-    else t = mk3(n.y, -n.x, 0);
-  } else {
-    if (fabs(n.y) < fabs(n.z)) t = mk3(-n.z, 0, n.x);
-    else t = mk3(n.y, -n.x, 0);
-  }
-  t = nrm3(t);
-  const d3 o = cross3(n, t);
-  const m33 cf = mat_cols(n, t, o);
-  double dist = R.dist[c];
-  if (dist < 0) {
-    atomicMax(maxpen, (unsigned long long)__double_as_longlong(-dist));
-    dist *= erp;
-  }
-  const double* s1 = B.sb + (size_t)kSB * i1;
-  const double* s2 = B.sb + (size_t)kSB * i2;
-  m33 I1, I2;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) { I1.a[k] = s1[SB_IW + k]; I2.a[k] = s2[SB_IW + k]; }
-  const double im1 = s1[SB_IM], im2 = s2[SB_IM];
-  const m33 A = mul_skew(skew_mul(r1, I1), r1);
-  const m33 Bm = mul_skew(skew_mul(r2, I2), r2);
-  m33 D;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) D.a[k] = -(A.a[k] + Bm.a[k]);
-  D.a[0] += im1 + im2;
-  D.a[4] += im1 + im2;
-  D.a[8] += im1 + im2;
-  D = mat_mul(mat_mul(mat_trans(cf), D), cf);
-  const m33 Di = mat_inv(D);
-  double det = D.a[4] * D.a[8] - D.a[5] * D.a[7];  // Matrix2x2::getInverse, Matrix2x2.h:977-988
-  det = 1.0 / det;
-  const size_t cap = C.cap;
-  double* f = C.f;
-  const d3 vals[5] = {r1, r2, n, t, o};
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    f[(3 * k) * cap + s] = vals[k].x;
-    f[(3 * k + 1) * cap + s] = vals[k].y;
-    f[(3 * k + 2) * cap + s] = vals[k].z;
-  }
-  f[CF_DDT * cap + s] = dist * dtinv;
-  f[CF_MU * cap + s] = R.mu[c];
-  f[CF_DNI * cap + s] = 1.0 / D.a[0];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) f[(CF_DINV + k) * cap + s] = Di.a[k];
-  f[(CF_DTOI + 0) * cap + s] = det * D.a[8];
-  f[(CF_DTOI + 1) * cap + s] = -det * D.a[5];
-  f[(CF_DTOI + 2) * cap + s] = -det * D.a[7];
-  f[(CF_DTOI + 3) * cap + s] = det * D.a[4];
-  f[(CF_DTO + 0) * cap + s] = D.a[4];
-  f[(CF_DTO + 1) * cap + s] = D.a[5];
-  f[(CF_DTO + 2) * cap + s] = D.a[7];
-  f[(CF_DTO + 3) * cap + s] = D.a[8];
-  f[CF_P * cap + s] = 0.0;
-  f[(CF_P + 1) * cap + s] = 0.0;
-  f[(CF_P + 2) * cap + s] = 0.0;
-  C.b1[s] = i1 | (B.fixed[i1] ? kFixedBit : 0u);
-  C.b2[s] = i2 | (B.fixed[i2] ? kFixedBit : 0u);
+  if (s < nc) setup_contact(B, R, perm[s], s, C, erp, dtinv, counters);
 }
 
 void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, uint32_t nc,

@@ -34,7 +34,7 @@ struct RawContacts {
 enum : uint32_t {
   ERR_NONFINITE = 1u,
   ERR_LEVEL_OVERFLOW = 2u,
-  ERR_RESERVED = 4u,
+  ERR_CAPACITY = 4u,
   ERR_COLOURS = 8u
 };
 

@@ -154,7 +154,7 @@ def test_contact_set_bitexact(name, steps):
     g = contact_map(w.detect())
     ex = contact_map(ow.detect(exhaustive=True))
     hg = contact_map(ow.detect(exhaustive=False))
-    assert len(ex) > 10
+    assert sum(len(v) for v in ex.values()) > 10
     assert g == ex, "GPU contact set differs from pe's ExhaustiveSearch"
     assert g == hg, "GPU contact set differs from pe's HashGrids"
 
@@ -215,6 +215,17 @@ def test_reference_order_step_matches_correctly_rounded_oracle(name):
     assert rel_err(w.impulses()[np.argsort(order_index(w.contacts(), c))], ow.impulses()) <= 1e-12
 
 
+def pair_ranks(c):
+    """Rank of each contact among the contacts of its body pair, in detection (narrowphase) order."""
+    seen = {}
+    r = np.zeros(len(c["b1"]), np.uint32)
+    for i, (a, b) in enumerate(zip(c["b1"], c["b2"])):
+        k = (int(a), int(b))
+        r[i] = seen.get(k, 0)
+        seen[k] = int(r[i]) + 1
+    return r
+
+
 def order_index(gc, oc):
     """Position of each GPU contact in the oracle's order (pairs matched in sequence)."""
     pos = {}
@@ -254,8 +265,10 @@ def test_reference_order_c1_window():
 def test_coloured_step_equals_gs_in_colour_order(name):
     # Contacts of one colour share no non-fixed body, so a coloured sweep is exactly a
     # sequential Gauss-Seidel sweep in colour-major order; replay that order on the oracle.
+    # Exact-trig mode is compared with the oracle in its correctly rounded trig mode: the
+    # host libm's rare non correctly rounded results would flip golden-section decisions.
     sc = SCENES[name]()
-    ow, st0 = oracle_state_after(sc, 10)
+    ow, st0 = oracle_state_after(sc, 10, cr_trig=True)
     w = gpu_for(sc, iters=30, exact_trig=True)
     w.setState(st0)
     w.simulationStep(sc.dt)
@@ -263,7 +276,7 @@ def test_coloured_step_equals_gs_in_colour_order(name):
     idx, offs = w.sweepOrder()
     gc = w.contacts()
     assert len(offs) >= 3 and offs[-1] == len(idx) == len(gc["b1"])
-    ow.set_sweep_order(gc["b1"][idx], gc["b2"][idx])
+    ow.set_sweep_order_ranked(gc["b1"][idx], gc["b2"][idx], pair_ranks(gc)[idx])
     ow.step(sc.dt)
     o1 = ow.get_state()
     for k in ("v", "w", "pos", "q"):
