@@ -1,10 +1,14 @@
 """Benchmark of the HCSITS time step on one MI355X.
 
 Workload (default): the polydisperse granular pile -- 200,000 spheres, radius U[0.25,0.5],
-jittered lattice in a 60x60x200 m plane box, v0 ~ U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps of
-InelasticCoulombContactByDecoupling, dt 5e-4, FP64.  A "step" is one full time step
-(detection, colouring, contact caching, 50 sweeps, integration) with all state resident in
-HBM.  Prints one JSON line.
+jittered lattice (spacing 1.05) in a 60x60x200 m plane box, v0 ~ U[-0.5,0.5]^3, mu 0.5, 50 NBGS
+sweeps of InelasticCoulombContactByDecoupling, dt 5e-4, FP64 -- run as the whole job: steps
+0..2000 from the initial state.  A "step" is one full time step (detection, colouring, contact
+caching, 50 sweeps, integration) with all state resident in HBM.  Prints one JSON line.
+
+Warmup steps run on a separate copy of the scene (same shapes) before the timed job.  The CPU
+baseline times the oracle (single-threaded restatement of pe's step) from the GPU states at the
+start, middle and end of the timed job and integrates the per-step times over the job.
 """
 import argparse
 import json
@@ -19,37 +23,63 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time steps/s and contact-iterations/s at 200k spheres FP64; batched scenes/s"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_c2_traffic.json")
 
 
-def cpu_baseline(sc, state, seconds, max_steps):
-    """The oracle (single-threaded CPU restatement of pe's step) on a bounded sample: steps
-    from the GPU state at the start of the timed region, until `seconds` have elapsed."""
+def cpu_step_time(sc, state, seconds, min_steps=1):
+    """Mean wall time of oracle steps from `state` (the first step, which also builds the hash
+    grids from scratch, is not timed); runs until `seconds` have elapsed (at least min_steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ow = O.OracleWorld(gravity=sc.gravity, max_iterations=sc.max_iterations, model=sc.model)
     sc.to_oracle(ow)
     ow.set_state(state)
-    ow.step(sc.dt, 1)          # first step also builds the hash grids from scratch; untimed
+    ow.step(sc.dt, 1)
     n = 0
     t0 = time.perf_counter()
-    while n < max_steps and time.perf_counter() - t0 < seconds:
+    while n < min_steps or time.perf_counter() - t0 < seconds:
         ow.step(sc.dt, 1)
         n += 1
-    t = time.perf_counter() - t0
-    return {"value": n / t, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the same workload from the GPU state at the start of the timed region "
-                      "(oracle/pe_oracle.cpp, g++ -O3, no FMA, 1 thread), %.1f s" % (n, t)}
+    return (time.perf_counter() - t0) / n, n
+
+
+def cpu_baseline(sc, states, steps, seconds):
+    """Oracle steps/s over the whole job: per-step times at the start, middle and end of the job
+    (from the GPU states there), integrated with the trapezoid rule over the job's steps."""
+    per = seconds / len(states)
+    ts, ns = [], []
+    for st in states:
+        t, n = cpu_step_time(sc, st, per)
+        ts.append(t)
+        ns.append(n)
+    total = steps * (ts[0] + 2 * ts[1] + ts[2]) / 4.0
+    return {"value": steps / total, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "oracle/pe_oracle.cpp (g++ -O3, no FMA, 1 thread) timed for %s steps from the GPU states at "
+                      "steps 0, %d and %d of the job (%.3f, %.3f, %.3f s/step), integrated over the %d-step job "
+                      "(trapezoid rule)" % ("/".join(str(n) for n in ns), steps // 2, steps, ts[0], ts[1], ts[2],
+                                           steps)}
+
+
+def save_state(path, st):
+    np.savez(path, **st)
+
+
+def load_state(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--spheres", type=int, default=200_000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
+    ap.add_argument("--load-state", default=None, help="start the timed steps from this saved state (.npz)")
+    ap.add_argument("--save-state", default=None, help="save the state after the timed steps (.npz)")
     args = ap.parse_args()
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world_size > 1:
@@ -61,55 +91,79 @@ def main():
     from pe_amd import World, scenes
 
     sc = scenes.config2(args.spheres)
-    w = World(device=0)
-    sc.to_world(w)
-    w.setMaxIterations(sc.max_iterations)
-    w.setRelaxationModel(sc.model)
+
+    def make_world():
+        w = World(device=0)
+        sc.to_world(w)
+        w.setMaxIterations(sc.max_iterations)
+        w.setRelaxationModel(sc.model)
+        return w
 
     if args.warmup:
-        w.run(args.warmup, sc.dt)
-    torch.cuda.synchronize()
+        ww = make_world()
+        ww.run(args.warmup, sc.dt)
+        torch.cuda.synchronize()
+        ww.close()
+    w = make_world()
+    if args.load_state:
+        w.setState(load_state(args.load_state))
     state0 = w.getState()
     if not args.no_profile:
         w.setProfiling(True)
-    contacts = []
-    colours = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        w.run(1, sc.dt)
-        s = w.stats()
-        contacts.append(s["num_contacts"])
-        colours.append(s["num_colours"])
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
+    contacts, colours, solve_ms = [], [], []
+    half = args.steps // 2
+    t = 0.0
+    state_mid = None
+    for seg in (half, args.steps - half):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(seg):
+            w.run(1, sc.dt)
+            s = w.stats()
+            contacts.append(s["num_contacts"])
+            colours.append(s["num_colours"])
+            solve_ms.append(s["t_solve_ms"])
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+        if state_mid is None:
+            state_mid = w.getState()           # between the two timed halves
     st = w.stats()
+    state_end = w.getState()
+    if args.save_state:
+        save_state(args.save_state, state_end)
 
     steps_per_s = args.steps / t
     ci = float(np.sum(contacts)) * sc.max_iterations
+    solve_s = float(np.sum(solve_ms)) * 1e-3
     out = {
         "metric": METRIC, "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "polydisperse granular pile, %d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, "
-                               "mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), dt 5e-4, "
-                               "steps %d..%d of the run" % (args.spheres, args.warmup, args.warmup + args.steps),
+                               "v0~U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), "
+                               "dt 5e-4, whole job: steps 0..%d%s" % (args.spheres, args.steps,
+                                                                      " from a saved state" if args.load_state else ""),
                    "bodies": int(w.size), "sweeps_per_step": sc.max_iterations, "dt": sc.dt,
                    "parallelism": "single GPU"},
-        "contact_iterations_per_s": ci / t,
-        "mean_contacts": float(np.mean(contacts)), "mean_colours": float(np.mean(colours)),
+        "contact_iterations_per_s": ci / solve_s if solve_s > 0 else None,
+        "mean_contacts": float(np.mean(contacts)), "max_contacts": int(np.max(contacts)),
+        "mean_colours": float(np.mean(colours)),
     }
     if not args.no_profile and st["prof_sweep_launches"]:
         avg_ms = st["prof_sweep_kernel_ms"] / st["prof_sweep_launches"]
         bytes_per_launch = st["prof_sweep_bytes"] / st["prof_sweep_launches"]
         achieved = st["prof_sweep_bytes"] / (st["prof_sweep_kernel_ms"] * 1e-3) / 1e9
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                            "kernel": "k_sweep<3> (one colour of one NBGS sweep)",
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
                            "launches": int(st["prof_sweep_launches"])}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, state0, args.cpu_seconds, args.steps)
+        out["cpu_baseline"] = cpu_baseline(sc, [state0, state_mid, state_end], args.steps, args.cpu_seconds)
     print(json.dumps(out))
 
 

@@ -183,7 +183,8 @@ pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);
 pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable);
 
 /* Evaluates the device trig of the solver on n arguments (checks against a host libm):
- * fn 0 = sin(x), 1 = cos(x), 2 = atan2(y, x), 3 = acos(x); exact selects the correctly
+ * fn 0 = sin(x), 1 = cos(x), 2 = atan2(y, x), 3 = acos(x), 4 / 5 = sin(x) / cos(x) as the
+ * golden-section search evaluates them (one sincos); exact selects the correctly
  * rounded implementation (see pehip_params.exact_trig).  y may be NULL unless fn = 2. */
 pehip_status pehip_eval_trig(pehip_ctx* ctx, int fn, int exact, size_t n, const double* x,
                              const double* y, double* out);

@@ -18,10 +18,9 @@ class BatchWorld:
     def __init__(self, scenes, device=0, contacts_per_body=8.0, scene_friction=None, max_iterations=None,
                  relaxation_model=None, abort_threshold=0.0, exact_trig=False):
         """scenes: list of pe_amd.scenes.Scene.  All scenes share the solver settings of the first
-        one (gravity, sweeps, relaxation model).  Materials: the union of the scenes' material
-        lists is not formed; scene k's materials are appended to one shared table and its bodies
-        re-indexed into it.  scene_friction: optional (static_mu, dynamic_mu) arrays, one value
-        per scene, replacing the material tables."""
+        one (gravity, sweeps, relaxation model).  The scenes' materials go into one shared pair
+        table (identical materials shared).  scene_friction: optional (static_mu, dynamic_mu)
+        arrays, one value per scene, replacing the pair tables' friction."""
         L = _abi.load()
         self.scenes = scenes
         first = scenes[0]
@@ -37,27 +36,38 @@ class BatchWorld:
         check_batch(L.pehip_batch_create(ctypes.byref(p), device, ctypes.byref(h)), None)
         self._b = h
         self.dt = first.dt
-        # shared material table
-        mats, offs = [], []
+        # shared material table: identical materials are shared between scenes; with per-scene
+        # friction only the densities matter
+        mats, index, maps = [], {}, []
+        dedup = not any(sc.pair_friction for sc in scenes)
         for sc in scenes:
-            offs.append(len(mats))
-            mats.extend(sc.materials)
+            local = []
+            for m in sc.materials:
+                key = (m["density"],) if scene_friction is not None else tuple(sorted(m.items()))
+                if not dedup or key not in index:
+                    index[key] = len(mats)
+                    mats.append(m)
+                local.append(index[key])
+            maps.append(np.array(local, np.uint32))
         nm = len(mats)
+        if nm > 4096:
+            raise ValueError("more than 4096 distinct materials; use scene_friction for per-scene friction")
         csf = np.zeros((nm, nm))
         cdf = np.zeros((nm, nm))
-        for i in range(nm):
-            for j in range(nm):
-                csf[i, j] = mats[i]["csf"] + mats[j]["csf"]   # Materials.cpp:237-243
-                cdf[i, j] = mats[i]["cdf"] + mats[j]["cdf"]
-        for sc, o in zip(scenes, offs):
-            for (m1, m2, s, d) in sc.pair_friction:
-                csf[o + m1, o + m2] = csf[o + m2, o + m1] = s
-                cdf[o + m1, o + m2] = cdf[o + m2, o + m1] = d
+        if scene_friction is None:
+            for i in range(nm):
+                for j in range(nm):
+                    csf[i, j] = mats[i]["csf"] + mats[j]["csf"]   # Materials.cpp:237-243
+                    cdf[i, j] = mats[i]["cdf"] + mats[j]["cdf"]
+            for sc, mp in zip(scenes, maps):
+                for (m1, m2, s_, d_) in sc.pair_friction:
+                    csf[mp[m1], mp[m2]] = csf[mp[m2], mp[m1]] = s_
+                    cdf[mp[m1], mp[m2]] = cdf[mp[m2], mp[m1]] = d_
         check_batch(L.pehip_batch_set_materials(self._b, nm, ptr(csf), ptr(cdf)), self._b)
         arrs = []
-        for sc, o in zip(scenes, offs):
+        for sc, mp in zip(scenes, maps):
             a = sc.arrays()
-            a["mat"] = a["mat"] + np.uint32(o)
+            a["mat"] = mp[a["mat"]]
             arrs.append(a)
         self.offsets = np.zeros(len(scenes) + 1, np.uint32)
         self.offsets[1:] = np.cumsum([len(a["shape"]) for a in arrs])

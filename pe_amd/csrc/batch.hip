@@ -261,6 +261,7 @@ static pehip_status batch_run(pehip_batch* b, double dt, uint32_t nsteps) {
   if (flags & ERR_NONFINITE) return bfail(b, PEHIP_NONFINITE, "non-finite body position or bounding box");
   if (flags & ERR_LEVEL_OVERFLOW) return bfail(b, PEHIP_INVALID_ARG, "body larger than its scene's grid cell");
   if (flags & ERR_COLOURS) return bfail(b, PEHIP_NOT_SUPPORTED, "more than 128 colours needed");
+  if (flags & ERR_INTERNAL) return bfail(b, PEHIP_HIP_ERROR, "internal error: inconsistent sweep order");
   return PEHIP_OK;
 }
 

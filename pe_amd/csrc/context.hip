@@ -812,7 +812,7 @@ pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable) {
 pehip_status pehip_eval_trig(pehip_ctx* ctx, int fn, int exact, size_t n, const double* x, const double* y,
                              double* out) {
   if (!ctx) return PEHIP_INVALID_ARG;
-  if (fn < 0 || fn > 3 || !x || !out || (fn == 2 && !y)) return fail(ctx, PEHIP_INVALID_ARG, "invalid trig request");
+  if (fn < 0 || fn > 5 || !x || !out || (fn == 2 && !y)) return fail(ctx, PEHIP_INVALID_ARG, "invalid trig request");
   if (!n) return PEHIP_OK;
   hipSetDevice(ctx->device);
   double *dx = nullptr, *dy = nullptr, *dout = nullptr;

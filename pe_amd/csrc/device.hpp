@@ -497,13 +497,91 @@ __device__ __forceinline__ d3 cf_ld3(const double* f, size_t cap, int fld, size_
   return mk3(f[fld * cap + i], f[(fld + 1) * cap + i], f[(fld + 2) * cap + i]);
 }
 
+// Golden-section search of relaxInelasticCoulombContactsByDecoupling (HCSITS.h:2429-2509)
+// over the angle of the tangential impulse on the circle of radius flimit.
+struct GoldenState {
+  double al, ar, am, fm, xm0, xm1;
+  bool left;  // "leftlarger"
+};
+
+__device__ __forceinline__ double golden_objective(double x0, double x1, double A00, double A01, double A10,
+                                                   double A11, double gt, double go) {
+  return 0.5 * x0 * (A00 * x0 + A01 * x1) + 0.5 * x1 * (A10 * x0 + A11 * x1) - (x0 * (-gt) + x1 * (-go));
+}
+
+// One iteration's update of the bracket given the new point an (with its impulse and objective)
+// and the comparison fn < fm.
+__device__ __forceinline__ void golden_step(GoldenState& S, double an, double x0, double x1, double fn, bool lt) {
+  if (S.left) {
+    if (lt) { S.ar = S.am; S.am = an; S.xm0 = x0; S.xm1 = x1; S.fm = fn; S.left = true; }
+    else { S.al = an; S.left = false; }
+  } else {
+    if (lt) { S.al = S.am; S.am = an; S.xm0 = x0; S.xm1 = x1; S.fm = fn; S.left = false; }
+    else { S.ar = an; S.left = true; }
+  }
+}
+
+// maxSub iterations.  SPEC = 0: one lane, as pe.  SPEC = d > 0: the 2^d lanes of a group run d
+// iterations per round -- lane j evaluates node j of the depth-d tree of possible next points
+// (heap order; the path to a node, read from the bits of j + 1 below the leading one, is a
+// sequence of comparison outcomes, and the bracket along it depends only on the points, not on
+// the objective values), then the group follows the actual comparisons through the tree.  Every
+// evaluated point and every comparison is the one the sequential search makes, bit for bit.
+template <class Trig, int SPEC>
+__device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, double flimit, double A00, double A01,
+                                               double A10, double A11, double gt, double go) {
+  if constexpr (SPEC == 0) {
+    for (int k = 0; k < maxSub; ++k) {
+      const double an = S.al + (S.ar - S.am);
+      double sn, cn;
+      Trig::sincos(an, sn, cn);
+      const double xn0 = flimit * cn, xn1 = flimit * sn;
+      const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
+      golden_step(S, an, xn0, xn1, fn, fn < S.fm);
+    }
+  } else {
+    constexpr int L = 1 << SPEC;
+    const uint32_t code = (threadIdx.x & (L - 1)) + 1;
+    const int depth = 31 - __clz(code);
+    for (int k = 0; k < maxSub; k += SPEC) {
+      const int levels = maxSub - k < SPEC ? maxSub - k : SPEC;
+      double al = S.al, ar = S.ar, am = S.am;
+      bool left = S.left;
+      for (int d = depth - 1; d >= 0; --d) {
+        const bool lt = ((code >> d) & 1u) != 0;
+        const double a = al + (ar - am);
+        if (left) {
+          if (lt) { ar = am; am = a; }
+          else { al = a; left = false; }
+        } else {
+          if (lt) { al = am; am = a; }
+          else { ar = a; left = true; }
+        }
+      }
+      const double an = al + (ar - am);
+      double sn, cn;
+      Trig::sincos(an, sn, cn);
+      const double xn0 = flimit * cn, xn1 = flimit * sn;
+      const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
+      int node = 0;
+      for (int d = 0; d < levels; ++d) {
+        const double a = __shfl(an, node, L), x0 = __shfl(xn0, node, L), x1 = __shfl(xn1, node, L);
+        const double f = __shfl(fn, node, L);
+        const bool lt = f < S.fm;
+        golden_step(S, a, x0, x1, f, lt);
+        node = 2 * node + (lt ? 2 : 1);
+      }
+    }
+  }
+}
+
 // One contact's NBGS update: HCSITS.h:2341-2536 (InelasticCoulombContactByDecoupling),
 // 2141-2290 (ApproximateInelasticCoulombContactByDecoupling), 2048-2093
 // (InelasticFrictionlessContact).  Returns |p_old - p_new|_inf (or |p_old|_inf for a
 // separating contact).  The caller guarantees that no other thread touches the
 // non-fixed bodies of contact i concurrently.  Trig = TrigFast (device libm) or TrigExact
 // (correctly rounded, reproduces the host libm's golden-section decisions).
-template <int MODEL, class Trig>
+template <int MODEL, class Trig, int SPEC = 0>
 __device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
                                        double* __restrict__ dvw, double dtinv, int maxSub) {
   const size_t cap = C.cap;
@@ -611,28 +689,19 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
               const double phi = 0.5 * (1.0 + sqrt(5.0));
               const double shift = Trig::atan2(-pc[2], pc[1]);
               const double acos_f = Trig::acos(fr);
-              double al = -acos_f - shift;
-              double ar = acos_f - shift;
-              double am = (ar + al * phi) / (1 + phi);
-              double xm0 = flimit * Trig::cos(am), xm1 = flimit * Trig::sin(am);
-              double fm = 0.5 * xm0 * (A00 * xm0 + A01 * xm1) + 0.5 * xm1 * (A10 * xm0 + A11 * xm1) -
-                          (xm0 * (-gt) + xm1 * (-go));
-              bool leftlarger = false;
-              for (int k = 0; k < maxSub; ++k) {
-                const double an = al + (ar - am);
-                const double xn0 = flimit * Trig::cos(an), xn1 = flimit * Trig::sin(an);
-                const double fn = 0.5 * xn0 * (A00 * xn0 + A01 * xn1) + 0.5 * xn1 * (A10 * xn0 + A11 * xn1) -
-                                  (xn0 * (-gt) + xn1 * (-go));
-                if (leftlarger) {
-                  if (fn < fm) { ar = am; am = an; xm0 = xn0; xm1 = xn1; fm = fn; leftlarger = true; }
-                  else { al = an; leftlarger = false; }
-                } else {
-                  if (fn < fm) { al = am; am = an; xm0 = xn0; xm1 = xn1; fm = fn; leftlarger = false; }
-                  else { ar = an; leftlarger = true; }
-                }
-              }
-              pc[1] = xm0;
-              pc[2] = xm1;
+              GoldenState gs;
+              gs.al = -acos_f - shift;
+              gs.ar = acos_f - shift;
+              gs.am = (gs.ar + gs.al * phi) / (1 + phi);
+              double sm, cm;
+              Trig::sincos(gs.am, sm, cm);
+              gs.xm0 = flimit * cm;
+              gs.xm1 = flimit * sm;
+              gs.fm = golden_objective(gs.xm0, gs.xm1, A00, A01, A10, A11, gt, go);
+              gs.left = false;
+              golden_section<Trig, SPEC>(gs, maxSub, flimit, A00, A01, A10, A11, gt, go);
+              pc[1] = gs.xm0;
+              pc[2] = gs.xm1;
             }
           }
         }
@@ -648,11 +717,13 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
     dv2 = dv2 - im2 * p;
     dw2 = dw2 - mat_vec(I2, cross3(r2, p));
   }
-  f[CF_P * cap + i] = p.x;
-  f[(CF_P + 1) * cap + i] = p.y;
-  f[(CF_P + 2) * cap + i] = p.z;
-  if (!fx1) { q1[0] = dv1.x; q1[1] = dv1.y; q1[2] = dv1.z; q1[3] = dw1.x; q1[4] = dw1.y; q1[5] = dw1.z; }
-  if (!fx2) { q2[0] = dv2.x; q2[1] = dv2.y; q2[2] = dv2.z; q2[3] = dw2.x; q2[4] = dw2.y; q2[5] = dw2.z; }
+  if (SPEC == 0 || (threadIdx.x & ((1u << SPEC) - 1)) == 0) {  // one lane of the contact's group stores
+    f[CF_P * cap + i] = p.x;
+    f[(CF_P + 1) * cap + i] = p.y;
+    f[(CF_P + 2) * cap + i] = p.z;
+    if (!fx1) { q1[0] = dv1.x; q1[1] = dv1.y; q1[2] = dv1.z; q1[3] = dw1.x; q1[4] = dw1.y; q1[5] = dw1.z; }
+    if (!fx2) { q2[0] = dv2.x; q2[1] = dv2.y; q2[2] = dv2.z; q2[3] = dw2.x; q2[4] = dw2.y; q2[5] = dw2.z; }
+  }
   return delta;
 }
 

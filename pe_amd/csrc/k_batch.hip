@@ -338,9 +338,18 @@ __device__ bool scene_step(const Bodies& B, const RawContacts& R, const SolverCo
   if (tid == 0)
     for (uint32_t k = 0; k <= ncol; ++k) A.colour_off[(size_t)sc * (kMaxColours + 1) + k] = coff[k];
 
-  // ---- contact caching in sweep order
-  for (uint32_t s = tid; s < nc; s += T) setup_contact(B, R, A.perm[c0 + s], c0 + s, C, A.erp, A.dtinv, O.counters);
+  // ---- contact caching in sweep order (a slot whose contact index is out of range -- an
+  // internal error -- stops the scene before any sweep reads it)
+  for (uint32_t s = tid; s < nc; s += T) {
+    const uint32_t c = A.perm[c0 + s];
+    if (c - c0 >= nc) { atomicOr(lflag, ERR_INTERNAL); continue; }
+    setup_contact(B, R, c, c0 + s, C, A.erp, A.dtinv, O.counters);
+  }
   __syncthreads();
+  if (*lflag) {
+    if (tid == 0) { O.flags |= *lflag; atomicOr(A.any_flags, *lflag); }
+    return false;
+  }
 
   // ---- sweeps
   uint32_t iters = 0;

@@ -59,7 +59,7 @@ void launch_integrate(const Bodies& B, double dt, double damping, bool exact_tri
 }
 
 // Device evaluation of one trig function over an array (checks of the device trig against
-// the host libm): fn 0 sin, 1 cos, 2 atan2(y, x), 3 acos.
+// the host libm): fn 0 sin, 1 cos, 2 atan2(y, x), 3 acos, 4 / 5 sin / cos from sincos.
 template <class Trig>
 __global__ void k_eval_trig(int fn, size_t n, const double* x, const double* y, double* out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,7 +69,12 @@ __global__ void k_eval_trig(int fn, size_t n, const double* x, const double* y, 
     case 0: r = Trig::sin(x[i]); break;
     case 1: r = Trig::cos(x[i]); break;
     case 2: r = Trig::atan2(y[i], x[i]); break;
-    default: r = Trig::acos(x[i]); break;
+    case 3: r = Trig::acos(x[i]); break;
+    default: {
+      double sn, cs;
+      Trig::sincos(x[i], sn, cs);
+      r = fn == 4 ? sn : cs;
+    }
   }
   out[i] = r;
 }

@@ -153,41 +153,53 @@ __device__ __forceinline__ void wave_max_atomic(double v, unsigned long long* ou
   if ((threadIdx.x & 63) == 0 && u) atomicMax(out, u);
 }
 
-// One colour: every contact in [begin, end) is independent of the others.
-template <int MODEL, class Trig>
+// One colour: every contact in [begin, end) is independent of the others.  A group of 2^SPEC
+// lanes relaxes one contact (the golden-section search runs SPEC iterations per round on it).
+template <int MODEL, class Trig, int SPEC>
 __global__ void __launch_bounds__(256) k_sweep(SolverContacts C, uint32_t begin, uint32_t end, const double* sb,
                                                double* dvw, double dtinv, int maxSub, unsigned long long* delta) {
-  const uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = begin + ((blockIdx.x * blockDim.x + threadIdx.x) >> SPEC);
   double d = 0;
-  if (i < end) d = relax_contact<MODEL, Trig>(C, i, sb, dvw, dtinv, maxSub);
+  if (i < end) d = relax_contact<MODEL, Trig, SPEC>(C, i, sb, dvw, dtinv, maxSub);
   wave_max_atomic(d, delta);
 }
 
-// All sweeps in the given contact order, one contact at a time (lane 0 of one wavefront).
-template <int MODEL, class Trig>
-__global__ void k_sweep_serial(SolverContacts C, uint32_t nc, uint32_t iters, const double* sb, double* dvw,
+// All sweeps in the given contact order, one contact at a time, by one group of 2^SPEC lanes.
+template <int MODEL, class Trig, int SPEC>
+__global__ void __launch_bounds__(64) k_sweep_serial(SolverContacts C, uint32_t nc, uint32_t iters, const double* sb, double* dvw,
                                double dtinv, int maxSub, double thr, unsigned long long* deltas) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x >= (1u << SPEC)) return;
   uint32_t it = 0;
   while (it < iters) {
     double dmax = 0;
-    for (uint32_t i = 0; i < nc; ++i) dmax = smax(dmax, relax_contact<MODEL, Trig>(C, i, sb, dvw, dtinv, maxSub));
-    deltas[it] = (unsigned long long)__double_as_longlong(dmax);
+    for (uint32_t i = 0; i < nc; ++i) dmax = smax(dmax, relax_contact<MODEL, Trig, SPEC>(C, i, sb, dvw, dtinv, maxSub));
+    if (threadIdx.x == 0) deltas[it] = (unsigned long long)__double_as_longlong(dmax);
     ++it;
     if (thr > 0 && dmax < thr) break;
   }
-  deltas[iters] = it;
+  if (threadIdx.x == 0) deltas[iters] = it;
 }
+
+// Lanes per contact: 16 while the colour fills at most kSpecLanes lanes that way, then 4, then 1.
+constexpr uint64_t kSpecLanes = 131072;
 
 template <class Trig>
 static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
                            double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
-  const uint32_t n = end - begin;
-  const dim3 g((n + 255) / 256), b(256);
+  const uint64_t n = end - begin;
+  const dim3 b(256);
+  auto grid = [&](int spec) { return dim3((unsigned)(((n << spec) + 255) / 256)); };
   switch (model) {
-    case 0: hipLaunchKernelGGL((k_sweep<0, Trig>), g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
-    case 1: hipLaunchKernelGGL((k_sweep<1, Trig>), g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
-    default: hipLaunchKernelGGL((k_sweep<3, Trig>), g, b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    default:
+      if ((n << 4) <= kSpecLanes)
+        hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+      else if ((n << 2) <= kSpecLanes)
+        hipLaunchKernelGGL((k_sweep<3, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+      else
+        hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+      break;
   }
 }
 
@@ -203,9 +215,9 @@ static void serial_dispatch(int model, const SolverContacts& C, uint32_t nc, uin
                             double dtinv, int maxSub, double thr, unsigned long long* deltas, hipStream_t s) {
   const dim3 g(1), b(64);
   switch (model) {
-    case 0: hipLaunchKernelGGL((k_sweep_serial<0, Trig>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
-    case 1: hipLaunchKernelGGL((k_sweep_serial<1, Trig>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
-    default: hipLaunchKernelGGL((k_sweep_serial<3, Trig>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 0: hipLaunchKernelGGL((k_sweep_serial<0, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 1: hipLaunchKernelGGL((k_sweep_serial<1, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    default: hipLaunchKernelGGL((k_sweep_serial<3, Trig, 5>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
 }
 

@@ -35,7 +35,8 @@ enum : uint32_t {
   ERR_NONFINITE = 1u,
   ERR_LEVEL_OVERFLOW = 2u,
   ERR_CAPACITY = 4u,
-  ERR_COLOURS = 8u
+  ERR_COLOURS = 8u,
+  ERR_INTERNAL = 16u
 };
 
 struct BroadphaseBuffers {

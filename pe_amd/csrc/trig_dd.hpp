@@ -117,6 +117,13 @@ struct TrigExact {
     dd::sincos_dd(x, s, c);
     return dd::round_dd(c);
   }
+  // sin(x) and cos(x) from one reduction and series: the same values as sin() and cos().
+  static PE_HD void sincos(double x, double& sn, double& cs) {
+    dd::ddv s, c;
+    dd::sincos_dd(x, s, c);
+    sn = x == 0.0 ? x : dd::round_dd(s);
+    cs = dd::round_dd(c);
+  }
   // atan2: device value refined by one Newton step on y cos(a) - x sin(a) = 0.
   static PE_HD double atan2(double y, double x) {
     const double a0 = ::atan2(y, x);
@@ -149,6 +156,7 @@ struct TrigExact {
 struct TrigFast {
   static PE_HD double sin(double x) { return ::sin(x); }
   static PE_HD double cos(double x) { return ::cos(x); }
+  static PE_HD void sincos(double x, double& sn, double& cs) { ::sincos(x, &sn, &cs); }
   static PE_HD double atan2(double y, double x) { return ::atan2(y, x); }
   static PE_HD double acos(double f) { return ::acos(f); }
 };

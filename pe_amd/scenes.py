@@ -195,3 +195,74 @@ def drop_scene(nx, ny, nz, r=0.5, spacing=1.2, jitter=0.05, box=(10.0, 10.0, 40.
                  [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": mu / 2, "cdf": mu / 2}],
                  box_planes(*box), pos, np.full(n, r), np.zeros(n, np.uint32), v0=v0, dt=dt,
                  max_iterations=max_iterations)
+
+
+def shoemake_quaternions(rng, n):
+    """Uniform random unit quaternions (w, x, y, z), Shoemake's method."""
+    u1, u2, u3 = rng.uniform(0.0, 1.0, size=(3, n))
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    return np.stack([b * np.cos(2 * np.pi * u3), a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                     b * np.sin(2 * np.pi * u3)], axis=1)
+
+
+def config3(n_spheres=950_000, n_boxes=50_000):
+    """Mixed shapes in a V-channel hopper: spheres r ~ U[0.2,0.4] and boxes with edges
+    U[0.3,0.8]^3 in Shoemake-uniform orientations, rained from a jittered lattice (spacing 1.2,
+    boxes at evenly spread sites, bottom layer at z = 33) above two planes inclined at +-30 deg whose lines meet 0.4 m
+    either side of the channel axis below the floor level (a 0.8 m gap at z = 0), plus the floor
+    and four side walls; mu = 0.6, 30 sweeps, dt = 2e-4, 1,000 steps.  pe planes are half-spaces,
+    so the outlet is the V-channel's 0.8 m gap at the floor rather than an opening."""
+    rng = np.random.default_rng(12345 + 3)
+    n = n_spheres + n_boxes
+    L = 120.0
+    per = 90
+    nz = (n + per * per - 1) // (per * per)
+    pos = lattice(per, per, nz, 1.2, (6.0, 6.0, 33.0), 0.05, rng, count=n)
+    is_box = np.zeros(n, bool)
+    is_box[np.linspace(0, n - 1, n_boxes).astype(int)] = True
+    c30, s30 = math.cos(math.radians(30.0)), math.sin(math.radians(30.0))
+    # inward normals of the two inclined planes: the channel axis is x = L/2, gap 0.8 at z = 0
+    cx = L / 2
+    n_left = (s30, 0.0, c30)        # plane through (cx - 0.4, 0) rising to the left
+    n_right = (-s30, 0.0, c30)
+    planes = [((0, 0, 1), 0.0, 0),
+              (n_left, s30 * (cx - 0.4), 0), (n_right, -s30 * (cx + 0.4), 0),
+              ((0, 1, 0), 0.0, 0), ((0, -1, 0), -L, 0), ((1, 0, 0), 0.0, 0), ((-1, 0, 0), -L, 0)]
+    sph = pos[~is_box]
+    box = pos[is_box]
+    return Scene("C3 %d spheres + %d boxes" % (n_spheres, n_boxes),
+                 [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.3, "cdf": 0.3}],
+                 planes, sph, rng.uniform(0.2, 0.4, size=len(sph)), np.zeros(len(sph), np.uint32),
+                 box_pos=box, box_len=rng.uniform(0.3, 0.8, size=(len(box), 3)),
+                 box_q=shoemake_quaternions(rng, len(box)), box_mat=np.zeros(len(box), np.uint32),
+                 dt=2e-4, steps=1000, max_iterations=30)
+
+
+def config4(n_scenes=4096):
+    """Batched scenes: scene sizes round(exp(U[ln 50, ln 2000])), r = 0.5 spheres on a jittered
+    8x8-per-layer lattice (spacing 1.1, jitter 0.05) in a 10x10x40 plane box each, per-scene
+    friction mu ~ U[0.2, 0.8] (material csf = cdf = mu/2, so the pair friction is mu), 40 sweeps,
+    dt = 1e-3, 300 steps.  Returns (scenes, mu)."""
+    rng = np.random.default_rng(12345 + 4)
+    sizes = np.rint(np.exp(rng.uniform(math.log(50.0), math.log(2000.0), size=n_scenes))).astype(int)
+    mus = rng.uniform(0.2, 0.8, size=n_scenes)
+    out = []
+    for k in range(n_scenes):
+        ns = int(sizes[k])
+        pos = lattice(8, 8, (ns + 63) // 64, 1.1, (1.15, 1.15, 0.6), 0.05, rng, count=ns)
+        out.append(Scene("C4 scene %d" % k,
+                         [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": mus[k] / 2, "cdf": mus[k] / 2}],
+                         box_planes(10.0, 10.0, 40.0), pos, np.full(ns, 0.5), np.zeros(ns, np.uint32),
+                         dt=1e-3, steps=300, max_iterations=40))
+    return out, mus
+
+
+def config5(n=37):
+    """Differentiable step: n^3 spheres r = 0.4 dropped as a jittered block (spacing 0.82, jitter
+    0.005, bottom layer 0.1 m above the floor) onto a floor plane, mu = 0.5, dt = 5e-4, 400 steps."""
+    rng = np.random.default_rng(12345 + 5)
+    pos = lattice(n, n, n, 0.82, (0.41, 0.41, 0.5), 0.005, rng)
+    ns = len(pos)
+    return Scene("C5 %d spheres" % ns, [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.25, "cdf": 0.25}],
+                 [((0, 0, 1), 0.0, 0)], pos, np.full(ns, 0.4), np.zeros(ns, np.uint32),
+                 dt=5e-4, steps=400, max_iterations=100)

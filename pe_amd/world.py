@@ -457,7 +457,8 @@ class World:
         check(_abi.load().pehip_set_profiling(self._ctx, 1 if on else 0), self._ctx)
 
     def evalTrig(self, fn, x, y=None, exact=True):
-        """Device sin (0), cos (1), atan2(y, x) (2) or acos (3) over an array."""
+        """Device sin (0), cos (1), atan2(y, x) (2), acos (3), or sin (4) / cos (5) from one
+        sincos, over an array."""
         self._ensure_device()
         xa = np.ascontiguousarray(x, float)
         ya = np.ascontiguousarray(y, float) if y is not None else None

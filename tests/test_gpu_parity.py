@@ -395,6 +395,18 @@ def test_device_trig_matches_host_libm():
     assert w.evalTrig(0, np.array([math.pi / 6]))[0] == math.sin(math.pi / 6)
 
 
+@pytest.mark.parametrize("exact", [True, False])
+def test_sincos_equals_sin_and_cos(exact):
+    # The golden-section search takes sin and cos of one angle from a single sincos; it must
+    # return exactly what separate sin and cos calls return.
+    w = gpu_for(dense_scene())
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-8.0, 8.0, 200_000), rng.uniform(-1e-3, 1e-3, 1000), [0.0, -0.0, 1e-300]])
+    assert np.array_equal(w.evalTrig(4, x, exact=exact), w.evalTrig(0, x, exact=exact))
+    assert np.array_equal(w.evalTrig(5, x, exact=exact), w.evalTrig(1, x, exact=exact))
+    assert np.array_equal(np.signbit(w.evalTrig(4, x, exact=exact)), np.signbit(w.evalTrig(0, x, exact=exact)))
+
+
 # --------------------------------------------------------------------------- closed forms / errors
 def test_free_fall_on_gpu():
     g, dt, n = 9.81, 1e-3, 40
