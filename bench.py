@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
     ap.add_argument("--load-state", default=None, help="start the timed steps from this saved state (.npz)")
     ap.add_argument("--save-state", default=None, help="save the state after the timed steps (.npz)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="before the run, 4 launches of a plain elementwise kernel that reads and writes 256 MiB "
+                         "(scale reference for hardware byte counters)")
     args = ap.parse_args()
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world_size > 1:
@@ -91,6 +94,13 @@ def main():
     from pe_amd import World, scenes
 
     sc = scenes.config2(args.spheres)
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     def make_world():
         w = World(device=0)

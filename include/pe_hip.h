@@ -200,6 +200,38 @@ const char* pehip_last_error(const pehip_ctx* ctx);
 const char* pehip_version(void);
 
 /* ------------------------------------------------------------------------------------------
+ * Differentiable time step (no counterpart in pe).  A forward run keeps checkpoints of the
+ * state; the reverse pass re-runs each segment between checkpoints and differentiates every
+ * step through its converged contact impulses at fixed active set (each contact keeps its mode:
+ * separating, static or sliding), solving the transposed linearised fixed-point system with
+ * Gauss-Seidel sweeps in the forward's colouring.  Supported: spheres and planes, relaxation
+ * models 0 and 1, coloured mode, no damping.  Derivatives are of the converged solve, so the
+ * forward should run to a tight abort_threshold with enough max_iterations.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+  uint64_t steps;          /* steps differentiated */
+  uint64_t total_sweeps;   /* adjoint sweeps summed over the steps */
+  uint64_t max_contacts;   /* largest contact count of a step */
+  double max_last_delta;   /* largest final L-inf change of an adjoint sweep */
+} pehip_adjoint_stats;
+
+/* World::run(nsteps, dt) as pehip_step, keeping a checkpoint of the state every
+ * checkpoint_every steps (the first at the start). */
+pehip_status pehip_adjoint_forward(pehip_ctx* ctx, double dt, uint32_t nsteps, uint32_t checkpoint_every);
+
+/* Reverse pass of the last pehip_adjoint_forward.  Inputs: dL/d(final positions, linear and
+ * angular velocities), 3n doubles each (NULL = 0).  Outputs (any may be NULL): dL/d(initial
+ * positions, velocities, angular velocities), and dL/dmu = the sum over all contacts of all steps
+ * of dL/d(contact friction) -- the derivative with respect to the friction coefficient when every
+ * contact's friction is that one coefficient.  adjoint_sweeps bounds the sweeps per step;
+ * adjoint_tol > 0 stops a step's sweeps when the L-inf change of the adjoint impulses drops below
+ * it.  The context is left at the forward run's final state. */
+pehip_status pehip_adjoint_backward(pehip_ctx* ctx, const double* dL_dpos, const double* dL_dv,
+                                    const double* dL_dw, double* dL_dpos0, double* dL_dv0,
+                                    double* dL_dw0, double* dL_dmu, uint32_t adjoint_sweeps,
+                                    double adjoint_tol, pehip_adjoint_stats* stats);
+
+/* ------------------------------------------------------------------------------------------
  * Batched scenes.  Many independent pe worlds (scenes) of uneven body counts advance together:
  * one workgroup runs one scene's whole time step (detection on a per-scene hash grid in LDS,
  * colouring, contact caching, the NBGS sweeps with workgroup barriers between colours,

@@ -28,6 +28,7 @@ EXPORTED = [
     "pehip_batch_download_state", "pehip_batch_scene_stats", "pehip_batch_get_contacts",
     "pehip_batch_get_sweep_order", "pehip_batch_get_stats", "pehip_batch_get_tuning", "pehip_batch_set_tuning",
     "pehip_batch_autotune", "pehip_batch_stream", "pehip_batch_last_error",
+    "pehip_adjoint_forward", "pehip_adjoint_backward",
 ]
 
 
@@ -62,6 +63,11 @@ class BatchStats(ctypes.Structure):
                 ("max_colours", ctypes.c_uint32), ("steps_done_min", ctypes.c_uint64),
                 ("capacity_retries", ctypes.c_uint32), ("launches", ctypes.c_uint32),
                 ("sweep_bytes", ctypes.c_double)]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("total_sweeps", ctypes.c_uint64), ("max_contacts", ctypes.c_uint64),
+                ("max_last_delta", ctypes.c_double)]
 
 
 class PeHipError(RuntimeError):
@@ -133,6 +139,9 @@ def load():
         "pehip_batch_autotune": (st, [_VP, ctypes.c_double, ctypes.c_uint32, ctypes.POINTER(BatchTuning)]),
         "pehip_batch_stream": (_VP, [_VP]),
         "pehip_batch_last_error": (ctypes.c_char_p, [_VP]),
+        "pehip_adjoint_forward": (st, [_VP, ctypes.c_double, ctypes.c_uint32, ctypes.c_uint32]),
+        "pehip_adjoint_backward": (st, [_VP, _D, _D, _D, _D, _D, _D, _D, ctypes.c_uint32, ctypes.c_double,
+                                        ctypes.POINTER(AdjointStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

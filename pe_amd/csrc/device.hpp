@@ -521,11 +521,20 @@ __device__ __forceinline__ void golden_step(GoldenState& S, double an, double x0
   }
 }
 
+// Value of x in lane `lane` of the wavefront (lane uniform across the wavefront).
+__device__ __forceinline__ double wave_read_lane(double x, int lane) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // maxSub iterations.  SPEC = 0: one lane, as pe.  SPEC = d > 0: the 2^d lanes of a group run d
 // iterations per round -- lane j evaluates node j of the depth-d tree of possible next points
 // (heap order; the path to a node, read from the bits of j + 1 below the leading one, is a
 // sequence of comparison outcomes, and the bracket along it depends only on the points, not on
-// the objective values), then the group follows the actual comparisons through the tree.  Every
+// the objective values), then the group follows the actual comparisons through the tree
+// (SPEC 6: the whole wavefront).  Every
 // evaluated point and every comparison is the one the sequential search makes, bit for bit.
 template <class Trig, int SPEC>
 __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, double flimit, double A00, double A01,
@@ -538,6 +547,40 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       const double xn0 = flimit * cn, xn1 = flimit * sn;
       const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
       golden_step(S, an, xn0, xn1, fn, fn < S.fm);
+    }
+  } else if constexpr (SPEC == 6) {
+    // A whole wavefront per contact: the path through the tree is wave-uniform, so the group
+    // follows it with scalar reads of single lanes instead of cross-lane shuffles.
+    const uint32_t code = (threadIdx.x & 63) + 1;
+    const int depth = 31 - __clz(code);
+    for (int k = 0; k < maxSub; k += SPEC) {
+      const int levels = maxSub - k < SPEC ? maxSub - k : SPEC;
+      double al = S.al, ar = S.ar, am = S.am;
+      bool left = S.left;
+      for (int d = depth - 1; d >= 0; --d) {
+        const bool lt = ((code >> d) & 1u) != 0;
+        const double a = al + (ar - am);
+        if (left) {
+          if (lt) { ar = am; am = a; }
+          else { al = a; left = false; }
+        } else {
+          if (lt) { al = am; am = a; }
+          else { ar = a; left = true; }
+        }
+      }
+      const double an = al + (ar - am);
+      double sn, cn;
+      Trig::sincos(an, sn, cn);
+      const double xn0 = flimit * cn, xn1 = flimit * sn;
+      const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
+      int node = 0;
+      for (int d = 0; d < levels; ++d) {
+        const double f = wave_read_lane(fn, node);
+        const double a = wave_read_lane(an, node), x0 = wave_read_lane(xn0, node), x1 = wave_read_lane(xn1, node);
+        const bool lt = __builtin_amdgcn_readfirstlane(f < S.fm ? 1 : 0) != 0;
+        golden_step(S, a, x0, x1, f, lt);
+        node = 2 * node + (lt ? 2 : 1);
+      }
     }
   } else {
     constexpr int L = 1 << SPEC;

@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(64) k_sweep_serial(SolverContacts C, uint32_t 
   if (threadIdx.x == 0) deltas[iters] = it;
 }
 
-// Lanes per contact: 16 while the colour fills at most kSpecLanes lanes that way, then 4, then 1.
+// Lanes per contact: 64 while the colour fills at most kSpecLanes lanes that way, then 16, 4, 1.
 constexpr uint64_t kSpecLanes = 131072;
 
 template <class Trig>
@@ -193,7 +193,9 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
     case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
     case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
     default:
-      if ((n << 4) <= kSpecLanes)
+      if ((n << 6) <= kSpecLanes)
+        hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+      else if ((n << 4) <= kSpecLanes)
         hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
       else if ((n << 2) <= kSpecLanes)
         hipLaunchKernelGGL((k_sweep<3, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
@@ -217,7 +219,7 @@ static void serial_dispatch(int model, const SolverContacts& C, uint32_t nc, uin
   switch (model) {
     case 0: hipLaunchKernelGGL((k_sweep_serial<0, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 1: hipLaunchKernelGGL((k_sweep_serial<1, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
-    default: hipLaunchKernelGGL((k_sweep_serial<3, Trig, 5>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    default: hipLaunchKernelGGL((k_sweep_serial<3, Trig, 6>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
 }
 
