@@ -60,6 +60,55 @@ def cpu_baseline(sc, states, steps, seconds):
                                            steps)}
 
 
+def batched_c4(torch):
+    """Batched scenes (4,096 independent sphere scenes of 50-2,000 spheres, 40 sweeps, 300 steps,
+    per-scene friction): the persistent batch kernel replayed from a hipGraph, against the
+    single-world path stepping one scene at a time (sampled: 8 random scenes, 10 steps each from
+    the batch's state at step 150, extrapolated to 300 steps x 4,096 scenes)."""
+    from pe_amd import World, scenes
+    from pe_amd.batch import BatchWorld
+    scs, mus = scenes.config4(4096)
+    bw = BatchWorld(scs, scene_friction=(mus, mus))
+    tun = bw.autotune(trial_steps=3)
+    bw.setTuning(use_graph=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bw.run(150)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mid = bw.getState()
+    t2 = time.perf_counter()
+    bw.run(150)
+    torch.cuda.synchronize()
+    tb = (t1 - t0) + (time.perf_counter() - t2)
+    st = bw.stats()
+    bw.close()
+    rng = np.random.default_rng(0)
+    per = []
+    for k in rng.choice(len(scs), 8, replace=False):
+        sc = scs[int(k)]
+        w = World()
+        sc.to_world(w)
+        w.setMaxIterations(40)
+        w.setRelaxationModel(3)
+        w.setState(bw.sceneState(mid, int(k)))
+        w.run(2, sc.dt)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        w.run(10, sc.dt)
+        torch.cuda.synchronize()
+        per.append((time.perf_counter() - a) / 10)
+        w.close()
+    unb = 1.0 / (float(np.mean(per)) * 300)
+    return {"workload": "4096 independent scenes, log-uniform 50-2000 spheres r 0.5 in 10x10x40 m plane boxes, "
+                        "per-scene mu U[0.2,0.8], 40 NBGS sweeps, dt 1e-3, 300 steps",
+            "scenes_per_s": 4096 / tb, "scene_steps_per_s": 4096 * 300 / tb, "seconds": tb,
+            "unbatched_scenes_per_s": unb, "batched_over_unbatched": (4096 / tb) / unb,
+            "tuning": tun, "contacts_last_step": int(st["num_contacts"]),
+            "sample": "unbatched: single-world path, one scene at a time, 8 random scenes x 10 steps from the "
+                      "batch state at step 150 (%.3f ms/step mean), extrapolated" % (1e3 * float(np.mean(per)))}
+
+
 def save_state(path, st):
     np.savez(path, **st)
 
@@ -75,9 +124,13 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--spheres", type=int, default=200_000)
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+                    help="c2: the polydisperse pile (default); c3: 950k spheres + 50k boxes in a V-channel hopper, "
+                         "30 sweeps, dt 2e-4 (use --steps 1000 for the whole job)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
     ap.add_argument("--load-state", default=None, help="start the timed steps from this saved state (.npz)")
     ap.add_argument("--save-state", default=None, help="save the state after the timed steps (.npz)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -93,7 +146,7 @@ def main():
     import torch
     from pe_amd import World, scenes
 
-    sc = scenes.config2(args.spheres)
+    sc = scenes.config2(args.spheres) if args.workload == "c2" else scenes.config3()
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -149,10 +202,12 @@ def main():
         "metric": METRIC, "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "polydisperse granular pile, %d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, "
-                               "v0~U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), "
-                               "dt 5e-4, whole job: steps 0..%d%s" % (args.spheres, args.steps,
-                                                                      " from a saved state" if args.load_state else ""),
+        "config": {"workload": (("polydisperse granular pile, %d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, "
+                                 "v0~U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), "
+                                 "dt 5e-4" % args.spheres) if args.workload == "c2" else
+                                ("mixed shapes, 950k spheres r~U[0.2,0.4] + 50k boxes edges U[0.3,0.8], V-channel "
+                                 "hopper (planes at +-30 deg, 0.8 m gap), mu 0.6, 30 NBGS sweeps, dt 2e-4")) +
+                               ", whole job: steps 0..%d%s" % (args.steps, " from a saved state" if args.load_state else ""),
                    "bodies": int(w.size), "sweeps_per_step": sc.max_iterations, "dt": sc.dt,
                    "parallelism": "single GPU"},
         "contact_iterations_per_s": ci / solve_s if solve_s > 0 else None,
@@ -172,6 +227,9 @@ def main():
                            "kernel": "k_sweep<3> (one colour of one NBGS sweep)",
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
                            "launches": int(st["prof_sweep_launches"])}
+    if not args.no_batched and args.workload == "c2" and not args.load_state:
+        w.close()
+        out["batched"] = batched_c4(torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, [state0, state_mid, state_end], args.steps, args.cpu_seconds)
     print(json.dumps(out))

@@ -150,6 +150,9 @@ pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** ou
   pehip_ctx* ctx = new pehip_ctx();
   ctx->prm = p;
   ctx->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    ctx->ncu = (uint32_t)ncu;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&ctx->hstage, 4096, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&ctx->flags, 64) != hipSuccess ||
@@ -517,7 +520,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
         for (uint32_t k = 0; k < ncol; ++k) {
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
           launch_sweep(P.relaxation_model, exact, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
-                       (int)P.max_sub_iterations, ctx->deltas + it, s);
+                       (int)P.max_sub_iterations, ctx->deltas + it, (uint64_t)ctx->ncu * 4 * 64, s);
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
         }
         iters = it + 1;

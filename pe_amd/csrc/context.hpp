@@ -30,6 +30,7 @@ using namespace pe_amd;  // internal header: the context code is written against
 struct pehip_ctx {
   pehip_params prm;
   int device = 0;
+  uint32_t ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
   bool profiling = false;

@@ -573,13 +573,33 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       Trig::sincos(an, sn, cn);
       const double xn0 = flimit * cn, xn1 = flimit * sn;
       const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
-      int node = 0;
+      // follow the comparisons (only the objective values cross lanes on this path) ...
+      int node = 0, last = -1;
+      uint32_t bits = 0;
+      double fm = S.fm;
       for (int d = 0; d < levels; ++d) {
         const double f = wave_read_lane(fn, node);
-        const double a = wave_read_lane(an, node), x0 = wave_read_lane(xn0, node), x1 = wave_read_lane(xn1, node);
-        const bool lt = __builtin_amdgcn_readfirstlane(f < S.fm ? 1 : 0) != 0;
-        golden_step(S, a, x0, x1, f, lt);
+        const bool lt = __builtin_amdgcn_readfirstlane(f < fm ? 1 : 0) != 0;
+        if (lt) { fm = f; last = node; }
+        bits = (bits << 1) | (lt ? 1u : 0u);
         node = 2 * node + (lt ? 2 : 1);
+      }
+      // ... then advance the bracket along the taken path and take the last improving point
+      for (int d = levels - 1; d >= 0; --d) {
+        const bool lt = ((bits >> d) & 1u) != 0;
+        const double a = S.al + (S.ar - S.am);
+        if (S.left) {
+          if (lt) { S.ar = S.am; S.am = a; }
+          else { S.al = a; S.left = false; }
+        } else {
+          if (lt) { S.al = S.am; S.am = a; }
+          else { S.ar = a; S.left = true; }
+        }
+      }
+      if (last >= 0) {
+        S.fm = fm;
+        S.xm0 = wave_read_lane(xn0, last);
+        S.xm1 = wave_read_lane(xn1, last);
       }
     }
   } else {

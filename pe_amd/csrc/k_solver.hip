@@ -180,12 +180,11 @@ __global__ void __launch_bounds__(64) k_sweep_serial(SolverContacts C, uint32_t 
   if (threadIdx.x == 0) deltas[iters] = it;
 }
 
-// Lanes per contact: 64 while the colour fills at most kSpecLanes lanes that way, then 16, 4, 1.
-constexpr uint64_t kSpecLanes = 131072;
-
+// Lanes per contact: 64 while the colour's waves all fit on the device at once (lanes <= budget,
+// one wavefront per SIMD: the model-3 kernels need 256 registers), then 16, 4, 1.
 template <class Trig>
 static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
-                           double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
+                           double dtinv, int maxSub, unsigned long long* delta, uint64_t kSpecLanes, hipStream_t s) {
   const uint64_t n = end - begin;
   const dim3 b(256);
   auto grid = [&](int spec) { return dim3((unsigned)(((n << spec) + 255) / 256)); };
@@ -206,10 +205,11 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
-                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, hipStream_t s) {
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint64_t lanes_budget,
+                  hipStream_t s) {
   if (end <= begin) return;
-  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, s);
-  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, s);
+  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, lanes_budget, s);
+  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, lanes_budget, s);
 }
 
 template <class Trig>

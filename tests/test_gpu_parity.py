@@ -133,7 +133,61 @@ def stack_scene():
                         box_mat=np.zeros(nb, np.uint32), v0=v0, dt=1e-3, max_iterations=30)
 
 
-SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene, "stack": stack_scene}
+def _box_R(q):
+    w_, x_, y_, z_ = q
+    return np.array([[1 - 2 * (y_ * y_ + z_ * z_), 2 * (x_ * y_ - w_ * z_), 2 * (x_ * z_ + w_ * y_)],
+                     [2 * (x_ * y_ + w_ * z_), 1 - 2 * (x_ * x_ + z_ * z_), 2 * (y_ * z_ - w_ * x_)],
+                     [2 * (x_ * z_ - w_ * y_), 2 * (y_ * z_ + w_ * x_), 1 - 2 * (x_ * x_ + y_ * y_)]])
+
+
+def hopper_scene():
+    # the mixed-shape hopper's geometry at small scale: two planes inclined at +-30 deg meeting
+    # a 0.8 m gap above the floor (inclined normals: every body is tested against them).  On
+    # each slope, slots of a box (edges U[0.3, 0.8], random orientation) or a sphere
+    # (r ~ U[0.2, 0.4]) resting on the plane with a small sphere resting on top of it; three
+    # spheres in the gap on the floor; mu = 0.6, sliding down the slopes
+    rng = np.random.default_rng(31)
+    L = 10.0
+    cx = L / 2
+    s30, c30 = 0.5, np.sqrt(0.75)
+    nl, nr = np.array([s30, 0.0, c30]), np.array([-s30, 0.0, c30])
+    planes = [((0, 0, 1), 0.0, 0), (tuple(nl), s30 * (cx - 0.4), 0), (tuple(nr), -s30 * (cx + 0.4), 0),
+              ((0, 1, 0), 0.0, 0), ((0, -1, 0), -L, 0), ((1, 0, 0), 0.0, 0), ((-1, 0, 0), -L, 0)]
+    sph, rad, bpos, blen, bq = [], [], [], [], []
+    e = 1e-6
+    for side, nrm, x0 in ((0, nl, cx - 0.4), (1, nr, cx + 0.4)):
+        up_slope = np.array([-c30 if side == 0 else c30, 0.0, s30])   # along the slope, away from the gap
+        for k in range(3):
+            for j in range(4):
+                foot = np.array([x0, 1.2 + 1.5 * j, 0.0]) + (0.9 + 1.5 * k) * up_slope
+                if k > 0 and (k + j + side) % 2 == 0:
+                    lens = rng.uniform(0.3, 0.8, 3)
+                    q = scenes.shoemake_quaternions(rng, 1)[0]
+                    h = 0.5 * np.sum(np.abs(nrm @ _box_R(q)) * lens)   # extent of the box along n
+                    c = foot + (h - e) * nrm
+                    bpos.append(c)
+                    blen.append(lens)
+                    bq.append(q)
+                else:
+                    h = rng.uniform(0.2, 0.4)
+                    c = foot + (h - e) * nrm
+                    sph.append(c)
+                    rad.append(h)
+                r2 = rng.uniform(0.15, 0.25)                           # resting on top, along n
+                sph.append(c + (h + r2 - e) * nrm + rng.uniform(-1e-7, 1e-7, 3))
+                rad.append(r2)
+    for j in range(3):
+        sph.append([cx + rng.uniform(-0.1, 0.1), 1.5 + 2.0 * j, 0.2 - e])
+        rad.append(0.2)
+    sph, rad = np.array(sph), np.array(rad)
+    ns = len(sph)
+    return scenes.Scene("hopper", [{"name": "m", "density": 2500.0, "cor": 0.0, "csf": 0.3, "cdf": 0.3}],
+                        planes, sph, rad, np.zeros(ns, np.uint32), box_pos=np.array(bpos), box_len=np.array(blen),
+                        box_q=np.array(bq), box_mat=np.zeros(len(bpos), np.uint32), dt=2e-4, max_iterations=30)
+
+
+SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene, "stack": stack_scene,
+          "hopper": hopper_scene}
 
 
 def oracle_state_after(sc, steps, iters=30, cr_trig=False):
