@@ -520,7 +520,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
         for (uint32_t k = 0; k < ncol; ++k) {
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
           launch_sweep(P.relaxation_model, exact, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
-                       (int)P.max_sub_iterations, ctx->deltas + it, (uint64_t)ctx->ncu * 4 * 64, s);
+                       (int)P.max_sub_iterations, ctx->deltas + it, ctx->ncu, s);
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
         }
         iters = it + 1;

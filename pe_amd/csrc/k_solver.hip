@@ -158,7 +158,10 @@ __device__ __forceinline__ void wave_max_atomic(double v, unsigned long long* ou
 template <int MODEL, class Trig, int SPEC>
 __global__ void __launch_bounds__(256) k_sweep(SolverContacts C, uint32_t begin, uint32_t end, const double* sb,
                                                double* dvw, double dtinv, int maxSub, unsigned long long* delta) {
-  const uint32_t i = begin + ((blockIdx.x * blockDim.x + threadIdx.x) >> SPEC);
+  uint32_t i = begin + ((blockIdx.x * blockDim.x + threadIdx.x) >> SPEC);
+  // A whole wavefront per contact: the index is wave-uniform, so the contact's and its bodies'
+  // records load through the scalar path into SGPRs (reads only), easing VGPR pressure.
+  if (SPEC == 6) i = __builtin_amdgcn_readfirstlane(i);
   double d = 0;
   if (i < end) d = relax_contact<MODEL, Trig, SPEC>(C, i, sb, dvw, dtinv, maxSub);
   wave_max_atomic(d, delta);
@@ -180,11 +183,28 @@ __global__ void __launch_bounds__(64) k_sweep_serial(SolverContacts C, uint32_t 
   if (threadIdx.x == 0) deltas[iters] = it;
 }
 
-// Lanes per contact: 64 while the colour's waves all fit on the device at once (lanes <= budget,
-// one wavefront per SIMD: the model-3 kernels need 256 registers), then 16, 4, 1.
+// Lanes per contact.  A wavefront issues one instruction every four cycles whatever its lanes
+// do, so a contact's relaxation costs its instruction count in either layout; the speculated
+// search (SPEC > 0) trades 20 sequential sin/cos evaluations per pass for 4 rounds plus tree
+// bookkeeping.  That pays with the correctly rounded trig (~6,000 cycles per sincos, measured
+// 2.4x faster per sliding contact) and not with the device libm (~600 cycles: 1.2x slower), so
+// the fast trig always runs one lane per contact and the exact trig the largest group (64, 16, 4)
+// with which every wavefront of the colour is resident at once (occupancy queried once).
+template <class Trig, int SPEC>
+static uint64_t resident_lanes(uint32_t ncu) {
+  static int blocks = -1;
+  if (blocks < 0) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)k_sweep<3, Trig, SPEC>, 256, 0) != hipSuccess || b < 1)
+      b = 1;
+    blocks = b;
+  }
+  return (uint64_t)blocks * 256u * ncu;
+}
+
 template <class Trig>
 static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
-                           double dtinv, int maxSub, unsigned long long* delta, uint64_t kSpecLanes, hipStream_t s) {
+                           double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s) {
   const uint64_t n = end - begin;
   const dim3 b(256);
   auto grid = [&](int spec) { return dim3((unsigned)(((n << spec) + 255) / 256)); };
@@ -192,11 +212,13 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
     case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
     case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
     default:
-      if ((n << 6) <= kSpecLanes)
+      if (!Trig::kCorrectlyRounded)
+        hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+      else if ((n << 6) <= resident_lanes<Trig, 6>(ncu))
         hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
-      else if ((n << 4) <= kSpecLanes)
+      else if ((n << 4) <= resident_lanes<Trig, 4>(ncu))
         hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
-      else if ((n << 2) <= kSpecLanes)
+      else if ((n << 2) <= resident_lanes<Trig, 2>(ncu))
         hipLaunchKernelGGL((k_sweep<3, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
       else
         hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
@@ -205,11 +227,10 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
-                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint64_t lanes_budget,
-                  hipStream_t s) {
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s) {
   if (end <= begin) return;
-  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, lanes_budget, s);
-  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, lanes_budget, s);
+  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, ncu, s);
+  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, ncu, s);
 }
 
 template <class Trig>

@@ -107,10 +107,9 @@ void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, u
                   SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen,
                   hipStream_t s);
 // One colour of one sweep.
-// lanes_budget: lanes the device runs at once for this kernel (sizes the lanes per contact).
+// ncu: compute units of the device (sizes the lanes per contact).
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
-                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint64_t lanes_budget,
-                  hipStream_t s);
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s);
 // All sweeps, strictly sequential, one wavefront.  deltas[it] receives the L-inf impulse
 // change of sweep it; deltas[iters] the number of sweeps run (stops early below thr > 0).
 void launch_sweep_serial(int model, bool exact_trig, const SolverContacts& C, uint32_t nc, uint32_t iters,

@@ -106,6 +106,7 @@ PE_HD double round_dd(ddv a) { return a.hi + a.lo; }
 
 // Correctly rounded trig (arguments |x| < 2^20).
 struct TrigExact {
+  static constexpr bool kCorrectlyRounded = true;
   static PE_HD double sin(double x) {
     if (x == 0.0) return x;
     dd::ddv s, c;
@@ -152,8 +153,10 @@ struct TrigExact {
   }
 };
 
-// The device libm (fast path of the production solver).
+// Fast trig of the production solver: the device libm (within about 1 ulp, not correctly
+// rounded).  sin and cos of one angle come from one sincos, which returns the same values.
 struct TrigFast {
+  static constexpr bool kCorrectlyRounded = false;
   static PE_HD double sin(double x) { return ::sin(x); }
   static PE_HD double cos(double x) { return ::cos(x); }
   static PE_HD void sincos(double x, double& sn, double& cs) { ::sincos(x, &sn, &cs); }

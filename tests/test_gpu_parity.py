@@ -449,6 +449,16 @@ def test_device_trig_matches_host_libm():
     assert w.evalTrig(0, np.array([math.pi / 6]))[0] == math.sin(math.pi / 6)
 
 
+def test_fast_sincos_within_two_ulp():
+    # The production sin/cos (device libm) against the correctly rounded device trig.
+    w = gpu_for(dense_scene())
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(-8.0, 8.0, 200_000), rng.uniform(-1e-3, 1e-3, 2000)])
+    for fn in (0, 1):
+        fast, cr = w.evalTrig(fn, x, exact=False), w.evalTrig(fn, x, exact=True)
+        assert np.all(np.abs(fast - cr) <= 2 * np.abs(np.spacing(cr))), fn
+
+
 @pytest.mark.parametrize("exact", [True, False])
 def test_sincos_equals_sin_and_cos(exact):
     # The golden-section search takes sin and cos of one angle from a single sincos; it must
