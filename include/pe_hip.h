@@ -75,7 +75,18 @@ typedef struct {
                                   search and the orientation update, reproducing the host libm
                                   (glibc) wherever it is correctly rounded; 0: device libm.  Always
                                   on in PEHIP_SWEEP_REFERENCE_ORDER. */
+  int32_t sweep_schedule;      /* coloured mode only -- pehip_sweep_schedule */
 } pehip_params;
+
+typedef enum {
+  /* dataflow when the step's contacts fit it (and abort_threshold <= 0), else colour launches */
+  PEHIP_SCHEDULE_AUTO = 0,
+  /* one kernel launch per colour and sweep */
+  PEHIP_SCHEDULE_COLOUR_LAUNCHES = 1,
+  /* all sweeps in one cooperative launch; each contact waits only for the earlier updates of its
+   * own bodies (same results as the colour launches, bit for bit) */
+  PEHIP_SCHEDULE_DATAFLOW = 2
+} pehip_sweep_schedule;
 
 /* Fills p with pe's defaults (HCSITS.h:408-431, Settings.cpp:46-48). */
 void pehip_default_params(pehip_params* p);

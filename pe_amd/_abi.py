@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libpehip.so")
 OK, INVALID_ARG, OUT_OF_MEMORY, CAPACITY_RETRY, HIP_ERROR, NONFINITE, NOT_SUPPORTED = range(7)
 SPHERE, BOX, PLANE = 1, 2, 3
 SWEEP_COLOURED, SWEEP_REFERENCE_ORDER = 0, 1
+SCHEDULE_AUTO, SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW = 0, 1, 2
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "CAPACITY_RETRY", 4: "HIP_ERROR",
                 5: "NONFINITE", 6: "NOT_SUPPORTED"}
@@ -37,7 +38,8 @@ class Params(ctypes.Structure):
                 ("erp", ctypes.c_double), ("abort_threshold", ctypes.c_double),
                 ("gravity", ctypes.c_double * 3), ("damping", ctypes.c_double),
                 ("relaxation_param", ctypes.c_double), ("max_sub_iterations", ctypes.c_uint32),
-                ("sweep_mode", ctypes.c_int32), ("exact_trig", ctypes.c_int32)]
+                ("sweep_mode", ctypes.c_int32), ("exact_trig", ctypes.c_int32),
+                ("sweep_schedule", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -15,6 +15,12 @@ using namespace pe_amd;
 namespace {
 
 const char* kVersion = "pe_hip 0.1.0 (gfx950)";
+// Automatic schedule: dataflow sweeps up to this many contacts (above it a colour holds enough
+// contacts to fill the device and the colour launches are throughput-bound, not latency-bound).
+constexpr uint32_t kDataflowMaxContacts = 65536;
+// Polls (each with a short sleep) before a waiting wavefront gives up: seconds, far above any
+// legitimate wait.
+constexpr uint32_t kDataflowSpinLimit = 1u << 22;
 std::string g_create_error;
 
 uint32_t next_pow2(uint64_t x) {
@@ -37,6 +43,7 @@ static pehip_status ensure_contact_capacity(pehip_ctx* ctx, size_t need) {
   CK(dalloc(ctx->col.cround, cap));
   CK(dalloc(ctx->perm, cap)); CK(dalloc(ctx->sort_keys, cap)); CK(dalloc(ctx->sort_keys_tmp, cap));
   CK(dalloc(ctx->sort_vals_tmp, cap));
+  CK(dalloc(ctx->df_inv, cap)); CK(dalloc(ctx->df_rank1, cap)); CK(dalloc(ctx->df_rank2, cap));
   CK(dalloc(ctx->cscan_scratch, scan_scratch_words(cap)));
   CK(dalloc(ctx->csort_scratch, sort_scratch_words(cap)));
   ctx->raw.cap = cap;
@@ -114,6 +121,10 @@ pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
     return PEHIP_INVALID_ARG;
   }
   if (p->damping > 1 || p->damping < 0) { why = "Invalid damping."; return PEHIP_INVALID_ARG; }
+  if (p->sweep_schedule < PEHIP_SCHEDULE_AUTO || p->sweep_schedule > PEHIP_SCHEDULE_DATAFLOW) {
+    why = "Invalid sweep schedule.";
+    return PEHIP_INVALID_ARG;
+  }
   return PEHIP_OK;
 }
 
@@ -130,6 +141,7 @@ void pehip_default_params(pehip_params* p) {
   p->max_sub_iterations = 20;
   p->sweep_mode = PEHIP_SWEEP_COLOURED;
   p->exact_trig = 0;
+  p->sweep_schedule = PEHIP_SCHEDULE_AUTO;
 }
 
 
@@ -312,6 +324,7 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   CK(dalloc(ctx->sort_scratch, sort_scratch_words(n)));
   CK(dalloc(ctx->col.deg, n)); CK(dalloc(ctx->col.adj_start, (size_t)n + 1)); CK(dalloc(ctx->col.adj_cursor, n));
   CK(dalloc(ctx->ke_partial, (n + 255) / 256));
+  CK(dalloc(ctx->df_done, n));
   P.planes = ctx->planes;
   P.nplanes = (uint32_t)hplanes.size();
   P.flags = ctx->flags;
@@ -513,6 +526,25 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + P.max_iterations, 8, hipMemcpyDeviceToHost, s));
       CKS(sync_check(ctx));
       iters = (uint32_t)(*(unsigned long long*)ctx->hstage);
+    } else if (P.abort_threshold <= 0 && (P.sweep_schedule == PEHIP_SCHEDULE_DATAFLOW ||
+                                          (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && nc <= kDataflowMaxContacts))) {
+      // all sweeps in one cooperative launch (same Gauss-Seidel order per body as the colours)
+      DataflowBuffers D;
+      D.perm = ctx->perm; D.adj_start = ctx->col.adj_start; D.adj = ctx->col.adj; D.deg = ctx->col.deg;
+      D.inv = ctx->df_inv; D.rank1 = ctx->df_rank1; D.rank2 = ctx->df_rank2; D.done = ctx->df_done;
+      D.deltas = ctx->deltas; D.flags = ctx->flags; D.spin_limit = kDataflowSpinLimit;
+      fill_u32(ctx->flags, 0u, 1, s);
+      if (ctx->profiling) CKS(record_launch_event(ctx, 0));
+      if (launch_sweep_dataflow(P.relaxation_model, exact, ctx->sol, nc, P.max_iterations, B, dtinv,
+                                (int)P.max_sub_iterations, D, ctx->ncu, s) != SWEEP_OK)
+        return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep launch failed");
+      if (ctx->profiling) CKS(record_launch_event(ctx, 1));
+      ctx->n_launch_events = ctx->profiling ? 2 : 0;
+      CK(hipMemcpyAsync(ctx->hstage, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+      CKS(sync_check(ctx));
+      if (ctx->hstage[0] & ERR_TIMEOUT) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
+      iters = P.max_iterations;
+      ++ctx->dataflow_steps;
     } else {
       const uint32_t ncol = (uint32_t)ctx->colour_off.size() - 1;
       size_t nev = 0;

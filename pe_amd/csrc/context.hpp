@@ -66,6 +66,8 @@ struct pehip_ctx {
   SolverContacts sol{};
   ColourBuffers col{};
   uint32_t *perm = nullptr, *sort_keys = nullptr, *sort_keys_tmp = nullptr, *sort_vals_tmp = nullptr;
+  uint32_t *df_inv = nullptr, *df_rank1 = nullptr, *df_rank2 = nullptr, *df_done = nullptr;  // dataflow sweeps
+  uint64_t dataflow_steps = 0;  // steps whose sweeps ran as one dataflow launch
   uint32_t *cscan_scratch = nullptr, *csort_scratch = nullptr;
   std::vector<uint32_t> colour_off;
   unsigned long long *deltas = nullptr, *maxpen = nullptr;
@@ -96,7 +98,7 @@ struct pehip_ctx {
     dfree(force); dfree(torque); dfree(sb); dfree(dvw); dfree(planes);
     dfree(bp.keys); dfree(bp.vals); dfree(bp.keys_tmp); dfree(bp.vals_tmp); dfree(bp.cell_start);
     dfree(bp.cell_end); dfree(bp.counts); dfree(bp.offsets); dfree(scan_scratch); dfree(sort_scratch);
-    dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial);
+    dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial); dfree(df_done);
     n = 0;
   }
   void free_contacts() {
@@ -104,6 +106,7 @@ struct pehip_ctx {
     dfree(sol.b1); dfree(sol.b2); dfree(sol.f);
     dfree(col.adj); dfree(col.colour); dfree(col.prio); dfree(col.cround);
     dfree(perm); dfree(sort_keys); dfree(sort_keys_tmp); dfree(sort_vals_tmp);
+    dfree(df_inv); dfree(df_rank1); dfree(df_rank2);
     dfree(cscan_scratch); dfree(csort_scratch);
     ccap = 0;
   }

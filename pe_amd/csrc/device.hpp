@@ -521,6 +521,27 @@ __device__ __forceinline__ void golden_step(GoldenState& S, double an, double x0
   }
 }
 
+// Bracket of the tree node `code` (heap index + 1, `depth` levels below the root): the bracket
+// updates of golden_step along the node's path, written with selects so that the lanes of a
+// wavefront, each on its own path, do not diverge.
+__device__ __forceinline__ void golden_path(uint32_t code, int depth, double& al, double& ar, double& am,
+                                            bool& left) {
+#pragma unroll
+  for (int lev = 0; lev < 6; ++lev) {
+    const bool act = lev < depth;
+    const bool lt = act && ((code >> (act ? depth - 1 - lev : 0)) & 1u) != 0;
+    const double a = al + (ar - am);
+    const double nal = lt ? (left ? al : am) : (left ? a : al);
+    const double nar = lt ? (left ? am : ar) : (left ? ar : a);
+    const double nam = lt ? a : am;
+    const bool nleft = lt ? left : !left;
+    al = act ? nal : al;
+    ar = act ? nar : ar;
+    am = act ? nam : am;
+    left = act ? nleft : left;
+  }
+}
+
 // Value of x in lane `lane` of the wavefront (lane uniform across the wavefront).
 __device__ __forceinline__ double wave_read_lane(double x, int lane) {
   const unsigned long long u = (unsigned long long)__double_as_longlong(x);
@@ -557,17 +578,7 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       const int levels = maxSub - k < SPEC ? maxSub - k : SPEC;
       double al = S.al, ar = S.ar, am = S.am;
       bool left = S.left;
-      for (int d = depth - 1; d >= 0; --d) {
-        const bool lt = ((code >> d) & 1u) != 0;
-        const double a = al + (ar - am);
-        if (left) {
-          if (lt) { ar = am; am = a; }
-          else { al = a; left = false; }
-        } else {
-          if (lt) { al = am; am = a; }
-          else { ar = a; left = true; }
-        }
-      }
+      golden_path(code, depth, al, ar, am, left);
       const double an = al + (ar - am);
       double sn, cn;
       Trig::sincos(an, sn, cn);
@@ -610,17 +621,7 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       const int levels = maxSub - k < SPEC ? maxSub - k : SPEC;
       double al = S.al, ar = S.ar, am = S.am;
       bool left = S.left;
-      for (int d = depth - 1; d >= 0; --d) {
-        const bool lt = ((code >> d) & 1u) != 0;
-        const double a = al + (ar - am);
-        if (left) {
-          if (lt) { ar = am; am = a; }
-          else { al = a; left = false; }
-        } else {
-          if (lt) { al = am; am = a; }
-          else { ar = a; left = true; }
-        }
-      }
+      golden_path(code, depth, al, ar, am, left);
       const double an = al + (ar - am);
       double sn, cn;
       Trig::sincos(an, sn, cn);
@@ -644,7 +645,21 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
 // separating contact).  The caller guarantees that no other thread touches the
 // non-fixed bodies of contact i concurrently.  Trig = TrigFast (device libm) or TrigExact
 // (correctly rounded, reproduces the host libm's golden-section decisions).
-template <int MODEL, class Trig, int SPEC = 0>
+// Loads and stores of the data a sweep writes (velocity corrections, impulses).  COHERENT:
+// agent-scope atomic accesses, for sweeps in which consecutive updates of one body may run on
+// different compute units (and L2 slices) within one kernel.
+template <bool COHERENT>
+__device__ __forceinline__ double sweep_ld(const double* p) {
+  if constexpr (COHERENT) return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else return *p;
+}
+template <bool COHERENT>
+__device__ __forceinline__ void sweep_st(double* p, double v) {
+  if constexpr (COHERENT) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
+template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
 __device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
                                        double* __restrict__ dvw, double dtinv, int maxSub) {
   const size_t cap = C.cap;
@@ -666,13 +681,20 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
   double* q1 = dvw + (size_t)kDVW * i1;
   double* q2 = dvw + (size_t)kDVW * i2;
   d3 dv1 = mk3(0, 0, 0), dw1 = mk3(0, 0, 0), dv2 = mk3(0, 0, 0), dw2 = mk3(0, 0, 0);
-  if (!fx1) { dv1 = mk3(q1[0], q1[1], q1[2]); dw1 = mk3(q1[3], q1[4], q1[5]); }
-  if (!fx2) { dv2 = mk3(q2[0], q2[1], q2[2]); dw2 = mk3(q2[3], q2[4], q2[5]); }
+  if (!fx1) {
+    dv1 = mk3(sweep_ld<COHERENT>(q1), sweep_ld<COHERENT>(q1 + 1), sweep_ld<COHERENT>(q1 + 2));
+    dw1 = mk3(sweep_ld<COHERENT>(q1 + 3), sweep_ld<COHERENT>(q1 + 4), sweep_ld<COHERENT>(q1 + 5));
+  }
+  if (!fx2) {
+    dv2 = mk3(sweep_ld<COHERENT>(q2), sweep_ld<COHERENT>(q2 + 1), sweep_ld<COHERENT>(q2 + 2));
+    dw2 = mk3(sweep_ld<COHERENT>(q2 + 3), sweep_ld<COHERENT>(q2 + 4), sweep_ld<COHERENT>(q2 + 5));
+  }
 
   const d3 r1 = cf_ld3(f, cap, CF_R1, i), r2 = cf_ld3(f, cap, CF_R2, i);
   const d3 n = cf_ld3(f, cap, CF_N, i), t = cf_ld3(f, cap, CF_T, i), o = cf_ld3(f, cap, CF_O, i);
   const double ddt = f[CF_DDT * cap + i];
-  d3 p = cf_ld3(f, cap, CF_P, i);
+  d3 p = mk3(sweep_ld<COHERENT>(f + CF_P * cap + i), sweep_ld<COHERENT>(f + (CF_P + 1) * cap + i),
+             sweep_ld<COHERENT>(f + (CF_P + 2) * cap + i));
 
   // Remove this contact's reaction (HCSITS.h:2343-2346).
   dv1 = dv1 - im1 * p;
@@ -780,12 +802,20 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
     dv2 = dv2 - im2 * p;
     dw2 = dw2 - mat_vec(I2, cross3(r2, p));
   }
-  if (SPEC == 0 || (threadIdx.x & ((1u << SPEC) - 1)) == 0) {  // one lane of the contact's group stores
-    f[CF_P * cap + i] = p.x;
-    f[(CF_P + 1) * cap + i] = p.y;
-    f[(CF_P + 2) * cap + i] = p.z;
-    if (!fx1) { q1[0] = dv1.x; q1[1] = dv1.y; q1[2] = dv1.z; q1[3] = dw1.x; q1[4] = dw1.y; q1[5] = dw1.z; }
-    if (!fx2) { q2[0] = dv2.x; q2[1] = dv2.y; q2[2] = dv2.z; q2[3] = dw2.x; q2[4] = dw2.y; q2[5] = dw2.z; }
+  // one lane of the contact's group stores (COHERENT: the whole wavefront owns the contact)
+  const uint32_t group_mask = COHERENT ? 63u : ((1u << SPEC) - 1u);
+  if ((threadIdx.x & group_mask) == 0) {
+    sweep_st<COHERENT>(f + CF_P * cap + i, p.x);
+    sweep_st<COHERENT>(f + (CF_P + 1) * cap + i, p.y);
+    sweep_st<COHERENT>(f + (CF_P + 2) * cap + i, p.z);
+    if (!fx1) {
+      sweep_st<COHERENT>(q1, dv1.x); sweep_st<COHERENT>(q1 + 1, dv1.y); sweep_st<COHERENT>(q1 + 2, dv1.z);
+      sweep_st<COHERENT>(q1 + 3, dw1.x); sweep_st<COHERENT>(q1 + 4, dw1.y); sweep_st<COHERENT>(q1 + 5, dw1.z);
+    }
+    if (!fx2) {
+      sweep_st<COHERENT>(q2, dv2.x); sweep_st<COHERENT>(q2 + 1, dv2.y); sweep_st<COHERENT>(q2 + 2, dv2.z);
+      sweep_st<COHERENT>(q2 + 3, dw2.x); sweep_st<COHERENT>(q2 + 4, dw2.y); sweep_st<COHERENT>(q2 + 5, dw2.z);
+    }
   }
   return delta;
 }

@@ -1,4 +1,6 @@
 // Solver kernels: contact-graph colouring, contact caching and the NBGS sweeps.
+#include <algorithm>
+
 #include "step_ops.hpp"
 
 namespace pe_amd {
@@ -183,6 +185,82 @@ __global__ void __launch_bounds__(64) k_sweep_serial(SolverContacts C, uint32_t 
   if (threadIdx.x == 0) deltas[iters] = it;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Dataflow sweeps.  The coloured sweep order (sweep-major, then colour-major slots) is one
+// sequential Gauss-Seidel order; what it fixes is, for every body, the order of the updates that
+// touch it.  Here each (sweep, slot) item waits only for the previous updates of its own bodies
+// (per-body counters) instead of for a whole colour, so a sliding contact's long golden-section
+// chain delays only the contacts that share its bodies.  Results equal the coloured launches bit
+// for bit.  Items go round-robin to the wavefronts of a co-resident (cooperative) grid, which
+// take them in increasing order: the smallest unfinished item always has its predecessors done
+// and its wavefront free, so the sweep progresses; every wait also has a spin limit and a shared
+// abort flag, so no wavefront can wait forever.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_inv_perm(const uint32_t* perm, uint32_t nc, uint32_t* inv) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nc) inv[perm[s]] = s;
+}
+
+// rank1[s] / rank2[s]: position of slot s among the slots of its body 1 / body 2 (non-fixed).
+__global__ void k_body_ranks(const SolverContacts C, const uint8_t* fixed, uint32_t n, const uint32_t* start,
+                             const uint32_t* adj, const uint32_t* inv, uint32_t* rank1, uint32_t* rank2) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n || fixed[b]) return;
+  const uint32_t st = start[b], en = start[b + 1];
+  for (uint32_t j = st; j < en; ++j) {
+    const uint32_t sj = inv[adj[j]];
+    uint32_t r = 0;
+    for (uint32_t k = st; k < en; ++k) r += inv[adj[k]] < sj ? 1u : 0u;
+    if ((C.b1[sj] & ~kFixedBit) == b) rank1[sj] = r;
+    else rank2[sj] = r;
+  }
+}
+
+__device__ __forceinline__ bool wait_count(const uint32_t* ctr, uint32_t target, const uint32_t* abort_flag,
+                                           uint32_t spin_limit) {
+  uint32_t spins = 0;
+  while (__hip_atomic_load(const_cast<uint32_t*>(ctr), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (++spins > spin_limit) return false;
+    if ((spins & 63u) == 0 &&
+        __hip_atomic_load(const_cast<uint32_t*>(abort_flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+template <int MODEL, class Trig, int SPEC>
+__global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32_t nc, uint32_t iters,
+                                                        const double* sb, double* dvw, double dtinv, int maxSub,
+                                                        const uint32_t* deg, const uint32_t* rank1,
+                                                        const uint32_t* rank2, uint32_t* done,
+                                                        unsigned long long* deltas, uint32_t* flags,
+                                                        uint32_t spin_limit) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const uint64_t total = (uint64_t)iters * nc;
+  for (uint64_t item = wave; item < total; item += nwaves) {
+    const uint32_t q = (uint32_t)(item / nc);
+    const uint32_t s = __builtin_amdgcn_readfirstlane((uint32_t)(item - (uint64_t)q * nc));
+    const uint32_t e1 = C.b1[s], e2 = C.b2[s];
+    const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
+    const bool f1 = (e1 & kFixedBit) != 0, f2 = (e2 & kFixedBit) != 0;
+    if ((!f1 && !wait_count(done + i1, q * deg[i1] + rank1[s], flags, spin_limit)) ||
+        (!f2 && !wait_count(done + i2, q * deg[i2] + rank2[s], flags, spin_limit))) {
+      // give up and raise the flag, so that the wavefronts waiting for this one stop as well
+      if (lane == 0) __hip_atomic_fetch_or(flags, ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    const double d = relax_contact<MODEL, Trig, SPEC, true>(C, s, sb, dvw, dtinv, maxSub);
+    if (lane == 0) {
+      if (d > 0) atomicMax(&deltas[q], (unsigned long long)__double_as_longlong(d));
+      if (!f1) __hip_atomic_fetch_add(done + i1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (!f2) __hip_atomic_fetch_add(done + i2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // Lanes per contact.  A wavefront issues one instruction every four cycles whatever its lanes
 // do, so a contact's relaxation costs its instruction count in either layout; the speculated
 // search (SPEC > 0) trades 20 sequential sin/cos evaluations per pass for 4 rounds plus tree
@@ -224,6 +302,43 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
         hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
       break;
   }
+}
+
+template <class Trig, int SPEC>
+static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, uint32_t nc, uint32_t iters,
+                                           const Bodies& B, double dtinv, int maxSub, const DataflowBuffers& D,
+                                           uint32_t ncu, hipStream_t s) {
+  void* args_fn = nullptr;
+  switch (model) {
+    case 0: args_fn = (void*)k_sweep_dataflow<0, Trig, 0>; break;
+    case 1: args_fn = (void*)k_sweep_dataflow<1, Trig, 0>; break;
+    default: args_fn = (void*)k_sweep_dataflow<3, Trig, SPEC>; break;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)args_fn, 256, 0) != hipSuccess || per_cu < 1)
+    return SWEEP_LAUNCH_FAILED;
+  const uint64_t want = ((uint64_t)iters * nc * 64 + 255) / 256;  // one wavefront per item at most
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * ncu, std::max<uint64_t>(want, 1));
+  SolverContacts Cc = C;
+  const double* sb = B.sb;
+  double* dvw = B.dvw;
+  uint32_t spin = D.spin_limit;
+  void* args[] = {&Cc, &nc, &iters, &sb, &dvw, &dtinv, &maxSub, (void*)&D.deg, (void*)&D.rank1, (void*)&D.rank2,
+                  (void*)&D.done, (void*)&D.deltas, (void*)&D.flags, &spin};
+  if (hipLaunchCooperativeKernel(args_fn, dim3(grid), dim3(256), args, 0, s) != hipSuccess) return SWEEP_LAUNCH_FAILED;
+  return SWEEP_OK;
+}
+
+pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
+                                         uint32_t iters, const Bodies& B, double dtinv, int maxSub,
+                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s) {
+  if (!nc || !iters) return SWEEP_OK;
+  hipLaunchKernelGGL(k_inv_perm, dim3((nc + 255) / 256), dim3(256), 0, s, D.perm, nc, D.inv);
+  hipLaunchKernelGGL(k_body_ranks, dim3((B.n + 255) / 256), dim3(256), 0, s, C, B.fixed, B.n, D.adj_start, D.adj,
+                     D.inv, D.rank1, D.rank2);
+  fill_u32(D.done, 0u, B.n, s);
+  if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
+  return dataflow_dispatch<TrigFast, 0>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,

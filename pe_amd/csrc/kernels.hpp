@@ -36,7 +36,8 @@ enum : uint32_t {
   ERR_LEVEL_OVERFLOW = 2u,
   ERR_CAPACITY = 4u,
   ERR_COLOURS = 8u,
-  ERR_INTERNAL = 16u
+  ERR_INTERNAL = 16u,
+  ERR_TIMEOUT = 32u
 };
 
 struct BroadphaseBuffers {
@@ -107,6 +108,26 @@ void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, u
                   SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen,
                   hipStream_t s);
 // One colour of one sweep.
+// Dataflow sweeps: all K sweeps in one cooperative launch, every (sweep, contact) waiting only
+// for the earlier updates of its own bodies.
+struct DataflowBuffers {
+  const uint32_t* perm;       // solver slot -> raw contact
+  const uint32_t* adj_start;  // body -> contacts (raw), from the colouring
+  const uint32_t* adj;
+  const uint32_t* deg;        // contacts per non-fixed body
+  uint32_t* inv;              // raw contact -> solver slot
+  uint32_t* rank1;            // per slot: position among its body 1's slots
+  uint32_t* rank2;
+  uint32_t* done;             // per body: updates done
+  unsigned long long* deltas; // per sweep: L-inf impulse change
+  uint32_t* flags;            // ERR_TIMEOUT
+  uint32_t spin_limit;
+};
+enum pehip_sweep_status { SWEEP_OK = 0, SWEEP_LAUNCH_FAILED = 1 };
+pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
+                                         uint32_t iters, const Bodies& B, double dtinv, int maxSub,
+                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s);
+
 // ncu: compute units of the device (sizes the lanes per contact).
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
                   const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s);

@@ -88,6 +88,7 @@ class World:
         self._max_sub = 20
         self._sweep_mode = _abi.SWEEP_COLOURED
         self._exact_trig = 0
+        self._schedule = 0
         self._next_sid = 1
         self._shape, self._sid, self._mat, self._fixed = [], [], [], []
         self._param, self._pos, self._q, self._v, self._w = [], [], [], [], []
@@ -144,6 +145,13 @@ class World:
 
     def setSweepMode(self, mode):
         self._sweep_mode = int(mode)
+        self._params_dirty = True
+
+    def setSweepSchedule(self, schedule):
+        """Coloured mode: 0 auto, 1 one launch per colour, 2 dataflow (pehip_sweep_schedule)."""
+        if schedule not in (0, 1, 2):
+            raise ValueError("Invalid sweep schedule.")
+        self._schedule = int(schedule)
         self._params_dirty = True
 
     def setExactTrig(self, on=True):
@@ -321,6 +329,7 @@ class World:
         p.max_sub_iterations = self._max_sub
         p.sweep_mode = self._sweep_mode
         p.exact_trig = self._exact_trig
+        p.sweep_schedule = self._schedule
         return p
 
     def _ensure_device(self):

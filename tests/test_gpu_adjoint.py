@@ -83,7 +83,7 @@ def test_adjoint_matches_central_differences(model):
     st = run.forward(v0, mu)
     assert np.array_equal(st["pos"], base["pos"])            # the checkpointed forward is the plain run
     L, gpos = loss_and_grad_pos(st, sc, target)
-    _, gv, _, gmu = run.backward(gpos)
+    gx, gv, _, gmu = run.backward(gpos)
     assert sum(len(s) for s in base_sets) > 100
     rng = np.random.default_rng(11)
     coords = [(int(i), int(k)) for i, k in zip(rng.integers(0, len(v0), 8), rng.integers(0, 3, 8))]
@@ -104,8 +104,23 @@ def test_adjoint_matches_central_differences(model):
             continue                        # a contact or a contact's mode changed: FD not comparable
         g_fd = (fd[0] - fd[1]) / (2 * H)
         g = gmu if i == "mu" else gv[1 + i, k]
-        scale = max(abs(g_fd), 1e-3 * np.abs(gv).max())
+        # relative to the coordinate's value or, for a coordinate much smaller than the others,
+        # to the gradient's max-norm: central differences with h = 1e-6 of a loss of size ~0.1
+        # carry ~1e-10 of rounding noise, above 1e-5 of the smallest components
+        scale = abs(gmu) if i == "mu" else max(abs(g_fd), np.abs(gv[1:]).max())
         rows.append((i, k, g, g_fd, abs(g - g_fd) / scale))
+    for (i, k) in coords[:4]:                                 # initial positions
+        fd = []
+        for sgn in (1, -1):
+            sc2, _ = block_scene(model)
+            sc2.sphere_pos = sc2.sphere_pos.copy()
+            sc2.sphere_pos[i, k] += sgn * H
+            s, sets = run_plain(sc2, v0, mu, model)
+            fd.append(loss_and_grad_pos(s, sc, target)[0] if sets == base_sets else np.nan)
+        if np.isnan(fd).any():
+            continue
+        g_fd = (fd[0] - fd[1]) / (2 * H)
+        rows.append(("x%d" % i, k, gx[1 + i, k], g_fd, abs(gx[1 + i, k] - g_fd) / max(abs(g_fd), np.abs(gx[1:]).max())))
     print("\n".join("%s %s adjoint %.12e fd %.12e rel %.2e" % r for r in rows))
     print("adjoint stats", run.last_stats)
     assert len(rows) >= 6

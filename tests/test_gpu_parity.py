@@ -337,6 +337,26 @@ def test_coloured_step_equals_gs_in_colour_order(name):
         assert rel_err(g1[k], o1[k]) <= 1e-12, k
 
 
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_dataflow_sweeps_equal_colour_launches(name, exact):
+    # All sweeps in one cooperative launch, each contact waiting only for its own bodies' earlier
+    # updates: the same per-body update order as the colour launches, so the same bits.
+    sc = SCENES[name]()
+    _, st0 = oracle_state_after(sc, 10)
+    out = []
+    for schedule in (1, 2):
+        w = gpu_for(sc, iters=30, exact_trig=exact)
+        w.setSweepSchedule(schedule)
+        w.setState(st0)
+        w.run(3, sc.dt)
+        out.append((w.getState(), w.impulses()))
+        w.close()
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_colouring_is_conflict_free(name):
     sc = SCENES[name]()
