@@ -523,11 +523,12 @@ __device__ __forceinline__ void golden_step(GoldenState& S, double an, double x0
 
 // Bracket of the tree node `code` (heap index + 1, `depth` levels below the root): the bracket
 // updates of golden_step along the node's path, written with selects so that the lanes of a
-// wavefront, each on its own path, do not diverge.
+// wavefront, each on its own path, do not diverge.  LEVELS bounds the depth of the nodes used.
+template <int LEVELS>
 __device__ __forceinline__ void golden_path(uint32_t code, int depth, double& al, double& ar, double& am,
                                             bool& left) {
 #pragma unroll
-  for (int lev = 0; lev < 6; ++lev) {
+  for (int lev = 0; lev < LEVELS; ++lev) {
     const bool act = lev < depth;
     const bool lt = act && ((code >> (act ? depth - 1 - lev : 0)) & 1u) != 0;
     const double a = al + (ar - am);
@@ -550,13 +551,30 @@ __device__ __forceinline__ double wave_read_lane(double x, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Value of x in lane `node` of the caller's group of 2^SPEC lanes (node uniform in the group;
+// SPEC 6: the whole wavefront, read by the scalar unit).
+template <int SPEC>
+__device__ __forceinline__ double group_read(double x, uint32_t node) {
+  if constexpr (SPEC == 6) return wave_read_lane(x, (int)node);
+  else return __shfl(x, (int)node, 1 << SPEC);
+}
+template <int SPEC>
+__device__ __forceinline__ bool group_read(bool x, uint32_t node) {
+  if constexpr (SPEC == 6) return __builtin_amdgcn_readlane(x ? 1 : 0, (int)node) != 0;
+  else return __shfl(x ? 1 : 0, (int)node, 1 << SPEC) != 0;
+}
+
 // maxSub iterations.  SPEC = 0: one lane, as pe.  SPEC = d > 0: the 2^d lanes of a group run d
 // iterations per round -- lane j evaluates node j of the depth-d tree of possible next points
 // (heap order; the path to a node, read from the bits of j + 1 below the leading one, is a
 // sequence of comparison outcomes, and the bracket along it depends only on the points, not on
-// the objective values), then the group follows the actual comparisons through the tree
-// (SPEC 6: the whole wavefront).  Every
-// evaluated point and every comparison is the one the sequential search makes, bit for bit.
+// the objective values).  The comparisons are then resolved for all nodes at once: the best
+// value the sequential search holds on reaching node j is the running minimum (by the same
+// strict comparison) of the objective values at j's ancestors, so every lane compares its own
+// value against that fold of its ancestors' values, one ballot collects the outcomes, the
+// group walks the tree along them, and the bracket after the round is one golden_step from the
+// state at the last node on the walk.  Every evaluated point and every comparison is the one
+// the sequential search makes, bit for bit.
 template <class Trig, int SPEC>
 __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, double flimit, double A00, double A01,
                                                double A10, double A11, double gt, double go) {
@@ -569,71 +587,57 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
       golden_step(S, an, xn0, xn1, fn, fn < S.fm);
     }
-  } else if constexpr (SPEC == 6) {
-    // A whole wavefront per contact: the path through the tree is wave-uniform, so the group
-    // follows it with scalar reads of single lanes instead of cross-lane shuffles.
-    const uint32_t code = (threadIdx.x & 63) + 1;
-    const int depth = 31 - __clz(code);
-    for (int k = 0; k < maxSub; k += SPEC) {
-      const int levels = maxSub - k < SPEC ? maxSub - k : SPEC;
-      double al = S.al, ar = S.ar, am = S.am;
-      bool left = S.left;
-      golden_path(code, depth, al, ar, am, left);
-      const double an = al + (ar - am);
-      double sn, cn;
-      Trig::sincos(an, sn, cn);
-      const double xn0 = flimit * cn, xn1 = flimit * sn;
-      const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
-      // follow the comparisons (only the objective values cross lanes on this path) ...
-      int node = 0, last = -1;
-      uint32_t bits = 0;
-      double fm = S.fm;
-      for (int d = 0; d < levels; ++d) {
-        const double f = wave_read_lane(fn, node);
-        const bool lt = __builtin_amdgcn_readfirstlane(f < fm ? 1 : 0) != 0;
-        if (lt) { fm = f; last = node; }
-        bits = (bits << 1) | (lt ? 1u : 0u);
-        node = 2 * node + (lt ? 2 : 1);
-      }
-      // ... then advance the bracket along the taken path and take the last improving point
-      for (int d = levels - 1; d >= 0; --d) {
-        const bool lt = ((bits >> d) & 1u) != 0;
-        const double a = S.al + (S.ar - S.am);
-        if (S.left) {
-          if (lt) { S.ar = S.am; S.am = a; }
-          else { S.al = a; S.left = false; }
-        } else {
-          if (lt) { S.al = S.am; S.am = a; }
-          else { S.ar = a; S.left = true; }
-        }
-      }
-      if (last >= 0) {
-        S.fm = fm;
-        S.xm0 = wave_read_lane(xn0, last);
-        S.xm1 = wave_read_lane(xn1, last);
-      }
-    }
   } else {
     constexpr int L = 1 << SPEC;
-    const uint32_t code = (threadIdx.x & (L - 1)) + 1;
-    const int depth = 31 - __clz(code);
+    const uint32_t lane = threadIdx.x & (L - 1);
+    const uint32_t code = lane + 1;
+    const int depth = 31 - __clz(code);   // nodes 0 .. L-2 are used: depth <= SPEC - 1
+    const uint32_t base = (threadIdx.x & 63u) & ~(uint32_t)(L - 1);
     for (int k = 0; k < maxSub; k += SPEC) {
       const int levels = maxSub - k < SPEC ? maxSub - k : SPEC;
       double al = S.al, ar = S.ar, am = S.am;
       bool left = S.left;
-      golden_path(code, depth, al, ar, am, left);
+      golden_path<SPEC - 1>(code, depth, al, ar, am, left);
       const double an = al + (ar - am);
       double sn, cn;
       Trig::sincos(an, sn, cn);
       const double xn0 = flimit * cn, xn1 = flimit * sn;
       const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
-      int node = 0;
+      // best value held on reaching this node: fold of the ancestors' values, root first
+      double m = S.fm;
+#pragma unroll
+      for (int l = 0; l < SPEC - 1; ++l) {
+        const bool act = l < depth;
+        const uint32_t src = act ? (code >> (depth - l)) - 1u : lane;
+        const double fa = __shfl(fn, (int)src, L);
+        m = (act && fa < m) ? fa : m;
+      }
+      const unsigned long long all = __ballot(fn < m);
+      const uint64_t bits = SPEC == 6 ? all : (all >> base) & ((1ull << L) - 1ull);
+      // walk the taken path: fin = the last node compared, last = the last improving one
+      uint32_t node = 0, fin = 0;
+      int last = -1;
+      bool ltf = false;
       for (int d = 0; d < levels; ++d) {
-        const double a = __shfl(an, node, L), x0 = __shfl(xn0, node, L), x1 = __shfl(xn1, node, L);
-        const double f = __shfl(fn, node, L);
-        const bool lt = f < S.fm;
-        golden_step(S, a, x0, x1, f, lt);
-        node = 2 * node + (lt ? 2 : 1);
+        fin = node;
+        ltf = ((bits >> node) & 1ull) != 0;
+        if (ltf) last = (int)node;
+        node = 2 * node + (ltf ? 2u : 1u);
+      }
+      const double fal = group_read<SPEC>(al, fin), far = group_read<SPEC>(ar, fin);
+      const double fam = group_read<SPEC>(am, fin), fa = group_read<SPEC>(an, fin);
+      const bool fleft = group_read<SPEC>(left, fin);
+      if (fleft) {
+        if (ltf) { S.al = fal; S.ar = fam; S.am = fa; S.left = true; }
+        else { S.al = fa; S.ar = far; S.am = fam; S.left = false; }
+      } else {
+        if (ltf) { S.al = fam; S.ar = far; S.am = fa; S.left = false; }
+        else { S.al = fal; S.ar = fa; S.am = fam; S.left = true; }
+      }
+      if (last >= 0) {
+        S.fm = group_read<SPEC>(fn, (uint32_t)last);
+        S.xm0 = group_read<SPEC>(xn0, (uint32_t)last);
+        S.xm1 = group_read<SPEC>(xn1, (uint32_t)last);
       }
     }
   }

@@ -262,12 +262,12 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
 }
 
 // Lanes per contact.  A wavefront issues one instruction every four cycles whatever its lanes
-// do, so a contact's relaxation costs its instruction count in either layout; the speculated
-// search (SPEC > 0) trades 20 sequential sin/cos evaluations per pass for 4 rounds plus tree
-// bookkeeping.  That pays with the correctly rounded trig (~6,000 cycles per sincos, measured
-// 2.4x faster per sliding contact) and not with the device libm (~600 cycles: 1.2x slower), so
-// the fast trig always runs one lane per contact and the exact trig the largest group (64, 16, 4)
-// with which every wavefront of the colour is resident at once (occupancy queried once).
+// do, so a lone contact's relaxation costs its instruction count; the speculated search with a
+// whole wavefront per contact (SPEC 6) cuts the sequential sin/cos evaluations of a pass from 20
+// to 4 rounds and measures 1.2x (device libm) to 2.8x (correctly rounded trig) faster per sliding
+// contact than one lane per contact.  Colours are small in the latency-bound regime, so the
+// largest group (64, 16, 4, 1) with which every wavefront of the colour is resident at once wins;
+// big colours fall back to more contacts per wavefront (occupancy queried once).
 template <class Trig, int SPEC>
 static uint64_t resident_lanes(uint32_t ncu) {
   static int blocks = -1;
@@ -290,9 +290,7 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
     case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
     case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
     default:
-      if (!Trig::kCorrectlyRounded)
-        hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
-      else if ((n << 6) <= resident_lanes<Trig, 6>(ncu))
+      if ((n << 6) <= resident_lanes<Trig, 6>(ncu))
         hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
       else if ((n << 4) <= resident_lanes<Trig, 4>(ncu))
         hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
@@ -337,8 +335,9 @@ pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const Solve
   hipLaunchKernelGGL(k_body_ranks, dim3((B.n + 255) / 256), dim3(256), 0, s, C, B.fixed, B.n, D.adj_start, D.adj,
                      D.inv, D.rank1, D.rank2);
   fill_u32(D.done, 0u, B.n, s);
+  // a whole wavefront owns each item, so the speculated search costs no parallelism here
   if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
-  return dataflow_dispatch<TrigFast, 0>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
+  return dataflow_dispatch<TrigFast, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
