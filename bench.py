@@ -23,7 +23,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time steps/s and contact-iterations/s at 200k spheres FP64; batched scenes/s"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_c2_traffic.json")
+# HBM bytes per launch of the sweep kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+# bench (scaled by the counters' reading for a copy kernel of known size)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_dataflow_c2_traffic.json")
+
+
+def progress(msg):
+    """One line on stderr per finished phase (the JSON result alone goes to stdout)."""
+    print("bench.py: " + msg, file=sys.stderr, flush=True)
 
 
 def cpu_step_time(sc, state, seconds, min_steps=1):
@@ -52,6 +59,7 @@ def cpu_baseline(sc, states, steps, seconds):
         t, n = cpu_step_time(sc, st, per)
         ts.append(t)
         ns.append(n)
+        progress("cpu baseline: %d oracle steps, %.3f s/step" % (n, t))
     total = steps * (ts[0] + 2 * ts[1] + ts[2]) / 4.0
     return {"value": steps / total, "unit": "steps/s", "cores": 1, "kind": "port",
             "sample": "oracle/pe_oracle.cpp (g++ -O3, no FMA, 1 thread) timed for %s steps from the GPU states at "
@@ -81,6 +89,7 @@ def batched_c4(torch):
     bw.run(150)
     torch.cuda.synchronize()
     tb = (t1 - t0) + (time.perf_counter() - t2)
+    progress("batched: 4096 scenes x 300 steps in %.2f s" % tb)
     st = bw.stats()
     bw.close()
     rng = np.random.default_rng(0)
@@ -145,6 +154,7 @@ def main():
 
     import torch
     from pe_amd import World, scenes
+    from pe_amd._abi import SCHEDULE_DATAFLOW
 
     sc = scenes.config2(args.spheres) if args.workload == "c2" else scenes.config3()
     if args.copy_calibration:
@@ -167,6 +177,7 @@ def main():
         ww.run(args.warmup, sc.dt)
         torch.cuda.synchronize()
         ww.close()
+        progress("%d warmup steps done" % args.warmup)
     w = make_world()
     if args.load_state:
         w.setState(load_state(args.load_state))
@@ -190,6 +201,7 @@ def main():
         t += time.perf_counter() - t0
         if state_mid is None:
             state_mid = w.getState()           # between the two timed halves
+        progress("%d of %d timed steps done" % (len(contacts), args.steps))
     st = w.stats()
     state_end = w.getState()
     if args.save_state:
@@ -218,13 +230,18 @@ def main():
         avg_ms = st["prof_sweep_kernel_ms"] / st["prof_sweep_launches"]
         bytes_per_launch = st["prof_sweep_bytes"] / st["prof_sweep_launches"]
         achieved = st["prof_sweep_bytes"] / (st["prof_sweep_kernel_ms"] * 1e-3) / 1e9
+        kernel = ("k_sweep_dataflow<3, TrigFast, 6>" if st["schedule_used"] == SCHEDULE_DATAFLOW
+                  else "k_sweep<3> (one colour of one NBGS sweep)")
         traffic = None
-        if os.path.exists(TRAFFIC_FILE):
+        if os.path.exists(TRAFFIC_FILE) and args.workload == "c2" and not args.load_state:
             with open(TRAFFIC_FILE) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tf = json.load(f)
+            if tf.get("kernel") == kernel:
+                traffic = tf.get("hbm_bytes_per_launch")
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                           "kernel": "k_sweep<3> (one colour of one NBGS sweep)",
+                           "kernel": kernel + (" (all %d NBGS sweeps of one step)" % sc.max_iterations
+                                               if st["schedule_used"] == SCHEDULE_DATAFLOW else ""),
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
                            "launches": int(st["prof_sweep_launches"])}
     if not args.no_batched and args.workload == "c2" and not args.load_state:

@@ -186,6 +186,9 @@ typedef struct {
   uint64_t prof_sweep_launches;
   double prof_sweep_kernel_ms;
   double prof_sweep_bytes;
+  int32_t schedule_used;      /* schedule of the last step's sweeps: PEHIP_SCHEDULE_COLOUR_LAUNCHES
+                                 or PEHIP_SCHEDULE_DATAFLOW; -1 for reference order or no sweeps */
+  int32_t reserved;
 } pehip_stats;
 pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);
 

@@ -51,7 +51,8 @@ class Stats(ctypes.Structure):
                 ("t_setup_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_integrate_ms", ctypes.c_double), ("prof_steps", ctypes.c_uint64),
                 ("prof_sweep_launches", ctypes.c_uint64), ("prof_sweep_kernel_ms", ctypes.c_double),
-                ("prof_sweep_bytes", ctypes.c_double)]
+                ("prof_sweep_bytes", ctypes.c_double), ("schedule_used", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class BatchTuning(ctypes.Structure):

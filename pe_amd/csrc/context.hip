@@ -503,6 +503,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
   launch_predict(B, g, dt, s);
   const uint32_t nc = ctx->nc;
   uint32_t iters = 0;
+  int32_t schedule_used = -1;
   double last_delta = 0;
   fill_u64(ctx->maxpen, 0ull, 2, s);
   if (nc) {
@@ -534,6 +535,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       D.inv = ctx->df_inv; D.rank1 = ctx->df_rank1; D.rank2 = ctx->df_rank2; D.done = ctx->df_done;
       D.deltas = ctx->deltas; D.flags = ctx->flags; D.spin_limit = kDataflowSpinLimit;
       fill_u32(ctx->flags, 0u, 1, s);
+      launch_dataflow_prepare(ctx->sol, nc, B, D, s);
       if (ctx->profiling) CKS(record_launch_event(ctx, 0));
       if (launch_sweep_dataflow(P.relaxation_model, exact, ctx->sol, nc, P.max_iterations, B, dtinv,
                                 (int)P.max_sub_iterations, D, ctx->ncu, s) != SWEEP_OK)
@@ -544,10 +546,12 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       CKS(sync_check(ctx));
       if (ctx->hstage[0] & ERR_TIMEOUT) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
       iters = P.max_iterations;
+      schedule_used = PEHIP_SCHEDULE_DATAFLOW;
       ++ctx->dataflow_steps;
     } else {
       const uint32_t ncol = (uint32_t)ctx->colour_off.size() - 1;
       size_t nev = 0;
+      schedule_used = PEHIP_SCHEDULE_COLOUR_LAUNCHES;
       for (uint32_t it = 0; it < P.max_iterations; ++it) {
         for (uint32_t k = 0; k < ncol; ++k) {
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
@@ -580,6 +584,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
   std::memcpy(&nfx, ctx->hstage + 2, 8);
   if (iters) std::memcpy(&last_delta, ctx->hstage + 4, 8);
   ctx->stats.num_contacts = nc;
+  ctx->stats.schedule_used = schedule_used;
   ctx->stats.max_penetration = mp;
   ctx->stats.iterations = iters;
   ctx->stats.last_delta = last_delta;

@@ -327,14 +327,19 @@ static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, 
   return SWEEP_OK;
 }
 
-pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
-                                         uint32_t iters, const Bodies& B, double dtinv, int maxSub,
-                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s) {
-  if (!nc || !iters) return SWEEP_OK;
+void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies& B, const DataflowBuffers& D,
+                             hipStream_t s) {
+  if (!nc) return;
   hipLaunchKernelGGL(k_inv_perm, dim3((nc + 255) / 256), dim3(256), 0, s, D.perm, nc, D.inv);
   hipLaunchKernelGGL(k_body_ranks, dim3((B.n + 255) / 256), dim3(256), 0, s, C, B.fixed, B.n, D.adj_start, D.adj,
                      D.inv, D.rank1, D.rank2);
   fill_u32(D.done, 0u, B.n, s);
+}
+
+pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
+                                         uint32_t iters, const Bodies& B, double dtinv, int maxSub,
+                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s) {
+  if (!nc || !iters) return SWEEP_OK;
   // a whole wavefront owns each item, so the speculated search costs no parallelism here
   if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
   return dataflow_dispatch<TrigFast, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);

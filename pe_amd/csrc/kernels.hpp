@@ -107,7 +107,6 @@ void launch_colour_hist(uint32_t nc, ColourBuffers& C, hipStream_t s);
 void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, uint32_t nc,
                   SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen,
                   hipStream_t s);
-// One colour of one sweep.
 // Dataflow sweeps: all K sweeps in one cooperative launch, every (sweep, contact) waiting only
 // for the earlier updates of its own bodies.
 struct DataflowBuffers {
@@ -124,11 +123,15 @@ struct DataflowBuffers {
   uint32_t spin_limit;
 };
 enum pehip_sweep_status { SWEEP_OK = 0, SWEEP_LAUNCH_FAILED = 1 };
+// The per-body ranks and cleared counters the dataflow sweeps wait on (three small kernels) ...
+void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies& B, const DataflowBuffers& D,
+                             hipStream_t s);
+// ... then all `iters` sweeps in one cooperative launch.
 pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
                                          uint32_t iters, const Bodies& B, double dtinv, int maxSub,
                                          const DataflowBuffers& D, uint32_t ncu, hipStream_t s);
 
-// ncu: compute units of the device (sizes the lanes per contact).
+// One colour of one sweep; ncu: compute units of the device (sizes the lanes per contact).
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
                   const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s);
 // All sweeps, strictly sequential, one wavefront.  deltas[it] receives the L-inf impulse

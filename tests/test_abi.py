@@ -89,3 +89,30 @@ def test_mat3_inverse_host():
     I[0], I[4], I[8] = 2.0, 3.0, 5.0
     inv = mat3_inverse(I)
     assert np.allclose(inv[[0, 4, 8]], [0.5, 1 / 3, 0.2], rtol=1e-15)
+
+
+def test_ctypes_structs_match_header_layout(tmp_path):
+    # The ctypes mirrors in pe_amd/_abi.py have the size and field offsets that a C compiler
+    # gives the structs of include/pe_hip.h (gcc on the header alone; no GPU, no library).
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    mirrors = {"pehip_params": _abi.Params, "pehip_stats": _abi.Stats, "pehip_batch_tuning": _abi.BatchTuning,
+               "pehip_batch_stats": _abi.BatchStats, "pehip_adjoint_stats": _abi.AdjointStats}
+    lines = ['#include <stdio.h>', '#include "pe_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for cname, py in mirrors.items():
+        assert int(got[cname + " size"]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
