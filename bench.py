@@ -217,8 +217,10 @@ def main():
         "config": {"workload": (("polydisperse granular pile, %d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, "
                                  "v0~U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), "
                                  "dt 5e-4" % args.spheres) if args.workload == "c2" else
-                                ("mixed shapes, 950k spheres r~U[0.2,0.4] + 50k boxes edges U[0.3,0.8], V-channel "
-                                 "hopper (planes at +-30 deg, 0.8 m gap), mu 0.6, 30 NBGS sweeps, dt 2e-4")) +
+                                ("mixed shapes, 950k spheres r~U[0.2,0.4] + 50k boxes edges U[0.3,0.8] (Shoemake "
+                                 "orientations) filling a V-channel hopper (planes at +-30 deg, 0.8 m gap) on a "
+                                 "0.85 m jittered lattice, poured at v0 = (0,0,-10) + U[-1,1]^3 m/s, mu 0.6, 30 NBGS sweeps, "
+                                 "dt 2e-4")) +
                                ", whole job: steps 0..%d%s" % (args.steps, " from a saved state" if args.load_state else ""),
                    "bodies": int(w.size), "sweeps_per_step": sc.max_iterations, "dt": sc.dt,
                    "parallelism": "single GPU"},

@@ -206,36 +206,70 @@ def shoemake_quaternions(rng, n):
 
 
 def config3(n_spheres=950_000, n_boxes=50_000):
-    """Mixed shapes in a V-channel hopper: spheres r ~ U[0.2,0.4] and boxes with edges
-    U[0.3,0.8]^3 in Shoemake-uniform orientations, rained from a jittered lattice (spacing 1.2,
-    boxes at evenly spread sites, bottom layer at z = 33) above two planes inclined at +-30 deg whose lines meet 0.4 m
-    either side of the channel axis below the floor level (a 0.8 m gap at z = 0), plus the floor
-    and four side walls; mu = 0.6, 30 sweeps, dt = 2e-4, 1,000 steps.  pe planes are half-spaces,
-    so the outlet is the V-channel's 0.8 m gap at the floor rather than an opening."""
+    """Mixed shapes filling a V-channel hopper: spheres r ~ U[0.2,0.4] and boxes with edges
+    U[0.3,0.8]^3 in Shoemake-uniform orientations, on a jittered lattice (spacing 0.85, jitter
+    0.02) that fills the hopper from the bottom and is poured into it: v0 = (0, 0, -10) +
+    U[-1,1]^3 m/s.  The job lasts 0.2 s; a fill at rest or stirred only forms a dilute gas of
+    brief contacts in that time (under 2,000 contacts at once), while the pour packs a layer
+    several bodies deep against the inclined planes and the floor; mu = 0.6, 30 sweeps,
+    dt = 2e-4, 1,000 steps.  The hopper: two planes inclined at +-30 deg whose lines meet 0.4 m
+    either side of the channel axis at z = 0 (a 0.8 m outlet gap at the floor; pe planes are
+    half-spaces, so the outlet is the gap rather than an opening), the floor, and four side walls
+    of a 120 x 120 m footprint.  Boxes sit on the even sites of the lattice (every other site
+    along each axis, spread evenly over the fill) with their six face neighbours left empty, so
+    no two bodies overlap at the start: sphere-sphere face neighbours are >= 0.81 m apart
+    (radii sum <= 0.8), a box's nearest spheres (edge neighbours) >= 1.14 m (extents sum
+    <= 0.693 + 0.4) and two boxes >= 1.58 m (<= 1.386); sites keep 0.43 m (spheres) or 0.73 m
+    (boxes) from every plane."""
     rng = np.random.default_rng(12345 + 3)
-    n = n_spheres + n_boxes
-    L = 120.0
-    per = 90
-    nz = (n + per * per - 1) // (per * per)
-    pos = lattice(per, per, nz, 1.2, (6.0, 6.0, 33.0), 0.05, rng, count=n)
-    is_box = np.zeros(n, bool)
-    is_box[np.linspace(0, n - 1, n_boxes).astype(int)] = True
+    L, sp, jit = 120.0, 0.85, 0.02
     c30, s30 = math.cos(math.radians(30.0)), math.sin(math.radians(30.0))
-    # inward normals of the two inclined planes: the channel axis is x = L/2, gap 0.8 at z = 0
     cx = L / 2
     n_left = (s30, 0.0, c30)        # plane through (cx - 0.4, 0) rising to the left
     n_right = (-s30, 0.0, c30)
     planes = [((0, 0, 1), 0.0, 0),
               (n_left, s30 * (cx - 0.4), 0), (n_right, -s30 * (cx + 0.4), 0),
               ((0, 1, 0), 0.0, 0), ((0, -1, 0), -L, 0), ((1, 0, 0), 0.0, 0), ((-1, 0, 0), -L, 0)]
-    sph = pos[~is_box]
-    box = pos[is_box]
-    return Scene("C3 %d spheres + %d boxes" % (n_spheres, n_boxes),
+    nx = int(L / sp)
+    nz = 1
+    while True:                       # enough layers for the bodies after the boxes' clearances
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(nx), np.arange(nx), indexing="ij")
+        c = np.stack([sp / 2 + sp * i, sp / 2 + sp * j, sp / 2 + sp * k], axis=-1).reshape(-1, 3)
+        dmin = np.full(len(c), np.inf)
+        for (nrm, d, _) in planes:
+            dmin = np.minimum(dmin, c @ np.asarray(nrm, float) - d)
+        ii, jj, kk = i.reshape(-1), j.reshape(-1), k.reshape(-1)
+        cand = np.flatnonzero((dmin >= 0.73) & (ii % 2 == 0) & (jj % 2 == 0) & (kk % 2 == 0))
+        if len(cand) >= n_boxes and (dmin >= 0.43).sum() - 7 * n_boxes >= n_spheres:
+            break
+        nz += 8
+    top = nz - 1
+    for _ in range(4):                # spread the boxes over the layers the spheres fill
+        sel = cand[kk[cand] <= top]
+        if len(sel) < n_boxes:
+            sel = cand
+        is_box = np.zeros(len(c), bool)
+        is_box[sel[np.linspace(0, len(sel) - 1, n_boxes).astype(np.int64)]] = True
+        empty = np.zeros(len(c), bool)
+        bi, bj, bk = ii[is_box], jj[is_box], kk[is_box]
+        for di, dj, dk in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            ni, nj, nk = bi + di, bj + dj, bk + dk
+            ok = (ni >= 0) & (ni < nx) & (nj >= 0) & (nj < nx) & (nk >= 0) & (nk < nz)
+            empty[(nk[ok] * nx + nj[ok]) * nx + ni[ok]] = True
+        sph_sites = np.flatnonzero((dmin >= 0.43) & ~is_box & ~empty)[:n_spheres]
+        top = int(kk[sph_sites[-1]])
+    box_sites = np.flatnonzero(is_box)
+    sph = c[sph_sites] + rng.uniform(-jit, jit, size=(len(sph_sites), 3))
+    box = c[box_sites] + rng.uniform(-jit, jit, size=(len(box_sites), 3))
+    v = rng.uniform(-1.0, 1.0, size=(len(sph) + len(box), 3))
+    v[:, 2] -= 10.0
+    v0 = np.concatenate([np.zeros((len(planes), 3)), v])
+    return Scene("C3 %d spheres + %d boxes" % (len(sph), len(box)),
                  [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.3, "cdf": 0.3}],
                  planes, sph, rng.uniform(0.2, 0.4, size=len(sph)), np.zeros(len(sph), np.uint32),
                  box_pos=box, box_len=rng.uniform(0.3, 0.8, size=(len(box), 3)),
                  box_q=shoemake_quaternions(rng, len(box)), box_mat=np.zeros(len(box), np.uint32),
-                 dt=2e-4, steps=1000, max_iterations=30)
+                 v0=v0, dt=2e-4, steps=1000, max_iterations=30)
 
 
 def config4(n_scenes=4096):
