@@ -118,6 +118,139 @@ def batched_c4(torch):
                       "batch state at step 150 (%.3f ms/step mean), extrapolated" % (1e3 * float(np.mean(per)))}
 
 
+def step_signature(w):
+    """Hash of the last step's active set: every contact's body pair and mode (separating,
+    static inside the friction cone, sliding on it)."""
+    import hashlib
+    c = w.contacts()
+    p = w.impulses()
+    pn = np.einsum("ij,ij->i", p, c["normal"])
+    pt = np.linalg.norm(p - pn[:, None] * c["normal"], axis=1)
+    mode = np.where(np.all(p == 0, axis=1), 0, np.where(pt < c["mu"] * pn * (1 - 1e-9), 1, 2)).astype(np.uint8)
+    key = (np.asarray(c["b1"], np.uint64) << np.uint64(32)) | np.asarray(c["b2"], np.uint64)
+    i = np.argsort(key, kind="stable")
+    return hashlib.sha1(key[i].tobytes() + mode[i].tobytes()).hexdigest(), len(key)
+
+
+def adjoint_c5(args):
+    """Differentiable step at full size: 37^3 spheres dropped onto a floor, 400 steps, loss
+    |COM(T) - x*|^2; gradients w.r.t. every initial sphere velocity and mu from the reverse
+    pass (checkpoints every 20 steps), checked against central differences (h = 1e-6) on
+    seeded random coordinates whose perturbed runs keep every step's active set."""
+    import torch
+    from pe_amd import World, scenes
+    from pe_amd.adjoint import DifferentiableRun
+    sc = scenes.config5()
+    model, mu, H = 1, 0.5, 1e-6
+    iters, abort = args.c5_sweeps, args.c5_abort
+    npl = len(sc.planes)
+    n = sc.n_spheres
+    m = 1.0 / sc.arrays()["inv_mass"][npl:]
+    v0 = np.zeros((n, 3))
+
+    def plain(vv, mm):
+        w = World()
+        sc.to_world(w)
+        w.setMaxIterations(iters)
+        w.setRelaxationModel(model)
+        w.setAbortThreshold(abort)
+        w.setPairFriction(0, 0, mm, mm)
+        w.host_arrays()["v"][npl:] = vv
+        sig, ncon = [], []
+        for _ in range(sc.steps):
+            w.run(1, sc.dt)
+            h, k = step_signature(w)
+            sig.append(h)
+            ncon.append(k)
+        pos = w.getState()["pos"][npl:]
+        w.close()
+        return pos, sig, ncon
+
+    def loss(pos, target):
+        com = (m[:, None] * pos).sum(0) / m.sum()
+        d = com - target
+        return float(d @ d), d
+
+    t0 = time.perf_counter()
+    base, base_sig, ncon = plain(v0, mu)
+    t_plain = time.perf_counter() - t0
+    progress("c5 plain forward: %.1f s, max contacts %d, steps with contacts %d" %
+             (t_plain, max(ncon), sum(1 for k in ncon if k)))
+    target = (m[:, None] * base).sum(0) / m.sum() + np.array([0.3, -0.2, 0.1])
+    run = DifferentiableRun(sc, sc.steps, checkpoint_every=20, model=model, max_iterations=iters,
+                            abort_threshold=abort, adjoint_sweeps=args.c5_adjoint_sweeps,
+                            adjoint_tol=args.c5_adjoint_tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = run.forward(v0, mu)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    assert np.array_equal(st["pos"][npl:], base), "checkpointed forward differs from the plain run"
+    L, d = loss(st["pos"][npl:], target)
+    gpos = np.zeros((npl + n, 3))
+    gpos[npl:] = 2.0 * d[None, :] * (m / m.sum())[:, None]
+    t0 = time.perf_counter()
+    _, gv, _, gmu = run.backward(gpos)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    progress("c5 checkpointed forward %.1f s, reverse pass %.1f s" % (t_fwd, t_bwd))
+    stats = dict(run.last_stats)
+    run.close()
+    # coordinates: seeded random ones over all 3n initial velocities, plus some of spheres that
+    # touch the floor at the end (the random ones are mostly spheres still in free fall)
+    rng = np.random.default_rng(64)
+    coords = [("mu", None)] + [(int(f) // 3, int(f) % 3) for f in rng.choice(3 * n, args.fd_coords, replace=False)]
+    touching = np.flatnonzero(base[:, 2] < 0.4 + 1e-6)
+    if len(touching) and args.fd_contact_coords:
+        coords += [(int(i), int(k)) for i, k in zip(rng.choice(touching, args.fd_contact_coords, replace=False),
+                                                    rng.integers(0, 3, args.fd_contact_coords))]
+    if args.fd_range:
+        a, b = (int(x) for x in args.fd_range.split(":"))
+        coords = coords[a:b]
+    rows, skipped = [], 0
+    M = m.sum()
+    s_free = 2.0 * float(np.linalg.norm(d)) * sc.steps * sc.dt / n
+    for c in coords:
+        runs, same = [], True
+        for sgn in (1, -1):
+            if c[0] == "mu":
+                pos, sig, _ = plain(v0, mu + sgn * H)
+            else:
+                vp = v0.copy()
+                vp[c] += sgn * H
+                pos, sig, _ = plain(vp, mu)
+            same = same and sig == base_sig
+            runs.append(pos)
+        if not same:
+            skipped += 1
+            progress("c5 FD %s: active set changed, not comparable" % (c,))
+            continue
+        # L+ - L- = (d+ - d-) . (d+ + d-), with d+ - d- summed from per-sphere differences
+        # (most spheres' final positions are identical in both runs, so no cancellation)
+        dd = (m[:, None] * (runs[0] - runs[1])).sum(0) / M
+        ds = loss(runs[0], target)[1] + loss(runs[1], target)[1]
+        g_fd = float(dd @ ds) / (2 * H)
+        g = gmu if c[0] == "mu" else float(gv[npl + c[0], c[1]])
+        # relative to the larger of the two, floored (for derivatives that vanish: a sphere whose
+        # push is absorbed by the floor, mu for a block landing flat) at the derivative a single
+        # free-flying sphere has, 2 |d| T / n, or for mu at the loss itself
+        scale = max(abs(g_fd), abs(g), L if c[0] == "mu" else s_free)
+        rows.append({"coord": "mu" if c[0] == "mu" else "v0[%d,%d]" % c, "adjoint": g, "fd": g_fd,
+                     "rel": abs(g - g_fd) / scale if scale > 0 else 0.0,
+                     "touching": bool(c[0] != "mu" and c[0] in set(touching.tolist()))})
+        progress("c5 FD %s: adjoint %.12e fd %.12e rel %.2e" % (rows[-1]["coord"], g, g_fd, rows[-1]["rel"]))
+    return {"workload": "differentiable step: %d spheres r 0.4 dropped as a jittered 37^3 block onto a floor, mu 0.5, "
+                        "relaxation model 1 (ApproximateInelasticCoulombContactByDecoupling), %s, adjoint sweeps to "
+                        "%g, dt 5e-4, 400 steps, checkpoints every 20 steps; loss |COM(T) - x*|^2"
+                        % (n, ("up to %d sweeps per step to |dp| < %g" % (iters, abort)) if abort > 0 else
+                           ("%d sweeps per step" % iters), args.c5_adjoint_tol),
+            "loss": L, "forward_s": t_fwd, "reverse_s": t_bwd, "plain_forward_s": t_plain,
+            "max_contacts": int(max(ncon)), "adjoint_stats": stats, "grad_mu": gmu,
+            "grad_v0_maxabs": float(np.abs(gv[npl:]).max()), "fd_h": H, "fd_checked": len(rows),
+            "fd_not_comparable": skipped, "fd_scale_floor": s_free,
+            "fd_max_rel": max((r["rel"] for r in rows), default=None), "fd": rows}
+
+
 def save_state(path, st):
     np.savez(path, **st)
 
@@ -133,10 +266,21 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--spheres", type=int, default=200_000)
-    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
                     help="c2: the polydisperse pile (default); c3: 950k spheres + 50k boxes in a V-channel hopper, "
                          "30 sweeps, dt 2e-4 (use --steps 1000 for the whole job)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
+    ap.add_argument("--fd-coords", type=int, default=64, help="c5: random v0 coordinates checked by central differences")
+    ap.add_argument("--fd-contact-coords", type=int, default=16,
+                    help="c5: further coordinates of spheres that touch the floor at the end")
+    ap.add_argument("--fd-range", default=None,
+                    help="c5: START:STOP slice of the checked coordinates (mu, the random ones, the touching ones)")
+    ap.add_argument("--c5-sweeps", type=int, default=20000, help="c5: sweep cap per step")
+    ap.add_argument("--c5-abort", type=float, default=1e-15,
+                    help="c5: sweep threshold on |dp|; differences of runs stopped early carry the truncation "
+                         "error / 2h, so the solves run close to the floating-point fixed point")
+    ap.add_argument("--c5-adjoint-sweeps", type=int, default=20000, help="c5: adjoint sweep cap per step")
+    ap.add_argument("--c5-adjoint-tol", type=float, default=1e-15, help="c5: adjoint sweep threshold")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
@@ -155,6 +299,11 @@ def main():
     import torch
     from pe_amd import World, scenes
     from pe_amd._abi import SCHEDULE_DATAFLOW
+
+    if args.workload == "c5":
+        print(json.dumps({"metric": "adjoint of the time step: forward + reverse seconds, FD agreement",
+                          "n_gpus": 1, "dtype": "f64", "data": "synthetic", **adjoint_c5(args)}))
+        return
 
     sc = scenes.config2(args.spheres) if args.workload == "c2" else scenes.config3()
     if args.copy_calibration:

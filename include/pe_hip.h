@@ -79,13 +79,18 @@ typedef struct {
 } pehip_params;
 
 typedef enum {
-  /* dataflow when the step's contacts fit it (and abort_threshold <= 0), else colour launches */
+  /* abort_threshold <= 0: dataflow when the step's contacts fit it (65,536), else colour launches;
+   * abort_threshold > 0: one workgroup for up to 512 contacts of relaxation models 0 and 1,
+   * else colour launches */
   PEHIP_SCHEDULE_AUTO = 0,
   /* one kernel launch per colour and sweep */
   PEHIP_SCHEDULE_COLOUR_LAUNCHES = 1,
   /* all sweeps in one cooperative launch; each contact waits only for the earlier updates of its
-   * own bodies (same results as the colour launches, bit for bit) */
-  PEHIP_SCHEDULE_DATAFLOW = 2
+   * own bodies (same results as the colour launches, bit for bit); abort_threshold must be <= 0 */
+  PEHIP_SCHEDULE_DATAFLOW = 2,
+  /* all sweeps in one launch of one workgroup, a barrier between colours and the abort test after
+   * every sweep on the device (same results as the colour launches, bit for bit); for small steps */
+  PEHIP_SCHEDULE_SINGLE_BLOCK = 3
 } pehip_sweep_schedule;
 
 /* Fills p with pe's defaults (HCSITS.h:408-431, Settings.cpp:46-48). */
@@ -186,8 +191,8 @@ typedef struct {
   uint64_t prof_sweep_launches;
   double prof_sweep_kernel_ms;
   double prof_sweep_bytes;
-  int32_t schedule_used;      /* schedule of the last step's sweeps: PEHIP_SCHEDULE_COLOUR_LAUNCHES
-                                 or PEHIP_SCHEDULE_DATAFLOW; -1 for reference order or no sweeps */
+  int32_t schedule_used;      /* schedule of the last step's sweeps: PEHIP_SCHEDULE_COLOUR_LAUNCHES,
+                                 _DATAFLOW or _SINGLE_BLOCK; -1 for reference order or no sweeps */
   int32_t reserved;
 } pehip_stats;
 pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);

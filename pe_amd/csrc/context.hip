@@ -21,6 +21,13 @@ constexpr uint32_t kDataflowMaxContacts = 65536;
 // Polls (each with a short sleep) before a waiting wavefront gives up: seconds, far above any
 // legitimate wait.
 constexpr uint32_t kDataflowSpinLimit = 1u << 22;
+// With an abort threshold the host reads the device's stop flag once per this many sweeps.
+constexpr uint32_t kAbortPoll = 32;
+// Automatic schedule with an abort threshold: one workgroup runs all sweeps up to this many
+// contacts (relaxation models 0 and 1: one pass of its 512 lanes per colour), where a colour
+// launch is mostly launch latency.  Beyond it the workgroup needs several passes per colour and
+// the colour launches are faster.
+constexpr uint32_t kSingleBlockMaxContacts = 512;
 std::string g_create_error;
 
 uint32_t next_pow2(uint64_t x) {
@@ -121,7 +128,7 @@ pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
     return PEHIP_INVALID_ARG;
   }
   if (p->damping > 1 || p->damping < 0) { why = "Invalid damping."; return PEHIP_INVALID_ARG; }
-  if (p->sweep_schedule < PEHIP_SCHEDULE_AUTO || p->sweep_schedule > PEHIP_SCHEDULE_DATAFLOW) {
+  if (p->sweep_schedule < PEHIP_SCHEDULE_AUTO || p->sweep_schedule > PEHIP_SCHEDULE_SINGLE_BLOCK) {
     why = "Invalid sweep schedule.";
     return PEHIP_INVALID_ARG;
   }
@@ -527,6 +534,22 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + P.max_iterations, 8, hipMemcpyDeviceToHost, s));
       CKS(sync_check(ctx));
       iters = (uint32_t)(*(unsigned long long*)ctx->hstage);
+    } else if (P.sweep_schedule == PEHIP_SCHEDULE_SINGLE_BLOCK ||
+               (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && P.abort_threshold > 0 &&
+                P.relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING &&
+                nc <= kSingleBlockMaxContacts)) {
+      ColourOffsets O;
+      O.ncol = (uint32_t)ctx->colour_off.size() - 1;
+      for (uint32_t k = 0; k <= O.ncol; ++k) O.off[k] = ctx->colour_off[k];
+      if (ctx->profiling) CKS(record_launch_event(ctx, 0));
+      launch_sweep_block(P.relaxation_model, exact, ctx->sol, O, P.max_iterations, B, dtinv,
+                         (int)P.max_sub_iterations, P.abort_threshold, ctx->deltas, s);
+      if (ctx->profiling) CKS(record_launch_event(ctx, 1));
+      ctx->n_launch_events = ctx->profiling ? 2 : 0;
+      CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + P.max_iterations, 8, hipMemcpyDeviceToHost, s));
+      CKS(sync_check(ctx));
+      iters = (uint32_t)(*(unsigned long long*)ctx->hstage);
+      schedule_used = PEHIP_SCHEDULE_SINGLE_BLOCK;
     } else if (P.abort_threshold <= 0 && (P.sweep_schedule == PEHIP_SCHEDULE_DATAFLOW ||
                                           (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && nc <= kDataflowMaxContacts))) {
       // all sweeps in one cooperative launch (same Gauss-Seidel order per body as the colours)
@@ -552,23 +575,33 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       const uint32_t ncol = (uint32_t)ctx->colour_off.size() - 1;
       size_t nev = 0;
       schedule_used = PEHIP_SCHEDULE_COLOUR_LAUNCHES;
+      // pe's abort test (HCSITS.h:1935-1946 semantics: stop after the first sweep whose impulse
+      // change is below the threshold) runs on the device after every sweep; the sweeps queued
+      // behind a stop return at once, and the host looks at the flag every kAbortPoll sweeps.
+      const bool abort_on = P.abort_threshold > 0;
+      uint32_t* stop = ctx->flags + 4;
+      if (abort_on) fill_u32(stop, 0u, 2, s);
+      iters = P.max_iterations;
       for (uint32_t it = 0; it < P.max_iterations; ++it) {
         for (uint32_t k = 0; k < ncol; ++k) {
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
           launch_sweep(P.relaxation_model, exact, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
-                       (int)P.max_sub_iterations, ctx->deltas + it, ctx->ncu, s);
+                       (int)P.max_sub_iterations, ctx->deltas + it, abort_on ? stop : nullptr, ctx->ncu, s);
           if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
         }
-        iters = it + 1;
-        if (P.abort_threshold > 0) {
-          CK(hipMemcpyAsync(ctx->hstage, ctx->deltas + it, 8, hipMemcpyDeviceToHost, s));
+        if (!abort_on) continue;
+        launch_abort_check(ctx->deltas, it, P.abort_threshold, stop, s);
+        if ((it + 1) % kAbortPoll == 0 || it + 1 == P.max_iterations) {
+          CK(hipMemcpyAsync(ctx->hstage, stop, 8, hipMemcpyDeviceToHost, s));
           CKS(sync_check(ctx));
-          double d;
-          std::memcpy(&d, ctx->hstage, 8);
-          if (d < P.abort_threshold) break;
+          if (ctx->hstage[0]) {
+            iters = ctx->hstage[1];
+            break;
+          }
         }
       }
-      ctx->n_launch_events = nev;
+      // events of the launches that returned at once (after the stop) are not sweep launches
+      ctx->n_launch_events = std::min(nev, (size_t)2 * ncol * iters);
     }
   }
   mark(ctx, 4);

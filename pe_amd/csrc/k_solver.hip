@@ -159,7 +159,9 @@ __device__ __forceinline__ void wave_max_atomic(double v, unsigned long long* ou
 // lanes relaxes one contact (the golden-section search runs SPEC iterations per round on it).
 template <int MODEL, class Trig, int SPEC>
 __global__ void __launch_bounds__(256) k_sweep(SolverContacts C, uint32_t begin, uint32_t end, const double* sb,
-                                               double* dvw, double dtinv, int maxSub, unsigned long long* delta) {
+                                               double* dvw, double dtinv, int maxSub, unsigned long long* delta,
+                                               const uint32_t* stop) {
+  if (stop != nullptr && *stop) return;   // an earlier sweep of this step met the abort threshold
   uint32_t i = begin + ((blockIdx.x * blockDim.x + threadIdx.x) >> SPEC);
   // A whole wavefront per contact: the index is wave-uniform, so the contact's and its bodies'
   // records load through the scalar path into SGPRs (reads only), easing VGPR pressure.
@@ -167,6 +169,72 @@ __global__ void __launch_bounds__(256) k_sweep(SolverContacts C, uint32_t begin,
   double d = 0;
   if (i < end) d = relax_contact<MODEL, Trig, SPEC>(C, i, sb, dvw, dtinv, maxSub);
   wave_max_atomic(d, delta);
+}
+
+// All sweeps of a step in one workgroup: colour k's contacts are spread over the workgroup's
+// groups of 2^SPEC lanes, a barrier separates colours (the contacts of one colour share no
+// non-fixed body, so this is the colour launches' computation, bit for bit), and the abort test
+// after each sweep needs no host.  For steps too small to fill the device.
+template <int MODEL, class Trig, int SPEC, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_sweep_block(SolverContacts C, ColourOffsets O, uint32_t iters,
+                                                      const double* sb, double* dvw, double dtinv, int maxSub,
+                                                      double thr, unsigned long long* deltas) {
+  __shared__ unsigned long long wmax[8];
+  __shared__ unsigned long long smax;
+  constexpr uint32_t G = 1u << SPEC;
+  const uint32_t groups = blockDim.x / G, g = threadIdx.x / G;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  uint32_t it = 0;
+  while (it < iters) {
+    // non-negative doubles (and NaN above +inf) order like their bit patterns, as in the
+    // colour launches' atomicMax
+    unsigned long long u = 0;
+    for (uint32_t k = 0; k < O.ncol; ++k) {
+      for (uint32_t i = O.off[k] + g; i < O.off[k + 1]; i += groups) {
+        const unsigned long long d =
+            (unsigned long long)__double_as_longlong(relax_contact<MODEL, Trig, SPEC>(C, i, sb, dvw, dtinv, maxSub));
+        u = d > u ? d : u;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      const unsigned long long o = __shfl_xor(u, d, 64);
+      u = o > u ? o : u;
+    }
+    if (lane == 0) wmax[wave] = u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long m = 0;
+      for (uint32_t w = 0; w < nwaves; ++w) m = wmax[w] > m ? wmax[w] : m;
+      deltas[it] = m;
+      smax = m;
+    }
+    __syncthreads();
+    ++it;
+    if (thr > 0 && __longlong_as_double((long long)smax) < thr) break;
+  }
+  if (threadIdx.x == 0) deltas[iters] = it;
+}
+
+template <class Trig>
+static void block_dispatch(int model, const SolverContacts& C, const ColourOffsets& O, uint32_t iters, const Bodies& B,
+                           double dtinv, int maxSub, double thr, unsigned long long* deltas, hipStream_t s) {
+  // 512 lanes (two wavefronts per SIMD) keep models 0 and 1 within their registers; model 3's
+  // search needs a SIMD's whole register file per wavefront, so 256 lanes
+  const dim3 g(1);
+  switch (model) {
+    case 0: hipLaunchKernelGGL((k_sweep_block<0, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 1: hipLaunchKernelGGL((k_sweep_block<1, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    default: hipLaunchKernelGGL((k_sweep_block<3, Trig, 4, 256>), g, dim3(256), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+  }
+}
+
+void launch_sweep_block(int model, bool exact_trig, const SolverContacts& C, const ColourOffsets& O, uint32_t iters,
+                        const Bodies& B, double dtinv, int maxSub, double thr, unsigned long long* deltas,
+                        hipStream_t s) {
+  if (exact_trig) block_dispatch<TrigExact>(model, C, O, iters, B, dtinv, maxSub, thr, deltas, s);
+  else block_dispatch<TrigFast>(model, C, O, iters, B, dtinv, maxSub, thr, deltas, s);
 }
 
 // All sweeps in the given contact order, one contact at a time, by one group of 2^SPEC lanes.
@@ -282,22 +350,23 @@ static uint64_t resident_lanes(uint32_t ncu) {
 
 template <class Trig>
 static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B,
-                           double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s) {
+                           double dtinv, int maxSub, unsigned long long* delta, const uint32_t* stop, uint32_t ncu,
+                           hipStream_t s) {
   const uint64_t n = end - begin;
   const dim3 b(256);
   auto grid = [&](int spec) { return dim3((unsigned)(((n << spec) + 255) / 256)); };
   switch (model) {
-    case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
-    case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta); break;
+    case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
+    case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
     default:
       if ((n << 6) <= resident_lanes<Trig, 6>(ncu))
-        hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+        hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
       else if ((n << 4) <= resident_lanes<Trig, 4>(ncu))
-        hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+        hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
       else if ((n << 2) <= resident_lanes<Trig, 2>(ncu))
-        hipLaunchKernelGGL((k_sweep<3, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+        hipLaunchKernelGGL((k_sweep<3, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
       else
-        hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta);
+        hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
       break;
   }
 }
@@ -346,10 +415,25 @@ pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const Solve
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
-                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s) {
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, const uint32_t* stop,
+                  uint32_t ncu, hipStream_t s) {
   if (end <= begin) return;
-  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, ncu, s);
-  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, ncu, s);
+  if (exact_trig) sweep_dispatch<TrigExact>(model, C, begin, end, B, dtinv, maxSub, delta, stop, ncu, s);
+  else sweep_dispatch<TrigFast>(model, C, begin, end, B, dtinv, maxSub, delta, stop, ncu, s);
+}
+
+// After sweep `it`: the first sweep whose L-inf impulse change is below thr raises stop[0] and
+// records the number of sweeps run in stop[1] (a NaN change never stops, as on the host).
+__global__ void k_abort_check(const unsigned long long* deltas, uint32_t it, double thr, uint32_t* stop) {
+  if (threadIdx.x != 0 || stop[0]) return;
+  if (__longlong_as_double((long long)deltas[it]) < thr) {
+    stop[1] = it + 1;
+    stop[0] = 1;
+  }
+}
+
+void launch_abort_check(const unsigned long long* deltas, uint32_t it, double thr, uint32_t* stop, hipStream_t s) {
+  hipLaunchKernelGGL(k_abort_check, dim3(1), dim3(64), 0, s, deltas, it, thr, stop);
 }
 
 template <class Trig>

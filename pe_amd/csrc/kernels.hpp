@@ -131,9 +131,25 @@ pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const Solve
                                          uint32_t iters, const Bodies& B, double dtinv, int maxSub,
                                          const DataflowBuffers& D, uint32_t ncu, hipStream_t s);
 
-// One colour of one sweep; ncu: compute units of the device (sizes the lanes per contact).
+// One colour of one sweep; ncu: compute units of the device (sizes the lanes per contact).  A
+// non-null stop makes the launch return at once once stop[0] is set (see launch_abort_check).
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,
-                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, uint32_t ncu, hipStream_t s);
+                  const Bodies& B, double dtinv, int maxSub, unsigned long long* delta, const uint32_t* stop,
+                  uint32_t ncu, hipStream_t s);
+// pe's abort test on the device: sets stop[0] = 1 and stop[1] = it + 1 if sweep it changed no
+// impulse by thr or more and no earlier sweep did (stop[0..1] cleared before the first sweep).
+void launch_abort_check(const unsigned long long* deltas, uint32_t it, double thr, uint32_t* stop, hipStream_t s);
+// Colour boundaries of the solver order, by value (a kernel argument).
+struct ColourOffsets {
+  uint32_t ncol;
+  uint32_t off[kMaxColours + 1];
+};
+// All `iters` sweeps in one workgroup, colour by colour with a barrier between colours; after
+// each sweep deltas[it] = its L-inf impulse change, and the sweeps stop after the first one
+// below thr > 0.  deltas[iters] receives the number of sweeps run.
+void launch_sweep_block(int model, bool exact_trig, const SolverContacts& C, const ColourOffsets& O, uint32_t iters,
+                        const Bodies& B, double dtinv, int maxSub, double thr, unsigned long long* deltas,
+                        hipStream_t s);
 // All sweeps, strictly sequential, one wavefront.  deltas[it] receives the L-inf impulse
 // change of sweep it; deltas[iters] the number of sweeps run (stops early below thr > 0).
 void launch_sweep_serial(int model, bool exact_trig, const SolverContacts& C, uint32_t nc, uint32_t iters,

@@ -148,8 +148,9 @@ class World:
         self._params_dirty = True
 
     def setSweepSchedule(self, schedule):
-        """Coloured mode: 0 auto, 1 one launch per colour, 2 dataflow (pehip_sweep_schedule)."""
-        if schedule not in (0, 1, 2):
+        """Coloured mode: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup
+        (pehip_sweep_schedule)."""
+        if schedule not in (0, 1, 2, 3):
             raise ValueError("Invalid sweep schedule.")
         self._schedule = int(schedule)
         self._params_dirty = True

@@ -357,6 +357,56 @@ def test_dataflow_sweeps_equal_colour_launches(name, exact):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("model,abort", [(3, 0.0), (1, 0.0), (0, 0.0), (1, 1e-3), (3, 1e-3)])
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_single_block_sweeps_equal_colour_launches(name, model, abort):
+    # All sweeps in one workgroup with a barrier between colours and the abort test on the
+    # device: the colour launches' computation, and the same number of sweeps when aborting.
+    sc = SCENES[name]()
+    _, st0 = oracle_state_after(sc, 10)
+    out = []
+    for schedule in (1, 3):
+        w = gpu_for(sc, iters=60, model=model, abort=abort)
+        w.setSweepSchedule(schedule)
+        w.setState(st0)
+        its = []
+        for _ in range(3):
+            w.simulationStep(sc.dt)
+            s = w.stats()
+            assert s["schedule_used"] == schedule
+            its.append(s["iterations"])
+        out.append((w.getState(), w.impulses(), its))
+        w.close()
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("model", [3, 1, 0])
+def test_single_block_abort_equals_colour_launches(model):
+    # A resting packing converges (static contacts): both schedules stop after the same sweep.
+    sc = scenes.drop_scene(3, 3, 3, r=0.5, spacing=1.0 - 2e-6, jitter=2e-7, box=(6.0, 6.0, 10.0), mu=0.4, seed=2,
+                           v_amp=0.0, max_iterations=4000, z0=0.5 - 1e-6)
+    ow = oracle_for(sc, iters=4000, abort=1e-10)
+    ow.step(sc.dt, 3)
+    st0 = ow.get_state()
+    out = []
+    for schedule in (1, 3):
+        w = gpu_for(sc, iters=4000, model=model, abort=1e-10)
+        w.setSweepSchedule(schedule)
+        w.setState(st0)
+        w.simulationStep(sc.dt)
+        s = w.stats()
+        assert s["schedule_used"] == schedule and s["num_contacts"] >= 20
+        out.append((w.getState(), w.impulses(), s["iterations"], s["last_delta"]))
+        w.close()
+    assert out[0][2] == out[1][2] < 4000 and out[0][3] == out[1][3] < 1e-10
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_colouring_is_conflict_free(name):
     sc = SCENES[name]()
