@@ -93,6 +93,7 @@ class World:
         self._shape, self._sid, self._mat, self._fixed = [], [], [], []
         self._param, self._pos, self._q, self._v, self._w = [], [], [], [], []
         self._inv_mass, self._I, self._Iinv, self._normal, self._d = [], [], [], [], []
+        self._uid = []
         self._arrays = None      # host SoA snapshot (numpy) when built
         self._ctx = None
         self._bodies_dirty = True
@@ -220,10 +221,12 @@ class World:
             self._param, self._pos, self._q = list(a["param"]), list(a["pos"]), list(a["q"])
             self._v, self._w, self._inv_mass = list(a["v"]), list(a["w"]), list(a["inv_mass"])
             self._I, self._Iinv, self._normal, self._d = list(a["I"]), list(a["Iinv"]), list(a["normal"]), list(a["d"])
+            self._uid = list(a["uid"]) if "uid" in a else [0] * len(self._shape)
             self._arrays = None
 
-    def _add(self, shape, mat, fixed, param, pos, q, inv_mass, I, Iinv, normal, d):
+    def _add(self, shape, mat, fixed, param, pos, q, inv_mass, I, Iinv, normal, d, uid=0):
         self._materialise_lists()
+        self._uid.append(int(uid))
         sid = self._next_sid
         self._next_sid += 1
         self._shape.append(shape); self._sid.append(sid); self._mat.append(mat); self._fixed.append(1 if fixed else 0)
@@ -243,7 +246,7 @@ class World:
         I = np.zeros(9)
         I[0] = I[4] = I[8] = 0.4 * m * r * r
         return self._add(SPHERE, material, False, (r, 0.0, 0.0), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
-                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0)
+                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0, uid)
 
     def createBox(self, uid, gpos, lengths, material):
         """createBox (Box.cpp:1312) with BoxBase's mass properties (BoxBase.cpp:60-160)."""
@@ -256,7 +259,7 @@ class World:
         I[4] = m / 12.0 * (lx * lx + lz * lz)
         I[8] = m / 12.0 * (lx * lx + ly * ly)
         return self._add(BOX, material, False, (lx, ly, lz), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
-                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0)
+                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0, uid)
 
     def createPlane(self, uid, normal, d, material):
         """createPlane (Plane.cpp:1202-1240) and the Plane ctor (Plane.cpp:77-102)."""
@@ -269,7 +272,7 @@ class World:
             q = quat_axis_angle((1.0, 0.0, 0.0), math.acos(n[2]))
         else:
             q = quat_axis_angle((-n[1], n[0], 0.0), math.acos(n[2]))
-        return self._add(PLANE, material, True, (0.0, 0.0, 0.0), n * d, q, 0.0, np.zeros(9), np.zeros(9), n, d)
+        return self._add(PLANE, material, True, (0.0, 0.0, 0.0), n * d, q, 0.0, np.zeros(9), np.zeros(9), n, d, uid)
 
     def setFixed(self, body, fixed=True):
         """RigidBody::fix (RigidBody.cpp:402-418): zero inverse mass and inertia, zero velocity."""
@@ -283,16 +286,21 @@ class World:
         self._w[body] = np.zeros(3)
         self._bodies_dirty = True
 
-    def add_bodies(self, arrays):
-        """Bulk creation from SoA arrays in body order (as a sequence of create* calls)."""
+    def add_bodies(self, arrays, sids=None, next_sid=None):
+        """Bulk creation from SoA arrays in body order (as a sequence of create* calls; or with
+        given system IDs, as a body file reader instantiates bodies)."""
         self._materialise_lists()
         base = len(self._shape)
         n = len(arrays["shape"])
         if base:
             raise ValueError("bulk creation only into an empty world")
         self._arrays = dict(arrays)
-        self._arrays["sid"] = np.arange(self._next_sid, self._next_sid + n, dtype=np.uint64)
-        self._next_sid += n
+        if sids is None:
+            self._arrays["sid"] = np.arange(self._next_sid, self._next_sid + n, dtype=np.uint64)
+            self._next_sid += n
+        else:
+            self._arrays["sid"] = np.asarray(sids, np.uint64).copy()
+            self._next_sid = int(next_sid) if next_sid is not None else int(self._arrays["sid"].max()) + 1
         self._bodies_dirty = True
         return np.arange(n)
 
@@ -312,6 +320,7 @@ class World:
                 "w": np.asarray(self._w, float).reshape(n, 3), "inv_mass": np.asarray(self._inv_mass, float),
                 "I": np.asarray(self._I, float).reshape(n, 9), "Iinv": np.asarray(self._Iinv, float).reshape(n, 9),
                 "normal": np.asarray(self._normal, float).reshape(n, 3), "d": np.asarray(self._d, float),
+                "uid": np.asarray(self._uid, np.uint64),
             }
         else:
             self._sync_from_device()

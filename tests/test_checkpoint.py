@@ -1,0 +1,186 @@
+"""pe binary body files (BodyBinaryWriter / BodyBinaryReader format): round trips through
+pe_amd.checkpoint, the byte layout fixed by the reference's writer, and restarts on the GPU."""
+import math
+import struct
+
+import numpy as np
+import pytest
+
+import oracle as O
+from pe_amd import World, checkpoint, scenes
+from pe_amd._abi import BOX, PLANE, SPHERE
+
+
+def mixed_world():
+    rng = np.random.default_rng(3)
+    w = World()
+    w.setGravity(0, 0, -9.81)
+    w.createMaterial("iron", 7800.0, 0.2, 0.3, 0.25)
+    w.createMaterial("wood", 600.0, 0.4, 0.2, 0.1)
+    w.createPlane(11, (0, 0, 1), 0.0, 0)
+    w.createPlane(12, (0, 0, -1), -20.0, 0)
+    w.createPlane(13, (0.5, 0.1, math.cos(math.radians(30))), 0.3, 1)
+    for k in range(12):
+        w.createSphere(100 + k, rng.uniform(1, 9, 3), rng.uniform(0.2, 0.6), k % 2)
+    for k in range(5):
+        w.createBox(200 + k, rng.uniform(1, 9, 3), rng.uniform(0.3, 0.9, 3), 1 - k % 2)
+    w.setFixed(5)
+    a = w.host_arrays()
+    n = len(a["shape"])
+    fin = a["shape"] != PLANE
+    a["v"][fin] = rng.uniform(-1, 1, (fin.sum(), 3))
+    a["w"][fin] = rng.uniform(-2, 2, (fin.sum(), 3))
+    boxes = a["shape"] == BOX
+    q = scenes.shoemake_quaternions(rng, int(boxes.sum()))
+    a["q"][boxes] = q
+    a["v"][5] = 0.0
+    a["w"][5] = 0.0
+    assert n == 20
+    return w
+
+
+def pe_reader_order(a):
+    """Body indices in the order pe's reader creates them: spheres, boxes (local), planes (global)."""
+    s = a["shape"]
+    return np.concatenate([np.flatnonzero(s == SPHERE), np.flatnonzero(s == BOX), np.flatnonzero(s == PLANE)])
+
+
+def test_round_trip_reproduces_every_body(tmp_path):
+    w = mixed_world()
+    f = tmp_path / "bodies.bin"
+    checkpoint.write_body_file(w, str(f))
+    w2 = World()
+    for m in w.materials:
+        w2.createMaterial(m.name, m.density, m.cor, m.csf, m.cdf)
+    info = checkpoint.load_body_file(w2, str(f))
+    a, b = w.host_arrays(), w2.host_arrays()
+    order = pe_reader_order(a)
+    assert len(info["bodies"]) == len(order) == len(b["shape"])
+    for k in ("shape", "sid", "uid", "mat", "fixed", "pos", "q", "v", "w", "param", "inv_mass", "I", "Iinv"):
+        assert np.array_equal(np.asarray(a[k])[order], np.asarray(b[k])), k
+    assert w2._next_sid == w._next_sid
+    # a read plane takes its normal from its orientation (Plane.cpp:121-142), as pe's reader
+    # does, not the exact normal the plane was created with: the -z plane gains a -1.2e-16
+    for i in np.flatnonzero(b["shape"] == PLANE):
+        nrm = checkpoint.quat_rotate(b["q"][i], (0.0, 0.0, 1.0))
+        assert np.array_equal(b["normal"][i], nrm)
+        assert b["d"][i] == nrm[0] * b["pos"][i][0] + nrm[1] * b["pos"][i][1] + nrm[2] * b["pos"][i][2]
+        assert np.abs(b["normal"][i] - np.asarray(a["normal"])[order][i]).max() < 1e-15
+
+
+def test_plane_orientation_gives_pe_normal(tmp_path):
+    # the reader's plane normal is q.rotate(0, 0, 1) (Plane.cpp:121-142): the oracle's
+    # restatement of Quaternion::rotate gives the same bits, including the -z plane's artefact
+    w = mixed_world()
+    a = w.host_arrays()
+    for i in np.flatnonzero(a["shape"] == PLANE):
+        nrm = checkpoint.quat_rotate(a["q"][i], (0.0, 0.0, 1.0))
+        assert np.array_equal(np.array(nrm), O.quat_rotate(a["q"][i], np.array([0.0, 0.0, 1.0])))
+    nz = checkpoint.quat_rotate(a["q"][1], (0.0, 0.0, 1.0))
+    assert nz[2] == -1.0 and nz[1] != 0.0           # acos(-1) = pi: cos(pi/2) is not 0 in binary
+
+
+def test_byte_layout(tmp_path):
+    w = World()
+    w.createMaterial("m", 1000.0, 0.0, 0.1, 0.1)
+    w.createPlane(7, (0, 0, 1), 0.0, 0)
+    w.createSphere(9, (1.0, 2.0, 3.0), 0.5, 0)
+    f = tmp_path / "one.bin"
+    checkpoint.write_body_file(w, str(f))
+    data = f.read_bytes()
+    # header: magic, version 0.1, sizes of real/int/size_t/id_t/bool, one process, offsets
+    assert data[:9] == bytes([ord("P"), ord("E"), 0, 1, 8, 4, 8, 8, 1])
+    p, off_g, off_l, end = struct.unpack(">IIII", data[9:25])
+    assert p == 1 and off_g == 25 and end == len(data)
+    # global chunk: counter, then (type byte, uint32 count) for sphere, box, capsule, cylinder,
+    # plane (one plane), mesh, union
+    g = data[off_g:off_l]
+    assert struct.unpack(">Q", g[:8])[0] == 1
+    assert g[8:13] == bytes([1, 0, 0, 0, 0]) and g[13:18] == bytes([2, 0, 0, 0, 0])
+    assert g[28:33] == bytes([5, 0, 0, 0, 1])
+    plane = g[33:33 + 8 + 8 + 2 + 13 * 8 + 8]
+    sid, uid, vis, fixed = struct.unpack(">QQBB", plane[:18])
+    assert (uid, vis, fixed) == (7, 1, 1)
+    # local chunk: counter, then the sphere record; 1.0 as a big-endian double is 3FF0000000000000
+    l = data[off_l:]
+    assert struct.unpack(">Q", l[:8])[0] == w._next_sid
+    assert l[8:13] == bytes([1, 0, 0, 0, 1])
+    rec = l[13:]
+    assert struct.unpack(">QQBB", rec[:18])[1:] == (9, 1, 0)
+    assert rec[18:26] == bytes.fromhex("3ff0000000000000")
+    assert struct.unpack(">3d", rec[18:42]) == (1.0, 2.0, 3.0)
+    assert struct.unpack(">4d", rec[42:74]) == (1.0, 0.0, 0.0, 0.0)
+    radius_at = 18 + 13 * 8 + 8
+    assert rec[radius_at:radius_at + 8] == bytes.fromhex("3fe0000000000000")   # 0.5
+    assert len(rec) == radius_at + 8 + 5 * 5                  # then box, capsule, cylinder, mesh, union: empty
+
+
+def test_single_precision_file(tmp_path):
+    w = mixed_world()
+    f = tmp_path / "single.bin"
+    checkpoint.write_body_file(w, str(f), fp_size=2)
+    info = checkpoint.read_body_file(str(f))
+    a = w.host_arrays()
+    order = pe_reader_order(a)
+    pos = np.array([b["pos"] for b in info["bodies"]])
+    assert np.array_equal(pos, a["pos"][order].astype(np.float32).astype(np.float64))
+
+
+def test_rejects_bad_files(tmp_path):
+    w = mixed_world()
+    f = tmp_path / "b.bin"
+    checkpoint.write_body_file(w, str(f))
+    data = bytearray(f.read_bytes())
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"XE" + bytes(data[2:]))
+    with pytest.raises(ValueError, match="Invalid file format"):
+        checkpoint.read_body_file(str(bad))
+    v = bytearray(data)
+    v[3] = 2
+    bad.write_bytes(bytes(v))
+    with pytest.raises(ValueError, match="Unsupported file format version"):
+        checkpoint.read_body_file(str(bad))
+    bad.write_bytes(bytes(data[:-7]))
+    with pytest.raises(ValueError):
+        checkpoint.read_body_file(str(bad))
+    # a capsule (type 3) in the local chunk: not a shape of this library
+    off_l = struct.unpack(">I", bytes(data[17:21]))[0]
+    v = bytearray(data)
+    n_sph = struct.unpack(">I", bytes(v[off_l + 9:off_l + 13]))[0]
+    rec = 18 + 13 * 8 + 8 + 8
+    box_hdr = off_l + 13 + n_sph * rec
+    n_box = struct.unpack(">I", bytes(v[box_hdr + 1:box_hdr + 5]))[0]
+    cap_hdr = box_hdr + 5 + n_box * (rec + 16)
+    assert v[cap_hdr] == 3
+    v[cap_hdr + 1:cap_hdr + 5] = struct.pack(">I", 1)
+    bad.write_bytes(bytes(v))
+    with pytest.raises(ValueError, match="Unsupported geometry type 3"):
+        checkpoint.read_body_file(str(bad))
+
+
+@pytest.mark.gpu
+def test_restart_from_file_on_gpu(tmp_path):
+    # a run interrupted by a body file continues with the same bodies: same state and the same
+    # contact set (up to pe's reader body order) as the world it was written from
+    sc = scenes.drop_scene(4, 4, 3, r=0.45, spacing=1.0, box=(8.0, 8.0, 20.0), v_amp=0.5, seed=4)
+    w = World()
+    sc.to_world(w)
+    w.run(60, sc.dt)
+    f = tmp_path / "mid.bin"
+    checkpoint.write_body_file(w, str(f))
+    w2 = World()
+    for m in w.materials:
+        w2.createMaterial(m.name, m.density, m.cor, m.csf, m.cdf)
+    w2.setGravity(*sc.gravity)
+    checkpoint.load_body_file(w2, str(f))
+    a = w.host_arrays()
+    order = pe_reader_order(a)
+    s1, s2 = w.getState(), w2.getState()
+    for k in ("pos", "q", "v", "w", "normal", "d"):
+        assert np.array_equal(s1[k][order], s2[k]), k
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    c1, c2 = w.detect(), w2.detect()
+    k1 = sorted(zip(inv[c1["b1"]].tolist(), inv[c1["b2"]].tolist(), map(tuple, c1["pos"].tolist())))
+    k2 = sorted(zip(c2["b1"].tolist(), c2["b2"].tolist(), map(tuple, c2["pos"].tolist())))
+    assert len(k1) > 10 and k1 == k2
