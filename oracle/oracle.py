@@ -17,7 +17,9 @@ _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 SPHERE, BOX, PLANE = 1, 2, 3
 INELASTIC_FRICTIONLESS = 0
 APPROX_INELASTIC_COULOMB_DECOUPLING = 1
+APPROX_INELASTIC_COULOMB_ORTHO = 2
 INELASTIC_COULOMB_DECOUPLING = 3
+INELASTIC_COULOMB_ORTHO = 4
 
 _D = ctypes.POINTER(ctypes.c_double)
 _U32 = ctypes.POINTER(ctypes.c_uint32)

@@ -615,6 +615,7 @@ struct orc_world {
   double relaxInelasticCoulombContactsByDecoupling(double dtinv);
   double relaxApproximateInelasticCoulombContactsByDecoupling(double dtinv);
   double relaxInelasticFrictionlessContacts(double dtinv);
+  double relaxInelasticCoulombContactsByOrthogonalProjections(double dtinv, bool approximate);
   void initializeVelocityCorrections(Body& b, V3& dv, V3& dw, double dt);
   void integratePositions(Body& b, V3 v, V3 w, double dt);
 };
@@ -1104,6 +1105,10 @@ void orc_world::resolveContacts(double dt) {
         delta = relaxApproximateInelasticCoulombContactsByDecoupling(dtinv); break;
       case ORC_INELASTIC_COULOMB_DECOUPLING:
         delta = relaxInelasticCoulombContactsByDecoupling(dtinv); break;
+      case ORC_APPROX_INELASTIC_COULOMB_ORTHO:
+        delta = relaxInelasticCoulombContactsByOrthogonalProjections(dtinv, true); break;
+      case ORC_INELASTIC_COULOMB_ORTHO:
+        delta = relaxInelasticCoulombContactsByOrthogonalProjections(dtinv, false); break;
       default: throw std::runtime_error("Unsupported relaxation model.");
     }
     if (abortThreshold > 0 && delta < abortThreshold) { ++it; break; }
@@ -1272,6 +1277,57 @@ double orc_world::relaxApproximateInelasticCoulombContactsByDecoupling(double dt
       dv_[i2] -= b2.invMass * p_[i];
       dw_[i2] -= I2 * cross(r2_[i], p_[i]);
     }
+  }
+  return delta_max;
+}
+
+// HCSITS.h:2631-2723: the impulse in the contact frame takes one step p - (D p + gdot_nto)
+// (approximate: Anitescu/Tasora) or p - (D p + gdot_nto + mu |(D p + gdot_nto)_to| e_n)
+// (De Saxce/Feng) and is projected onto the friction cone by the reference's three cases.
+double orc_world::relaxInelasticCoulombContactsByOrthogonalProjections(double dtinv, bool approximate) {
+  double delta_max = 0;
+  const size_t nc = p_.size();
+  for (size_t i = 0; i < nc; ++i) {
+    const int i1 = body1_[i], i2 = body2_[i];
+    const Body& b1 = bodies[i1];
+    const Body& b2 = bodies[i2];
+    const M3 I1 = b1.invInertiaWF(), I2 = b2.invInertiaWF();
+    dv_[i1] -= b1.invMass * p_[i];
+    dw_[i1] -= I1 * cross(r1_[i], p_[i]);
+    dv_[i2] += b2.invMass * p_[i];
+    dw_[i2] += I2 * cross(r2_[i], p_[i]);
+    const V3 gdot = (v_[i1] + dv_[i1]) - (v_[i2] + dv_[i2]) + cross(w_[i1] + dw_[i1], r1_[i]) -
+                    cross(w_[i2] + dw_[i2], r2_[i]);
+    const M3 cf = M3::cols(n_[i], t_[i], o_[i]);
+    V3 gdot_nto = trans(cf) * gdot;
+    gdot_nto[0] += dist_[i] * dtinv;
+    const double step = 1.0;   // the reference's w
+    const double mu = mu_[i];
+    V3 p_cf = trans(cf) * p_[i];
+    V3 corr = diag_nto_[i] * p_cf + gdot_nto;
+    if (!approximate) corr[0] += std::sqrt(corr[1] * corr[1] + corr[2] * corr[2]) * mu;
+    p_cf = p_cf - step * corr;
+    const double flimit = mu * p_cf[0];
+    const double fsq = p_cf[1] * p_cf[1] + p_cf[2] * p_cf[2];
+    if (p_cf[0] > 0 && fsq < flimit * flimit) {
+      // static: no projection
+    } else if (p_cf[0] < 0 && fsq < p_cf[0] / (mu * mu)) {
+      p_cf = V3();
+    } else {
+      const double fn = std::sqrt(fsq);
+      p_cf[0] = (fn * mu + p_cf[0]) / (mu * mu + 1);
+      const double factor = mu * p_cf[0] / fn;
+      p_cf[1] *= factor;
+      p_cf[2] *= factor;
+    }
+    const V3 p_wf = cf * p_cf;
+    const V3 dp = p_[i] - p_wf;
+    delta_max = std::max(delta_max, std::max(std::fabs(dp[0]), std::max(std::fabs(dp[1]), std::fabs(dp[2]))));
+    p_[i] = p_wf;
+    dv_[i1] += b1.invMass * p_[i];
+    dw_[i1] += I1 * cross(r1_[i], p_[i]);
+    dv_[i2] -= b2.invMass * p_[i];
+    dw_[i2] -= I2 * cross(r2_[i], p_[i]);
   }
   return delta_max;
 }

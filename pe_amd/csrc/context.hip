@@ -118,7 +118,9 @@ static void plan_grid(pehip_ctx* ctx, const uint8_t* shape, const double* param,
 pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
   if (p->relaxation_model != PEHIP_INELASTIC_FRICTIONLESS &&
       p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
-      p->relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING) {
+      p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_ORTHO &&
+      p->relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING &&
+      p->relaxation_model != PEHIP_INELASTIC_COULOMB_ORTHO) {
     why = "Unsupported relaxation model.";  // HCSITS.h:1926
     return PEHIP_NOT_SUPPORTED;
   }

@@ -483,7 +483,8 @@ enum ContactField {
   CF_DTOI = 27,  // 4: inverse of its tangential 2x2 block
   CF_DTO = 31,   // 4: the tangential 2x2 block (golden-section objective)
   CF_P = 35,     // 3: impulse, world frame
-  CF_COUNT = 38
+  CF_D0 = 38,    // 5: the 3x3 block's first row and column, D00 D01 D02 D10 D20 (projections)
+  CF_COUNT = 43
 };
 
 struct SolverContacts {
@@ -713,7 +714,43 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
   g.x += ddt;
 
   double delta;
-  if (g.x >= 0) {
+  if constexpr (MODEL == 2 || MODEL == 4) {
+    // Orthogonal projection onto the friction cone (HCSITS.h:2631-2723): one step from the
+    // current impulse, p_cf - (D p_cf + gdot_nto), Anitescu/Tasora (2); De Saxce/Feng (4) adds
+    // mu |(D p_cf + gdot_nto)_to| to the normal component first.  pe's branches are kept as
+    // written: its separating test compares fsq with p_n / mu^2 < 0 and so never holds.
+    const double mu = f[CF_MU * cap + i];
+    const double D00 = f[CF_D0 * cap + i], D01 = f[(CF_D0 + 1) * cap + i], D02 = f[(CF_D0 + 2) * cap + i];
+    const double D10 = f[(CF_D0 + 3) * cap + i], D20 = f[(CF_D0 + 4) * cap + i];
+    const double D11 = f[CF_DTO * cap + i], D12 = f[(CF_DTO + 1) * cap + i];
+    const double D21 = f[(CF_DTO + 2) * cap + i], D22 = f[(CF_DTO + 3) * cap + i];
+    const d3 pc = mat_tvec(cfm, p);
+    d3 tmp = mk3((D00 * pc.x + D01 * pc.y + D02 * pc.z) + g.x, (D10 * pc.x + D11 * pc.y + D12 * pc.z) + g.y,
+                 (D20 * pc.x + D21 * pc.y + D22 * pc.z) + g.z);
+    if (MODEL == 4) tmp.x += sqrt(tmp.y * tmp.y + tmp.z * tmp.z) * mu;
+    d3 pcf = pc - tmp;
+    const double flimit = mu * pcf.x;
+    const double fsq = pcf.y * pcf.y + pcf.z * pcf.z;
+    if (pcf.x > 0 && fsq < flimit * flimit) {
+      // inside the cone: static
+    } else if (pcf.x < 0 && fsq < pcf.x / (mu * mu)) {
+      pcf = mk3(0, 0, 0);
+    } else {
+      const double fn = sqrt(fsq);
+      pcf.x = (fn * mu + pcf.x) / (mu * mu + 1.0);
+      const double factor = mu * pcf.x / fn;
+      pcf.y *= factor;
+      pcf.z *= factor;
+    }
+    const d3 pwf = mat_vec(cfm, pcf);
+    const d3 dp = p - pwf;
+    delta = fmax(fabs(dp.x), fmax(fabs(dp.y), fabs(dp.z)));
+    p = pwf;
+    dv1 = dv1 + im1 * p;
+    dw1 = dw1 + mat_vec(I1, cross3(r1, p));
+    dv2 = dv2 - im2 * p;
+    dw2 = dw2 - mat_vec(I2, cross3(r2, p));
+  } else if (g.x >= 0) {
     delta = fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
     p = mk3(0, 0, 0);
   } else {

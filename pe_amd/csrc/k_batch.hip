@@ -433,6 +433,8 @@ static void batch_dispatch(int model, const Bodies& B, const RawContacts& R, con
   switch (model) {
     case 0: hipLaunchKernelGGL((k_batch<0, Trig>), g, b, 0, s, B, R, C, A); break;
     case 1: hipLaunchKernelGGL((k_batch<1, Trig>), g, b, 0, s, B, R, C, A); break;
+    case 2: hipLaunchKernelGGL((k_batch<2, Trig>), g, b, 0, s, B, R, C, A); break;
+    case 4: hipLaunchKernelGGL((k_batch<4, Trig>), g, b, 0, s, B, R, C, A); break;
     default: hipLaunchKernelGGL((k_batch<3, Trig>), g, b, 0, s, B, R, C, A); break;
   }
 }

@@ -226,6 +226,8 @@ static void block_dispatch(int model, const SolverContacts& C, const ColourOffse
   switch (model) {
     case 0: hipLaunchKernelGGL((k_sweep_block<0, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 1: hipLaunchKernelGGL((k_sweep_block<1, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 2: hipLaunchKernelGGL((k_sweep_block<2, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 4: hipLaunchKernelGGL((k_sweep_block<4, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     default: hipLaunchKernelGGL((k_sweep_block<3, Trig, 4, 256>), g, dim3(256), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
 }
@@ -358,6 +360,8 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
   switch (model) {
     case 0: hipLaunchKernelGGL((k_sweep<0, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
     case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
+    case 2: hipLaunchKernelGGL((k_sweep<2, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
+    case 4: hipLaunchKernelGGL((k_sweep<4, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
     default:
       if ((n << 6) <= resident_lanes<Trig, 6>(ncu))
         hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
@@ -379,6 +383,8 @@ static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, 
   switch (model) {
     case 0: args_fn = (void*)k_sweep_dataflow<0, Trig, 0>; break;
     case 1: args_fn = (void*)k_sweep_dataflow<1, Trig, 0>; break;
+    case 2: args_fn = (void*)k_sweep_dataflow<2, Trig, 0>; break;
+    case 4: args_fn = (void*)k_sweep_dataflow<4, Trig, 0>; break;
     default: args_fn = (void*)k_sweep_dataflow<3, Trig, SPEC>; break;
   }
   int per_cu = 0;
@@ -443,6 +449,8 @@ static void serial_dispatch(int model, const SolverContacts& C, uint32_t nc, uin
   switch (model) {
     case 0: hipLaunchKernelGGL((k_sweep_serial<0, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 1: hipLaunchKernelGGL((k_sweep_serial<1, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 2: hipLaunchKernelGGL((k_sweep_serial<2, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 4: hipLaunchKernelGGL((k_sweep_serial<4, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     default: hipLaunchKernelGGL((k_sweep_serial<3, Trig, 6>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
 }

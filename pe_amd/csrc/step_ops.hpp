@@ -158,6 +158,11 @@ __device__ __forceinline__ void setup_contact(const Bodies& B, const RawContacts
   f[(CF_DTO + 1) * cap + s] = D.a[5];
   f[(CF_DTO + 2) * cap + s] = D.a[7];
   f[(CF_DTO + 3) * cap + s] = D.a[8];
+  f[CF_D0 * cap + s] = D.a[0];
+  f[(CF_D0 + 1) * cap + s] = D.a[1];
+  f[(CF_D0 + 2) * cap + s] = D.a[2];
+  f[(CF_D0 + 3) * cap + s] = D.a[3];
+  f[(CF_D0 + 4) * cap + s] = D.a[6];
   f[CF_P * cap + s] = 0.0;
   f[(CF_P + 1) * cap + s] = 0.0;
   f[(CF_P + 2) * cap + s] = 0.0;

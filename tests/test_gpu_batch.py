@@ -76,6 +76,28 @@ def test_batch_step_equals_gs_in_colour_order():
             assert rel_err(gk[key], o1[key]) <= 1e-12, (k, scs[k].name, key)
 
 
+@pytest.mark.parametrize("model", [2, 4, 1, 0])
+def test_batch_other_models_equal_gs_in_colour_order(model):
+    # the batch kernel runs every relaxation model of the single-world path: the projections
+    # (2, 4), the approximate decoupling (1) and the frictionless contact (0)
+    scs = batch_scenes()
+    ows = oracle_states(scs, 10)
+    bw = BatchWorld(scs, relaxation_model=model)
+    bw.setState(concat_states([st for _, st in ows]))
+    bw.run(1)
+    g1 = bw.getState()
+    for k, (ow, _) in enumerate(ows):
+        idx, _ = bw.sweepOrder(k)
+        gc = bw.contacts(k)
+        ow.L.orc_set_relaxation_model(ow.h, model)
+        ow.set_sweep_order_ranked(gc["b1"][idx], gc["b2"][idx], pair_ranks(gc)[idx])
+        ow.step(scs[k].dt)
+        o1 = ow.get_state()
+        gk = bw.sceneState(g1, k)
+        for key in ("v", "w", "pos", "q"):
+            assert rel_err(gk[key], o1[key]) <= 1e-12, (k, scs[k].name, key)
+
+
 def test_batch_colouring_is_conflict_free():
     scs = batch_scenes()
     bw = BatchWorld(scs)

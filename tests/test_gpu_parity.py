@@ -227,7 +227,7 @@ def test_contact_set_bitexact_c1_late_state():
 
 # --------------------------------------------------------------------------- one step, pe's order
 @pytest.mark.parametrize("name", sorted(SCENES))
-@pytest.mark.parametrize("model", [3, 1, 0])
+@pytest.mark.parametrize("model", [3, 1, 0, 2, 4])
 def test_reference_order_step_matches_oracle(name, model):
     sc = SCENES[name]()
     ow, st0 = oracle_state_after(sc, 12)
@@ -337,6 +337,28 @@ def test_coloured_step_equals_gs_in_colour_order(name):
         assert rel_err(g1[k], o1[k]) <= 1e-12, k
 
 
+@pytest.mark.parametrize("model", [2, 4])
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_coloured_projection_step_equals_gs_in_colour_order(name, model):
+    # The orthogonal-projection models (HCSITS.h:2631-2723) in colour order: no trig, so the
+    # device and the oracle agree without the correctly rounded mode.
+    sc = SCENES[name]()
+    ow, st0 = oracle_state_after(sc, 10)
+    ow.L.orc_set_relaxation_model(ow.h, model)
+    w = gpu_for(sc, iters=30, model=model)
+    w.setState(st0)
+    w.simulationStep(sc.dt)
+    g1 = w.getState()
+    idx, offs = w.sweepOrder()
+    gc = w.contacts()
+    ow.set_sweep_order_ranked(gc["b1"][idx], gc["b2"][idx], pair_ranks(gc)[idx])
+    ow.step(sc.dt)
+    o1 = ow.get_state()
+    assert len(gc["b1"]) > 10
+    for k in ("v", "w", "pos", "q"):
+        assert rel_err(g1[k], o1[k]) <= 1e-12, k
+
+
 @pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_dataflow_sweeps_equal_colour_launches(name, exact):
@@ -357,7 +379,7 @@ def test_dataflow_sweeps_equal_colour_launches(name, exact):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("model,abort", [(3, 0.0), (1, 0.0), (0, 0.0), (1, 1e-3), (3, 1e-3)])
+@pytest.mark.parametrize("model,abort", [(3, 0.0), (1, 0.0), (0, 0.0), (1, 1e-3), (3, 1e-3), (2, 0.0), (4, 1e-3)])
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_single_block_sweeps_equal_colour_launches(name, model, abort):
     # All sweeps in one workgroup with a barrier between colours and the abort test on the

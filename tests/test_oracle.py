@@ -409,3 +409,75 @@ def test_ranked_sweep_order_replay():
     rev.set_sweep_order_ranked(c["b1"][k], c["b2"][k], rank[k])
     rev.step(1e-3)
     assert not np.array_equal(rev.get_state()["v"], ref.get_state()["v"])
+
+
+def _cone_projection_brute(y, mu):
+    """Euclidean projection of y = (y_n, y_t1, y_t2) onto {|x_t| <= mu x_n} by minimising over
+    the cone's boundary rays and its tip (a dense search refined by a scalar minimisation)."""
+    from scipy.optimize import minimize_scalar
+    yn, yt = y[0], np.asarray(y[1:])
+    ty = np.linalg.norm(yt)
+    if ty <= mu * yn:
+        return np.array(y, float)
+    e = yt / ty if ty > 0 else np.array([1.0, 0.0])
+    # on the boundary the optimum lies in the plane of y and the axis: x = (s, mu s e), s >= 0
+    f = lambda s: (s - yn) ** 2 + (mu * s - ty) ** 2
+    r = minimize_scalar(f, bounds=(0.0, abs(yn) + ty + 1.0), method="bounded", options={"xatol": 1e-15})
+    s = max(r.x, 0.0)
+    return np.array([s, mu * s * e[0], mu * s * e[1]])
+
+
+@pytest.mark.parametrize("vel,sliding", [((3.0, -2.0, -1.5), True), ((0.3, -0.2, -1.5), False)])
+@pytest.mark.parametrize("model", [O.APPROX_INELASTIC_COULOMB_ORTHO, O.INELASTIC_COULOMB_ORTHO])
+def test_orthogonal_projection_first_sweep(model, vel, sliding):
+    # A sphere of unit mass sliding and pressing on a plane, one sweep from p = 0: the contact
+    # impulse in the contact frame is the Euclidean projection onto the friction cone of
+    # -gdot_nto (Anitescu/Tasora) or of -(gdot_nto + mu |gdot_t| e_n) (De Saxce/Feng).
+    mu, dt, g = 0.4, 1e-3, 9.81
+    r = 0.5
+    rho = 1.0 / (4.0 / 3.0 * math.pi * r ** 3)            # m = 1
+    w = make_world(gravity=(0, 0, -g), model=model, iters=1, mu=mu, density=rho)
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_sphere([0, 0, r], r, 0)
+    st = w.get_state()
+    st["v"][1] = vel
+    st["w"][1] = [0.4, 0.1, 0.0]
+    w.set_state(st)
+    w.step(dt)
+    c = w.contacts()
+    n = c["normal"][0]
+    assert np.array_equal(n, [0.0, 0.0, 1.0])
+    # relative contact velocity after gravity, in the contact frame (r1 = -r n)
+    v = np.array([vel[0], vel[1], vel[2] - g * dt])
+    om = np.array([0.4, 0.1, 0.0])
+    gdot = v + np.cross(om, np.array([0.0, 0.0, -r]))
+    p = w.impulses()[0]
+    # the contact frame's tangents: n = +z gives t = n x e_x normalised = (0, 1, 0), o = n x t
+    t = np.array([0.0, 1.0, 0.0])
+    o = np.cross(n, t)
+    y = -np.array([gdot @ n, gdot @ t, gdot @ o])
+    if model == O.INELASTIC_COULOMB_ORTHO:
+        y[0] -= mu * math.hypot(y[1], y[2])
+    x = _cone_projection_brute(y, mu)
+    pc = np.array([p @ n, p @ t, p @ o])
+    assert np.allclose(pc, x, rtol=1e-9, atol=1e-12)
+    if sliding:
+        assert math.hypot(pc[1], pc[2]) == pytest.approx(mu * pc[0], rel=1e-12)   # on the cone
+    else:
+        assert math.hypot(pc[1], pc[2]) < mu * pc[0]                             # inside: y itself
+
+
+@pytest.mark.parametrize("model", [O.APPROX_INELASTIC_COULOMB_ORTHO, O.INELASTIC_COULOMB_ORTHO])
+def test_orthogonal_projection_resting_unit_sphere(model):
+    # m = 1 makes the normal entry of D one: a resting sphere's contact reaches p = m g dt in a
+    # single sweep and stays (the static branch), whatever the form of the iteration
+    dt, g = 1e-3, 9.81
+    r = 0.5
+    rho = 1.0 / (4.0 / 3.0 * math.pi * r ** 3)
+    w = make_world(gravity=(0, 0, -g), model=model, iters=5, density=rho)
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_sphere([0, 0, r], r, 0)
+    w.step(dt)
+    p = w.impulses()[0]
+    assert p[2] == pytest.approx(g * dt, rel=1e-14) and p[0] == 0 and p[1] == 0
+    assert np.all(np.abs(w.get_state()["v"][1]) < 1e-15)
