@@ -531,16 +531,18 @@ __device__ __forceinline__ void golden_path(uint32_t code, int depth, double& al
 #pragma unroll
   for (int lev = 0; lev < LEVELS; ++lev) {
     const bool act = lev < depth;
-    const bool lt = act && ((code >> (act ? depth - 1 - lev : 0)) & 1u) != 0;
+    const bool lt = ((code >> (act ? depth - 1 - lev : 0)) & 1u) != 0;
     const double a = al + (ar - am);
-    const double nal = lt ? (left ? al : am) : (left ? a : al);
-    const double nar = lt ? (left ? am : ar) : (left ? ar : a);
-    const double nam = lt ? a : am;
-    const bool nleft = lt ? left : !left;
-    al = act ? nal : al;
-    ar = act ? nar : ar;
-    am = act ? nam : am;
-    left = act ? nleft : left;
+    // golden_step's four cases, inactive levels leaving everything as it is
+    const bool to_am = act && lt;                   // a becomes the interior point
+    const bool l_am = to_am && !left, r_am = to_am && left;
+    const bool l_a = act && !lt && left, r_a = act && !lt && !left;
+    const double nal = l_am ? am : (l_a ? a : al);
+    const double nar = r_am ? am : (r_a ? a : ar);
+    am = to_am ? a : am;
+    al = nal;
+    ar = nar;
+    left = (act && !lt) ? !left : left;
   }
 }
 
@@ -599,11 +601,19 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       double al = S.al, ar = S.ar, am = S.am;
       bool left = S.left;
       golden_path<SPEC - 1>(code, depth, al, ar, am, left);
-      const double an = al + (ar - am);
+      // lane L-1 holds no node of the tree: in the first round it evaluates the search's
+      // initial point (the bracket's interior point am), which the comparisons need
+      const bool init = k == 0 && lane == (uint32_t)(L - 1);
+      const double an = init ? S.am : al + (ar - am);
       double sn, cn;
       Trig::sincos(an, sn, cn);
       const double xn0 = flimit * cn, xn1 = flimit * sn;
       const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
+      if (k == 0) {
+        S.fm = group_read<SPEC>(fn, L - 1);
+        S.xm0 = group_read<SPEC>(xn0, L - 1);
+        S.xm1 = group_read<SPEC>(xn1, L - 1);
+      }
       // best value held on reaching this node: fold of the ancestors' values, root first
       double m = S.fm;
 #pragma unroll
@@ -819,12 +829,15 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
               gs.al = -acos_f - shift;
               gs.ar = acos_f - shift;
               gs.am = (gs.ar + gs.al * phi) / (1 + phi);
-              double sm, cm;
-              Trig::sincos(gs.am, sm, cm);
-              gs.xm0 = flimit * cm;
-              gs.xm1 = flimit * sm;
-              gs.fm = golden_objective(gs.xm0, gs.xm1, A00, A01, A10, A11, gt, go);
               gs.left = false;
+              if (SPEC == 0 || maxSub <= 0) {
+                double sm, cm;
+                Trig::sincos(gs.am, sm, cm);
+                gs.xm0 = flimit * cm;
+                gs.xm1 = flimit * sm;
+                gs.fm = golden_objective(gs.xm0, gs.xm1, A00, A01, A10, A11, gt, go);
+              }
+              // (with SPEC > 0 the search's first round evaluates the initial point as well)
               golden_section<Trig, SPEC>(gs, maxSub, flimit, A00, A01, A10, A11, gt, go);
               pc[1] = gs.xm0;
               pc[2] = gs.xm1;
