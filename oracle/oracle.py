@@ -20,6 +20,7 @@ APPROX_INELASTIC_COULOMB_DECOUPLING = 1
 APPROX_INELASTIC_COULOMB_ORTHO = 2
 INELASTIC_COULOMB_DECOUPLING = 3
 INELASTIC_COULOMB_ORTHO = 4
+INELASTIC_GMD = 5
 
 _D = ctypes.POINTER(ctypes.c_double)
 _U32 = ctypes.POINTER(ctypes.c_uint32)

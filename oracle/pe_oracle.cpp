@@ -616,6 +616,7 @@ struct orc_world {
   double relaxApproximateInelasticCoulombContactsByDecoupling(double dtinv);
   double relaxInelasticFrictionlessContacts(double dtinv);
   double relaxInelasticCoulombContactsByOrthogonalProjections(double dtinv, bool approximate);
+  double relaxInelasticGeneralizedMaximumDissipationContacts(double dtinv);
   void initializeVelocityCorrections(Body& b, V3& dv, V3& dw, double dt);
   void integratePositions(Body& b, V3 v, V3 w, double dt);
 };
@@ -1109,6 +1110,8 @@ void orc_world::resolveContacts(double dt) {
         delta = relaxInelasticCoulombContactsByOrthogonalProjections(dtinv, true); break;
       case ORC_INELASTIC_COULOMB_ORTHO:
         delta = relaxInelasticCoulombContactsByOrthogonalProjections(dtinv, false); break;
+      case ORC_INELASTIC_GMD:
+        delta = relaxInelasticGeneralizedMaximumDissipationContacts(dtinv); break;
       default: throw std::runtime_error("Unsupported relaxation model.");
     }
     if (abortThreshold > 0 && delta < abortThreshold) { ++it; break; }
@@ -1319,6 +1322,142 @@ double orc_world::relaxInelasticCoulombContactsByOrthogonalProjections(double dt
       const double factor = mu * p_cf[0] / fn;
       p_cf[1] *= factor;
       p_cf[2] *= factor;
+    }
+    const V3 p_wf = cf * p_cf;
+    const V3 dp = p_[i] - p_wf;
+    delta_max = std::max(delta_max, std::max(std::fabs(dp[0]), std::max(std::fabs(dp[1]), std::fabs(dp[2]))));
+    p_[i] = p_wf;
+    dv_[i1] += b1.invMass * p_[i];
+    dw_[i1] += I1 * cross(r1_[i], p_[i]);
+    dv_[i2] -= b2.invMass * p_[i];
+    dw_[i2] -= I2 * cross(r2_[i], p_[i]);
+  }
+  return delta_max;
+}
+
+// HCSITS.h:2745-3055 (relaxInelasticGeneralizedMaximumDissipationContacts).  A sliding contact
+// takes the impulse on the friction cone's boundary that satisfies the normal constraint
+// (gdot_n + D p = 0 in the normal row) and minimises the energy 1/2 x^T A_ x - x^T b_ of the
+// tangential part x, with A_ and b_ the tangential block and load after eliminating the normal
+// row.  On the cone boundary |x| = mu p_n, so x = r(alpha) (cos alpha, sin alpha) with
+// r(alpha) = mu (-gdot_n) / (D00 + mu a3 cos(alpha + shiftI)), and a golden-section search over
+// alpha finds the minimum; the bracket is where r(alpha) is positive (angleI) intersected with
+// where the cone boundary is cheaper than the unconstrained direction (angleJ).
+double orc_world::relaxInelasticGeneralizedMaximumDissipationContacts(double dtinv) {
+  const double kPi = 3.141592653589793;   // M_PI (pe/math/Constants.h:163)
+  double delta_max = 0;
+  const size_t nc = p_.size();
+  for (size_t i = 0; i < nc; ++i) {
+    const int i1 = body1_[i], i2 = body2_[i];
+    const Body& b1 = bodies[i1];
+    const Body& b2 = bodies[i2];
+    const M3 I1 = b1.invInertiaWF(), I2 = b2.invInertiaWF();
+    dv_[i1] -= b1.invMass * p_[i];
+    dw_[i1] -= I1 * cross(r1_[i], p_[i]);
+    dv_[i2] += b2.invMass * p_[i];
+    dw_[i2] += I2 * cross(r2_[i], p_[i]);
+    const V3 gdot = (v_[i1] + dv_[i1]) - (v_[i2] + dv_[i2]) + cross(w_[i1] + dw_[i1], r1_[i]) -
+                    cross(w_[i2] + dw_[i2], r2_[i]);
+    const M3 cf = M3::cols(n_[i], t_[i], o_[i]);
+    V3 gdot_nto = trans(cf) * gdot;
+    gdot_nto[0] += dist_[i] * dtinv;
+    if (gdot_nto[0] >= 0) {
+      delta_max = std::max(delta_max, std::max(std::fabs(p_[i][0]), std::max(std::fabs(p_[i][1]), std::fabs(p_[i][2]))));
+      p_[i] = V3();
+      continue;
+    }
+    V3 p_cf = -(diag_nto_inv_[i] * gdot_nto);
+    const double mu = mu_[i];
+    const double flimit = mu * p_cf[0];
+    const double fsq = p_cf[1] * p_cf[1] + p_cf[2] * p_cf[2];
+    if (fsq > flimit * flimit || p_cf[0] < 0) {
+      const M3& D = diag_nto_[i];
+      const double D00 = D[0], D01 = D[1], D02 = D[2], D11 = D[4], D12 = D[5], D22 = D[8];
+      const double invA_nn = 1.0 / D00;   // inv(real), pe/math/shims/Invert.h:62-65
+      const double offdiag = D12 - invA_nn * D01 * D02;
+      // Mat2 A_breve (row-major) and Vec2 b_breve (HCSITS.h:2831-2832)
+      const double A00 = D11 - invA_nn * (D01 * D01), A01 = offdiag, A10 = offdiag,
+                   A11 = D22 - invA_nn * (D02 * D02);
+      const double bb0 = -gdot_nto[1] + invA_nn * D01 * gdot_nto[0];
+      const double bb1 = -gdot_nto[2] + invA_nn * D02 * gdot_nto[0];
+      const double shiftI = t_atan2(-D02, D01, crTrig);
+      const double shiftJ = t_atan2(-p_cf[2], p_cf[1], crTrig);
+      const double a3 = std::sqrt(D01 * D01 + D02 * D02);
+      const double fractionI = -D00 / (mu * a3);
+      const double fj = invA_nn * mu * ((-gdot_nto[0]) / std::sqrt(fsq) - a3 * t_cos(shiftI - shiftJ, crTrig));
+      const double fractionJ = std::min(fj, 1.0);
+      double alpha_left, alpha_right;
+      if (fractionJ < -1) {
+        const double angleI = t_acos(fractionI, crTrig);
+        alpha_left = -angleI - shiftI;
+        alpha_right = +angleI - shiftI;
+        if (alpha_left < 0) { alpha_left += 2 * kPi; alpha_right += 2 * kPi; }
+      } else if (D00 > mu * a3) {
+        const double angleJ = t_acos(fractionJ, crTrig);
+        alpha_left = -angleJ - shiftJ;
+        alpha_right = +angleJ - shiftJ;
+        if (alpha_left < 0) { alpha_left += 2 * kPi; alpha_right += 2 * kPi; }
+      } else {
+        const double angleJ = t_acos(fractionJ, crTrig);
+        double a1l = -angleJ - shiftJ, a1r = +angleJ - shiftJ;
+        if (a1l < 0) { a1l += 2 * kPi; a1r += 2 * kPi; }
+        const double angleI = t_acos(fractionI, crTrig);
+        double a2l = -angleI - shiftI, a2r = +angleI - shiftI;
+        if (a2l < 0) { a2l += 2 * kPi; a2r += 2 * kPi; }
+        if (a1l > a2l) { std::swap(a1l, a2l); std::swap(a1r, a2r); }
+        if (a2l > a1r) {
+          a2r -= 2 * kPi;
+          if (!(a2r > a1r)) a1r = a2r;
+        } else {
+          a1l = a2l;
+          if (!(a2r > a1r)) a1r = a2r;
+        }
+        alpha_left = a1l;
+        alpha_right = a1r;
+      }
+      // a point of the cone boundary at angle alpha and its energy (HCSITS.h:2924-2932)
+      auto eval = [&](double alpha, double& x0, double& x1) {
+        double r_ub = mu * (-gdot_nto[0]) / (D00 + mu * a3 * t_cos(alpha + shiftI, crTrig));
+        if (r_ub < 0) r_ub = kInf;
+        x0 = r_ub * t_cos(alpha, crTrig);
+        x1 = r_ub * t_sin(alpha, crTrig);
+        return 0.5 * x0 * (A00 * x0 + A01 * x1) + 0.5 * x1 * (A10 * x0 + A11 * x1) - (x0 * bb0 + x1 * bb1);
+      };
+      const double phi = 0.5 * (1.0 + std::sqrt(5.0));
+      double alpha_mid = (alpha_right + alpha_left * phi) / (1 + phi);
+      double xm0, xm1;
+      double f_mid = eval(alpha_mid, xm0, xm1);
+      bool leftlarger = false;
+      for (size_t k = 0; k < maxSubIterations; ++k) {
+        const double alpha_next = alpha_left + (alpha_right - alpha_mid);
+        double xn0, xn1;
+        const double f_next = eval(alpha_next, xn0, xn1);
+        if (leftlarger) {
+          if (f_next < f_mid) {
+            alpha_right = alpha_mid; alpha_mid = alpha_next; xm0 = xn0; xm1 = xn1; f_mid = f_next;
+            leftlarger = true;
+          } else {
+            alpha_left = alpha_next; leftlarger = false;
+          }
+        } else {
+          if (f_next < f_mid) {
+            alpha_left = alpha_mid; alpha_mid = alpha_next; xm0 = xn0; xm1 = xn1; f_mid = f_next;
+            leftlarger = false;
+          } else {
+            alpha_right = alpha_next; leftlarger = true;
+          }
+        }
+      }
+      // the direction of the unconstrained impulse, if cheaper (HCSITS.h:3001-3015)
+      {
+        const double alpha_init = t_atan2(p_cf[2], p_cf[1], crTrig);
+        double xi0, xi1;
+        const double f_init = eval(alpha_init, xi0, xi1);
+        if (f_init < f_mid) { xm0 = xi0; xm1 = xi1; }
+      }
+      p_cf[0] = invA_nn * (-gdot_nto[0] - D01 * xm0 - D02 * xm1);
+      p_cf[1] = xm0;
+      p_cf[2] = xm1;
     }
     const V3 p_wf = cf * p_cf;
     const V3 dp = p_[i] - p_wf;

@@ -3,6 +3,7 @@
 // detection -> body caching -> colouring -> contact caching -> sweeps -> integration.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -552,58 +553,79 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       CKS(sync_check(ctx));
       iters = (uint32_t)(*(unsigned long long*)ctx->hstage);
       schedule_used = PEHIP_SCHEDULE_SINGLE_BLOCK;
-    } else if (P.abort_threshold <= 0 && (P.sweep_schedule == PEHIP_SCHEDULE_DATAFLOW ||
-                                          (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && nc <= kDataflowMaxContacts))) {
-      // all sweeps in one cooperative launch (same Gauss-Seidel order per body as the colours)
-      DataflowBuffers D;
-      D.perm = ctx->perm; D.adj_start = ctx->col.adj_start; D.adj = ctx->col.adj; D.deg = ctx->col.deg;
-      D.inv = ctx->df_inv; D.rank1 = ctx->df_rank1; D.rank2 = ctx->df_rank2; D.done = ctx->df_done;
-      D.deltas = ctx->deltas; D.flags = ctx->flags; D.spin_limit = kDataflowSpinLimit;
-      fill_u32(ctx->flags, 0u, 1, s);
-      launch_dataflow_prepare(ctx->sol, nc, B, D, s);
-      if (ctx->profiling) CKS(record_launch_event(ctx, 0));
-      if (launch_sweep_dataflow(P.relaxation_model, exact, ctx->sol, nc, P.max_iterations, B, dtinv,
-                                (int)P.max_sub_iterations, D, ctx->ncu, s) != SWEEP_OK)
-        return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep launch failed");
-      if (ctx->profiling) CKS(record_launch_event(ctx, 1));
-      ctx->n_launch_events = ctx->profiling ? 2 : 0;
-      CK(hipMemcpyAsync(ctx->hstage, ctx->flags, 4, hipMemcpyDeviceToHost, s));
-      CKS(sync_check(ctx));
-      if (ctx->hstage[0] & ERR_TIMEOUT) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
-      iters = P.max_iterations;
-      schedule_used = PEHIP_SCHEDULE_DATAFLOW;
-      ++ctx->dataflow_steps;
     } else {
-      const uint32_t ncol = (uint32_t)ctx->colour_off.size() - 1;
-      size_t nev = 0;
-      schedule_used = PEHIP_SCHEDULE_COLOUR_LAUNCHES;
-      // pe's abort test (HCSITS.h:1935-1946 semantics: stop after the first sweep whose impulse
-      // change is below the threshold) runs on the device after every sweep; the sweeps queued
-      // behind a stop return at once, and the host looks at the flag every kAbortPoll sweeps.
-      const bool abort_on = P.abort_threshold > 0;
-      uint32_t* stop = ctx->flags + 4;
-      if (abort_on) fill_u32(stop, 0u, 2, s);
-      iters = P.max_iterations;
-      for (uint32_t it = 0; it < P.max_iterations; ++it) {
-        for (uint32_t k = 0; k < ncol; ++k) {
-          if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
-          launch_sweep(P.relaxation_model, exact, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
-                       (int)P.max_sub_iterations, ctx->deltas + it, abort_on ? stop : nullptr, ctx->ncu, s);
-          if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
-        }
-        if (!abort_on) continue;
-        launch_abort_check(ctx->deltas, it, P.abort_threshold, stop, s);
-        if ((it + 1) % kAbortPoll == 0 || it + 1 == P.max_iterations) {
-          CK(hipMemcpyAsync(ctx->hstage, stop, 8, hipMemcpyDeviceToHost, s));
+      // All sweeps in one cooperative launch (same Gauss-Seidel order per body as the colours).
+      // AUTO falls back to the colour launches (the same bits) when the device refuses the
+      // cooperative launch, e.g. while another process holds the device's cooperative-launch
+      // resources; nothing has run then.  An explicit DATAFLOW schedule reports the refusal.
+      bool swept = false;
+      if (P.abort_threshold <= 0 &&
+          (P.sweep_schedule == PEHIP_SCHEDULE_DATAFLOW ||
+           (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && nc <= kDataflowMaxContacts && !ctx->dataflow_refused))) {
+        DataflowBuffers D;
+        D.perm = ctx->perm; D.adj_start = ctx->col.adj_start; D.adj = ctx->col.adj; D.deg = ctx->col.deg;
+        D.inv = ctx->df_inv; D.rank1 = ctx->df_rank1; D.rank2 = ctx->df_rank2; D.done = ctx->df_done;
+        D.deltas = ctx->deltas; D.flags = ctx->flags; D.spin_limit = kDataflowSpinLimit;
+        fill_u32(ctx->flags, 0u, 1, s);
+        launch_dataflow_prepare(ctx->sol, nc, B, D, s);
+        if (ctx->profiling) CKS(record_launch_event(ctx, 0));
+        DataflowLaunchInfo info;
+        if (launch_sweep_dataflow(P.relaxation_model, exact, ctx->sol, nc, P.max_iterations, B, dtinv,
+                                  (int)P.max_sub_iterations, D, ctx->ncu, s, &info) != SWEEP_OK) {
+          const std::string why = std::string("dataflow sweep launch refused: occupancy query ") +
+                                  hipGetErrorName(info.occupancy_err) + ", " + std::to_string(info.per_cu) +
+                                  " workgroups per CU, " + std::to_string(ctx->ncu) + " CUs, grid " +
+                                  std::to_string(info.grid) + ", cooperative launch " +
+                                  hipGetErrorName(info.launch_err);
+          (void)hipGetLastError();  // the refusal is not a sticky error
+          if (P.sweep_schedule == PEHIP_SCHEDULE_DATAFLOW) return fail(ctx, PEHIP_HIP_ERROR, why);
+          ctx->dataflow_refused = true;
+          ctx->dataflow_refusal = why;
+          std::fprintf(stderr, "pe_amd: %s; this context uses colour launches from now on\n", why.c_str());
+        } else {
+          if (ctx->profiling) CKS(record_launch_event(ctx, 1));
+          ctx->n_launch_events = ctx->profiling ? 2 : 0;
+          CK(hipMemcpyAsync(ctx->hstage, ctx->flags, 4, hipMemcpyDeviceToHost, s));
           CKS(sync_check(ctx));
-          if (ctx->hstage[0]) {
-            iters = ctx->hstage[1];
-            break;
-          }
+          if (ctx->hstage[0] & ERR_TIMEOUT) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
+          iters = P.max_iterations;
+          schedule_used = PEHIP_SCHEDULE_DATAFLOW;
+          ++ctx->dataflow_steps;
+          swept = true;
         }
       }
-      // events of the launches that returned at once (after the stop) are not sweep launches
-      ctx->n_launch_events = std::min(nev, (size_t)2 * ncol * iters);
+      if (!swept) {
+        const uint32_t ncol = (uint32_t)ctx->colour_off.size() - 1;
+        size_t nev = 0;
+        schedule_used = PEHIP_SCHEDULE_COLOUR_LAUNCHES;
+        // pe's abort test (HCSITS.h:1935-1946 semantics: stop after the first sweep whose impulse
+        // change is below the threshold) runs on the device after every sweep; the sweeps queued
+        // behind a stop return at once, and the host looks at the flag every kAbortPoll sweeps.
+        const bool abort_on = P.abort_threshold > 0;
+        uint32_t* stop = ctx->flags + 4;
+        if (abort_on) fill_u32(stop, 0u, 2, s);
+        iters = P.max_iterations;
+        for (uint32_t it = 0; it < P.max_iterations; ++it) {
+          for (uint32_t k = 0; k < ncol; ++k) {
+            if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+            launch_sweep(P.relaxation_model, exact, ctx->sol, ctx->colour_off[k], ctx->colour_off[k + 1], B, dtinv,
+                         (int)P.max_sub_iterations, ctx->deltas + it, abort_on ? stop : nullptr, ctx->ncu, s);
+            if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+          }
+          if (!abort_on) continue;
+          launch_abort_check(ctx->deltas, it, P.abort_threshold, stop, s);
+          if ((it + 1) % kAbortPoll == 0 || it + 1 == P.max_iterations) {
+            CK(hipMemcpyAsync(ctx->hstage, stop, 8, hipMemcpyDeviceToHost, s));
+            CKS(sync_check(ctx));
+            if (ctx->hstage[0]) {
+              iters = ctx->hstage[1];
+              break;
+            }
+          }
+        }
+        // events of the launches that returned at once (after the stop) are not sweep launches
+        ctx->n_launch_events = std::min(nev, (size_t)2 * ncol * iters);
+      }
     }
   }
   mark(ctx, 4);

@@ -68,6 +68,8 @@ struct pehip_ctx {
   uint32_t *perm = nullptr, *sort_keys = nullptr, *sort_keys_tmp = nullptr, *sort_vals_tmp = nullptr;
   uint32_t *df_inv = nullptr, *df_rank1 = nullptr, *df_rank2 = nullptr, *df_done = nullptr;  // dataflow sweeps
   uint64_t dataflow_steps = 0;  // steps whose sweeps ran as one dataflow launch
+  bool dataflow_refused = false;  // the device refused a cooperative launch: AUTO uses colour launches
+  std::string dataflow_refusal;   // what the refused launch saw
   uint32_t *cscan_scratch = nullptr, *csort_scratch = nullptr;
   std::vector<uint32_t> colour_off;
   unsigned long long *deltas = nullptr, *maxpen = nullptr;

@@ -567,27 +567,58 @@ __device__ __forceinline__ bool group_read(bool x, uint32_t node) {
   else return __shfl(x ? 1 : 0, (int)node, 1 << SPEC) != 0;
 }
 
-// maxSub iterations.  SPEC = 0: one lane, as pe.  SPEC = d > 0: the 2^d lanes of a group run d
-// iterations per round -- lane j evaluates node j of the depth-d tree of possible next points
-// (heap order; the path to a node, read from the bits of j + 1 below the leading one, is a
-// sequence of comparison outcomes, and the bracket along it depends only on the points, not on
-// the objective values).  The comparisons are then resolved for all nodes at once: the best
-// value the sequential search holds on reaching node j is the running minimum (by the same
-// strict comparison) of the objective values at j's ancestors, so every lane compares its own
-// value against that fold of its ancestors' values, one ballot collects the outcomes, the
-// group walks the tree along them, and the bracket after the round is one golden_step from the
-// state at the last node on the walk.  Every evaluated point and every comparison is the one
-// the sequential search makes, bit for bit.
-template <class Trig, int SPEC>
-__device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, double flimit, double A00, double A01,
-                                               double A10, double A11, double gt, double go) {
+// Points of the decoupled model's search (HCSITS.h:2450-2470): the circle of radius flimit,
+// energy 1/2 x^T A x - x^T (-g) of the tangential block.
+template <class Trig>
+struct CircleEval {
+  double flimit, A00, A01, A10, A11, gt, go;
+  __device__ __forceinline__ double operator()(double a, double& x0, double& x1) const {
+    double sn, cn;
+    Trig::sincos(a, sn, cn);
+    x0 = flimit * cn;
+    x1 = flimit * sn;
+    return golden_objective(x0, x1, A00, A01, A10, A11, gt, go);
+  }
+};
+
+// Points of the generalized-maximum-dissipation search (HCSITS.h:2924-2932): the friction
+// cone's boundary under the normal constraint, radius num / (D00 + ma3 cos(a + shiftI)) with
+// num = mu (-gdot_n) and ma3 = mu a3 (infinite where negative), energy of the tangential
+// block after eliminating the normal row (A_, b_ = -(gt, go)).
+template <class Trig>
+struct ConeEval {
+  double num, D00, ma3, shiftI, A00, A01, A10, A11, gt, go;
+  __device__ __forceinline__ double operator()(double a, double& x0, double& x1) const {
+    double r_ub = num / (D00 + ma3 * Trig::cos(a + shiftI));
+    if (r_ub < 0) r_ub = __builtin_inf();
+    double sn, cn;
+    Trig::sincos(a, sn, cn);
+    x0 = r_ub * cn;
+    x1 = r_ub * sn;
+    return golden_objective(x0, x1, A00, A01, A10, A11, gt, go);
+  }
+};
+
+// maxSub iterations of the golden-section search from the bracket in S (S.am its interior
+// point); `eval` gives a point's impulse and energy.  SPEC = 0: one lane, as pe (the caller has
+// evaluated S.am).  SPEC = d > 0: the 2^d lanes of a group run d iterations per round -- lane
+// j evaluates node j of the depth-d tree of possible next points (heap order; the path to a
+// node, read from the bits of j + 1 below the leading one, is a sequence of comparison
+// outcomes, and the bracket along it depends only on the points, not on the energies).  The
+// comparisons are then resolved for all nodes at once: the best value the sequential search
+// holds on reaching node j is the running minimum (by the same strict comparison) of the
+// energies at j's ancestors, so every lane compares its own value against that fold of its
+// ancestors' values, one ballot collects the outcomes, the group walks the tree along them, and
+// the bracket after the round is one golden_step from the state at the last node on the walk.
+// Every evaluated point and every comparison is the one the sequential search makes, bit for
+// bit.  In the first round the group's spare lane evaluates S.am.
+template <int SPEC, class Eval>
+__device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const Eval& eval) {
   if constexpr (SPEC == 0) {
     for (int k = 0; k < maxSub; ++k) {
       const double an = S.al + (S.ar - S.am);
-      double sn, cn;
-      Trig::sincos(an, sn, cn);
-      const double xn0 = flimit * cn, xn1 = flimit * sn;
-      const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
+      double xn0, xn1;
+      const double fn = eval(an, xn0, xn1);
       golden_step(S, an, xn0, xn1, fn, fn < S.fm);
     }
   } else {
@@ -605,10 +636,8 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, doubl
       // initial point (the bracket's interior point am), which the comparisons need
       const bool init = k == 0 && lane == (uint32_t)(L - 1);
       const double an = init ? S.am : al + (ar - am);
-      double sn, cn;
-      Trig::sincos(an, sn, cn);
-      const double xn0 = flimit * cn, xn1 = flimit * sn;
-      const double fn = golden_objective(xn0, xn1, A00, A01, A10, A11, gt, go);
+      double xn0, xn1;
+      const double fn = eval(an, xn0, xn1);
       if (k == 0) {
         S.fm = group_read<SPEC>(fn, L - 1);
         S.xm0 = group_read<SPEC>(xn0, L - 1);
@@ -803,15 +832,41 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
             pc[2] *= fr;
           }
         } else {
-          // InelasticCoulombContactByDecoupling: 20 decoupled passes, HCSITS.h:2395-2520
+          // InelasticCoulombContactByDecoupling: 20 decoupled passes, HCSITS.h:2395-2520.
+          // Pass j computes the normal impulse c_j from the previous pass's tangential impulse
+          // t_{j-1} and then t_j from c_j alone, so the passes are a deterministic map of c.
+          // Once c_j equals c_{j-P} bit for bit (P = 1, 2, 3) the passes repeat with period P
+          // from pass j-P on, the result of pass 19 is that of pass j-P+((19-j+P) mod P), and
+          // the remaining passes are skipped: the same bits as running all 20.
           const double A00 = f[(CF_DTO + 0) * cap + i], A01 = f[(CF_DTO + 1) * cap + i];
           const double A10 = f[(CF_DTO + 2) * cap + i], A11 = f[(CF_DTO + 3) * cap + i];
+          double c1 = 0.0, c2 = 0.0, c3 = 0.0;  // c_{j-1}, c_{j-2}, c_{j-3}
+          double u1 = 0.0, u2 = 0.0;            // t_{j-2}
+          double w1 = 0.0, w2 = 0.0;            // t_{j-3}
           for (int j = 0; j < 20; ++j) {
+            const double tq1 = pc[1], tq2 = pc[2];  // t_{j-1}
             pc[0] = 0;
             const d3 tp = t * pc[1] + o * pc[2];
             d3 gc = gdot + cross3(mat_vec(I1, cross3(r1, tp)), r1) + cross3(mat_vec(I2, cross3(r2, tp)), r2);
             const double gcn = dot3(n, gc) + ddt;
             pc[0] = smax(0.0, -(dni * gcn));
+            const long long cb = __double_as_longlong(pc[0]);
+            if (j >= 1 && cb == __double_as_longlong(c1)) break;  // pass 19 = pass j: (c_j, t_{j-1})
+            if (j >= 2 && cb == __double_as_longlong(c2)) {
+              if (((19 - j) & 1) == 0) { pc[1] = u1; pc[2] = u2; }  // pass j: (c_j, t_{j-2})
+              else { pc[0] = c1; }                                  // pass j-1: (c_{j-1}, t_{j-1})
+              break;
+            }
+            if (j >= 3 && cb == __double_as_longlong(c3)) {
+              const int r = (19 - j) % 3;
+              if (r == 0) { pc[1] = w1; pc[2] = w2; }                   // pass j: (c_j, t_{j-3})
+              else if (r == 1) { pc[0] = c2; pc[1] = u1; pc[2] = u2; }  // pass j-2: (c_{j-2}, t_{j-2})
+              else { pc[0] = c1; }                                      // pass j-1: (c_{j-1}, t_{j-1})
+              break;
+            }
+            c3 = c2; c2 = c1; c1 = pc[0];
+            w1 = u1; w2 = u2;
+            u1 = tq1; u2 = tq2;
             pc[1] = pc[2] = 0.0;
             const d3 np0 = n * pc[0];
             gc = gdot + cross3(mat_vec(I1, cross3(r1, np0)), r1) + cross3(mat_vec(I2, cross3(r2, np0)), r2);
@@ -830,15 +885,10 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
               gs.ar = acos_f - shift;
               gs.am = (gs.ar + gs.al * phi) / (1 + phi);
               gs.left = false;
-              if (SPEC == 0 || maxSub <= 0) {
-                double sm, cm;
-                Trig::sincos(gs.am, sm, cm);
-                gs.xm0 = flimit * cm;
-                gs.xm1 = flimit * sm;
-                gs.fm = golden_objective(gs.xm0, gs.xm1, A00, A01, A10, A11, gt, go);
-              }
-              // (with SPEC > 0 the search's first round evaluates the initial point as well)
-              golden_section<Trig, SPEC>(gs, maxSub, flimit, A00, A01, A10, A11, gt, go);
+              const CircleEval<Trig> ev{flimit, A00, A01, A10, A11, gt, go};
+              // (with SPEC > 0 the search's first round evaluates the initial point)
+              if (SPEC == 0 || maxSub <= 0) gs.fm = ev(gs.am, gs.xm0, gs.xm1);
+              golden_section<SPEC>(gs, maxSub, ev);
               pc[1] = gs.xm0;
               pc[2] = gs.xm1;
             }

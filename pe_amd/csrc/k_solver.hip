@@ -378,7 +378,7 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
 template <class Trig, int SPEC>
 static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, uint32_t nc, uint32_t iters,
                                            const Bodies& B, double dtinv, int maxSub, const DataflowBuffers& D,
-                                           uint32_t ncu, hipStream_t s) {
+                                           uint32_t ncu, hipStream_t s, DataflowLaunchInfo* info) {
   void* args_fn = nullptr;
   switch (model) {
     case 0: args_fn = (void*)k_sweep_dataflow<0, Trig, 0>; break;
@@ -388,18 +388,20 @@ static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, 
     default: args_fn = (void*)k_sweep_dataflow<3, Trig, SPEC>; break;
   }
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)args_fn, 256, 0) != hipSuccess || per_cu < 1)
-    return SWEEP_LAUNCH_FAILED;
+  info->occupancy_err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)args_fn, 256, 0);
+  info->per_cu = per_cu;
+  if (info->occupancy_err != hipSuccess || per_cu < 1) return SWEEP_LAUNCH_FAILED;
   const uint64_t want = ((uint64_t)iters * nc * 64 + 255) / 256;  // one wavefront per item at most
   const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)per_cu * ncu, std::max<uint64_t>(want, 1));
+  info->grid = grid;
   SolverContacts Cc = C;
   const double* sb = B.sb;
   double* dvw = B.dvw;
   uint32_t spin = D.spin_limit;
   void* args[] = {&Cc, &nc, &iters, &sb, &dvw, &dtinv, &maxSub, (void*)&D.deg, (void*)&D.rank1, (void*)&D.rank2,
                   (void*)&D.done, (void*)&D.deltas, (void*)&D.flags, &spin};
-  if (hipLaunchCooperativeKernel(args_fn, dim3(grid), dim3(256), args, 0, s) != hipSuccess) return SWEEP_LAUNCH_FAILED;
-  return SWEEP_OK;
+  info->launch_err = hipLaunchCooperativeKernel(args_fn, dim3(grid), dim3(256), args, 0, s);
+  return info->launch_err == hipSuccess ? SWEEP_OK : SWEEP_LAUNCH_FAILED;
 }
 
 void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies& B, const DataflowBuffers& D,
@@ -413,11 +415,12 @@ void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies&
 
 pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
                                          uint32_t iters, const Bodies& B, double dtinv, int maxSub,
-                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s) {
+                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s,
+                                         DataflowLaunchInfo* info) {
   if (!nc || !iters) return SWEEP_OK;
   // a whole wavefront owns each item, so the speculated search costs no parallelism here
-  if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
-  return dataflow_dispatch<TrigFast, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s);
+  if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s, info);
+  return dataflow_dispatch<TrigFast, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s, info);
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,

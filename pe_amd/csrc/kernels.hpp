@@ -123,13 +123,21 @@ struct DataflowBuffers {
   uint32_t spin_limit;
 };
 enum pehip_sweep_status { SWEEP_OK = 0, SWEEP_LAUNCH_FAILED = 1 };
+// What a failed dataflow launch saw: the HIP error of the occupancy query and of the launch,
+// the resident workgroups per compute unit and the grid asked for.
+struct DataflowLaunchInfo {
+  hipError_t occupancy_err = hipSuccess, launch_err = hipSuccess;
+  int per_cu = 0;
+  uint32_t grid = 0;
+};
 // The per-body ranks and cleared counters the dataflow sweeps wait on (three small kernels) ...
 void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies& B, const DataflowBuffers& D,
                              hipStream_t s);
 // ... then all `iters` sweeps in one cooperative launch.
 pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
                                          uint32_t iters, const Bodies& B, double dtinv, int maxSub,
-                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s);
+                                         const DataflowBuffers& D, uint32_t ncu, hipStream_t s,
+                                         DataflowLaunchInfo* info);
 
 // One colour of one sweep; ncu: compute units of the device (sizes the lanes per contact).  A
 // non-null stop makes the launch return at once once stop[0] is set (see launch_abort_check).

@@ -3,12 +3,13 @@
 // Usage: granular_dropin <nx> <ny> <nz> <steps>; prints the final state as hex floats.
 #include <cstdio>
 #include <cstdlib>
+#include <exception>
 
 #include <pe_amd/pe.hpp>
 
 using namespace pe;
 
-int main(int argc, char** argv) {
+static int run(int argc, char** argv) {
   const int nx = argc > 1 ? std::atoi(argv[1]) : 5;
   const int ny = argc > 2 ? std::atoi(argv[2]) : 5;
   const int nz = argc > 3 ? std::atoi(argv[3]) : 5;
@@ -47,4 +48,14 @@ int main(int argc, char** argv) {
     std::printf("%zu %a %a %a %a %a %a\n", b, p[0], p[1], p[2], v[0], v[1], v[2]);
   }
   return 0;
+}
+
+int main(int argc, char** argv) {
+  // pe reports failures as exceptions (std::invalid_argument, std::runtime_error)
+  try {
+    return run(argc, argv);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "granular_dropin: %s\n", e.what());
+    return 1;
+  }
 }

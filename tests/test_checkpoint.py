@@ -162,7 +162,9 @@ def test_rejects_bad_files(tmp_path):
 def test_restart_from_file_on_gpu(tmp_path):
     # a run interrupted by a body file continues with the same bodies: same state and the same
     # contact set (up to pe's reader body order) as the world it was written from
-    sc = scenes.drop_scene(4, 4, 3, r=0.45, spacing=1.0, box=(8.0, 8.0, 20.0), v_amp=0.5, seed=4)
+    # a nearly touching lattice just above the floor: dozens of contacts by step 60
+    sc = scenes.drop_scene(4, 4, 3, r=0.45, spacing=0.902, jitter=0.001, z0=0.451, box=(8.0, 8.0, 20.0),
+                           v_amp=0.1, seed=4)
     w = World()
     sc.to_world(w)
     w.run(60, sc.dt)

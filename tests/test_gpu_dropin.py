@@ -37,8 +37,10 @@ def test_cpp_dropin_matches_python_world():
     if not os.path.exists(EXE):
         pytest.fail("drop-in example not built")
     nx, ny, nz, steps = 5, 4, 3, 220
-    out = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps)], capture_output=True, text=True, check=True,
-                         timeout=120).stdout.split("\n")
+    r = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps)], capture_output=True, text=True, timeout=120)
+    print(r.stderr)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout.split("\n")
     head = out[0].split()
     rows = [l.split() for l in out[1:] if l.strip()]
     pos = np.array([[float.fromhex(x) for x in r[1:4]] for r in rows])

@@ -481,3 +481,94 @@ def test_orthogonal_projection_resting_unit_sphere(model):
     p = w.impulses()[0]
     assert p[2] == pytest.approx(g * dt, rel=1e-14) and p[0] == 0 and p[1] == 0
     assert np.all(np.abs(w.get_state()["v"][1]) < 1e-15)
+
+
+# ------------------------------------------- generalized maximum dissipation (model 5)
+def test_gmd_sliding_sphere_on_cone():
+    # A unit-mass sphere sliding and pressing on a plane, one sweep from p = 0.  Its lever arm is
+    # parallel to n, so the normal row decouples: p_n = -gdot_n / D_nn with D_nn = 1/m, and the
+    # tangential block is isotropic, so the energy on the circle |p_t| = mu p_n is least where
+    # p_t points along -gdot_t: the friction impulse opposes the slip with magnitude mu p_n.
+    mu, dt, g, r = 0.4, 1e-3, 9.81, 0.5
+    rho = 1.0 / (4.0 / 3.0 * math.pi * r ** 3)            # m = 1
+    w = make_world(gravity=(0, 0, -g), model=O.INELASTIC_GMD, iters=1, mu=mu, density=rho)
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_sphere([0, 0, r], r, 0)
+    st = w.get_state()
+    vel = np.array([3.0, -2.0, -1.5])
+    st["v"][1] = vel
+    w.set_state(st)
+    w.step(dt)
+    p = w.impulses()[0]
+    gdot = vel + np.array([0.0, 0.0, -g * dt])
+    assert p[2] == pytest.approx(-gdot[2], rel=1e-14)
+    pt = np.hypot(p[0], p[1])
+    assert pt == pytest.approx(mu * p[2], rel=1e-12)
+    slip = gdot[:2] / np.hypot(gdot[0], gdot[1])
+    assert np.dot(p[:2] / pt, -slip) > 1 - 1e-9
+    # the sphere stops pressing: zero normal velocity after the step
+    assert abs(w.get_state()["v"][1][2]) < 1e-14
+
+
+def _skew(r):
+    return np.array([[0.0, -r[2], r[1]], [r[2], 0.0, -r[0]], [-r[1], r[0], 0.0]])
+
+
+def test_gmd_sphere_on_box_face_maximum_dissipation():
+    # A sphere sliding onto a box's top face off the box's centre: the box's lever arm is not
+    # parallel to n, so normal and tangential impulses couple (D_nt != 0).  One contact, one
+    # sweep from p = 0 from a state with no predictor terms (no gravity, no spin).  Checked
+    # against the mathematics, not the oracle's formulas: (1) the normal relative velocity
+    # vanishes after the solve, (2) the impulse is on the friction cone, (3) among impulses on the
+    # cone boundary that satisfy (1), it has the least energy 1/2 p^T K p + p^T gdot (brute force
+    # over the boundary's angle with K built from the bodies' masses and inertias).
+    mu, dt = 0.5, 1e-3
+    w = make_world(model=O.INELASTIC_GMD, iters=1, mu=mu)
+    lx, ly, lz = 2.0, 1.5, 0.6
+    rs = 0.3
+    xs = np.array([0.45, -0.3, lz / 2 + rs])
+    w.create_box([0.0, 0.0, 0.0], [lx, ly, lz], 0)
+    w.create_sphere(xs.tolist(), rs, 0)
+    st = w.get_state()
+    v1 = np.array([2.0, -1.0, -1.0])
+    st["v"][1] = v1
+    w.set_state(st)
+    w.step(dt)
+    c = w.contacts()
+    assert len(c["b1"]) == 1 and int(c["b1"][0]) == 1 and int(c["b2"][0]) == 0
+    n, xc, dist = c["normal"][0], c["pos"][0], c["dist"][0]
+    p = w.impulses()[0]
+    info = w.body_info()
+    m1, m2 = 1.0 / info["inv_mass"][1], 1.0 / info["inv_mass"][0]
+    I1inv = np.linalg.inv(0.4 * m1 * rs ** 2 * np.eye(3))
+    I2inv = np.linalg.inv(np.diag([m2 / 12 * (ly ** 2 + lz ** 2), m2 / 12 * (lx ** 2 + lz ** 2),
+                                   m2 / 12 * (lx ** 2 + ly ** 2)]))
+    r1, r2 = xc - xs, xc - np.zeros(3)
+    K = (1 / m1 + 1 / m2) * np.eye(3) - _skew(r1) @ I1inv @ _skew(r1) - _skew(r2) @ I2inv @ _skew(r2)
+    gdot = v1.copy()
+    gdot_n = gdot @ n + dist / dt
+    # (1) normal relative velocity after the solve
+    post = w.get_state()
+    vv1, ww1, vv2, ww2 = post["v"][1], post["w"][1], post["v"][0], post["w"][0]
+    g_after = (vv1 - vv2) + np.cross(ww1, r1) - np.cross(ww2, r2)
+    assert abs(g_after @ n + dist / dt) < 1e-12 * np.linalg.norm(v1)
+    # (2) on the cone
+    pn = p @ n
+    pt = p - pn * n
+    assert pn > 0 and np.linalg.norm(pt) == pytest.approx(mu * pn, rel=1e-12)
+    # (3) least energy on the boundary curve
+    e1 = pt / np.linalg.norm(pt)
+    e2 = np.cross(n, e1)
+    th = np.linspace(-math.pi, math.pi, 200001)
+    u = np.outer(np.cos(th), e1) + np.outer(np.sin(th), e2)
+    knn, knt = n @ K @ n, u @ (K @ n)
+    rad = mu * (-gdot_n) / (knn + mu * knt)
+    ok = rad > 0
+    pts = rad[ok, None] * u[ok]
+    pns = (-gdot_n - pts @ (K @ n)) / knn
+    P = pts + pns[:, None] * n
+    energy = 0.5 * np.einsum("ij,jk,ik->i", P, K, P) + P @ gdot
+    e_res = 0.5 * p @ K @ p + p @ gdot
+    k = int(np.argmin(energy))
+    assert e_res <= energy[k] + 1e-9 * abs(energy[k])
+    assert abs(th[ok][k]) < 1e-3   # the minimiser's direction is the returned impulse's (th = 0)
