@@ -121,7 +121,8 @@ pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
       p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
       p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_ORTHO &&
       p->relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING &&
-      p->relaxation_model != PEHIP_INELASTIC_COULOMB_ORTHO) {
+      p->relaxation_model != PEHIP_INELASTIC_COULOMB_ORTHO &&
+      p->relaxation_model != PEHIP_INELASTIC_GMD) {
     why = "Unsupported relaxation model.";  // HCSITS.h:1926
     return PEHIP_NOT_SUPPORTED;
   }
@@ -540,7 +541,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
     } else if (P.sweep_schedule == PEHIP_SCHEDULE_SINGLE_BLOCK ||
                (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && P.abort_threshold > 0 &&
                 P.relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING &&
-                nc <= kSingleBlockMaxContacts)) {
+                P.relaxation_model != PEHIP_INELASTIC_GMD && nc <= kSingleBlockMaxContacts)) {
       ColourOffsets O;
       O.ncol = (uint32_t)ctx->colour_off.size() - 1;
       for (uint32_t k = 0; k <= O.ncol; ++k) O.off[k] = ctx->colour_off[k];

@@ -685,7 +685,8 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const
 
 // One contact's NBGS update: HCSITS.h:2341-2536 (InelasticCoulombContactByDecoupling),
 // 2141-2290 (ApproximateInelasticCoulombContactByDecoupling), 2048-2093
-// (InelasticFrictionlessContact).  Returns |p_old - p_new|_inf (or |p_old|_inf for a
+// (InelasticFrictionlessContact), 2631-2723 (orthogonal projections, models 2 and 4),
+// 2756-3040 (InelasticGeneralizedMaximumDissipationContact, model 5).  Returns |p_old - p_new|_inf (or |p_old|_inf for a
 // separating contact).  The caller guarantees that no other thread touches the
 // non-fixed bodies of contact i concurrently.  Trig = TrigFast (device libm) or TrigExact
 // (correctly rounded, reproduces the host libm's golden-section decisions).
@@ -831,6 +832,74 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
             pc[1] *= fr;
             pc[2] *= fr;
           }
+        } else if constexpr (MODEL == 5) {
+          // InelasticGeneralizedMaximumDissipationContact, HCSITS.h:2806-3022: the impulse on
+          // the friction cone's boundary that meets the normal constraint with the least energy
+          // of its tangential part, by golden section over the angle, bracketed where the
+          // boundary's radius is positive (angleI) and cheaper than the unconstrained direction
+          // (angleJ); the unconstrained direction replaces the result if cheaper.
+          constexpr double kTwoPi = 2.0 * 3.141592653589793;   // 2 * M_PI, pe/math/Constants.h:163
+          const double D00 = f[CF_D0 * cap + i], D01 = f[(CF_D0 + 1) * cap + i], D02 = f[(CF_D0 + 2) * cap + i];
+          const double D11 = f[CF_DTO * cap + i], D12 = f[(CF_DTO + 1) * cap + i];
+          const double D22 = f[(CF_DTO + 3) * cap + i];
+          const double invA_nn = 1.0 / D00;
+          const double offdiag = D12 - invA_nn * D01 * D02;
+          const double A00 = D11 - invA_nn * (D01 * D01), A11 = D22 - invA_nn * (D02 * D02);
+          const double bb0 = -g.y + invA_nn * D01 * g.x;
+          const double bb1 = -g.z + invA_nn * D02 * g.x;
+          const double shiftI = Trig::atan2(-D02, D01);
+          const double shiftJ = Trig::atan2(-pc[2], pc[1]);
+          const double a3 = sqrt(D01 * D01 + D02 * D02);
+          const double fractionI = -D00 / (mu * a3);
+          const double fj = invA_nn * mu * ((-g.x) / sqrt(fsq) - a3 * Trig::cos(shiftI - shiftJ));
+          const double fractionJ = (1.0 < fj) ? 1.0 : fj;   // std::min(fj, 1)
+          GoldenState gs;
+          if (fractionJ < -1) {
+            const double angleI = Trig::acos(fractionI);
+            gs.al = -angleI - shiftI;
+            gs.ar = +angleI - shiftI;
+            if (gs.al < 0) { gs.al += kTwoPi; gs.ar += kTwoPi; }
+          } else if (D00 > mu * a3) {
+            const double angleJ = Trig::acos(fractionJ);
+            gs.al = -angleJ - shiftJ;
+            gs.ar = +angleJ - shiftJ;
+            if (gs.al < 0) { gs.al += kTwoPi; gs.ar += kTwoPi; }
+          } else {
+            const double angleJ = Trig::acos(fractionJ);
+            double a1l = -angleJ - shiftJ, a1r = +angleJ - shiftJ;
+            if (a1l < 0) { a1l += kTwoPi; a1r += kTwoPi; }
+            const double angleI = Trig::acos(fractionI);
+            double a2l = -angleI - shiftI, a2r = +angleI - shiftI;
+            if (a2l < 0) { a2l += kTwoPi; a2r += kTwoPi; }
+            if (a1l > a2l) {
+              const double tl = a1l, tr = a1r;
+              a1l = a2l; a1r = a2r; a2l = tl; a2r = tr;
+            }
+            if (a2l > a1r) {
+              a2r -= kTwoPi;
+              if (!(a2r > a1r)) a1r = a2r;
+            } else {
+              a1l = a2l;
+              if (!(a2r > a1r)) a1r = a2r;
+            }
+            gs.al = a1l;
+            gs.ar = a1r;
+          }
+          const double phi = 0.5 * (1.0 + sqrt(5.0));
+          gs.am = (gs.ar + gs.al * phi) / (1 + phi);
+          gs.left = false;
+          const ConeEval<Trig> ev{mu * (-g.x), D00, mu * a3, shiftI, A00, offdiag, offdiag, A11, -bb0, -bb1};
+          // (with SPEC > 0 the search's first round evaluates the initial point)
+          if (SPEC == 0 || maxSub <= 0) gs.fm = ev(gs.am, gs.xm0, gs.xm1);
+          golden_section<SPEC>(gs, maxSub, ev);
+          {
+            double xi0, xi1;
+            const double f_init = ev(Trig::atan2(pc[2], pc[1]), xi0, xi1);
+            if (f_init < gs.fm) { gs.xm0 = xi0; gs.xm1 = xi1; }
+          }
+          pc[0] = invA_nn * (-g.x - D01 * gs.xm0 - D02 * gs.xm1);
+          pc[1] = gs.xm0;
+          pc[2] = gs.xm1;
         } else {
           // InelasticCoulombContactByDecoupling: 20 decoupled passes, HCSITS.h:2395-2520.
           // Pass j computes the normal impulse c_j from the previous pass's tangential impulse

@@ -435,6 +435,7 @@ static void batch_dispatch(int model, const Bodies& B, const RawContacts& R, con
     case 1: hipLaunchKernelGGL((k_batch<1, Trig>), g, b, 0, s, B, R, C, A); break;
     case 2: hipLaunchKernelGGL((k_batch<2, Trig>), g, b, 0, s, B, R, C, A); break;
     case 4: hipLaunchKernelGGL((k_batch<4, Trig>), g, b, 0, s, B, R, C, A); break;
+    case 5: hipLaunchKernelGGL((k_batch<5, Trig>), g, b, 0, s, B, R, C, A); break;
     default: hipLaunchKernelGGL((k_batch<3, Trig>), g, b, 0, s, B, R, C, A); break;
   }
 }

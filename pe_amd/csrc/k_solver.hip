@@ -228,6 +228,7 @@ static void block_dispatch(int model, const SolverContacts& C, const ColourOffse
     case 1: hipLaunchKernelGGL((k_sweep_block<1, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 2: hipLaunchKernelGGL((k_sweep_block<2, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 4: hipLaunchKernelGGL((k_sweep_block<4, Trig, 0, 512>), g, dim3(512), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 5: hipLaunchKernelGGL((k_sweep_block<5, Trig, 4, 256>), g, dim3(256), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     default: hipLaunchKernelGGL((k_sweep_block<3, Trig, 4, 256>), g, dim3(256), 0, s, C, O, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
 }
@@ -338,16 +339,35 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
 // contact than one lane per contact.  Colours are small in the latency-bound regime, so the
 // largest group (64, 16, 4, 1) with which every wavefront of the colour is resident at once wins;
 // big colours fall back to more contacts per wavefront (occupancy queried once).
-template <class Trig, int SPEC>
+template <int MODEL, class Trig, int SPEC>
 static uint64_t resident_lanes(uint32_t ncu) {
   static int blocks = -1;
   if (blocks < 0) {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)k_sweep<3, Trig, SPEC>, 256, 0) != hipSuccess || b < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (const void*)k_sweep<MODEL, Trig, SPEC>, 256, 0) != hipSuccess ||
+        b < 1)
       b = 1;
     blocks = b;
   }
   return (uint64_t)blocks * 256u * ncu;
+}
+
+// A colour of a searching model (3, 5): the largest lane group per contact with which every
+// wavefront of the colour is resident at once.
+template <int MODEL, class Trig>
+static void search_sweep(const SolverContacts& C, uint32_t begin, uint32_t end, const Bodies& B, double dtinv,
+                         int maxSub, unsigned long long* delta, const uint32_t* stop, uint32_t ncu, hipStream_t s) {
+  const uint64_t n = end - begin;
+  const dim3 b(256);
+  auto grid = [&](int spec) { return dim3((unsigned)(((n << spec) + 255) / 256)); };
+  if ((n << 6) <= resident_lanes<MODEL, Trig, 6>(ncu))
+    hipLaunchKernelGGL((k_sweep<MODEL, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
+  else if ((n << 4) <= resident_lanes<MODEL, Trig, 4>(ncu))
+    hipLaunchKernelGGL((k_sweep<MODEL, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
+  else if ((n << 2) <= resident_lanes<MODEL, Trig, 2>(ncu))
+    hipLaunchKernelGGL((k_sweep<MODEL, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
+  else
+    hipLaunchKernelGGL((k_sweep<MODEL, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
 }
 
 template <class Trig>
@@ -362,16 +382,8 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
     case 1: hipLaunchKernelGGL((k_sweep<1, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
     case 2: hipLaunchKernelGGL((k_sweep<2, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
     case 4: hipLaunchKernelGGL((k_sweep<4, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop); break;
-    default:
-      if ((n << 6) <= resident_lanes<Trig, 6>(ncu))
-        hipLaunchKernelGGL((k_sweep<3, Trig, 6>), grid(6), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
-      else if ((n << 4) <= resident_lanes<Trig, 4>(ncu))
-        hipLaunchKernelGGL((k_sweep<3, Trig, 4>), grid(4), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
-      else if ((n << 2) <= resident_lanes<Trig, 2>(ncu))
-        hipLaunchKernelGGL((k_sweep<3, Trig, 2>), grid(2), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
-      else
-        hipLaunchKernelGGL((k_sweep<3, Trig, 0>), grid(0), b, 0, s, C, begin, end, B.sb, B.dvw, dtinv, maxSub, delta, stop);
-      break;
+    case 5: search_sweep<5, Trig>(C, begin, end, B, dtinv, maxSub, delta, stop, ncu, s); break;
+    default: search_sweep<3, Trig>(C, begin, end, B, dtinv, maxSub, delta, stop, ncu, s); break;
   }
 }
 
@@ -385,6 +397,7 @@ static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, 
     case 1: args_fn = (void*)k_sweep_dataflow<1, Trig, 0>; break;
     case 2: args_fn = (void*)k_sweep_dataflow<2, Trig, 0>; break;
     case 4: args_fn = (void*)k_sweep_dataflow<4, Trig, 0>; break;
+    case 5: args_fn = (void*)k_sweep_dataflow<5, Trig, SPEC>; break;
     default: args_fn = (void*)k_sweep_dataflow<3, Trig, SPEC>; break;
   }
   int per_cu = 0;
@@ -454,6 +467,7 @@ static void serial_dispatch(int model, const SolverContacts& C, uint32_t nc, uin
     case 1: hipLaunchKernelGGL((k_sweep_serial<1, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 2: hipLaunchKernelGGL((k_sweep_serial<2, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     case 4: hipLaunchKernelGGL((k_sweep_serial<4, Trig, 0>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
+    case 5: hipLaunchKernelGGL((k_sweep_serial<5, Trig, 6>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
     default: hipLaunchKernelGGL((k_sweep_serial<3, Trig, 6>), g, b, 0, s, C, nc, iters, B.sb, B.dvw, dtinv, maxSub, thr, deltas); break;
   }
 }

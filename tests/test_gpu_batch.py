@@ -57,10 +57,13 @@ def test_batch_contact_sets_bitexact():
         assert g == ex, "scene %d (%s)" % (k, scs[k].name)
 
 
-def test_batch_step_equals_gs_in_colour_order():
+@pytest.mark.parametrize("model", [3, 5])
+def test_batch_step_equals_gs_in_colour_order(model):
     scs = batch_scenes()
     ows = oracle_states(scs, 10, cr_trig=True)
-    bw = BatchWorld(scs, exact_trig=True)
+    for ow, _ in ows:
+        ow.L.orc_set_relaxation_model(ow.h, model)
+    bw = BatchWorld(scs, exact_trig=True, relaxation_model=model)
     bw.setState(concat_states([st for _, st in ows]))
     bw.run(1)
     g1 = bw.getState()

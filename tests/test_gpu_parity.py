@@ -250,20 +250,24 @@ def test_reference_order_step_matches_oracle(name, model):
     assert perr <= (1e-12 if model != 3 else 1e-7), perr
 
 
+@pytest.mark.parametrize("model", [3, 5])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_reference_order_step_matches_correctly_rounded_oracle(name):
+def test_reference_order_step_matches_correctly_rounded_oracle(name, model):
     # Against the oracle with correctly rounded trig (quad precision), the device's correctly
     # rounded trig leaves nothing to chance: velocities and every impulse agree to 1e-12.
+    # Model 5 (generalized maximum dissipation) searches with trig as model 3 does.
     sc = SCENES[name]()
     ow, st0 = oracle_state_after(sc, 12, cr_trig=True)
+    ow.L.orc_set_relaxation_model(ow.h, model)
     ow.step(sc.dt)
     c = ow.contacts()
     st1 = ow.get_state()
-    w = gpu_for(sc, mode=SWEEP_REFERENCE_ORDER)
+    w = gpu_for(sc, model=model, mode=SWEEP_REFERENCE_ORDER)
     w.setState(st0)
     w.setContactOrder(c["b1"], c["b2"])
     w.simulationStep(sc.dt)
     g1 = w.getState()
+    assert len(c["b1"]) > 10
     for k in ("v", "w", "pos", "q"):
         assert rel_err(g1[k], st1[k]) <= 1e-12, k
     assert rel_err(w.impulses()[np.argsort(order_index(w.contacts(), c))], ow.impulses()) <= 1e-12
@@ -315,15 +319,17 @@ def test_reference_order_c1_window():
 
 
 # --------------------------------------------------------------------------- coloured mode
+@pytest.mark.parametrize("model", [3, 5])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_coloured_step_equals_gs_in_colour_order(name):
+def test_coloured_step_equals_gs_in_colour_order(name, model):
     # Contacts of one colour share no non-fixed body, so a coloured sweep is exactly a
     # sequential Gauss-Seidel sweep in colour-major order; replay that order on the oracle.
     # Exact-trig mode is compared with the oracle in its correctly rounded trig mode: the
     # host libm's rare non correctly rounded results would flip golden-section decisions.
     sc = SCENES[name]()
     ow, st0 = oracle_state_after(sc, 10, cr_trig=True)
-    w = gpu_for(sc, iters=30, exact_trig=True)
+    ow.L.orc_set_relaxation_model(ow.h, model)
+    w = gpu_for(sc, iters=30, model=model, exact_trig=True)
     w.setState(st0)
     w.simulationStep(sc.dt)
     g1 = w.getState()
@@ -359,16 +365,17 @@ def test_coloured_projection_step_equals_gs_in_colour_order(name, model):
         assert rel_err(g1[k], o1[k]) <= 1e-12, k
 
 
+@pytest.mark.parametrize("model", [3, 5])
 @pytest.mark.parametrize("exact", [False, True])
 @pytest.mark.parametrize("name", sorted(SCENES))
-def test_dataflow_sweeps_equal_colour_launches(name, exact):
+def test_dataflow_sweeps_equal_colour_launches(name, exact, model):
     # All sweeps in one cooperative launch, each contact waiting only for its own bodies' earlier
     # updates: the same per-body update order as the colour launches, so the same bits.
     sc = SCENES[name]()
     _, st0 = oracle_state_after(sc, 10)
     out = []
     for schedule in (1, 2):
-        w = gpu_for(sc, iters=30, exact_trig=exact)
+        w = gpu_for(sc, iters=30, model=model, exact_trig=exact)
         w.setSweepSchedule(schedule)
         w.setState(st0)
         w.run(3, sc.dt)
@@ -379,7 +386,8 @@ def test_dataflow_sweeps_equal_colour_launches(name, exact):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("model,abort", [(3, 0.0), (1, 0.0), (0, 0.0), (1, 1e-3), (3, 1e-3), (2, 0.0), (4, 1e-3)])
+@pytest.mark.parametrize("model,abort", [(3, 0.0), (1, 0.0), (0, 0.0), (1, 1e-3), (3, 1e-3), (2, 0.0), (4, 1e-3),
+                                         (5, 0.0), (5, 1e-3)])
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_single_block_sweeps_equal_colour_launches(name, model, abort):
     # All sweeps in one workgroup with a barrier between colours and the abort test on the
