@@ -6,7 +6,8 @@
 //   theWorld(), World::run/simulationStep/setGravity/setDamping   (pe/core/World.h, src/core/World.cpp:330-378)
 //   theCollisionSystem() and its HCSITS setters/getters            (HCSITS.h:198-260, 715-840)
 //   createMaterial                                                  (src/core/Materials.cpp:175-247)
-//   createSphere / createBox / createPlane                          (Sphere.cpp:1187, Box.cpp:1312, Plane.cpp:1202)
+//   createSphere / createBox / createCapsule / createPlane          (Sphere.cpp:1187, Box.cpp:1312,
+//                                                                    Capsule.cpp:1077, Plane.cpp:1202)
 // Body handles (SphereID, BoxID, PlaneID) read and write their state through the device;
 // the state is downloaded lazily after a step and uploaded before the next one.
 #ifndef PE_AMD_PE_HPP
@@ -62,7 +63,7 @@ struct Quat {
 
 typedef size_t MaterialID;
 
-enum BodyType { sphereType = PEHIP_SPHERE, boxType = PEHIP_BOX, planeType = PEHIP_PLANE };
+enum BodyType { sphereType = PEHIP_SPHERE, boxType = PEHIP_BOX, planeType = PEHIP_PLANE, capsuleType = PEHIP_CAPSULE };
 
 namespace detail {
 
@@ -135,6 +136,14 @@ class Box : public RigidBody {
   friend class detail::Engine;
   using RigidBody::RigidBody;
 };
+class Capsule : public RigidBody {
+ public:
+  real getRadius() const;
+  real getLength() const;
+ private:
+  friend class detail::Engine;
+  using RigidBody::RigidBody;
+};
 class Plane : public RigidBody {
  public:
   Vec3 getNormal() const;
@@ -147,6 +156,7 @@ class Plane : public RigidBody {
 typedef RigidBody* BodyID;
 typedef Sphere* SphereID;
 typedef Box* BoxID;
+typedef Capsule* CapsuleID;
 typedef Plane* PlaneID;
 
 namespace detail {
@@ -374,6 +384,8 @@ inline void RigidBody::addTorque(const Vec3& t) {
   e.forces_pending = true;
 }
 inline real Sphere::getRadius() const { return detail::E().param[3 * index_]; }
+inline real Capsule::getRadius() const { return detail::E().param[3 * index_]; }
+inline real Capsule::getLength() const { return detail::E().param[3 * index_ + 1]; }
 inline Vec3 Box::getLengths() const {
   const detail::Engine& e = detail::E();
   return Vec3(e.param[3 * index_], e.param[3 * index_ + 1], e.param[3 * index_ + 2]);
@@ -421,6 +433,26 @@ inline BoxID createBox(id_t uid, const Vec3& gpos, const Vec3& lengths, Material
   I[8] = m / static_cast<real>(12) * (lengths[0] * lengths[0] + lengths[1] * lengths[1]);
   const real prm[3] = {lengths[0], lengths[1], lengths[2]};
   return e.add<Box>(boxType, uid, material, false, prm, gpos, Quat(), m, real(1) / m, I, Vec3(), 0);
+}
+
+// createCapsule (Capsule.cpp:1077) with CapsuleBase's mass properties (CapsuleBase.cpp:60-92,
+// 147-168): the axis is the body x-axis, the moments combine the cylinder and the end caps.
+inline CapsuleID createCapsule(id_t uid, const Vec3& gpos, real radius, real length, MaterialID material,
+                               bool visible = true) {
+  (void)visible;
+  if (radius <= real(0)) throw std::invalid_argument("Invalid capsule radius");
+  if (length <= real(0)) throw std::invalid_argument("Invalid capsule length");
+  detail::Engine& e = detail::E();
+  const real m = radius * radius * M_PI * (real(4) / real(3) * radius + length) * e.materials.at(material).density;
+  const real density = m / (M_PI * radius * radius * (real(4) / real(3) * radius + length));
+  const real sphereMass = real(4) / real(3) * M_PI * radius * radius * radius * density;
+  const real cylinderMass = M_PI * radius * radius * length * density;
+  real I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  I[0] = radius * radius * (real(0.5) * cylinderMass + real(0.4) * sphereMass);
+  I[4] = I[8] = cylinderMass * (real(0.25) * radius * radius + real(1) / real(12) * length * length) +
+                sphereMass * (real(0.4) * radius * radius + real(0.375) * radius * length + real(0.25) * length * length);
+  const real prm[3] = {radius, length, 0};
+  return e.add<Capsule>(capsuleType, uid, material, false, prm, gpos, Quat(), m, real(1) / m, I, Vec3(), 0);
 }
 
 // createPlane (Plane.cpp:1202-1240) and the Plane ctor (Plane.cpp:77-102)

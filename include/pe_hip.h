@@ -37,7 +37,7 @@ typedef enum {
   PEHIP_NOT_SUPPORTED = 6
 } pehip_status;
 
-typedef enum { PEHIP_SPHERE = 1, PEHIP_BOX = 2, PEHIP_PLANE = 3 } pehip_shape;
+typedef enum { PEHIP_SPHERE = 1, PEHIP_BOX = 2, PEHIP_PLANE = 3, PEHIP_CAPSULE = 4 } pehip_shape;
 
 /* CollisionSystem<HCSITS>::RelaxationModel, same numbering
  * (pe/core/collisionsystem/HardContactSemiImplicitTimesteppingSolvers.h:177-184). */

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
-SPHERE, BOX, PLANE = 1, 2, 3
+SPHERE, BOX, PLANE, CAPSULE = 1, 2, 3, 4
 INELASTIC_FRICTIONLESS = 0
 APPROX_INELASTIC_COULOMB_DECOUPLING = 1
 APPROX_INELASTIC_COULOMB_ORTHO = 2
@@ -65,6 +65,7 @@ def lib():
             "orc_set_pair_friction": (None, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             "orc_create_sphere": (ctypes.c_int, [vp] + [ctypes.c_double] * 4 + [ctypes.c_int]),
             "orc_create_box": (ctypes.c_int, [vp] + [ctypes.c_double] * 6 + [ctypes.c_int]),
+            "orc_create_capsule": (ctypes.c_int, [vp] + [ctypes.c_double] * 5 + [ctypes.c_int]),
             "orc_create_plane": (ctypes.c_int, [vp] + [ctypes.c_double] * 4 + [ctypes.c_int]),
             "orc_num_bodies": (ctypes.c_size_t, [vp]),
             "orc_get_state": (None, [vp, _D, _D, _D, _D, _D, _D]),
@@ -129,6 +130,10 @@ class OracleWorld:
 
     def create_box(self, pos, lengths, mat):
         return self.L.orc_create_box(self.h, *[float(x) for x in pos], *[float(x) for x in lengths], mat)
+
+    def create_capsule(self, pos, r, length, mat):
+        """A capsule whose axis is the body x-axis (CapsuleBase.cpp:60-92)."""
+        return self.L.orc_create_capsule(self.h, *[float(x) for x in pos], float(r), float(length), mat)
 
     def create_plane(self, normal, d, mat):
         return self.L.orc_create_plane(self.h, *[float(x) for x in normal], float(d), mat)

@@ -23,7 +23,7 @@ extern "C" {
 
 typedef struct orc_world orc_world;
 
-enum { ORC_SPHERE = 1, ORC_BOX = 2, ORC_PLANE = 3 };
+enum { ORC_SPHERE = 1, ORC_BOX = 2, ORC_PLANE = 3, ORC_CAPSULE = 4 };
 
 /* Relaxation models, numbered as the reference's RelaxationModel enum
  * (pe/core/collisionsystem/HardContactSemiImplicitTimesteppingSolvers.h:177-184). */
@@ -63,6 +63,7 @@ void orc_set_pair_friction(orc_world* w, int m1, int m2, double csf, double cdf)
 /* Body factories; return the body index (= BodyStorage order = creation order). */
 int orc_create_sphere(orc_world* w, double x, double y, double z, double r, int mat);
 int orc_create_box(orc_world* w, double x, double y, double z, double lx, double ly, double lz, int mat);
+int orc_create_capsule(orc_world* w, double x, double y, double z, double r, double length, int mat);
 int orc_create_plane(orc_world* w, double nx, double ny, double nz, double d, int mat);
 
 size_t orc_num_bodies(const orc_world* w);

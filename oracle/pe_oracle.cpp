@@ -29,6 +29,7 @@ namespace {
 const double kContactThreshold = 1e-8;
 const double kFrictionThreshold = 1e-8;
 const double kAccuracy = 1e-8;
+const double kParallelThreshold = 1e-8;   // pe/core/Thresholds.h:170
 const size_t kCellCount = 16;
 const size_t kMinimalGridDensity = 8;
 const size_t kGridActivationThreshold = 32;
@@ -211,7 +212,8 @@ struct Body {
   M3 R = M3(1, 0, 0, 0, 1, 0, 0, 0, 1);
   double mass = 0, invMass = 0;
   M3 I, Iinv;                 // body frame
-  double radius = 0;          // sphere
+  double radius = 0;          // sphere, capsule
+  double length = 0;          // capsule: length of the cylinder part, along the body x-axis
   V3 lengths;                 // box
   V3 normal; double d = 0;    // plane
   double aabb[6] = {0, 0, 0, 0, 0, 0};
@@ -235,6 +237,12 @@ struct Body {
       const double xl = 0.5 * (std::fabs(R[0] * lengths[0]) + std::fabs(R[1] * lengths[1]) + std::fabs(R[2] * lengths[2])) + kContactThreshold;
       const double yl = 0.5 * (std::fabs(R[3] * lengths[0]) + std::fabs(R[4] * lengths[1]) + std::fabs(R[5] * lengths[2])) + kContactThreshold;
       const double zl = 0.5 * (std::fabs(R[6] * lengths[0]) + std::fabs(R[7] * lengths[1]) + std::fabs(R[8] * lengths[2])) + kContactThreshold;
+      aabb[0] = gpos[0] - xl; aabb[1] = gpos[1] - yl; aabb[2] = gpos[2] - zl;
+      aabb[3] = gpos[0] + xl; aabb[4] = gpos[1] + yl; aabb[5] = gpos[2] + zl;
+    } else if (shape == ORC_CAPSULE) {  // CapsuleBase.cpp:124-138
+      const double xl = std::fabs(R[0] * length) * 0.5 + radius + kContactThreshold;
+      const double yl = std::fabs(R[3] * length) * 0.5 + radius + kContactThreshold;
+      const double zl = std::fabs(R[6] * length) * 0.5 + radius + kContactThreshold;
       aabb[0] = gpos[0] - xl; aabb[1] = gpos[1] - yl; aabb[2] = gpos[2] - zl;
       aabb[3] = gpos[0] + xl; aabb[4] = gpos[1] + yl; aabb[5] = gpos[2] + zl;
     } else {  // PlaneBase.cpp:132-148
@@ -285,6 +293,10 @@ struct Fine {
   void spherePlane(int s, int p);
   void boxPlane(int b, int p);
   void boxBox(int b1, int b2);
+  void sphereCapsule(int s, int c);
+  void boxCapsule(int b, int c);
+  void capsuleCapsule(int c1, int c2);
+  void capsulePlane(int c, int p);
   void collide(int a, int b);
 };
 
@@ -936,20 +948,234 @@ void Fine::boxBox(int i1, int i2) {
   }
 }
 
-// MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/plane subset)
+// getClosestLineSegmentPoints, src/core/GeomTools.cpp:259-385: the closest points of the
+// segments [a1, a2] and [b1, b2]; the end-point cases first, then the end point against the
+// other segment's interior, then the two interiors.
+void closestSegmentPoints(const V3& a1, const V3& a2, const V3& b1, const V3& b2, V3& cp1, V3& cp2) {
+  const V3 a1a2 = a2 - a1, b1b2 = b2 - b1, a1b1 = b1 - a1;
+  const double da1 = dot(a1a2, a1b1), db1 = dot(b1b2, a1b1);
+  if (da1 <= 0.0 && db1 >= 0.0) { cp1 = a1; cp2 = b1; return; }
+  const V3 a1b2 = b2 - a1;
+  const double da2 = dot(a1a2, a1b2), db2 = dot(b1b2, a1b2);
+  if (da2 <= 0.0 && db2 <= 0.0) { cp1 = a1; cp2 = b2; return; }
+  const V3 a2b1 = b1 - a2;
+  const double da3 = dot(a1a2, a2b1), db3 = dot(b1b2, a2b1);
+  if (da3 >= 0.0 && db3 >= 0.0) { cp1 = a2; cp2 = b1; return; }
+  const V3 a2b2 = b2 - a2;
+  const double da4 = dot(a1a2, a2b2), db4 = dot(b1b2, a2b2);
+  if (da4 >= 0.0 && db4 <= 0.0) { cp1 = a2; cp2 = b2; return; }
+  const double la = dot(a1a2, a1a2);
+  if (da1 >= 0.0 && da3 <= 0.0) {
+    const V3 k = (da1 / la) * a1a2;
+    if (dot(b1b2, a1b1 - k) >= 0.0) { cp1 = a1 + k; cp2 = b1; return; }
+  }
+  if (da2 >= 0.0 && da4 <= 0.0) {
+    const V3 k = (da2 / la) * a1a2;
+    if (dot(b1b2, a1b2 - k) <= 0.0) { cp1 = a1 + k; cp2 = b2; return; }
+  }
+  const double lb = dot(b1b2, b1b2);
+  if (db1 <= 0.0 && db2 >= 0.0) {
+    const V3 k = (-db1 / lb) * b1b2;
+    if (dot(a1a2, -a1b1 - k) >= 0.0) { cp1 = a1; cp2 = b1 + k; return; }
+  }
+  if (db3 <= 0.0 && db4 >= 0.0) {
+    const V3 k = (-db3 / lb) * b1b2;
+    if (dot(a1a2, -a2b1 - k) <= 0.0) { cp1 = a2; cp2 = b1 + k; return; }
+  }
+  const double scale = dot(a1a2, b1b2);
+  double div = la * lb - scale * scale;
+  div = 1.0 / div;
+  const double sa = (lb * da1 - scale * db1) * div;
+  const double tb = (scale * da1 - la * db1) * div;
+  cp1 = a1 + sa * a1a2;
+  cp2 = b1 + tb * b1b2;
+}
+
+// getClosestLineBoxPoints, src/core/GeomTools.cpp:73-257: the closest points of the segment
+// [p1, p2] and the box (centre c, rotation R, side lengths `side`).  In the box frame, with the
+// segment's direction mirrored into the positive octant, the squared distance is a convex
+// piecewise quadratic in the segment parameter t whose pieces change where the mirrored point
+// crosses a face plane (tanchor); the walk follows the pieces from t = 0 until the derivative
+// changes sign, then clamps the segment point to the box.
+void closestLineBoxPoints(const V3& p1, const V3& p2, const V3& c, const M3& R, const V3& side, V3& lret,
+                          V3& bret) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  const V3 l = p2 - p1;
+  V3 tmp = trans(R) * l;
+  const double sgn[3] = {tmp[0] < 0 ? -1.0 : 1.0, tmp[1] < 0 ? -1.0 : 1.0, tmp[2] < 0 ? -1.0 : 1.0};
+  const double v[3] = {sgn[0] * tmp[0], sgn[1] * tmp[1], sgn[2] * tmp[2]};
+  const double v2[3] = {v[0] * v[0], v[1] * v[1], v[2] * v[2]};
+  tmp = p1 - c;
+  const double s[3] = {sgn[0] * (R[0] * tmp[0] + R[3] * tmp[1] + R[6] * tmp[2]),
+                       sgn[1] * (R[1] * tmp[0] + R[4] * tmp[1] + R[7] * tmp[2]),
+                       sgn[2] * (R[2] * tmp[0] + R[5] * tmp[1] + R[8] * tmp[2])};
+  const double h[3] = {0.5 * side[0], 0.5 * side[1], 0.5 * side[2]};
+  int region[3] = {0, 0, 0};
+  double tanchor[3] = {2, 2, 2};
+  double dd2dt = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (v[i] > eps) {
+      if (s[i] < -h[i]) {
+        region[i] = -1;
+        tanchor[i] = (-h[i] - s[i]) / v[i];
+        dd2dt -= v2[i] * tanchor[i];
+      } else if (s[i] > h[i]) {
+        region[i] = 1;
+        tanchor[i] = (h[i] - s[i]) / v[i];
+        dd2dt -= v2[i] * tanchor[i];
+      } else {
+        tanchor[i] = (h[i] - s[i]) / v[i];
+      }
+    }
+  }
+  double t = 0;
+  if (dd2dt < -kAccuracy) {
+    do {
+      double next_t = 1;
+      for (int i = 0; i < 3; ++i)
+        if (tanchor[i] > t && tanchor[i] < 1.0 && tanchor[i] < next_t) next_t = tanchor[i];
+      double next_dd2dt = 0;
+      for (int i = 0; i < 3; ++i)
+        if (region[i] != 0) next_dd2dt += v2[i] * (next_t - tanchor[i]);
+      if (next_dd2dt >= 0) {
+        t -= dd2dt / ((next_dd2dt - dd2dt) / (next_t - t));
+        break;
+      }
+      for (int i = 0; i < 3; ++i)
+        if (tanchor[i] == next_t) {
+          tanchor[i] = (h[i] - s[i]) / v[i];
+          ++region[i];
+        }
+      t = next_t;
+      dd2dt = next_dd2dt;
+    } while (t < 1.0);
+  }
+  lret = p1 + t * l;
+  for (int i = 0; i < 3; ++i) {
+    double x = sgn[i] * (s[i] + t * v[i]);
+    if (x < -h[i]) x = -h[i];
+    else if (x > h[i]) x = h[i];
+    tmp[i] = x;
+  }
+  bret = (R * tmp) + c;
+}
+
+// Vector3 operator==, Vector3.h:1421-1428: every component within 1e-8 (Equal.h:172-175)
+bool nearly_equal(const V3& a, const V3& b) {
+  return std::fabs(a[0] - b[0]) < 1e-8 && std::fabs(a[1] - b[1]) < 1e-8 && std::fabs(a[2] - b[2]) < 1e-8;
+}
+
+// collideSphereCapsule, MaxContacts.h:690-725: the sphere against the closest point of the
+// capsule's axis segment.  The sphere is body1.
+void Fine::sphereCapsule(int si, int ci) {
+  const Body& S = world->bodies[si];
+  const Body& C = world->bodies[ci];
+  const double hl = 0.5 * C.length;
+  const V3& spos = S.gpos;
+  const V3& cpos = C.gpos;
+  const M3& R = C.R;
+  double sx = R[0] * (spos[0] - cpos[0]) + R[3] * (spos[1] - cpos[1]) + R[6] * (spos[2] - cpos[2]);
+  if (sx > hl) sx = hl;
+  else if (sx < -hl) sx = -hl;
+  const V3 spos2(sx * R[0] + cpos[0], sx * R[3] + cpos[1], sx * R[6] + cpos[2]);
+  V3 normal = spos - spos2;
+  const double dist = length(normal) - S.radius - C.radius;
+  if (dist < kContactThreshold) {
+    normal = normalized(normal);
+    const double k = C.radius + 0.5 * dist;
+    add(si, ci, spos2 + normal * k, normal, dist);
+  }
+}
+
+// collideBoxCapsule, MaxContacts.h:2273-2330: the closest points of the axis segment and the
+// box, from each end of the segment; the second contact is added whenever its points differ
+// from the first's (without a distance test, as in the reference).  The box is body1.
+void Fine::boxCapsule(int bi, int ci) {
+  const Body& B = world->bodies[bi];
+  const Body& C = world->bodies[ci];
+  const M3& R = C.R;
+  const V3 c_up(0.5 * C.length * R[0], 0.5 * C.length * R[3], 0.5 * C.length * R[6]);
+  const V3 up = C.gpos + c_up, down = C.gpos - c_up;
+  V3 cp1, bp1;
+  closestLineBoxPoints(up, down, B.gpos, B.R, B.lengths, cp1, bp1);
+  V3 normal1 = bp1 - cp1;
+  const double dist1 = length(normal1) - C.radius;
+  if (dist1 > kContactThreshold) return;
+  normal1 = normalized(normal1);
+  add(bi, ci, bp1 - 0.5 * dist1 * normal1, normal1, dist1);
+  V3 cp2, bp2;
+  closestLineBoxPoints(down, up, B.gpos, B.R, B.lengths, cp2, bp2);
+  if (nearly_equal(cp1, cp2)) return;
+  V3 normal2 = bp2 - cp2;
+  const double dist2 = length(normal2) - C.radius;
+  normal2 = normalized(normal2);
+  add(bi, ci, bp2 - 0.5 * dist2 * normal2, normal2, dist2);
+}
+
+// collideCapsuleCapsule, MaxContacts.h:2603-2690: body1 is the capsule with the smaller SID.
+// Nearly parallel, overlapping axes give two contacts at the ends of the overlap; otherwise
+// one contact between the axes' closest points.
+void Fine::capsuleCapsule(int c1i, int c2i) {
+  if (world->bodies[c2i].sid < world->bodies[c1i].sid) std::swap(c1i, c2i);
+  const Body& C1 = world->bodies[c1i];
+  const Body& C2 = world->bodies[c2i];
+  const V3 d1(C1.R[0], C1.R[3], C1.R[6]), d2(C2.R[0], C2.R[3], C2.R[6]);
+  const V3 up1 = (0.5 * C1.length) * d1, up2 = (0.5 * C2.length) * d2;
+  V3 cp1, cp2;
+  closestSegmentPoints(C1.gpos + up1, C1.gpos - up1, C2.gpos + up2, C2.gpos - up2, cp1, cp2);
+  V3 normal = cp1 - cp2;
+  const double dist = length(normal) - C1.radius - C2.radius;
+  if (!(dist < kContactThreshold)) return;
+  normal = normalized(normal);
+  const double c2x = dot(d1, C2.gpos - C1.gpos);
+  if ((std::fabs(dot(d1, d2)) - 1.0) > -kParallelThreshold && c2x > -C1.length && c2x < C1.length) {
+    const double k = C1.radius + 0.5 * dist;
+    const double hl1 = 0.5 * C1.length, hl2 = 0.5 * C2.length;
+    if (hl1 < c2x + hl2) add(c1i, c2i, C1.gpos + up1 - k * normal, normal, dist);
+    else add(c1i, c2i, C2.gpos + hl2 * d1 + k * normal, normal, dist);
+    if (-hl1 > c2x - hl2) add(c1i, c2i, C1.gpos - up1 - k * normal, normal, dist);
+    else add(c1i, c2i, C2.gpos - hl2 * d1 + k * normal, normal, dist);
+  } else {
+    const double k = C2.radius + 0.5 * dist;
+    add(c1i, c2i, cp2 + normal * k, normal, dist);
+  }
+}
+
+// collideCapsulePlane, MaxContacts.h:2744-2780: each end sphere against the plane.  The
+// capsule is body1.
+void Fine::capsulePlane(int ci, int pi) {
+  const Body& C = world->bodies[ci];
+  const Body& P = world->bodies[pi];
+  const V3 c_up = (0.5 * C.length) * V3(C.R[0], C.R[3], C.R[6]);
+  const V3 posUp = C.gpos + c_up, posDn = C.gpos - c_up;
+  const double distUp = dot(posUp, P.normal) - P.d - C.radius;
+  const double distDn = dot(posDn, P.normal) - P.d - C.radius;
+  if (distUp < kContactThreshold) add(ci, pi, posUp - C.radius * P.normal, P.normal, distUp);
+  if (distDn < kContactThreshold) add(ci, pi, posDn - C.radius * P.normal, P.normal, distDn);
+}
+
+// MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/capsule/plane subset)
 void Fine::collide(int a, int b) {
   const int ta = world->bodies[a].shape, tb = world->bodies[b].shape;
   if (ta == ORC_SPHERE) {
     if (tb == ORC_SPHERE) sphereSphere(a, b);
     else if (tb == ORC_BOX) sphereBox(a, b);
+    else if (tb == ORC_CAPSULE) sphereCapsule(a, b);
     else spherePlane(a, b);
   } else if (ta == ORC_BOX) {
     if (tb == ORC_SPHERE) sphereBox(b, a);
     else if (tb == ORC_BOX) boxBox(a, b);
+    else if (tb == ORC_CAPSULE) boxCapsule(a, b);
     else boxPlane(a, b);
+  } else if (ta == ORC_CAPSULE) {
+    if (tb == ORC_SPHERE) sphereCapsule(b, a);
+    else if (tb == ORC_BOX) boxCapsule(b, a);
+    else if (tb == ORC_CAPSULE) capsuleCapsule(a, b);
+    else capsulePlane(a, b);
   } else {
     if (tb == ORC_SPHERE) spherePlane(b, a);
     else if (tb == ORC_BOX) boxPlane(b, a);
+    else if (tb == ORC_CAPSULE) capsulePlane(b, a);
     // plane-plane: no contact
   }
 }
@@ -1601,6 +1827,33 @@ int orc_create_box(orc_world* w, double x, double y, double z, double lx, double
   return register_body(w, b);
 }
 
+// createCapsule / CapsuleBase ctor, CapsuleBase.cpp:60-92, 147-168; CapsuleBase.h:182-200.  The
+// axis is the body x-axis; mass = pi r^2 (4/3 r + length) rho; the moments combine the cylinder
+// and the two half-sphere caps, with the density recomputed from the mass as the reference does.
+int orc_create_capsule(orc_world* w, double x, double y, double z, double r, double len, int mat) {
+  Body b;
+  b.shape = ORC_CAPSULE;
+  b.mat = mat;
+  b.radius = r;
+  b.length = len;
+  b.gpos = V3(x, y, z);
+  b.mass = r * r * M_PI * (4.0 / 3.0 * r + len) * w->materials[mat].density;
+  b.invMass = 1.0 / b.mass;
+  const double density = b.mass / (M_PI * r * r * (4.0 / 3.0 * r + len));
+  const double sphereMass = 4.0 / 3.0 * M_PI * r * r * r * density;
+  const double cylinderMass = M_PI * r * r * len * density;
+  const double Ia = r * r * (0.5 * cylinderMass + 0.4 * sphereMass);
+  const double Ib = cylinderMass * (0.25 * r * r + 1.0 / 12.0 * len * len) +
+                    sphereMass * (0.4 * r * r + 0.375 * r * len + 0.25 * len * len);
+  b.I = M3();
+  b.I[0] = Ia;
+  b.I[4] = Ib;
+  b.I[8] = Ib;
+  b.Iinv = inverse(b.I);
+  b.calcBoundingBox();
+  return register_body(w, b);
+}
+
 // createPlane, Plane.cpp:1202-1240; Plane ctor, Plane.cpp:77-102; PlaneBase.cpp:73-100
 int orc_create_plane(orc_world* w, double nx, double ny, double nz, double d, int mat) {
   Body b;
@@ -1667,6 +1920,7 @@ void orc_get_body_info(const orc_world* w, int* shape, uint64_t* sid, int* mat, 
     fixed[i] = b.fixed ? 1 : 0;
     if (b.shape == ORC_SPHERE) { param[3 * i] = b.radius; param[3 * i + 1] = 0; param[3 * i + 2] = 0; }
     else if (b.shape == ORC_BOX) { for (int k = 0; k < 3; ++k) param[3 * i + k] = b.lengths[k]; }
+    else if (b.shape == ORC_CAPSULE) { param[3 * i] = b.radius; param[3 * i + 1] = b.length; param[3 * i + 2] = 0; }
     else { param[3 * i] = param[3 * i + 1] = param[3 * i + 2] = 0; }
     inv_mass[i] = b.invMass;
     for (int k = 0; k < 9; ++k) inv_inertia_body[9 * i + k] = b.Iinv[k];

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libpehip.so")
 
 OK, INVALID_ARG, OUT_OF_MEMORY, CAPACITY_RETRY, HIP_ERROR, NONFINITE, NOT_SUPPORTED = range(7)
-SPHERE, BOX, PLANE = 1, 2, 3
+SPHERE, BOX, PLANE, CAPSULE = 1, 2, 3, 4
 SWEEP_COLOURED, SWEEP_REFERENCE_ORDER = 0, 1
 SCHEDULE_AUTO, SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW, SCHEDULE_SINGLE_BLOCK = 0, 1, 2, 3
 

@@ -95,7 +95,8 @@ static pehip_status check_supported(pehip_ctx* ctx, const pehip_params& P) {
   if (P.sweep_mode != PEHIP_SWEEP_COLOURED) return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint runs in coloured mode");
   if (P.damping < 1.0) return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint does not support damping");
   for (uint32_t i = 0; i < ctx->n; ++i)
-    if (ctx->h_shape[i] == PEHIP_BOX) return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint supports spheres and planes");
+    if (ctx->h_shape[i] != PEHIP_SPHERE && ctx->h_shape[i] != PEHIP_PLANE)
+      return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint supports spheres and planes");
   return PEHIP_OK;
 }
 

@@ -374,7 +374,7 @@ pehip_status pehip_batch_upload(pehip_batch* b, uint32_t nscenes, const uint32_t
     double mx = 0;
     double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
     for (uint32_t i = lo; i < hi; ++i) {
-      if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_PLANE) return bfail(b, PEHIP_INVALID_ARG, "invalid shape");
+      if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_CAPSULE) return bfail(b, PEHIP_INVALID_ARG, "invalid shape");
       if (material[i] >= b->nmat) return bfail(b, PEHIP_INVALID_ARG, "Invalid material ID");
       const double* p = param + 3 * (size_t)i;
       if (shape[i] == PEHIP_PLANE) {
@@ -386,9 +386,12 @@ pehip_status pehip_batch_upload(pehip_batch* b, uint32_t nscenes, const uint32_t
       if (shape[i] == PEHIP_SPHERE && !(p[0] > 0)) return bfail(b, PEHIP_INVALID_ARG, "Invalid sphere radius");
       if (shape[i] == PEHIP_BOX && !(p[0] > 0 && p[1] > 0 && p[2] > 0))
         return bfail(b, PEHIP_INVALID_ARG, "Invalid side length");
+      if (shape[i] == PEHIP_CAPSULE && !(p[0] > 0 && p[1] > 0))
+        return bfail(b, PEHIP_INVALID_ARG, "Invalid capsule radius or length");
       if (fx[i] && inv_mass[i] != 0) return bfail(b, PEHIP_INVALID_ARG, "fixed body with non-zero inverse mass");
-      const double sz = shape[i] == PEHIP_SPHERE ? 2.0 * (p[0] + 1e-8)
-                                                 : std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2e-8;
+      const double sz = shape[i] == PEHIP_SPHERE    ? 2.0 * (p[0] + 1e-8)
+                        : shape[i] == PEHIP_CAPSULE ? p[1] + 2.0 * (p[0] + 1e-8)
+                                                    : std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2e-8;
       mx = std::max(mx, sz);
       for (int k = 0; k < 3; ++k) {
         blo[k] = std::min(blo[k], pos[3 * (size_t)i + k]);

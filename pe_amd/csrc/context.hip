@@ -73,6 +73,10 @@ static void plan_grid(pehip_ctx* ctx, const uint8_t* shape, const double* param,
     if (shape[i] == PEHIP_SPHERE) {
       smax_b = 2.0 * (p[0] + kContactThreshold);
       smin_b = smax_b;
+    } else if (shape[i] == PEHIP_CAPSULE) {
+      // AABB size between its diameter (lower bound) and axis-aligned length + diameter
+      smax_b = p[1] + 2.0 * (p[0] + kContactThreshold);
+      smin_b = 2.0 * p[0];
     } else {
       smax_b = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2.0 * kContactThreshold;
       smin_b = std::max(p[0], std::max(p[1], p[2]));
@@ -266,7 +270,7 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   std::vector<uint8_t> fx(fixed, fixed + n);
   std::vector<uint32_t> hplanes;
   for (uint32_t i = 0; i < n; ++i) {
-    if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_PLANE) return fail(ctx, PEHIP_INVALID_ARG, "invalid shape");
+    if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_CAPSULE) return fail(ctx, PEHIP_INVALID_ARG, "invalid shape");
     if (material[i] >= ctx->nmat) return fail(ctx, PEHIP_INVALID_ARG, "Invalid material ID");
     if (shape[i] == PEHIP_PLANE) {
       hplanes.push_back(i);
@@ -277,6 +281,8 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
       if (shape[i] == PEHIP_SPHERE && !(p[0] > 0)) return fail(ctx, PEHIP_INVALID_ARG, "Invalid sphere radius");
       if (shape[i] == PEHIP_BOX && !(p[0] > 0 && p[1] > 0 && p[2] > 0))
         return fail(ctx, PEHIP_INVALID_ARG, "Invalid side length");
+      if (shape[i] == PEHIP_CAPSULE && !(p[0] > 0 && p[1] > 0))
+        return fail(ctx, PEHIP_INVALID_ARG, "Invalid capsule radius or length");
       if (fx[i] && inv_mass[i] != 0) return fail(ctx, PEHIP_INVALID_ARG, "fixed body with non-zero inverse mass");
     }
   }

@@ -12,7 +12,8 @@ namespace pe_amd {
 constexpr double kContactThreshold = 1e-8;   // pe/core/Thresholds.h:151
 constexpr double kFrictionThreshold = 1e-8;  // pe/core/Thresholds.h:161
 constexpr uint32_t kFixedBit = 0x80000000u;  // set in solver body indices of fixed bodies
-constexpr int kShapeSphere = 1, kShapeBox = 2, kShapePlane = 3;
+constexpr int kShapeSphere = 1, kShapeBox = 2, kShapePlane = 3, kShapeCapsule = 4;
+constexpr double kParallelThreshold = 1e-8;  // pe/core/Thresholds.h:170
 
 // Solver body record (128 B, one L2 line): predicted v, w; inverse mass; world-frame
 // inverse inertia R*Iinv*R^T as 9 doubles (pe recomputes the full product on every
@@ -87,6 +88,13 @@ PE_HD void body_aabb(int shape, d3 gp, const m33& R, const double* prm, d3 nrm, 
     const double xl = 0.5 * (fabs(r[0] * prm[0]) + fabs(r[1] * prm[1]) + fabs(r[2] * prm[2])) + kContactThreshold;
     const double yl = 0.5 * (fabs(r[3] * prm[0]) + fabs(r[4] * prm[1]) + fabs(r[5] * prm[2])) + kContactThreshold;
     const double zl = 0.5 * (fabs(r[6] * prm[0]) + fabs(r[7] * prm[1]) + fabs(r[8] * prm[2])) + kContactThreshold;
+    o[0] = gp.x - xl; o[1] = gp.y - yl; o[2] = gp.z - zl;
+    o[3] = gp.x + xl; o[4] = gp.y + yl; o[5] = gp.z + zl;
+  } else if (shape == kShapeCapsule) {  // CapsuleBase.cpp:124-138 (axis = body x-axis)
+    const double* r = R.a;
+    const double xl = fabs(r[0] * prm[1]) * 0.5 + prm[0] + kContactThreshold;
+    const double yl = fabs(r[3] * prm[1]) * 0.5 + prm[0] + kContactThreshold;
+    const double zl = fabs(r[6] * prm[1]) * 0.5 + prm[0] + kContactThreshold;
     o[0] = gp.x - xl; o[1] = gp.y - yl; o[2] = gp.z - zl;
     o[3] = gp.x + xl; o[4] = gp.y + yl; o[5] = gp.z + zl;
   } else {
@@ -433,20 +441,257 @@ __device__ void np_box_box(const Bodies& B, uint32_t i1, uint32_t i2, Emit& emit
 
 // MaxContacts::collide dispatch for (a, b) as handed over by the broadphase
 // (MaxContacts.h:242-526).
+// getClosestLineSegmentPoints, src/core/GeomTools.cpp:259-385: closest points of the segments
+// [a1, a2] and [b1, b2] (end-point cases, end point against interior, then both interiors).
+__device__ inline void closest_segment_points(d3 a1, d3 a2, d3 b1, d3 b2, d3& cp1, d3& cp2) {
+  const d3 a1a2 = a2 - a1, b1b2 = b2 - b1, a1b1 = b1 - a1;
+  const double da1 = dot3(a1a2, a1b1), db1 = dot3(b1b2, a1b1);
+  if (da1 <= 0.0 && db1 >= 0.0) { cp1 = a1; cp2 = b1; return; }
+  const d3 a1b2 = b2 - a1;
+  const double da2 = dot3(a1a2, a1b2), db2 = dot3(b1b2, a1b2);
+  if (da2 <= 0.0 && db2 <= 0.0) { cp1 = a1; cp2 = b2; return; }
+  const d3 a2b1 = b1 - a2;
+  const double da3 = dot3(a1a2, a2b1), db3 = dot3(b1b2, a2b1);
+  if (da3 >= 0.0 && db3 >= 0.0) { cp1 = a2; cp2 = b1; return; }
+  const d3 a2b2 = b2 - a2;
+  const double da4 = dot3(a1a2, a2b2), db4 = dot3(b1b2, a2b2);
+  if (da4 >= 0.0 && db4 <= 0.0) { cp1 = a2; cp2 = b2; return; }
+  const double la = dot3(a1a2, a1a2);
+  if (da1 >= 0.0 && da3 <= 0.0) {
+    const d3 k = (da1 / la) * a1a2;
+    if (dot3(b1b2, a1b1 - k) >= 0.0) { cp1 = a1 + k; cp2 = b1; return; }
+  }
+  if (da2 >= 0.0 && da4 <= 0.0) {
+    const d3 k = (da2 / la) * a1a2;
+    if (dot3(b1b2, a1b2 - k) <= 0.0) { cp1 = a1 + k; cp2 = b2; return; }
+  }
+  const double lb = dot3(b1b2, b1b2);
+  if (db1 <= 0.0 && db2 >= 0.0) {
+    const d3 k = (-db1 / lb) * b1b2;
+    if (dot3(a1a2, -a1b1 - k) >= 0.0) { cp1 = a1; cp2 = b1 + k; return; }
+  }
+  if (db3 <= 0.0 && db4 >= 0.0) {
+    const d3 k = (-db3 / lb) * b1b2;
+    if (dot3(a1a2, -a2b1 - k) <= 0.0) { cp1 = a2; cp2 = b1 + k; return; }
+  }
+  const double scale = dot3(a1a2, b1b2);
+  double div = la * lb - scale * scale;
+  div = 1.0 / div;
+  const double sa = (lb * da1 - scale * db1) * div;
+  const double tb = (scale * da1 - la * db1) * div;
+  cp1 = a1 + sa * a1a2;
+  cp2 = b1 + tb * b1b2;
+}
+
+// getClosestLineBoxPoints, src/core/GeomTools.cpp:73-257: closest points of the segment [p1, p2]
+// and a box (centre c, rotation R, sides).  In the box frame, the segment's direction mirrored
+// into the positive octant, the squared distance is piecewise quadratic in the parameter t with
+// breaks where the point crosses a face plane; the walk follows the pieces from t = 0 until the
+// derivative changes sign, then clamps the segment point to the box.
+__device__ inline void closest_line_box_points(d3 p1, d3 p2, d3 c, const m33& R, const double* side, d3& lret,
+                                               d3& bret) {
+  const double eps = 2.220446049250313e-16;  // pe's `epsilon` (machine epsilon)
+  const d3 l = p2 - p1;
+  const d3 tl = mat_tvec(R, l);
+  const double tv[3] = {tl.x, tl.y, tl.z};
+  double sgn[3], v[3], v2[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    sgn[i] = tv[i] < 0 ? -1.0 : 1.0;
+    v[i] = sgn[i] * tv[i];
+    v2[i] = v[i] * v[i];
+  }
+  const d3 u = p1 - c;
+  const double* r = R.a;
+  const double sv[3] = {sgn[0] * (r[0] * u.x + r[3] * u.y + r[6] * u.z), sgn[1] * (r[1] * u.x + r[4] * u.y + r[7] * u.z),
+                        sgn[2] * (r[2] * u.x + r[5] * u.y + r[8] * u.z)};
+  const double h[3] = {0.5 * side[0], 0.5 * side[1], 0.5 * side[2]};
+  int region[3] = {0, 0, 0};
+  double tanchor[3] = {2, 2, 2};
+  double dd2dt = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (v[i] > eps) {
+      if (sv[i] < -h[i]) {
+        region[i] = -1;
+        tanchor[i] = (-h[i] - sv[i]) / v[i];
+        dd2dt -= v2[i] * tanchor[i];
+      } else if (sv[i] > h[i]) {
+        region[i] = 1;
+        tanchor[i] = (h[i] - sv[i]) / v[i];
+        dd2dt -= v2[i] * tanchor[i];
+      } else {
+        tanchor[i] = (h[i] - sv[i]) / v[i];
+      }
+    }
+  }
+  double t = 0;
+  if (dd2dt < -1e-8) {   // pe's `accuracy`
+    // at most one region step per axis and direction: the walk ends within 7 pieces
+    for (int it = 0; it < 8; ++it) {
+      double next_t = 1;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (tanchor[i] > t && tanchor[i] < 1.0 && tanchor[i] < next_t) next_t = tanchor[i];
+      double next_dd2dt = 0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (region[i] != 0) next_dd2dt += v2[i] * (next_t - tanchor[i]);
+      if (next_dd2dt >= 0) {
+        t -= dd2dt / ((next_dd2dt - dd2dt) / (next_t - t));
+        break;
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (tanchor[i] == next_t) {
+          tanchor[i] = (h[i] - sv[i]) / v[i];
+          ++region[i];
+        }
+      t = next_t;
+      dd2dt = next_dd2dt;
+      if (!(t < 1.0)) break;
+    }
+  }
+  lret = p1 + t * l;
+  double bx[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double x = sgn[i] * (sv[i] + t * v[i]);
+    if (x < -h[i]) x = -h[i];
+    else if (x > h[i]) x = h[i];
+    bx[i] = x;
+  }
+  bret = mat_vec(R, mk3(bx[0], bx[1], bx[2])) + c;
+}
+
+// Vector3 operator== (Vector3.h:1421-1428): every component within 1e-8 (Equal.h:172-175).
+__device__ inline bool nearly_equal3(d3 a, d3 b) {
+  return fabs(a.x - b.x) < 1e-8 && fabs(a.y - b.y) < 1e-8 && fabs(a.z - b.z) < 1e-8;
+}
+
+// collideSphereCapsule, MaxContacts.h:690-725: the sphere (body1) against the closest point of
+// the capsule's axis.
+template <class Emit>
+__device__ void np_sphere_capsule(const Bodies& B, uint32_t s, uint32_t c, Emit& emit) {
+  const double hl = 0.5 * B.param[3 * (size_t)c + 1];
+  const double rc = B.param[3 * (size_t)c];
+  const d3 spos = ld3(B.pos, s), cpos = ld3(B.pos, c);
+  const double* R = B.R + 9 * (size_t)c;
+  double sx = R[0] * (spos.x - cpos.x) + R[3] * (spos.y - cpos.y) + R[6] * (spos.z - cpos.z);
+  if (sx > hl) sx = hl;
+  else if (sx < -hl) sx = -hl;
+  const d3 spos2 = mk3(sx * R[0] + cpos.x, sx * R[3] + cpos.y, sx * R[6] + cpos.z);
+  d3 normal = spos - spos2;
+  const double dist = len3(normal) - B.param[3 * (size_t)s] - rc;
+  if (dist < kContactThreshold) {
+    normal = nrm3(normal);
+    const double k = rc + 0.5 * dist;
+    emit(s, c, spos2 + normal * k, normal, dist);
+  }
+}
+
+// collideBoxCapsule, MaxContacts.h:2273-2330: closest points of the axis and the box from each
+// end of the axis; the second contact needs only points that differ from the first's (no
+// distance test, as in the reference).  The box is body1.
+template <class Emit>
+__device__ void np_box_capsule(const Bodies& B, uint32_t b, uint32_t c, Emit& emit) {
+  const double* Rc = B.R + 9 * (size_t)c;
+  const double rc = B.param[3 * (size_t)c], len = B.param[3 * (size_t)c + 1];
+  const d3 c_up = mk3(0.5 * len * Rc[0], 0.5 * len * Rc[3], 0.5 * len * Rc[6]);
+  const d3 cpos = ld3(B.pos, c);
+  const d3 up = cpos + c_up, down = cpos - c_up;
+  const d3 bpos = ld3(B.pos, b);
+  const m33 Rb = ld9(B.R, b);
+  const double* side = B.param + 3 * (size_t)b;
+  d3 cp1, bp1;
+  closest_line_box_points(up, down, bpos, Rb, side, cp1, bp1);
+  d3 normal1 = bp1 - cp1;
+  const double dist1 = len3(normal1) - rc;
+  if (dist1 > kContactThreshold) return;
+  normal1 = nrm3(normal1);
+  emit(b, c, bp1 - 0.5 * dist1 * normal1, normal1, dist1);
+  d3 cp2, bp2;
+  closest_line_box_points(down, up, bpos, Rb, side, cp2, bp2);
+  if (nearly_equal3(cp1, cp2)) return;
+  d3 normal2 = bp2 - cp2;
+  const double dist2 = len3(normal2) - rc;
+  normal2 = nrm3(normal2);
+  emit(b, c, bp2 - 0.5 * dist2 * normal2, normal2, dist2);
+}
+
+// collideCapsuleCapsule, MaxContacts.h:2603-2690: body1 has the smaller SID.  Nearly parallel,
+// overlapping axes give two contacts at the ends of the overlap, otherwise one contact between
+// the axes' closest points.
+template <class Emit>
+__device__ void np_capsule_capsule(const Bodies& B, uint32_t c1, uint32_t c2, Emit& emit) {
+  if (B.sid[c2] < B.sid[c1]) { uint32_t t = c1; c1 = c2; c2 = t; }
+  const double* R1 = B.R + 9 * (size_t)c1;
+  const double* R2 = B.R + 9 * (size_t)c2;
+  const double r1 = B.param[3 * (size_t)c1], L1 = B.param[3 * (size_t)c1 + 1];
+  const double r2 = B.param[3 * (size_t)c2], L2 = B.param[3 * (size_t)c2 + 1];
+  const d3 d1 = mk3(R1[0], R1[3], R1[6]), d2 = mk3(R2[0], R2[3], R2[6]);
+  const d3 up1 = (0.5 * L1) * d1, up2 = (0.5 * L2) * d2;
+  const d3 x1 = ld3(B.pos, c1), x2 = ld3(B.pos, c2);
+  d3 cp1, cp2;
+  closest_segment_points(x1 + up1, x1 - up1, x2 + up2, x2 - up2, cp1, cp2);
+  d3 normal = cp1 - cp2;
+  const double dist = len3(normal) - r1 - r2;
+  if (!(dist < kContactThreshold)) return;
+  normal = nrm3(normal);
+  const double c2x = dot3(d1, x2 - x1);
+  if ((fabs(dot3(d1, d2)) - 1.0) > -kParallelThreshold && c2x > -L1 && c2x < L1) {
+    const double k = r1 + 0.5 * dist;
+    const double hl1 = 0.5 * L1, hl2 = 0.5 * L2;
+    if (hl1 < c2x + hl2) emit(c1, c2, x1 + up1 - k * normal, normal, dist);
+    else emit(c1, c2, x2 + hl2 * d1 + k * normal, normal, dist);
+    if (-hl1 > c2x - hl2) emit(c1, c2, x1 - up1 - k * normal, normal, dist);
+    else emit(c1, c2, x2 - hl2 * d1 + k * normal, normal, dist);
+  } else {
+    const double k = r2 + 0.5 * dist;
+    emit(c1, c2, cp2 + normal * k, normal, dist);
+  }
+}
+
+// collideCapsulePlane, MaxContacts.h:2744-2780: each end sphere against the plane; the capsule
+// is body1.
+template <class Emit>
+__device__ void np_capsule_plane(const Bodies& B, uint32_t c, uint32_t p, Emit& emit) {
+  const double* R = B.R + 9 * (size_t)c;
+  const double rc = B.param[3 * (size_t)c], len = B.param[3 * (size_t)c + 1];
+  const d3 c_up = (0.5 * len) * mk3(R[0], R[3], R[6]);
+  const d3 cpos = ld3(B.pos, c);
+  const d3 posUp = cpos + c_up, posDn = cpos - c_up;
+  const d3 n = ld3(B.pnormal, p);
+  const double pd = B.pd[p];
+  const double distUp = dot3(posUp, n) - pd - rc;
+  const double distDn = dot3(posDn, n) - pd - rc;
+  if (distUp < kContactThreshold) emit(c, p, posUp - rc * n, n, distUp);
+  if (distDn < kContactThreshold) emit(c, p, posDn - rc * n, n, distDn);
+}
+
+// MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/capsule/plane).
 template <class Emit>
 __device__ void narrowphase(const Bodies& B, uint32_t a, uint32_t b, Emit& emit) {
   const int ta = B.shape[a], tb = B.shape[b];
   if (ta == kShapeSphere) {
     if (tb == kShapeSphere) np_sphere_sphere(B, a, b, emit);
     else if (tb == kShapeBox) np_sphere_box(B, a, b, emit);
+    else if (tb == kShapeCapsule) np_sphere_capsule(B, a, b, emit);
     else np_sphere_plane(B, a, b, emit);
   } else if (ta == kShapeBox) {
     if (tb == kShapeSphere) np_sphere_box(B, b, a, emit);
     else if (tb == kShapeBox) np_box_box(B, a, b, emit);
+    else if (tb == kShapeCapsule) np_box_capsule(B, a, b, emit);
     else if (tb == kShapePlane) np_box_plane(B, a, b, emit);
+  } else if (ta == kShapeCapsule) {
+    if (tb == kShapeSphere) np_sphere_capsule(B, b, a, emit);
+    else if (tb == kShapeBox) np_box_capsule(B, b, a, emit);
+    else if (tb == kShapeCapsule) np_capsule_capsule(B, a, b, emit);
+    else if (tb == kShapePlane) np_capsule_plane(B, a, b, emit);
   } else {
     if (tb == kShapeSphere) np_sphere_plane(B, b, a, emit);
     else if (tb == kShapeBox) np_box_plane(B, b, a, emit);
+    else if (tb == kShapeCapsule) np_capsule_plane(B, b, a, emit);
   }
 }
 

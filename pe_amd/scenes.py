@@ -10,7 +10,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import BOX, PLANE, SPHERE
+from ._abi import BOX, CAPSULE, PLANE, SPHERE
 from .world import mat3_inverse, normalize, quat_axis_angle, ACCURACY
 
 
@@ -26,6 +26,11 @@ class Scene:
     box_len: np.ndarray = None           # (nb, 3)
     box_q: np.ndarray = None             # (nb, 4) orientation
     box_mat: np.ndarray = None
+    capsule_pos: np.ndarray = None       # (nc, 3); capsules follow the boxes in creation order
+    capsule_r: np.ndarray = None         # (nc,)
+    capsule_len: np.ndarray = None       # (nc,) length of the cylinder part (body x-axis)
+    capsule_q: np.ndarray = None         # (nc, 4) orientation
+    capsule_mat: np.ndarray = None
     v0: np.ndarray = None                # (n_total, 3) initial linear velocities
     gravity: tuple = (0.0, 0.0, -9.81)
     dt: float = 1e-3
@@ -43,14 +48,18 @@ class Scene:
         return 0 if self.box_pos is None else len(self.box_pos)
 
     @property
+    def n_capsules(self):
+        return 0 if self.capsule_pos is None else len(self.capsule_pos)
+
+    @property
     def n_bodies(self):
-        return len(self.planes) + self.n_spheres + self.n_boxes
+        return len(self.planes) + self.n_spheres + self.n_boxes + self.n_capsules
 
     def arrays(self):
         """Body arrays (SoA) in creation order with pe's mass properties."""
         dens = np.array([m["density"] for m in self.materials])
-        npl, ns, nb = len(self.planes), self.n_spheres, self.n_boxes
-        n = npl + ns + nb
+        npl, ns, nb, nc = len(self.planes), self.n_spheres, self.n_boxes, self.n_capsules
+        n = npl + ns + nb + nc
         a = {"shape": np.zeros(n, np.uint8), "mat": np.zeros(n, np.uint32), "fixed": np.zeros(n, np.uint8),
              "param": np.zeros((n, 3)), "pos": np.zeros((n, 3)), "q": np.zeros((n, 4)), "v": np.zeros((n, 3)),
              "w": np.zeros((n, 3)), "inv_mass": np.zeros(n), "I": np.zeros((n, 9)), "Iinv": np.zeros((n, 9)),
@@ -84,7 +93,7 @@ class Scene:
             a["I"][s] = I
             a["Iinv"][s] = mat3_inverse(I)
         if nb:
-            s = slice(npl + ns, n)
+            s = slice(npl + ns, npl + ns + nb)
             L = np.asarray(self.box_len, float)
             rho = dens[np.asarray(self.box_mat)]
             m = L[:, 0] * L[:, 1] * L[:, 2] * rho               # BoxBase.h:200-203
@@ -97,6 +106,27 @@ class Scene:
             a["param"][s] = L
             a["pos"][s] = self.box_pos
             a["q"][s] = self.box_q
+            a["inv_mass"][s] = 1.0 / m
+            a["I"][s] = I
+            a["Iinv"][s] = mat3_inverse(I)
+        if nc:
+            s = slice(npl + ns + nb, n)
+            r = np.asarray(self.capsule_r, float)
+            L = np.asarray(self.capsule_len, float)
+            rho = dens[np.asarray(self.capsule_mat)]
+            m = r * r * math.pi * (4.0 / 3.0 * r + L) * rho     # CapsuleBase.cpp:60-92
+            dn = m / (math.pi * r * r * (4.0 / 3.0 * r + L))     # CapsuleBase.h:196-199
+            ms = 4.0 / 3.0 * math.pi * r * r * r * dn            # CapsuleBase.cpp:147-168
+            mc = math.pi * r * r * L * dn
+            I = np.zeros((nc, 9))
+            I[:, 0] = r * r * (0.5 * mc + 0.4 * ms)
+            I[:, 4] = I[:, 8] = mc * (0.25 * r * r + 1.0 / 12.0 * L * L) + ms * (0.4 * r * r + 0.375 * r * L + 0.25 * L * L)
+            a["shape"][s] = CAPSULE
+            a["mat"][s] = self.capsule_mat
+            a["param"][s, 0] = r
+            a["param"][s, 1] = L
+            a["pos"][s] = self.capsule_pos
+            a["q"][s] = self.capsule_q
             a["inv_mass"][s] = 1.0 / m
             a["I"][s] = I
             a["Iinv"][s] = mat3_inverse(I)
@@ -128,12 +158,17 @@ class Scene:
             ow.create_sphere(self.sphere_pos[k], self.sphere_r[k], int(self.sphere_mat[k]))
         for k in range(self.n_boxes):
             ow.create_box(self.box_pos[k], self.box_len[k], int(self.box_mat[k]))
-        if self.v0 is not None or self.n_boxes:
+        for k in range(self.n_capsules):
+            ow.create_capsule(self.capsule_pos[k], self.capsule_r[k], self.capsule_len[k], int(self.capsule_mat[k]))
+        if self.v0 is not None or self.n_boxes or self.n_capsules:
             st = ow.get_state()
             if self.v0 is not None:
                 st["v"][:] = self.v0
+            b0 = len(self.planes) + self.n_spheres
             if self.n_boxes:
-                st["q"][len(self.planes) + self.n_spheres:] = self.box_q
+                st["q"][b0:b0 + self.n_boxes] = self.box_q
+            if self.n_capsules:
+                st["q"][b0 + self.n_boxes:] = self.capsule_q
             ow.set_state(st)
         return ow
 

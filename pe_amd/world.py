@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from . import _abi
-from ._abi import BOX, PLANE, SPHERE, check, ptr
+from ._abi import BOX, CAPSULE, PLANE, SPHERE, check, ptr
 
 CONTACT_THRESHOLD = 1e-8   # pe/core/Thresholds.h:151
 ACCURACY = 1e-8            # pe/util/Limits.h:385
@@ -259,6 +259,26 @@ class World:
         I[4] = m / 12.0 * (lx * lx + lz * lz)
         I[8] = m / 12.0 * (lx * lx + ly * ly)
         return self._add(BOX, material, False, (lx, ly, lz), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
+                         mat3_inverse(I), (0.0, 0.0, 0.0), 0.0, uid)
+
+    def createCapsule(self, uid, gpos, radius, length, material):
+        """createCapsule (Capsule.cpp:1077) with CapsuleBase's mass properties (CapsuleBase.cpp:60-92,
+        147-168): the axis is the body x-axis; the moments combine the cylinder and the end caps,
+        with the density recomputed from the mass as the reference does."""
+        r, L = float(radius), float(length)
+        if r <= 0:
+            raise ValueError("Invalid capsule radius")
+        if L <= 0:
+            raise ValueError("Invalid capsule length")
+        m = r * r * math.pi * (4.0 / 3.0 * r + L) * self.materials[material].density
+        density = m / (math.pi * r * r * (4.0 / 3.0 * r + L))
+        sphere_mass = 4.0 / 3.0 * math.pi * r * r * r * density
+        cylinder_mass = math.pi * r * r * L * density
+        I = np.zeros(9)
+        I[0] = r * r * (0.5 * cylinder_mass + 0.4 * sphere_mass)
+        I[4] = I[8] = (cylinder_mass * (0.25 * r * r + 1.0 / 12.0 * L * L) +
+                       sphere_mass * (0.4 * r * r + 0.375 * r * L + 0.25 * L * L))
+        return self._add(CAPSULE, material, False, (r, L, 0.0), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
                          mat3_inverse(I), (0.0, 0.0, 0.0), 0.0, uid)
 
     def createPlane(self, uid, normal, d, material):

@@ -13,7 +13,8 @@ import oracle as O
 from pe_amd import scenes, PeHipError
 from pe_amd.batch import BatchWorld
 
-from test_gpu_parity import contact_map, pair_ranks, rel_err, dense_scene, poly_scene, box_scene, stack_scene
+from test_gpu_parity import (contact_map, pair_ranks, rel_err, dense_scene, poly_scene, box_scene, stack_scene,
+                             capsule_scene)
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +27,7 @@ def small_scene(n, seed, mu=0.4):
 
 def batch_scenes():
     return [small_scene(3, 1), dense_scene(), small_scene(2, 3), poly_scene(), box_scene(), small_scene(4, 5),
-            stack_scene()]
+            stack_scene(), capsule_scene()]
 
 
 def oracle_states(scs, steps, cr_trig=False):

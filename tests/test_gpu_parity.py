@@ -101,6 +101,41 @@ def _qaxis(axis, ang):
     return np.concatenate([[np.cos(ang / 2)], np.sin(ang / 2) * ax / np.linalg.norm(ax)])
 
 
+def capsule_scene():
+    # capsules lying on the floor (capsule-plane, both end spheres), parallel pairs side by side
+    # (two capsule-capsule contacts), capsules crossed on top of others (one contact), spheres
+    # resting on capsules (sphere-capsule), and capsules lying on boxes (box-capsule, from both
+    # ends of the axis); small horizontal velocities
+    rng = np.random.default_rng(31)
+    e, r, L = 1e-6, 0.2, 0.8
+    cpos, cq = [], []
+    for k in range(4):                                          # floor row, parallel pairs
+        for j in range(2):
+            cpos.append([1.0 + 1.3 * k, 1.0 + j * (2 * r - e), r - e])
+            cq.append([1.0, 0, 0, 0])
+    for k in range(4):                                          # crossed on top of the pairs
+        cpos.append([1.0 + 1.3 * k, 1.0 + 0.5 * (2 * r - e), 3 * r - 3 * e])
+        cq.append(_qaxis((0, 0, 1), np.pi / 2 + rng.uniform(-0.2, 0.2)))
+    bpos = np.array([[2.0, 4.0, 0.3 - e], [4.5, 4.0, 0.3 - e]])
+    for b in bpos:                                              # lying on the boxes' top faces
+        cpos.append([b[0], b[1], 0.6 - e + r - e])
+        cq.append(_qaxis((0, 0, 1), rng.uniform(-0.5, 0.5)))
+    cpos = np.array(cpos) + rng.uniform(-1e-7, 1e-7, size=(len(cpos), 3))
+    nc = len(cpos)
+    # spheres on top of the crossed capsules, over their centres
+    sph = np.array([cpos[8 + k] + [0.0, 0.0, r + 0.25 - e] for k in range(4)])
+    ns = len(sph)
+    nb = len(bpos)
+    v0 = np.concatenate([np.zeros((6, 3)), rng.uniform(-0.1, 0.1, size=(ns, 3)) * [1, 1, 0], np.zeros((nb, 3)),
+                         rng.uniform(-0.2, 0.2, size=(nc, 3)) * [1, 1, 0]])
+    return scenes.Scene("capsules", [{"name": "m", "density": 2000.0, "cor": 0.0, "csf": 0.3, "cdf": 0.3}],
+                        scenes.box_planes(7.0, 6.0, 20.0), sph, np.full(ns, 0.25), np.zeros(ns, np.uint32),
+                        box_pos=bpos, box_len=np.array([[1.4, 1.0, 0.6]] * nb), box_q=np.array([[1.0, 0, 0, 0]] * nb),
+                        box_mat=np.zeros(nb, np.uint32), capsule_pos=cpos, capsule_r=np.full(nc, r),
+                        capsule_len=np.full(nc, L), capsule_q=np.array(cq), capsule_mat=np.zeros(nc, np.uint32),
+                        v0=v0, dt=1e-3, max_iterations=30)
+
+
 def stack_scene():
     # box-box contacts of every kind collideBoxBox distinguishes (MaxContacts.h:867-2268):
     # face-face between boxes turned about z (incident-face clipping), an edge resting on a
@@ -187,7 +222,7 @@ def hopper_scene():
 
 
 SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene, "stack": stack_scene,
-          "hopper": hopper_scene}
+          "hopper": hopper_scene, "capsules": capsule_scene}
 
 
 def oracle_state_after(sc, steps, iters=30, cr_trig=False):

@@ -572,3 +572,282 @@ def test_gmd_sphere_on_box_face_maximum_dissipation():
     k = int(np.argmin(energy))
     assert e_res <= energy[k] + 1e-9 * abs(energy[k])
     assert abs(th[ok][k]) < 1e-3   # the minimiser's direction is the returned impulse's (th = 0)
+
+
+# --------------------------------------------------------------------------- capsules
+def _quat_to_axis_x(q):
+    return O.quat_to_rotation(q)[:, 0]
+
+
+def _capsule_world(mu=0.4):
+    return make_world(mu=mu)
+
+
+def _set_orientation(w, idx, q):
+    st = w.get_state()
+    st["q"][idx] = q
+    w.set_state(st)
+
+
+def _segment_point_dist(p, a, b):
+    # distance from p to the segment [a, b] by projection (closed form)
+    ab = b - a
+    t = np.clip(np.dot(p - a, ab) / np.dot(ab, ab), 0.0, 1.0)
+    q = a + t * ab
+    return np.linalg.norm(p - q), q
+
+
+def test_capsule_mass_properties():
+    # m = rho pi r^2 (4/3 r + L); moments about the centre from the cylinder and two solid
+    # hemispheres (each of mass m_s/2, centroid 3r/8 from its face, face at L/2) by the
+    # parallel-axis theorem -- derived independently of the reference's combined formula.
+    r, L, rho = 0.3, 1.1, 2500.0
+    w = make_world(density=rho)
+    w.create_capsule([0, 0, 0], r, L, 0)
+    info = w.body_info()
+    m_cyl = rho * math.pi * r * r * L
+    m_s = rho * 4.0 / 3.0 * math.pi * r ** 3
+    m = m_cyl + m_s
+    assert 1.0 / info["inv_mass"][0] == pytest.approx(m, rel=1e-14)
+    Ia = 0.5 * m_cyl * r * r + 0.4 * m_s * r * r
+    hemi_face = 0.4 * (m_s / 2) * r * r                       # about a diameter of the flat face
+    hemi_cm = hemi_face - (m_s / 2) * (3 * r / 8) ** 2        # about its own centroid
+    hemi = hemi_cm + (m_s / 2) * (L / 2 + 3 * r / 8) ** 2     # about the capsule centre
+    Ib = m_cyl * (r * r / 4 + L * L / 12) + 2 * hemi
+    Iinv = info["inv_inertia"][0].reshape(3, 3)
+    assert np.allclose(np.diag(Iinv), [1 / Ia, 1 / Ib, 1 / Ib], rtol=1e-13)
+    assert np.count_nonzero(Iinv - np.diag(np.diag(Iinv))) == 0
+
+
+def test_capsule_aabb_rotated():
+    r, L = 0.25, 1.3
+    w = make_world()
+    w.create_capsule([1.0, 2.0, 3.0], r, L, 0)
+    q = O.quat_from_axis_angle([0.3, -0.5, 0.8], 0.9)
+    _set_orientation(w, 0, q)
+    ax = _quat_to_axis_x(q)
+    half = np.abs(ax) * L / 2 + r + 1e-8                      # extreme points of the end spheres
+    box = w.aabb()[0]
+    assert np.allclose(box[:3], np.array([1.0, 2.0, 3.0]) - half, atol=1e-15)
+    assert np.allclose(box[3:], np.array([1.0, 2.0, 3.0]) + half, atol=1e-15)
+
+
+def test_sphere_capsule_contact_closed_form():
+    rng = np.random.default_rng(11)
+    r_c, L, r_s = 0.3, 1.2, 0.4
+    q = O.quat_from_axis_angle([0.2, 1.0, -0.4], 0.7)
+    ax = _quat_to_axis_x(q)
+    hits = 0
+    for _ in range(40):
+        w = make_world()
+        w.create_capsule([0, 0, 0], r_c, L, 0)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        t = rng.uniform(-1.0, 1.0) * L / 2
+        p = t * ax + d * rng.uniform(0.6, 0.75)              # near the capsule surface
+        w.create_sphere(p, r_s, 0)
+        _set_orientation(w, 0, q)
+        c = w.detect(exhaustive=True)
+        dist, foot = _segment_point_dist(p, -ax * L / 2, ax * L / 2)
+        dist -= r_c + r_s
+        if dist >= 1e-8:
+            assert len(c["b1"]) == 0
+            continue
+        hits += 1
+        assert len(c["b1"]) == 1 and c["b1"][0] == 1 and c["b2"][0] == 0   # the sphere is body1
+        assert c["dist"][0] == pytest.approx(dist, abs=1e-14)
+        n = (p - foot) / np.linalg.norm(p - foot)
+        assert np.allclose(c["normal"][0], n, atol=1e-14)
+        assert np.allclose(c["pos"][0], foot + n * (r_c + dist / 2), atol=1e-14)
+    assert hits > 10
+
+
+def test_capsule_plane_end_contacts():
+    r, L = 0.3, 2.0
+    w = make_world()
+    w.create_plane([0, 0, 1], 0.0, 0)
+    w.create_capsule([0.5, -0.2, 0.31], r, L, 0)
+    q = O.quat_from_axis_angle([0, 1, 0], 0.02)             # axis tilted 0.02 rad about y
+    _set_orientation(w, 1, q)
+    c = w.detect()
+    ax = _quat_to_axis_x(q)
+    ends = [np.array([0.5, -0.2, 0.31]) + ax * L / 2, np.array([0.5, -0.2, 0.31]) - ax * L / 2]
+    expect = [(e, e[2] - r) for e in ends if e[2] - r < 1e-8]
+    assert len(expect) == 1 and len(c["b1"]) == 1             # one end down, one up
+    assert c["b1"][0] == 1 and c["b2"][0] == 0                # the capsule is body1
+    e, dist = expect[0]
+    assert c["dist"][0] == pytest.approx(dist, abs=1e-15)
+    assert np.allclose(c["pos"][0], e - r * np.array([0, 0, 1.0]), atol=1e-15)
+
+
+def _segment_segment_dist(a1, a2, b1, b2):
+    # brute force: a fine grid over both parameters, refined around the best cell
+    s = np.linspace(0, 1, 401)
+    A = a1 + s[:, None] * (a2 - a1)
+    B = b1 + s[:, None] * (b2 - b1)
+    D = np.linalg.norm(A[:, None] - B[None], axis=-1)
+    i, j = np.unravel_index(np.argmin(D), D.shape)
+    lo_s, hi_s = s[max(i - 1, 0)], s[min(i + 1, 400)]
+    lo_t, hi_t = s[max(j - 1, 0)], s[min(j + 1, 400)]
+    s2 = np.linspace(lo_s, hi_s, 801)
+    t2 = np.linspace(lo_t, hi_t, 801)
+    A = a1 + s2[:, None] * (a2 - a1)
+    B = b1 + t2[:, None] * (b2 - b1)
+    return float(np.min(np.linalg.norm(A[:, None] - B[None], axis=-1)))
+
+
+def test_capsule_capsule_crossed_single_contact():
+    rng = np.random.default_rng(5)
+    r1, r2, L1, L2 = 0.3, 0.25, 1.4, 1.0
+    count = 0
+    for _ in range(25):
+        w = make_world()
+        w.create_capsule([0, 0, 0], r1, L1, 0)
+        q1 = O.quat_from_axis_angle(rng.normal(size=3), rng.uniform(0, 3))
+        x2 = rng.normal(size=3) * 0.35
+        w.create_capsule(x2, r2, L2, 0)
+        q2 = O.quat_from_axis_angle(rng.normal(size=3), rng.uniform(0, 3))
+        st = w.get_state()
+        st["q"][0], st["q"][1] = q1, q2
+        w.set_state(st)
+        a, b = _quat_to_axis_x(q1), _quat_to_axis_x(q2)
+        if abs(abs(a @ b) - 1) < 1e-6:
+            continue
+        dseg = _segment_segment_dist(-a * L1 / 2, a * L1 / 2, x2 - b * L2 / 2, x2 + b * L2 / 2)
+        c = w.detect(exhaustive=True)
+        if dseg - r1 - r2 > 1e-6:
+            assert len(c["b1"]) == 0
+            continue
+        if dseg - r1 - r2 > -1e-6 and len(c["b1"]) == 0:
+            continue
+        count += 1
+        assert len(c["b1"]) == 1 and c["b1"][0] == 0 and c["b2"][0] == 1   # smaller SID first
+        assert c["dist"][0] == pytest.approx(dseg - r1 - r2, abs=2e-6)
+        assert np.linalg.norm(c["normal"][0]) == pytest.approx(1.0, abs=1e-14)
+    assert count > 8
+
+
+def test_capsule_capsule_parallel_two_contacts():
+    # two parallel capsules side by side along x, overlapping over [0.2, 0.7]: contacts at the
+    # ends of the overlap, on the midline between the surfaces
+    r, L1, L2 = 0.3, 1.4, 1.0
+    w = make_world()
+    w.create_capsule([0.0, 0.0, 0.0], r, L1, 0)       # axis [-0.7, 0.7]
+    w.create_capsule([1.2, 0.0, 0.59], r, L2, 0)      # axis [0.7, 1.7]: overlap ends at 0.7
+    w.create_capsule([0.2, 0.0, -0.59], r, L2, 0)     # axis [-0.3, 0.7] below
+    c = w.detect(exhaustive=True)
+    pairs = sorted(zip(c["b1"].tolist(), c["b2"].tolist(), c["pos"].tolist(), c["dist"].tolist()))
+    below = [p for p in pairs if p[1] == 2]
+    assert len(below) == 2
+    assert all(abs(p[3] + 0.01) < 1e-14 for p in below)
+    xs = sorted(p[2][0] for p in below)
+    assert xs == pytest.approx([-0.3, 0.7], abs=1e-14)
+    for p in below:
+        assert p[2][2] == pytest.approx(-0.295, abs=1e-14) and p[2][1] == 0.0
+    # the first capsule's own ends bound the overlap (a longer capsule below it)
+    w = make_world()
+    w.create_capsule([0.0, 0.0, 0.0], r, L1, 0)       # axis [-0.7, 0.7]
+    w.create_capsule([0.0, 0.0, -0.59], r, 2.0, 0)    # axis [-1.0, 1.0]
+    c = w.detect(exhaustive=True)
+    assert len(c["b1"]) == 2 and np.all(c["b1"] == 0) and np.all(c["b2"] == 1)
+    assert sorted(p[0] for p in c["pos"]) == pytest.approx([-0.7, 0.7], abs=1e-14)
+    assert np.allclose(c["pos"][:, 2], -0.295, atol=1e-14) and np.allclose(c["normal"], [[0, 0, 1.0]] * 2)
+
+
+def _segment_box_dist(a, b, half):
+    # brute force distance between the segment [a, b] and the axis-aligned box |x_k| <= half_k
+    s = np.linspace(0, 1, 20001)
+    P = a + s[:, None] * (b - a)
+    Q = np.clip(P, -half, half)
+    d = np.linalg.norm(P - Q, axis=1)
+    k = int(np.argmin(d))
+    return float(d[k]), P[k], Q[k]
+
+
+def test_box_capsule_contacts():
+    half = np.array([1.0, 0.6, 0.4])
+    r, L = 0.2, 1.0
+    # (1) lying on the top face: two contacts, one from each end of the axis
+    w = make_world()
+    w.create_box([0, 0, 0], 2 * half, 0)
+    w.create_capsule([0.1, 0.1, 0.4 + 0.2 - 0.003], r, L, 0)
+    c = w.detect(exhaustive=True)
+    assert len(c["b1"]) == 2 and np.all(c["b1"] == 0) and np.all(c["b2"] == 1)   # the box is body1
+    assert np.allclose(c["dist"], -0.003, atol=1e-14)
+    assert np.allclose(c["normal"], [[0, 0, -1.0]] * 2, atol=1e-14)   # from the capsule to the box
+    assert sorted(round(p[0], 12) for p in c["pos"]) == [-0.4, 0.6]
+    # (2) tilted over an edge: the first contact is the closest approach of axis and box
+    rng = np.random.default_rng(9)
+    hits = 0
+    for _ in range(20):
+        w = make_world()
+        w.create_box([0, 0, 0], 2 * half, 0)
+        q = O.quat_from_axis_angle(rng.normal(size=3), rng.uniform(0.3, 2.5))
+        ax = _quat_to_axis_x(q)
+        x = rng.normal(size=3)
+        x = x / np.linalg.norm(x) * 1.25
+        w.create_capsule(x, r, L, 0)
+        _set_orientation(w, 1, q)
+        up, dn = x + ax * L / 2, x - ax * L / 2
+        d, P, Q = _segment_box_dist(up, dn, half)
+        c = w.detect(exhaustive=True)
+        if d - r > 1e-4:
+            assert len(c["b1"]) == 0
+            continue
+        if d - r > -1e-4 or d < 1e-6:
+            continue
+        hits += 1
+        assert 1 <= len(c["b1"]) <= 2
+        assert c["dist"][0] == pytest.approx(d - r, abs=1e-6)
+        n = (Q - P) / np.linalg.norm(Q - P)
+        assert np.allclose(c["normal"][0], n, atol=1e-3)
+    assert hits >= 3
+
+
+def test_hashgrids_pair_set_with_capsules_and_boxes():
+    rng = np.random.default_rng(21)
+    n, Lx = 400, 9.0
+    w = make_world()
+    for p in [((0, 0, 1), 0.0), ((1, 0, 0), 0.0)]:
+        w.create_plane(p[0], p[1], 0)
+    kinds = rng.integers(0, 3, n)
+    pos = rng.uniform(0, Lx, size=(n, 3))
+    for k in range(n):
+        if kinds[k] == 0:
+            w.create_sphere(pos[k], rng.uniform(0.2, 0.5), 0)
+        elif kinds[k] == 1:
+            w.create_box(pos[k], rng.uniform(0.3, 0.9, size=3), 0)
+        else:
+            w.create_capsule(pos[k], rng.uniform(0.1, 0.3), rng.uniform(0.2, 1.2), 0)
+    st = w.get_state()
+    for k in range(n):
+        st["q"][2 + k] = O.quat_from_axis_angle(rng.normal(size=3), rng.uniform(0, 3))
+    w.set_state(st)
+    hg = w.detect(exhaustive=False)
+    ex = w.detect(exhaustive=True)
+    key = lambda c: sorted((int(a), int(b), tuple(p), tuple(nn), dd) for a, b, p, nn, dd in
+                           zip(c["b1"], c["b2"], c["pos"], c["normal"], c["dist"]))
+    kc = key(hg)
+    assert len(kc) > 50 and kc == key(ex)
+    shapes = w.body_info()["shape"]
+    kinds_seen = {tuple(sorted((int(shapes[a]), int(shapes[b])))) for a, b, *_ in kc}
+    assert {(1, 4), (2, 4), (4, 4), (3, 4)} <= kinds_seen
+
+
+def test_capsule_host_mass_properties_equal_oracle():
+    # the host mirrors (Scene.arrays, World.createCapsule) compute pe's capsule mass properties
+    # in the reference's operation order: bit for bit the oracle's
+    from pe_amd import scenes
+    rng = np.random.default_rng(3)
+    n = 20
+    sc = scenes.Scene("caps", [{"name": "m", "density": 1234.5, "cor": 0.0, "csf": 0.2, "cdf": 0.2}], [],
+                      capsule_pos=rng.uniform(0, 5, size=(n, 3)), capsule_r=rng.uniform(0.05, 0.6, n),
+                      capsule_len=rng.uniform(0.1, 2.0, n), capsule_q=np.tile([1.0, 0, 0, 0], (n, 1)),
+                      capsule_mat=np.zeros(n, np.uint32))
+    a = sc.arrays()
+    ow = O.OracleWorld()
+    sc.to_oracle(ow)
+    info = ow.body_info()
+    assert np.array_equal(a["inv_mass"], info["inv_mass"])
+    assert np.array_equal(a["Iinv"], info["inv_inertia"])
+    assert np.array_equal(a["param"][:, :2], info["param"][:, :2])
