@@ -949,12 +949,23 @@ __device__ __forceinline__ void sweep_st(double* p, double v) {
   else *p = v;
 }
 
+// e1, e2: the contact's body entries C.b1[i], C.b2[i] (the caller has them already).
+// COHERENT: the whole wavefront owns contact i (the dataflow kernel).
+template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
+__device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32_t e1, uint32_t e2,
+                                       const double* __restrict__ sb, double* __restrict__ dvw, double dtinv,
+                                       int maxSub);
 template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
 __device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
                                        double* __restrict__ dvw, double dtinv, int maxSub) {
+  return relax_contact<MODEL, Trig, SPEC, COHERENT>(C, i, C.b1[i], C.b2[i], sb, dvw, dtinv, maxSub);
+}
+template <int MODEL, class Trig, int SPEC, bool COHERENT>
+__device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32_t e1, uint32_t e2,
+                                       const double* __restrict__ sb, double* __restrict__ dvw, double dtinv,
+                                       int maxSub) {
   const size_t cap = C.cap;
   double* f = C.f;
-  const uint32_t e1 = C.b1[i], e2 = C.b2[i];
   const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
   const bool fx1 = (e1 & kFixedBit) != 0, fx2 = (e2 & kFixedBit) != 0;
 
@@ -983,8 +994,17 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, const 
   const d3 r1 = cf_ld3(f, cap, CF_R1, i), r2 = cf_ld3(f, cap, CF_R2, i);
   const d3 n = cf_ld3(f, cap, CF_N, i), t = cf_ld3(f, cap, CF_T, i), o = cf_ld3(f, cap, CF_O, i);
   const double ddt = f[CF_DDT * cap + i];
-  d3 p = mk3(sweep_ld<COHERENT>(f + CF_P * cap + i), sweep_ld<COHERENT>(f + (CF_P + 1) * cap + i),
-             sweep_ld<COHERENT>(f + (CF_P + 2) * cap + i));
+  d3 p;
+  if constexpr (COHERENT) {
+    // one load instruction for the three components (lanes 0-2), then broadcast: issued one
+    // after another, the three agent-scope loads cost three memory round trips
+    const uint32_t lane = threadIdx.x & 63u;
+    double pl = 0.0;
+    if (lane < 3) pl = sweep_ld<true>(f + (CF_P + lane) * cap + i);
+    p = mk3(wave_read_lane(pl, 0), wave_read_lane(pl, 1), wave_read_lane(pl, 2));
+  } else {
+    p = mk3(f[CF_P * cap + i], f[(CF_P + 1) * cap + i], f[(CF_P + 2) * cap + i]);
+  }
 
   // Remove this contact's reaction (HCSITS.h:2343-2346).
   dv1 = dv1 - im1 * p;

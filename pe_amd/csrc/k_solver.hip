@@ -326,13 +326,17 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
     const uint32_t e1 = C.b1[s], e2 = C.b2[s];
     const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
     const bool f1 = (e1 & kFixedBit) != 0, f2 = (e2 & kFixedBit) != 0;
-    if ((!f1 && !wait_count(done + i1, q * deg[i1] + rank1[s], flags, spin_limit)) ||
-        (!f2 && !wait_count(done + i2, q * deg[i2] + rank2[s], flags, spin_limit))) {
+    // both bodies' targets before either wait, so that the second body's loads are not one more
+    // round trip after the first wait
+    // (the loads are unconditional, so they issue together: indices of fixed bodies are valid too)
+    const uint32_t d1 = deg[i1], k1 = rank1[s], d2 = deg[i2], k2 = rank2[s];
+    const uint32_t t1 = q * d1 + k1, t2 = q * d2 + k2;
+    if ((!f1 && !wait_count(done + i1, t1, flags, spin_limit)) || (!f2 && !wait_count(done + i2, t2, flags, spin_limit))) {
       // give up and raise the flag, so that the wavefronts waiting for this one stop as well
       if (lane == 0) __hip_atomic_fetch_or(flags, ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
     }
-    const double d = relax_contact<MODEL, Trig, SPEC, true>(C, s, sb, dvw, dtinv, maxSub);
+    const double d = relax_contact<MODEL, Trig, SPEC, true>(C, s, e1, e2, sb, dvw, dtinv, maxSub);
     if (lane == 0) {
       if (d > 0) atomicMax(&deltas[q], (unsigned long long)__double_as_longlong(d));
       // Release without an agent-scope release: that writes back the XCD's whole L2 (measured
