@@ -17,31 +17,33 @@ pe/util/Endianness.h:308):
 
 A top-level body is sid (id_t), uid (id_t), visible (byte), fixed (byte), position (3 reals),
 orientation quaternion (4), linear and angular velocity (3 + 3), material (size_t), then the
-shape's parameters: a sphere's radius, a box's three edge lengths, nothing for a plane
-(Marshalling.h:184-210, 290-300, 321-330, 416-420, 498-501).  Reals are doubles unless the file
+shape's parameters: a sphere's radius, a box's three edge lengths, a capsule's radius and
+length, nothing for a plane (Marshalling.h:184-210, 290-300, 321-330, 352-356, 416-420, 498-501).  Reals are doubles unless the file
 was written with a smaller floating-point size (the header's first size byte: 4 = single).
 
 The reader creates the local chunk's bodies first, then the global chunk's
 (BodyBinaryReader.cpp:265-268), keeping their system IDs; a plane's normal is its orientation
 applied to +z and its displacement the normal's dot product with its position
-(src/core/rigidbody/Plane.cpp:121-142).  Capsules, cylinders, meshes and unions are not
-shapes of this library: reading a file that holds any raises ``ValueError``.
+(src/core/rigidbody/Plane.cpp:121-142).  Cylinders, meshes and unions are not shapes of this
+library: reading a file that holds any raises ``ValueError``.
 """
 import math
 import struct
 
 import numpy as np
 
-from ._abi import BOX, PLANE, SPHERE
+from ._abi import BOX, CAPSULE, PLANE, SPHERE
 from .world import mat3_inverse
 
 # pe/core/GeomType.h:45-53
 SPHERE_TYPE, BOX_TYPE, CAPSULE_TYPE, CYLINDER_TYPE, PLANE_TYPE, MESH_TYPE, UNION_TYPE = 1, 2, 3, 4, 5, 6, 7
 _LOCAL_ORDER = (SPHERE_TYPE, BOX_TYPE, CAPSULE_TYPE, CYLINDER_TYPE, MESH_TYPE, UNION_TYPE)
 _GLOBAL_ORDER = (SPHERE_TYPE, BOX_TYPE, CAPSULE_TYPE, CYLINDER_TYPE, PLANE_TYPE, MESH_TYPE, UNION_TYPE)
-_SHAPE_OF = {SPHERE_TYPE: SPHERE, BOX_TYPE: BOX, PLANE_TYPE: PLANE}
-_TYPE_OF = {SPHERE: SPHERE_TYPE, BOX: BOX_TYPE, PLANE: PLANE_TYPE}
-_NPARAM = {SPHERE_TYPE: 1, BOX_TYPE: 3, PLANE_TYPE: 0}
+_SHAPE_OF = {SPHERE_TYPE: SPHERE, BOX_TYPE: BOX, CAPSULE_TYPE: CAPSULE, PLANE_TYPE: PLANE}
+_TYPE_OF = {SPHERE: SPHERE_TYPE, BOX: BOX_TYPE, CAPSULE: CAPSULE_TYPE, PLANE: PLANE_TYPE}
+# reals after the common record: radius (Marshalling.h:296-300); lengths (:318-322); radius,
+# length (:352-356); none for a plane
+_NPARAM = {SPHERE_TYPE: 1, BOX_TYPE: 3, CAPSULE_TYPE: 2, PLANE_TYPE: 0}
 
 
 def quat_rotate(q, v):
@@ -229,6 +231,18 @@ def load_body_file(world, filename):
             a["I"][i, 0] = m / 12.0 * (L[1] * L[1] + L[2] * L[2])  # BoxBase.cpp:152-158
             a["I"][i, 4] = m / 12.0 * (L[0] * L[0] + L[2] * L[2])
             a["I"][i, 8] = m / 12.0 * (L[0] * L[0] + L[1] * L[1])
+            a["inv_mass"][i] = 1.0 / m
+            a["fixed"][i] = 1 if b["fixed"] else 0
+        elif t == CAPSULE_TYPE:
+            r, L = b["param"]
+            m = r * r * math.pi * (4.0 / 3.0 * r + L) * rho        # CapsuleBase.cpp:60-92
+            dn = m / (math.pi * r * r * (4.0 / 3.0 * r + L))       # CapsuleBase.h:196-199
+            ms = 4.0 / 3.0 * math.pi * r * r * r * dn              # CapsuleBase.cpp:147-168
+            mc = math.pi * r * r * L * dn
+            a["param"][i, :2] = (r, L)
+            a["I"][i, 0] = r * r * (0.5 * mc + 0.4 * ms)
+            a["I"][i, 4] = a["I"][i, 8] = (mc * (0.25 * r * r + 1.0 / 12.0 * L * L) +
+                                           ms * (0.4 * r * r + 0.375 * r * L + 0.25 * L * L))
             a["inv_mass"][i] = 1.0 / m
             a["fixed"][i] = 1 if b["fixed"] else 0
         else:                                                      # Plane.cpp:121-142

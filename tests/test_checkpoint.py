@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 from pe_amd import World, checkpoint, scenes
-from pe_amd._abi import BOX, PLANE, SPHERE
+from pe_amd._abi import BOX, CAPSULE, PLANE, SPHERE
 
 
 def mixed_world():
@@ -40,9 +40,11 @@ def mixed_world():
 
 
 def pe_reader_order(a):
-    """Body indices in the order pe's reader creates them: spheres, boxes (local), planes (global)."""
+    """Body indices in the order pe's reader creates them: spheres, boxes, capsules (local), planes
+    (global)."""
     s = a["shape"]
-    return np.concatenate([np.flatnonzero(s == SPHERE), np.flatnonzero(s == BOX), np.flatnonzero(s == PLANE)])
+    return np.concatenate([np.flatnonzero(s == SPHERE), np.flatnonzero(s == BOX), np.flatnonzero(s == CAPSULE),
+                           np.flatnonzero(s == PLANE)])
 
 
 def test_round_trip_reproduces_every_body(tmp_path):
@@ -143,7 +145,7 @@ def test_rejects_bad_files(tmp_path):
     bad.write_bytes(bytes(data[:-7]))
     with pytest.raises(ValueError):
         checkpoint.read_body_file(str(bad))
-    # a capsule (type 3) in the local chunk: not a shape of this library
+    # a cylinder (type 4) in the local chunk: not a shape of this library
     off_l = struct.unpack(">I", bytes(data[17:21]))[0]
     v = bytearray(data)
     n_sph = struct.unpack(">I", bytes(v[off_l + 9:off_l + 13]))[0]
@@ -151,11 +153,60 @@ def test_rejects_bad_files(tmp_path):
     box_hdr = off_l + 13 + n_sph * rec
     n_box = struct.unpack(">I", bytes(v[box_hdr + 1:box_hdr + 5]))[0]
     cap_hdr = box_hdr + 5 + n_box * (rec + 16)
-    assert v[cap_hdr] == 3
-    v[cap_hdr + 1:cap_hdr + 5] = struct.pack(">I", 1)
+    assert v[cap_hdr] == 3 and struct.unpack(">I", bytes(v[cap_hdr + 1:cap_hdr + 5]))[0] == 0
+    cyl_hdr = cap_hdr + 5
+    assert v[cyl_hdr] == 4
+    v[cyl_hdr + 1:cyl_hdr + 5] = struct.pack(">I", 1)
     bad.write_bytes(bytes(v))
-    with pytest.raises(ValueError, match="Unsupported geometry type 3"):
+    with pytest.raises(ValueError, match="Unsupported geometry type 4"):
         checkpoint.read_body_file(str(bad))
+
+
+def capsule_world():
+    rng = np.random.default_rng(8)
+    w = World()
+    w.createMaterial("rubber", 1100.0, 0.5, 0.6, 0.5)
+    w.createPlane(1, (0, 0, 1), 0.0, 0)
+    for k in range(3):
+        w.createSphere(10 + k, rng.uniform(1, 5, 3), 0.3, 0)
+    for k in range(4):
+        w.createCapsule(20 + k, rng.uniform(1, 5, 3), rng.uniform(0.1, 0.4), rng.uniform(0.3, 1.5), 0)
+    a = w.host_arrays()
+    caps = a["shape"] == CAPSULE
+    a["q"][caps] = scenes.shoemake_quaternions(rng, int(caps.sum()))
+    a["v"][caps] = rng.uniform(-1, 1, (int(caps.sum()), 3))
+    return w
+
+
+def test_capsule_records_round_trip(tmp_path):
+    # capsules are local bodies after spheres and boxes, their record ending in radius and length
+    # (Marshalling.h:352-356); reading them back gives pe's capsule mass properties
+    w = capsule_world()
+    f = tmp_path / "caps.bin"
+    checkpoint.write_body_file(w, str(f))
+    data = f.read_bytes()
+    off_l = struct.unpack(">I", data[17:21])[0]
+    n_sph = struct.unpack(">I", data[off_l + 9:off_l + 13])[0]
+    rec = 18 + 13 * 8 + 8 + 8
+    box_hdr = off_l + 13 + n_sph * rec
+    assert data[box_hdr] == 2 and struct.unpack(">I", data[box_hdr + 1:box_hdr + 5])[0] == 0
+    cap_hdr = box_hdr + 5
+    assert data[cap_hdr] == 3 and struct.unpack(">I", data[cap_hdr + 1:cap_hdr + 5])[0] == 4
+    a = w.host_arrays()
+    ci = np.flatnonzero(a["shape"] == CAPSULE)
+    first = cap_hdr + 5
+    crec = rec + 8                                        # radius and length: one real more
+    for k, i in enumerate(ci):
+        tail = data[first + k * crec + crec - 16:first + k * crec + crec]
+        assert struct.unpack(">dd", tail) == (a["param"][i][0], a["param"][i][1])
+    w2 = World()
+    for m in w.materials:
+        w2.createMaterial(m.name, m.density, m.cor, m.csf, m.cdf)
+    checkpoint.load_body_file(w2, str(f))
+    b = w2.host_arrays()
+    order = pe_reader_order(a)
+    for key in ("shape", "sid", "uid", "pos", "q", "v", "w", "param", "inv_mass", "I", "Iinv"):
+        assert np.array_equal(np.asarray(a[key])[order], np.asarray(b[key])), key
 
 
 @pytest.mark.gpu
