@@ -1,6 +1,7 @@
 // A pe-style program on the drop-in API: spheres on a lattice fall into a box of planes
 // (the setup of examples/lattice/Lattice.cpp:197-217 with a deterministic jitter).
-// Usage: granular_dropin <nx> <ny> <nz> <steps>; prints the final state as hex floats.
+// Usage: granular_dropin <nx> <ny> <nz> <steps> [capsules]; the optional capsules (radius 0.25,
+// length 0.8, axis along x) drop onto the spheres.  Prints the final state as hex floats.
 #include <cstdio>
 #include <cstdlib>
 #include <exception>
@@ -14,6 +15,7 @@ static int run(int argc, char** argv) {
   const int ny = argc > 2 ? std::atoi(argv[2]) : 5;
   const int nz = argc > 3 ? std::atoi(argv[3]) : 5;
   const unsigned steps = argc > 4 ? (unsigned)std::atoi(argv[4]) : 100;
+  const int ncaps = argc > 5 ? std::atoi(argv[5]) : 0;
   const real r = 0.5, spacing = 1.2, dt = 1e-3;
   const real lx = 2.0 + nx * spacing, ly = 2.0 + ny * spacing, lz = 40.0;
 
@@ -37,6 +39,8 @@ static int run(int argc, char** argv) {
         const real jit = 0.001 * (real)((i * 7 + j * 13 + k * 17) % 11 - 5);
         createSphere(6, Vec3(1.5 + spacing * i + jit, 1.5 + spacing * j - jit, 0.6 + spacing * k + jit), r, granular);
       }
+  for (int c = 0; c < ncaps; ++c)
+    createCapsule(7, Vec3(1.6 + 1.1 * c, 0.5 * ly + 0.05 * c, 0.6 + spacing * (nz - 1) + 0.77), 0.25, 0.8, granular);
 
   world->run(steps, dt);
 

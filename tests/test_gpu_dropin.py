@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "pe_amd", "_lib", "granular_dropin")
 
 
-def python_run(nx, ny, nz, steps):
+def python_run(nx, ny, nz, steps, ncaps=0):
     w = World()
     w.setGravity(0, 0, -9.81)
     w.setMaxIterations(50)
@@ -29,15 +29,20 @@ def python_run(nx, ny, nz, steps):
             for i in range(nx):
                 jit = 0.001 * float((i * 7 + j * 13 + k * 17) % 11 - 5)
                 w.createSphere(6, (1.5 + 1.2 * i + jit, 1.5 + 1.2 * j - jit, 0.6 + 1.2 * k + jit), 0.5, m)
+    for c in range(ncaps):
+        w.createCapsule(7, (1.6 + 1.1 * c, 0.5 * ly + 0.05 * c, 0.6 + 1.2 * (nz - 1) + 0.77), 0.25, 0.8, m)
     w.run(steps, 1e-3)
     return w
 
 
-def test_cpp_dropin_matches_python_world():
+@pytest.mark.parametrize("ncaps", [0, 3])
+def test_cpp_dropin_matches_python_world(ncaps):
+    # with capsules: pe::createCapsule's mass properties equal World.createCapsule's bit for bit
     if not os.path.exists(EXE):
         pytest.fail("drop-in example not built")
     nx, ny, nz, steps = 5, 4, 3, 220
-    r = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps), str(ncaps)], capture_output=True, text=True,
+                       timeout=120)
     print(r.stderr)
     assert r.returncode == 0, r.stderr
     out = r.stdout.split("\n")
@@ -45,7 +50,7 @@ def test_cpp_dropin_matches_python_world():
     rows = [l.split() for l in out[1:] if l.strip()]
     pos = np.array([[float.fromhex(x) for x in r[1:4]] for r in rows])
     vel = np.array([[float.fromhex(x) for x in r[4:7]] for r in rows])
-    w = python_run(nx, ny, nz, steps)
+    w = python_run(nx, ny, nz, steps, ncaps)
     st = w.getState()
     assert len(rows) == w.size
     assert int(head[1]) == w.getNumberOfContacts() and int(head[1]) > 0
