@@ -334,24 +334,34 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
   for (int l = (int)la; l < G.nlevels; ++l) {
     long long c[3];
     bucket_of(G, l, abox, c);
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const uint32_t bucket = ((uint32_t)((c[2] + dz) & (G.dimz - 1)) * G.dimy +
-                                   (uint32_t)((c[1] + dy) & (G.dimy - 1))) * G.dimx +
-                                  (uint32_t)((c[0] + dx) & (G.dimx - 1));
-          const uint32_t key = (uint32_t)l * G.table + bucket;
-          const uint32_t st = P.cell_start[key];
-          if (st == kEmpty) continue;
-          const uint32_t en = P.cell_end[key];
-          for (uint32_t t = st; t < en; ++t) {
-            const uint32_t b = P.vals[t];
-            if (l == (int)la && b <= a) continue;
-            if (fa && B.fixed[b]) continue;
-            if (!aabb_overlap(abox, B.aabb + 6 * (size_t)b)) continue;
-            narrowphase(B, a, b, emit);
-          }
-        }
+    // The 27 cells' start slots are loaded together (independent loads, one round trip) into a
+    // mask of the non-empty cells; the walk then takes those in the order dz, dy, dx.
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+      const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      const uint32_t bucket = ((uint32_t)((c[2] + dz) & (G.dimz - 1)) * G.dimy +
+                               (uint32_t)((c[1] + dy) & (G.dimy - 1))) * G.dimx +
+                              (uint32_t)((c[0] + dx) & (G.dimx - 1));
+      if (P.cell_start[(uint32_t)l * G.table + bucket] != kEmpty) mask |= 1u << k;
+    }
+    while (mask) {
+      const int k = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      const uint32_t bucket = ((uint32_t)((c[2] + dz) & (G.dimz - 1)) * G.dimy +
+                               (uint32_t)((c[1] + dy) & (G.dimy - 1))) * G.dimx +
+                              (uint32_t)((c[0] + dx) & (G.dimx - 1));
+      const uint32_t key = (uint32_t)l * G.table + bucket;
+      const uint32_t st = P.cell_start[key], en = P.cell_end[key];
+      for (uint32_t t = st; t < en; ++t) {
+        const uint32_t b = P.vals[t];
+        if (l == (int)la && b <= a) continue;
+        if (fa && B.fixed[b]) continue;
+        if (!aabb_overlap(abox, B.aabb + 6 * (size_t)b)) continue;
+        narrowphase(B, a, b, emit);
+      }
+    }
   }
   for (uint32_t k = 0; k < P.nplanes; ++k) {
     const uint32_t p = P.planes[k];
@@ -372,7 +382,9 @@ __global__ void k_pairs_count(Bodies B, GridParams G, BroadphaseBuffers P, uint3
 __global__ void k_pairs_write(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid, RawContacts R,
                               const double* csf, const double* cdf, uint32_t nmat) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= ngrid) return;
+  // the count pass found this body's contacts; most bodies have none (the enumeration is
+  // deterministic, so the walk would emit nothing) and their wavefronts end at once
+  if (s >= ngrid || P.counts[s] == 0) return;
   WriteEmit e{&B, &R, csf, cdf, nmat, P.offsets[s]};
   enumerate_pairs(B, G, P, s, e, P.flags);
 }
