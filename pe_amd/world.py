@@ -411,6 +411,9 @@ class World:
         """World::run (World.cpp:374-378)."""
         if dt <= 0:
             raise ValueError("Invalid time step size")
+        if self.size == 0:            # nothing to simulate: only the step counter advances
+            self.timestep += int(steps)
+            return
         self._ensure_device()
         check(_abi.load().pehip_step(self._ctx, float(dt), int(steps)), self._ctx)
         self._state_on_device = True
@@ -488,6 +491,8 @@ class World:
 
     def stats(self):
         s = _abi.Stats()
+        if self._ctx is None:         # never stepped (e.g. a world without bodies): all zero
+            return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
         check(_abi.load().pehip_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
         return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
 
