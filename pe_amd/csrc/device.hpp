@@ -888,15 +888,17 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const
         S.xm0 = group_read<SPEC>(xn0, L - 1);
         S.xm1 = group_read<SPEC>(xn1, L - 1);
       }
-      // best value held on reaching this node: fold of the ancestors' values, root first
-      double m = S.fm;
+      // best value held on reaching this node: fold of the ancestors' values, root first (all
+      // the ancestors' values are read before the fold, so the cross-lane reads overlap)
+      double fanc[SPEC - 1 > 0 ? SPEC - 1 : 1];
 #pragma unroll
       for (int l = 0; l < SPEC - 1; ++l) {
-        const bool act = l < depth;
-        const uint32_t src = act ? (code >> (depth - l)) - 1u : lane;
-        const double fa = __shfl(fn, (int)src, L);
-        m = (act && fa < m) ? fa : m;
+        const uint32_t src = l < depth ? (code >> (depth - l)) - 1u : lane;
+        fanc[l] = __shfl(fn, (int)src, L);
       }
+      double m = S.fm;
+#pragma unroll
+      for (int l = 0; l < SPEC - 1; ++l) m = (l < depth && fanc[l] < m) ? fanc[l] : m;
       const unsigned long long all = __ballot(fn < m);
       const uint64_t bits = SPEC == 6 ? all : (all >> base) & ((1ull << L) - 1ull);
       // walk the taken path: fin = the last node compared, last = the last improving one
