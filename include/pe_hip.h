@@ -37,7 +37,7 @@ typedef enum {
   PEHIP_NOT_SUPPORTED = 6
 } pehip_status;
 
-typedef enum { PEHIP_SPHERE = 1, PEHIP_BOX = 2, PEHIP_PLANE = 3, PEHIP_CAPSULE = 4 } pehip_shape;
+typedef enum { PEHIP_SPHERE = 1, PEHIP_BOX = 2, PEHIP_PLANE = 3, PEHIP_CAPSULE = 4, PEHIP_UNION = 5 } pehip_shape;
 
 /* CollisionSystem<HCSITS>::RelaxationModel, same numbering
  * (pe/core/collisionsystem/HardContactSemiImplicitTimesteppingSolvers.h:177-184). */
@@ -115,7 +115,10 @@ pehip_status pehip_set_materials(pehip_ctx* ctx, uint32_t n, const double* csf_p
  *   sid[i]             system id (UniqueID.h:143); sphere-sphere contacts order their bodies by it
  *   material[i]        row of the pair tables
  *   fixed[i]           1 = fixed (planes are always fixed)
- *   param[3i..]        sphere: radius,0,0; box: side lengths; plane: unused
+ *   param[3i..]        sphere: radius,0,0; box: side lengths; capsule: radius,length,0;
+ *                      union: bounding radius about its centre of mass (largest |rpos| + member
+ *                      radius), smallest member radius, 0 (used to size the hash grids);
+ *                      plane: unused
  *   pos[3i], q[4i], v[3i], w[3i]   state (R is derived from q as R = q.toRotationMatrix())
  *   inv_mass[i], inertia_body[9i], inv_inertia_body[9i]   mass properties (body frame)
  *   plane_normal[3i], plane_d[i]   plane parameters, host-computed (meaningful for planes)
@@ -128,7 +131,27 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
                                  const double* inv_inertia_body, const double* plane_normal,
                                  const double* plane_d);
 
-/* Overwrites the dynamic state of all n bodies (checkpoint restore). */
+/* Overwrites the dynamic state of all n bodies (checkpoint restore).  Union members keep their
+ * positions (pehip_upload_members places them). */
+/* The spheres contained in the union bodies (pe's Union of Spheres, pe/core/rigidbody/Union.h;
+ * createGranularParticle, Union.cpp:2376-2402), replacing Union::add (Union.cpp:1482-1545).  Called
+ * after pehip_upload_bodies whenever that call holds a union; until then pehip_step and
+ * pehip_detect fail.  Per member k, grouped by union in body order and, within a union, in the
+ * union's order:
+ *   owner[k]     body index of its union (non-decreasing; every union owns at least one member)
+ *   sid[k]       the member's own system id (sphere-sphere contacts order by it)
+ *   material[k]  row of the pair tables (a union's contacts take their members' materials)
+ *   radius[k]    sphere radius
+ *   rpos[3k]     position in the union's body frame (RigidBody::calcRelPosition, RigidBody.h:1370)
+ *   gpos[3k]     global position
+ * The union bodies themselves carry the union's centre of mass, orientation, velocities and
+ * mass properties (UnionBase::calcCenterOfMass / calcInertia, UnionBase.cpp:240-360). */
+pehip_status pehip_upload_members(pehip_ctx* ctx, uint32_t nm, const uint32_t* owner,
+                                  const uint64_t* sid, const uint32_t* material, const double* radius,
+                                  const double* rpos, const double* gpos);
+/* Global positions of the union members (they follow their unions each step, Sphere.cpp:1052). */
+pehip_status pehip_download_members(pehip_ctx* ctx, uint32_t nm, double* gpos);
+
 pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, const double* q,
                                 const double* v, const double* w, const double* plane_normal,
                                 const double* plane_d);

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
-SPHERE, BOX, PLANE, CAPSULE = 1, 2, 3, 4
+SPHERE, BOX, PLANE, CAPSULE, UNION = 1, 2, 3, 4, 5
 INELASTIC_FRICTIONLESS = 0
 APPROX_INELASTIC_COULOMB_DECOUPLING = 1
 APPROX_INELASTIC_COULOMB_ORTHO = 2
@@ -67,6 +67,10 @@ def lib():
             "orc_create_box": (ctypes.c_int, [vp] + [ctypes.c_double] * 6 + [ctypes.c_int]),
             "orc_create_capsule": (ctypes.c_int, [vp] + [ctypes.c_double] * 5 + [ctypes.c_int]),
             "orc_create_plane": (ctypes.c_int, [vp] + [ctypes.c_double] * 4 + [ctypes.c_int]),
+            "orc_create_union": (ctypes.c_int, [vp] + [ctypes.c_double] * 3 + [ctypes.c_int, _D, _D, _I32]),
+            "orc_num_members": (ctypes.c_size_t, [vp]),
+            "orc_get_members": (None, [vp, _U32, _U64, _I32, _D, _D, _D]),
+            "orc_set_members": (None, [vp, _D]),
             "orc_num_bodies": (ctypes.c_size_t, [vp]),
             "orc_get_state": (None, [vp, _D, _D, _D, _D, _D, _D]),
             "orc_set_state": (None, [vp, _D, _D, _D, _D, _D, _D]),
@@ -138,22 +142,51 @@ class OracleWorld:
     def create_plane(self, normal, d, mat):
         return self.L.orc_create_plane(self.h, *[float(x) for x in normal], float(d), mat)
 
+    def create_union(self, pos, disp, radius, mat):
+        """A union of spheres at offsets disp (k x 3) whose centre of mass goes to pos."""
+        d = np.ascontiguousarray(disp, np.float64).reshape(-1, 3)
+        r = np.ascontiguousarray(radius, np.float64).reshape(-1)
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mat, np.int32), r.shape))
+        idx = self.L.orc_create_union(self.h, *[float(x) for x in pos], len(r), _p(d), _p(r), _p(m, _I32))
+        if idx < 0:
+            raise ValueError("a union needs at least one sphere")
+        return idx
+
+    def members(self):
+        """Union members: owner body, system ID, material, radius, relative and global position."""
+        k = int(self.L.orc_num_members(self.h))
+        m = {"owner": np.zeros(k, np.uint32), "sid": np.zeros(k, np.uint64), "mat": np.zeros(k, np.int32),
+             "radius": np.zeros(k), "rpos": np.zeros((k, 3)), "gpos": np.zeros((k, 3))}
+        if k:
+            self.L.orc_get_members(self.h, _p(m["owner"], _U32), _p(m["sid"], _U64), _p(m["mat"], _I32),
+                                   _p(m["radius"]), _p(m["rpos"]), _p(m["gpos"]))
+        return m
+
+    def set_members(self, gpos):
+        g = np.ascontiguousarray(gpos, np.float64)
+        self.L.orc_set_members(self.h, _p(g))
+
     @property
     def n(self):
         return int(self.L.orc_num_bodies(self.h))
 
     # --- state -------------------------------------------------------------------------
     def get_state(self):
+        """Dynamic state; with unions also "mgpos", the members' global positions."""
         n = self.n
         st = {"pos": np.zeros((n, 3)), "q": np.zeros((n, 4)), "v": np.zeros((n, 3)),
               "w": np.zeros((n, 3)), "normal": np.zeros((n, 3)), "d": np.zeros(n)}
         self.L.orc_get_state(self.h, _p(st["pos"]), _p(st["q"]), _p(st["v"]), _p(st["w"]),
                              _p(st["normal"]), _p(st["d"]))
+        if self.L.orc_num_members(self.h):
+            st["mgpos"] = self.members()["gpos"]
         return st
 
     def set_state(self, st):
         arrs = [np.ascontiguousarray(st[k], dtype=np.float64) for k in ("pos", "q", "v", "w", "normal", "d")]
         self.L.orc_set_state(self.h, *[_p(a) for a in arrs])
+        if "mgpos" in st and self.L.orc_num_members(self.h):
+            self.set_members(st["mgpos"])
 
     def body_info(self):
         n = self.n

@@ -23,7 +23,7 @@ extern "C" {
 
 typedef struct orc_world orc_world;
 
-enum { ORC_SPHERE = 1, ORC_BOX = 2, ORC_PLANE = 3, ORC_CAPSULE = 4 };
+enum { ORC_SPHERE = 1, ORC_BOX = 2, ORC_PLANE = 3, ORC_CAPSULE = 4, ORC_UNION = 5 };
 
 /* Relaxation models, numbered as the reference's RelaxationModel enum
  * (pe/core/collisionsystem/HardContactSemiImplicitTimesteppingSolvers.h:177-184). */
@@ -65,6 +65,15 @@ int orc_create_sphere(orc_world* w, double x, double y, double z, double r, int 
 int orc_create_box(orc_world* w, double x, double y, double z, double lx, double ly, double lz, int mat);
 int orc_create_capsule(orc_world* w, double x, double y, double z, double r, double length, int mat);
 int orc_create_plane(orc_world* w, double nx, double ny, double nz, double d, int mat);
+/* A union of n spheres (offsets disp[3n] before placement, radii, materials) whose centre of mass
+ * is placed at (x, y, z); returns the union's body index, -1 if n < 1. */
+int orc_create_union(orc_world* w, double x, double y, double z, int n, const double* disp,
+                     const double* radius, const int* mat);
+/* Union members (spheres), grouped by union in creation order; any output may be NULL. */
+size_t orc_num_members(const orc_world* w);
+void orc_get_members(const orc_world* w, uint32_t* owner, uint64_t* sid, int* mat, double* radius,
+                     double* rpos, double* gpos);
+void orc_set_members(orc_world* w, const double* gpos);
 
 size_t orc_num_bodies(const orc_world* w);
 

@@ -216,6 +216,9 @@ struct Body {
   double length = 0;          // capsule: length of the cylinder part, along the body x-axis
   V3 lengths;                 // box
   V3 normal; double d = 0;    // plane
+  int super = -1;             // union member: index of its union in the body storage
+  V3 rpos;                    // union member: position in the union's body frame
+  int mbegin = 0, mcount = 0; // union: its members, orc_world::members[mbegin, mbegin + mcount)
   double aabb[6] = {0, 0, 0, 0, 0, 0};
   // HashGrids bookkeeping (BodyTrait<HashGrids>): grid, hash, cell id
   int grid = -1;
@@ -287,6 +290,10 @@ struct Contact {
 struct Fine {
   orc_world* world;
   std::vector<Contact>* out;
+  // Geometry ids: g < number of bodies names a body of the body storage, larger ids name union
+  // members (g - number of bodies indexes orc_world::members).
+  const Body& G(int g) const;
+  int top(int g) const;  // RigidBody::getTopSuperBody
   void add(int b1, int b2, const V3& gpos, const V3& n, double dist);  // Contact.cpp:72-119
   void sphereSphere(int s1, int s2);
   void sphereBox(int s, int b);
@@ -298,6 +305,7 @@ struct Fine {
   void capsuleCapsule(int c1, int c2);
   void capsulePlane(int c, int p);
   void collide(int a, int b);
+  void collideUnion(int a, int u);  // collide{Sphere,Box,Capsule,Plane}Union
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -444,6 +452,7 @@ void HashGrid::enlarge(std::vector<Body>& bodies) {
 // ---------------------------------------------------------------------------------------------
 struct orc_world {
   std::vector<Body> bodies;
+  std::vector<Body> members;  // spheres contained in unions, grouped by union
   std::vector<Material> materials;
   std::vector<double> csf, cdf;  // pair tables, row-major n x n
   uint64_t nextSid = 1;
@@ -631,42 +640,66 @@ struct orc_world {
   double relaxInelasticGeneralizedMaximumDissipationContacts(double dtinv);
   void initializeVelocityCorrections(Body& b, V3& dv, V3& dw, double dt);
   void integratePositions(Body& b, V3 v, V3 w, double dt);
+  // Union::calcBoundingBox (UnionBase.cpp:208-230): the members' boxes merged in member order
+  // (BoundingBox::operator+=, BoundingBox.h:339-348); every other body's own box.
+  void calcBoundingBox(Body& b) {
+    if (b.shape != ORC_UNION) { b.calcBoundingBox(); return; }
+    for (int k = 0; k < b.mcount; ++k) {
+      const double* a = members[b.mbegin + k].aabb;
+      if (k == 0) { for (int j = 0; j < 6; ++j) b.aabb[j] = a[j]; continue; }
+      for (int j = 0; j < 3; ++j) b.aabb[j] = (a[j] < b.aabb[j]) ? a[j] : b.aabb[j];
+      for (int j = 3; j < 6; ++j) b.aabb[j] = (b.aabb[j] < a[j]) ? a[j] : b.aabb[j];
+    }
+  }
 };
 
 namespace {
 
 // Contact::Contact, src/core/contact/Contact.cpp:72-119 (friction choice 102-105).
-void Fine::add(int b1, int b2, const V3& gpos, const V3& n, double dist) {
-  const Body& A = world->bodies[b1];
-  const Body& B = world->bodies[b2];
+const Body& Fine::G(int g) const {
+  const int nb = (int)world->bodies.size();
+  return g < nb ? world->bodies[g] : world->members[g - nb];
+}
+int Fine::top(int g) const {
+  const int nb = (int)world->bodies.size();
+  return g < nb ? g : world->members[g - nb].super;
+}
+
+// The contact joins the top-level bodies (ContactBase.h:209-210) and takes the friction of the
+// colliding geometries' materials (Contact.cpp:102-105).
+void Fine::add(int g1, int g2, const V3& gpos, const V3& n, double dist) {
+  const int b1 = top(g1), b2 = top(g2);
+  const Body& A = G(b1);
+  const Body& B = G(b2);
   const V3 rvel = A.velFromWF(gpos) - B.velFromWF(gpos);
   const double nvel = dot(n, rvel);
   const double tvel = dot(n, rvel - n * nvel);
   Contact c;
   c.b1 = b1; c.b2 = b2; c.pos = gpos; c.normal = n; c.dist = dist;
-  c.mu = (std::fabs(tvel) > kFrictionThreshold) ? world->pairCdf(A.mat, B.mat)
-                                                 : world->pairCsf(A.mat, B.mat);
+  const int m1 = G(g1).mat, m2 = G(g2).mat;
+  c.mu = (std::fabs(tvel) > kFrictionThreshold) ? world->pairCdf(m1, m2) : world->pairCsf(m1, m2);
   out->push_back(c);
 }
 
 // MaxContacts::collideSphereSphere, MaxContacts.h:547-568
 void Fine::sphereSphere(int s1, int s2) {
-  std::vector<Body>& bs = world->bodies;
-  if (bs[s2].sid < bs[s1].sid) std::swap(s1, s2);
-  V3 normal = bs[s1].gpos - bs[s2].gpos;
-  const double dist = length(normal) - bs[s1].radius - bs[s2].radius;
+  if (G(s2).sid < G(s1).sid) std::swap(s1, s2);
+  const Body& S1 = G(s1);
+  const Body& S2 = G(s2);
+  V3 normal = S1.gpos - S2.gpos;
+  const double dist = length(normal) - S1.radius - S2.radius;
   if (dist < kContactThreshold) {
     normal = normalized(normal);
-    const double k = bs[s2].radius + 0.5 * dist;
-    const V3 gPos = bs[s2].gpos + normal * k;
+    const double k = S2.radius + 0.5 * dist;
+    const V3 gPos = S2.gpos + normal * k;
     add(s1, s2, gPos, normal, dist);
   }
 }
 
 // MaxContacts::collideSphereBox, MaxContacts.h:589-667
 void Fine::sphereBox(int s, int b) {
-  const Body& S = world->bodies[s];
-  const Body& B = world->bodies[b];
+  const Body& S = G(s);
+  const Body& B = G(b);
   const V3 l = 0.5 * B.lengths;
   const V3& spos = S.gpos;
   const V3& bpos = B.gpos;
@@ -703,8 +736,8 @@ void Fine::sphereBox(int s, int b) {
 
 // MaxContacts::collideSpherePlane, MaxContacts.h:777-792
 void Fine::spherePlane(int s, int p) {
-  const Body& S = world->bodies[s];
-  const Body& P = world->bodies[p];
+  const Body& S = G(s);
+  const Body& P = G(p);
   const double k = dot(P.normal, S.gpos);
   const double dist = k - S.radius - P.d;
   if (dist < kContactThreshold) {
@@ -715,8 +748,8 @@ void Fine::spherePlane(int s, int p) {
 
 // MaxContacts::collideBoxPlane, MaxContacts.h:2380-2528 (8 corners in the reference's order)
 void Fine::boxPlane(int b, int p) {
-  const Body& B = world->bodies[b];
-  const Body& P = world->bodies[p];
+  const Body& B = G(b);
+  const Body& P = G(p);
   const V3& bpos = B.gpos;
   const V3& L = B.lengths;
   const M3& R = B.R;
@@ -760,10 +793,9 @@ void intersectLines(const V3& o1, const V3& d1, const V3& o2, const V3& d2, doub
 // MaxContacts::collideBoxBox, MaxContacts.h:867-2268: separating-axis test over 3 + 3 face
 // axes and 9 edge axes, then one edge-edge contact or a clipped face contact (up to 8).
 void Fine::boxBox(int i1, int i2) {
-  std::vector<Body>& bs = world->bodies;
-  if (bs[i2].sid < bs[i1].sid) std::swap(i1, i2);
-  const Body* B1 = &bs[i1];
-  const Body* B2 = &bs[i2];
+  if (G(i2).sid < G(i1).sid) std::swap(i1, i2);
+  const Body* B1 = &G(i1);
+  const Body* B2 = &G(i2);
   double maxDepth = -kInf;
   bool invertNormal = false;
   unsigned contactCase = 0;
@@ -1068,8 +1100,8 @@ bool nearly_equal(const V3& a, const V3& b) {
 // collideSphereCapsule, MaxContacts.h:690-725: the sphere against the closest point of the
 // capsule's axis segment.  The sphere is body1.
 void Fine::sphereCapsule(int si, int ci) {
-  const Body& S = world->bodies[si];
-  const Body& C = world->bodies[ci];
+  const Body& S = G(si);
+  const Body& C = G(ci);
   const double hl = 0.5 * C.length;
   const V3& spos = S.gpos;
   const V3& cpos = C.gpos;
@@ -1091,8 +1123,8 @@ void Fine::sphereCapsule(int si, int ci) {
 // box, from each end of the segment; the second contact is added whenever its points differ
 // from the first's (without a distance test, as in the reference).  The box is body1.
 void Fine::boxCapsule(int bi, int ci) {
-  const Body& B = world->bodies[bi];
-  const Body& C = world->bodies[ci];
+  const Body& B = G(bi);
+  const Body& C = G(ci);
   const M3& R = C.R;
   const V3 c_up(0.5 * C.length * R[0], 0.5 * C.length * R[3], 0.5 * C.length * R[6]);
   const V3 up = C.gpos + c_up, down = C.gpos - c_up;
@@ -1116,9 +1148,9 @@ void Fine::boxCapsule(int bi, int ci) {
 // Nearly parallel, overlapping axes give two contacts at the ends of the overlap; otherwise
 // one contact between the axes' closest points.
 void Fine::capsuleCapsule(int c1i, int c2i) {
-  if (world->bodies[c2i].sid < world->bodies[c1i].sid) std::swap(c1i, c2i);
-  const Body& C1 = world->bodies[c1i];
-  const Body& C2 = world->bodies[c2i];
+  if (G(c2i).sid < G(c1i).sid) std::swap(c1i, c2i);
+  const Body& C1 = G(c1i);
+  const Body& C2 = G(c2i);
   const V3 d1(C1.R[0], C1.R[3], C1.R[6]), d2(C2.R[0], C2.R[3], C2.R[6]);
   const V3 up1 = (0.5 * C1.length) * d1, up2 = (0.5 * C2.length) * d2;
   V3 cp1, cp2;
@@ -1144,8 +1176,8 @@ void Fine::capsuleCapsule(int c1i, int c2i) {
 // collideCapsulePlane, MaxContacts.h:2744-2780: each end sphere against the plane.  The
 // capsule is body1.
 void Fine::capsulePlane(int ci, int pi) {
-  const Body& C = world->bodies[ci];
-  const Body& P = world->bodies[pi];
+  const Body& C = G(ci);
+  const Body& P = G(pi);
   const V3 c_up = (0.5 * C.length) * V3(C.R[0], C.R[3], C.R[6]);
   const V3 posUp = C.gpos + c_up, posDn = C.gpos - c_up;
   const double distUp = dot(posUp, P.normal) - P.d - C.radius;
@@ -1155,9 +1187,32 @@ void Fine::capsulePlane(int ci, int pi) {
 }
 
 // MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/capsule/plane subset)
+// collideSphereUnion, collideBoxUnion, collideCapsuleUnion, collidePlaneUnion (MaxContacts.h:843-848,
+// 2576-2581, 2828-2833, 3453-3457): the other body against every member, in the union's order.
+void Fine::collideUnion(int a, int u) {
+  const Body& U = G(u);
+  const int nb = (int)world->bodies.size();
+  for (int k = 0; k < U.mcount; ++k) {
+    const int m = nb + U.mbegin + k;
+    if (G(a).shape == ORC_PLANE) collide(m, a);
+    else collide(a, m);
+  }
+}
+
 void Fine::collide(int a, int b) {
-  const int ta = world->bodies[a].shape, tb = world->bodies[b].shape;
-  if (ta == ORC_SPHERE) {
+  const int ta = G(a).shape, tb = G(b).shape;
+  if (ta == ORC_UNION && tb == ORC_UNION) {  // collideUnionUnion, MaxContacts.h:3569-3581
+    if (G(b).sid < G(a).sid) std::swap(a, b);
+    const Body& U1 = G(a);
+    const Body& U2 = G(b);
+    const int nb = (int)world->bodies.size();
+    for (int k1 = 0; k1 < U1.mcount; ++k1)
+      for (int k2 = 0; k2 < U2.mcount; ++k2) collide(nb + U1.mbegin + k1, nb + U2.mbegin + k2);
+  } else if (tb == ORC_UNION) {
+    collideUnion(a, b);
+  } else if (ta == ORC_UNION) {
+    collideUnion(b, a);
+  } else if (ta == ORC_SPHERE) {
     if (tb == ORC_SPHERE) sphereSphere(a, b);
     else if (tb == ORC_BOX) sphereBox(a, b);
     else if (tb == ORC_CAPSULE) sphereCapsule(a, b);
@@ -1172,7 +1227,7 @@ void Fine::collide(int a, int b) {
     else if (tb == ORC_BOX) boxCapsule(b, a);
     else if (tb == ORC_CAPSULE) capsuleCapsule(a, b);
     else capsulePlane(a, b);
-  } else {
+  } else if (ta == ORC_PLANE) {
     if (tb == ORC_SPHERE) spherePlane(b, a);
     else if (tb == ORC_BOX) boxPlane(b, a);
     else if (tb == ORC_CAPSULE) capsulePlane(b, a);
@@ -1242,11 +1297,21 @@ void orc_world::integratePositions(Body& b, V3 v, V3 w, double dt) {
     }
     b.v = v;
     b.w = w;
-    if (b.shape == ORC_PLANE) {
+    if (b.shape == ORC_UNION) {
+      // SuperBody::updateBody -> Sphere::update(dq), Sphere.cpp:1052-1070: each member at the
+      // union's position plus its rotated relative position
+      for (int k = 0; k < b.mcount; ++k) {
+        Body& m = members[b.mbegin + k];
+        m.gpos = b.gpos + b.R * m.rpos;
+        m.q = quat_mul(dq, m.q);
+        m.R = quat_to_R(m.q);
+        m.calcBoundingBox();
+      }
+    } else if (b.shape == ORC_PLANE) {
       b.normal = quat_rotate(b.q, V3(0, 0, 1));
       b.d = dot(b.normal, b.gpos);
     }
-    b.calcBoundingBox();
+    calcBoundingBox(b);
     // calcMotion never puts a body to sleep: sleepThreshold = 0 (pe/config/SleepMode.h:37)
   }
 }
@@ -1854,6 +1919,114 @@ int orc_create_capsule(orc_world* w, double x, double y, double z, double r, dou
   return register_body(w, b);
 }
 
+// A union of spheres as pe builds one inside pe_CREATE_UNION and places it with setPosition
+// (createGranularParticle, Union.cpp:2376-2402): the union takes the next system ID, then each
+// sphere is created at its offset disp[k] (its own system ID; SphereBase's mass properties) and
+// added (Union::add, Union.cpp:1482-1545).  After the last add: the total mass and centre of mass
+// (UnionBase::calcCenterOfMass, UnionBase.cpp:240-313), the members' relative positions
+// (RigidBody::calcRelPosition, RigidBody.h:1370-1373) and the moment of inertia about the centre of
+// mass, rotated into the body frame (UnionBase::calcInertia, UnionBase.cpp:323-360), all with the
+// union's identity orientation.  Union::setPosition (Union.cpp:251-272) then moves the centre of
+// mass to (x, y, z) and every member by the same displacement.
+int orc_create_union(orc_world* w, double x, double y, double z, int n, const double* disp,
+                     const double* radius, const int* mat) {
+  if (n < 1) return -1;
+  Body u;
+  u.shape = ORC_UNION;
+  u.mat = mat[0];
+  u.sid = w->nextSid++;
+  u.mbegin = (int)w->members.size();
+  u.mcount = n;
+  const int uidx = (int)w->bodies.size();
+  for (int k = 0; k < n; ++k) {
+    Body m;
+    m.shape = ORC_SPHERE;
+    m.mat = mat[k];
+    m.radius = radius[k];
+    m.gpos = V3(disp[3 * k], disp[3 * k + 1], disp[3 * k + 2]);
+    m.mass = 4.0 / 3.0 * M_PI * m.radius * m.radius * m.radius * w->materials[mat[k]].density;
+    m.invMass = 1.0 / m.mass;
+    m.I = M3();
+    m.I[0] = m.I[4] = m.I[8] = 0.4 * m.mass * m.radius * m.radius;
+    m.Iinv = inverse(m.I);
+    m.sid = w->nextSid++;
+    m.super = uidx;
+    w->members.push_back(m);
+  }
+  Body* ms = &w->members[u.mbegin];
+  if (n == 1) {
+    u.gpos = ms[0].gpos;
+    u.mass = ms[0].mass;
+    u.invMass = 1.0 / u.mass;
+  } else {
+    u.gpos = V3();
+    u.mass = 0;
+    for (int k = 0; k < n; ++k) {
+      u.mass += ms[k].mass;
+      u.gpos += ms[k].gpos * ms[k].mass;
+    }
+    u.invMass = 1.0 / u.mass;
+    const double tmp = 1.0 / u.mass;  // Vector3::operator/=, Vector3.h:836-848
+    u.gpos = V3(u.gpos[0] * tmp, u.gpos[1] * tmp, u.gpos[2] * tmp);
+  }
+  for (int k = 0; k < n; ++k) ms[k].rpos = trans(u.R) * (ms[k].gpos - u.gpos);
+  M3 I;  // zero
+  for (int k = 0; k < n; ++k) {
+    const double mass = ms[k].mass;
+    const V3 pos = ms[k].gpos - u.gpos;
+    for (int j = 0; j < 9; ++j) I[j] += ms[k].I[j];
+    I[0] += mass * (pos[1] * pos[1] + pos[2] * pos[2]);
+    I[1] -= mass * pos[0] * pos[1];
+    I[2] -= mass * pos[0] * pos[2];
+    I[3] -= mass * pos[0] * pos[1];
+    I[4] += mass * (pos[0] * pos[0] + pos[2] * pos[2]);
+    I[5] -= mass * pos[1] * pos[2];
+    I[6] -= mass * pos[0] * pos[2];
+    I[7] -= mass * pos[1] * pos[2];
+    I[8] += mass * (pos[0] * pos[0] + pos[1] * pos[1]);
+  }
+  u.I = trans(u.R) * I * u.R;
+  u.Iinv = inverse(u.I);
+  const V3 gpos(x, y, z);
+  const V3 dp = gpos - u.gpos;
+  u.gpos = gpos;
+  for (int k = 0; k < n; ++k) {
+    ms[k].gpos += dp;  // Sphere::update(dp), Sphere.cpp:1024-1038
+    ms[k].calcBoundingBox();
+  }
+  w->calcBoundingBox(u);
+  w->bodies.push_back(u);
+  w->hgAdd(uidx);
+  return uidx;
+}
+
+size_t orc_num_members(const orc_world* w) { return w->members.size(); }
+
+void orc_get_members(const orc_world* w, uint32_t* owner, uint64_t* sid, int* mat, double* radius,
+                     double* rpos, double* gpos) {
+  for (size_t k = 0; k < w->members.size(); ++k) {
+    const Body& m = w->members[k];
+    if (owner) owner[k] = (uint32_t)m.super;
+    if (sid) sid[k] = m.sid;
+    if (mat) mat[k] = m.mat;
+    if (radius) radius[k] = m.radius;
+    for (int j = 0; j < 3; ++j) {
+      if (rpos) rpos[3 * k + j] = m.rpos[j];
+      if (gpos) gpos[3 * k + j] = m.gpos[j];
+    }
+  }
+}
+
+void orc_set_members(orc_world* w, const double* gpos) {
+  for (size_t k = 0; k < w->members.size(); ++k) {
+    Body& m = w->members[k];
+    m.gpos = V3(gpos[3 * k], gpos[3 * k + 1], gpos[3 * k + 2]);
+    m.calcBoundingBox();
+  }
+  for (Body& b : w->bodies)
+    if (b.shape == ORC_UNION) w->calcBoundingBox(b);
+}
+
 // createPlane, Plane.cpp:1202-1240; Plane ctor, Plane.cpp:77-102; PlaneBase.cpp:73-100
 int orc_create_plane(orc_world* w, double nx, double ny, double nz, double d, int mat) {
   Body b;
@@ -1906,7 +2079,7 @@ void orc_set_state(orc_world* w, const double* pos, const double* q, const doubl
       for (int k = 0; k < 3; ++k) b.normal[k] = normal[3 * i + k];
       b.d = d[i];
     }
-    b.calcBoundingBox();
+    w->calcBoundingBox(b);  // a union's members keep their positions (orc_set_members moves them)
   }
 }
 
@@ -1921,6 +2094,7 @@ void orc_get_body_info(const orc_world* w, int* shape, uint64_t* sid, int* mat, 
     if (b.shape == ORC_SPHERE) { param[3 * i] = b.radius; param[3 * i + 1] = 0; param[3 * i + 2] = 0; }
     else if (b.shape == ORC_BOX) { for (int k = 0; k < 3; ++k) param[3 * i + k] = b.lengths[k]; }
     else if (b.shape == ORC_CAPSULE) { param[3 * i] = b.radius; param[3 * i + 1] = b.length; param[3 * i + 2] = 0; }
+    else if (b.shape == ORC_UNION) { param[3 * i] = b.mbegin; param[3 * i + 1] = b.mcount; param[3 * i + 2] = 0; }
     else { param[3 * i] = param[3 * i + 1] = param[3 * i + 2] = 0; }
     inv_mass[i] = b.invMass;
     for (int k = 0; k < 9; ++k) inv_inertia_body[9 * i + k] = b.Iinv[k];

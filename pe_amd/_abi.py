@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libpehip.so")
 
 OK, INVALID_ARG, OUT_OF_MEMORY, CAPACITY_RETRY, HIP_ERROR, NONFINITE, NOT_SUPPORTED = range(7)
-SPHERE, BOX, PLANE, CAPSULE = 1, 2, 3, 4
+SPHERE, BOX, PLANE, CAPSULE, UNION = 1, 2, 3, 4, 5
 SWEEP_COLOURED, SWEEP_REFERENCE_ORDER = 0, 1
 SCHEDULE_AUTO, SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW, SCHEDULE_SINGLE_BLOCK = 0, 1, 2, 3
 
@@ -20,7 +20,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "CAPACITY_RETR
 # Every function declared in include/pe_hip.h.
 EXPORTED = [
     "pehip_default_params", "pehip_create", "pehip_destroy", "pehip_set_params", "pehip_get_params",
-    "pehip_set_materials", "pehip_upload_bodies", "pehip_upload_state", "pehip_set_forces",
+    "pehip_set_materials", "pehip_upload_bodies", "pehip_upload_members", "pehip_download_members", "pehip_upload_state", "pehip_set_forces",
     "pehip_set_contact_order", "pehip_step", "pehip_detect", "pehip_download_state", "pehip_get_contacts",
     "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_set_profiling", "pehip_eval_trig",
     "pehip_stream", "pehip_last_error", "pehip_version",
@@ -108,6 +108,8 @@ def load():
         "pehip_upload_bodies": (st, [_VP, ctypes.c_uint32, _U8, _U64, _U32, _U8, _D, _D, _D, _D, _D, _D, _D, _D,
                                      _D, _D]),
         "pehip_upload_state": (st, [_VP, ctypes.c_uint32, _D, _D, _D, _D, _D, _D]),
+        "pehip_upload_members": (st, [_VP, ctypes.c_uint32, _U32, _U64, _U32, _D, _D, _D]),
+        "pehip_download_members": (st, [_VP, ctypes.c_uint32, _D]),
         "pehip_set_forces": (st, [_VP, ctypes.c_uint32, _D, _D]),
         "pehip_set_contact_order": (st, [_VP, ctypes.c_uint32, _U32, _U32]),
         "pehip_step": (st, [_VP, ctypes.c_double, ctypes.c_uint32]),

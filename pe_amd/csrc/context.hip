@@ -77,6 +77,10 @@ static void plan_grid(pehip_ctx* ctx, const uint8_t* shape, const double* param,
       // AABB size between its diameter (lower bound) and axis-aligned length + diameter
       smax_b = p[1] + 2.0 * (p[0] + kContactThreshold);
       smin_b = 2.0 * p[0];
+    } else if (shape[i] == PEHIP_UNION) {
+      // inside the bounding sphere about the centre of mass; at least one member's diameter
+      smax_b = 2.0 * (p[0] + kContactThreshold);
+      smin_b = 2.0 * p[1];
     } else {
       smax_b = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) + 2.0 * kContactThreshold;
       smin_b = std::max(p[0], std::max(p[1], p[2]));
@@ -270,7 +274,7 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   std::vector<uint8_t> fx(fixed, fixed + n);
   std::vector<uint32_t> hplanes;
   for (uint32_t i = 0; i < n; ++i) {
-    if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_CAPSULE) return fail(ctx, PEHIP_INVALID_ARG, "invalid shape");
+    if (shape[i] < PEHIP_SPHERE || shape[i] > PEHIP_UNION) return fail(ctx, PEHIP_INVALID_ARG, "invalid shape");
     if (material[i] >= ctx->nmat) return fail(ctx, PEHIP_INVALID_ARG, "Invalid material ID");
     if (shape[i] == PEHIP_PLANE) {
       hplanes.push_back(i);
@@ -283,6 +287,9 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
         return fail(ctx, PEHIP_INVALID_ARG, "Invalid side length");
       if (shape[i] == PEHIP_CAPSULE && !(p[0] > 0 && p[1] > 0))
         return fail(ctx, PEHIP_INVALID_ARG, "Invalid capsule radius or length");
+      if (shape[i] == PEHIP_UNION && !(p[1] > 0 && p[0] >= p[1] && std::isfinite(p[0])))
+        return fail(ctx, PEHIP_INVALID_ARG, "Invalid union bounding radius");
+      if (shape[i] == PEHIP_UNION && fx[i]) return fail(ctx, PEHIP_INVALID_ARG, "fixed unions are not supported");
       if (fx[i] && inv_mass[i] != 0) return fail(ctx, PEHIP_INVALID_ARG, "fixed body with non-zero inverse mass");
     }
   }
@@ -345,9 +352,62 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   P.planes = ctx->planes;
   P.nplanes = (uint32_t)hplanes.size();
   P.flags = ctx->flags;
-  launch_aabb(ctx->bodies(), ctx->stream);
+  ctx->members_missing = std::find(shape, shape + n, (uint8_t)PEHIP_UNION) != shape + n;
+  if (!ctx->members_missing) launch_aabb(ctx->bodies(), ctx->stream);  // unions: after their members
   ctx->nc = 0;
   ctx->order_pending = false;
+  return sync_check(ctx);
+}
+
+pehip_status pehip_upload_members(pehip_ctx* ctx, uint32_t nm, const uint32_t* owner, const uint64_t* sid,
+                                  const uint32_t* material, const double* radius, const double* rpos,
+                                  const double* gpos) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (ctx->n == 0) return fail(ctx, PEHIP_INVALID_ARG, "members need their union bodies uploaded first");
+  if (nm == 0 || !owner || !sid || !material || !radius || !rpos || !gpos)
+    return fail(ctx, PEHIP_INVALID_ARG, "NULL or empty member array");
+  std::vector<uint32_t> start((size_t)ctx->n + 1, 0);
+  for (uint32_t k = 0; k < nm; ++k) {
+    if (owner[k] >= ctx->n || ctx->h_shape[owner[k]] != PEHIP_UNION)
+      return fail(ctx, PEHIP_INVALID_ARG, "member owner is not a union body");
+    if (k && owner[k] < owner[k - 1]) return fail(ctx, PEHIP_INVALID_ARG, "members not grouped by union in body order");
+    if (material[k] >= ctx->nmat) return fail(ctx, PEHIP_INVALID_ARG, "Invalid material ID");
+    if (!(radius[k] > 0)) return fail(ctx, PEHIP_INVALID_ARG, "Invalid sphere radius");
+    for (int j = 0; j < 3; ++j)
+      if (!std::isfinite(rpos[3 * (size_t)k + j]) || !std::isfinite(gpos[3 * (size_t)k + j]))
+        return fail(ctx, PEHIP_INVALID_ARG, "non-finite member position");
+    ++start[owner[k] + 1];
+  }
+  for (uint32_t i = 0; i < ctx->n; ++i)
+    if (ctx->h_shape[i] == PEHIP_UNION && start[i + 1] == 0) return fail(ctx, PEHIP_INVALID_ARG, "union without members");
+  for (uint32_t i = 0; i < ctx->n; ++i) start[i + 1] += start[i];
+  hipSetDevice(ctx->device);
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->free_members();
+  const size_t D = sizeof(double);
+  CK(dalloc(ctx->mstart, (size_t)ctx->n + 1)); CK(dalloc(ctx->mowner, nm)); CK(dalloc(ctx->mmat, nm));
+  CK(dalloc(ctx->msid, nm)); CK(dalloc(ctx->mrad, nm)); CK(dalloc(ctx->mrpos, 3 * (size_t)nm));
+  CK(dalloc(ctx->mpos, 3 * (size_t)nm));
+  CK(hipMemcpy(ctx->mstart, start.data(), 4 * start.size(), hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mowner, owner, 4 * (size_t)nm, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mmat, material, 4 * (size_t)nm, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->msid, sid, 8 * (size_t)nm, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mrad, radius, D * nm, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mrpos, rpos, 3 * D * nm, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mpos, gpos, 3 * D * nm, hipMemcpyHostToDevice));
+  ctx->nm = nm;
+  ctx->members_missing = false;
+  launch_aabb(ctx->bodies(), ctx->stream);
+  ctx->nc = 0;
+  return sync_check(ctx);
+}
+
+pehip_status pehip_download_members(pehip_ctx* ctx, uint32_t nm, double* gpos) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (nm != ctx->nm || (nm && !gpos)) return fail(ctx, PEHIP_INVALID_ARG, "member count mismatch");
+  if (nm == 0) return PEHIP_OK;
+  hipSetDevice(ctx->device);
+  CK(hipMemcpyAsync(gpos, ctx->mpos, 3 * sizeof(double) * nm, hipMemcpyDeviceToHost, ctx->stream));
   return sync_check(ctx);
 }
 
@@ -356,6 +416,7 @@ pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, c
   if (!ctx) return PEHIP_INVALID_ARG;
   if (n != ctx->n) return fail(ctx, PEHIP_INVALID_ARG, "body count mismatch");
   if (!pos || !q || !v || !w || !plane_normal || !plane_d) return fail(ctx, PEHIP_INVALID_ARG, "NULL state array");
+  if (ctx->members_missing) return fail(ctx, PEHIP_INVALID_ARG, "union members not uploaded");
   hipSetDevice(ctx->device);
   CK(hipStreamSynchronize(ctx->stream));
   std::vector<double> hR(9 * (size_t)n);
@@ -688,6 +749,7 @@ pehip_status pehip_step(pehip_ctx* ctx, double dt, uint32_t nsteps) {
   if (!ctx) return PEHIP_INVALID_ARG;
   if (!(dt > 0)) return fail(ctx, PEHIP_INVALID_ARG, "Invalid time step size");  // World.cpp:333-334
   if (ctx->n == 0) return fail(ctx, PEHIP_INVALID_ARG, "no bodies uploaded");
+  if (ctx->members_missing) return fail(ctx, PEHIP_INVALID_ARG, "union members not uploaded");
   hipSetDevice(ctx->device);
   for (uint32_t k = 0; k < nsteps; ++k) CKS(pe_one_step(ctx, dt));
   return PEHIP_OK;
@@ -696,6 +758,7 @@ pehip_status pehip_step(pehip_ctx* ctx, double dt, uint32_t nsteps) {
 pehip_status pehip_detect(pehip_ctx* ctx) {
   if (!ctx) return PEHIP_INVALID_ARG;
   if (ctx->n == 0) return fail(ctx, PEHIP_INVALID_ARG, "no bodies uploaded");
+  if (ctx->members_missing) return fail(ctx, PEHIP_INVALID_ARG, "union members not uploaded");
   hipSetDevice(ctx->device);
   CKS(detect(ctx));
   return sync_check(ctx);

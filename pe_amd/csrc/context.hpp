@@ -47,6 +47,12 @@ struct pehip_ctx {
   double *pnormal = nullptr, *pd = nullptr, *aabb = nullptr, *inv_mass = nullptr, *Ib = nullptr, *Ibinv = nullptr;
   double *force = nullptr, *torque = nullptr, *sb = nullptr, *dvw = nullptr;
   std::vector<uint8_t> h_fixed, h_shape;
+  // union members (spheres), grouped by union; mstart has n + 1 entries when nm > 0
+  uint32_t nm = 0;
+  uint32_t *mstart = nullptr, *mowner = nullptr, *mmat = nullptr;
+  uint64_t* msid = nullptr;
+  double *mrad = nullptr, *mrpos = nullptr, *mpos = nullptr;
+  bool members_missing = false;  // unions uploaded, their members not yet
 
   // materials
   uint32_t nmat = 0;
@@ -91,10 +97,19 @@ struct pehip_ctx {
     B.pos = pos; B.q = q; B.R = R; B.v = v; B.w = w; B.pnormal = pnormal; B.pd = pd; B.aabb = aabb;
     B.inv_mass = inv_mass; B.Ib = Ib; B.Ibinv = Ibinv; B.force = force; B.torque = torque;
     B.sb = sb; B.dvw = dvw;
+    B.nm = nm; B.mstart = mstart; B.mowner = mowner; B.msid = msid; B.mmat = mmat; B.mrad = mrad;
+    B.mrpos = mrpos; B.mpos = mpos;
     return B;
   }
 
+  void free_members() {
+    dfree(mstart); dfree(mowner); dfree(mmat); dfree(msid); dfree(mrad); dfree(mrpos); dfree(mpos);
+    nm = 0;
+    members_missing = false;
+  }
+
   void free_bodies() {
+    free_members();
     dfree(shape); dfree(fixed); dfree(mat); dfree(sid); dfree(param); dfree(pos); dfree(q); dfree(R);
     dfree(v); dfree(w); dfree(pnormal); dfree(pd); dfree(aabb); dfree(inv_mass); dfree(Ib); dfree(Ibinv);
     dfree(force); dfree(torque); dfree(sb); dfree(dvw); dfree(planes);

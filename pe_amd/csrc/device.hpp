@@ -12,7 +12,7 @@ namespace pe_amd {
 constexpr double kContactThreshold = 1e-8;   // pe/core/Thresholds.h:151
 constexpr double kFrictionThreshold = 1e-8;  // pe/core/Thresholds.h:161
 constexpr uint32_t kFixedBit = 0x80000000u;  // set in solver body indices of fixed bodies
-constexpr int kShapeSphere = 1, kShapeBox = 2, kShapePlane = 3, kShapeCapsule = 4;
+constexpr int kShapeSphere = 1, kShapeBox = 2, kShapePlane = 3, kShapeCapsule = 4, kShapeUnion = 5;
 constexpr double kParallelThreshold = 1e-8;  // pe/core/Thresholds.h:170
 
 // Solver body record (128 B, one L2 line): predicted v, w; inverse mass; world-frame
@@ -46,6 +46,16 @@ struct Bodies {
   double* torque;          // 3n
   double* sb;              // kSB * n
   double* dvw;             // kDVW * n
+  // Union members (spheres), grouped by union in body order: the members of body i are
+  // [mstart[i], mstart[i+1]) (empty for every other shape).  Geometry id n + k names member k.
+  uint32_t nm = 0;
+  const uint32_t* mstart = nullptr;  // n + 1 (null when nm == 0)
+  const uint32_t* mowner = nullptr;  // nm: the union's body index
+  const uint64_t* msid = nullptr;    // nm
+  const uint32_t* mmat = nullptr;    // nm
+  const double* mrad = nullptr;      // nm
+  const double* mrpos = nullptr;     // 3nm: position in the union's body frame
+  double* mpos = nullptr;            // 3nm: global position
 };
 
 PE_HD d3 ld3(const double* p, uint32_t i) { return mk3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
@@ -75,6 +85,40 @@ PE_HD double aabb_size(const double* a) {
   if (y > s) s = y;
   if (z > s) s = z;
   return s;
+}
+
+// A sphere taking part in a contact: a sphere body or a union member.  g is its geometry id
+// (the body index, or n + member index); contacts join the top-level bodies (ContactBase.h:209-210).
+struct SphereRef {
+  d3 x;
+  double r;
+  uint64_t sid;
+  uint32_t g;
+};
+PE_HD SphereRef body_sphere(const Bodies& B, uint32_t i) {
+  return SphereRef{ld3(B.pos, i), B.param[3 * (size_t)i], B.sid[i], i};
+}
+PE_HD SphereRef member_sphere(const Bodies& B, uint32_t m) {
+  return SphereRef{ld3(B.mpos, m), B.mrad[m], B.msid[m], B.n + m};
+}
+PE_HD uint32_t geom_body(const Bodies& B, uint32_t g) { return g < B.n ? g : B.mowner[g - B.n]; }
+PE_HD uint32_t geom_mat(const Bodies& B, uint32_t g) { return g < B.n ? B.mat[g] : B.mmat[g - B.n]; }
+
+// Union::calcBoundingBox (UnionBase.cpp:208-230): the members' sphere boxes (SphereBase.h:208-221)
+// merged in member order (BoundingBox::operator+=, BoundingBox.h:339-348).
+PE_HD void union_aabb(const Bodies& B, uint32_t i, double* o) {
+  const uint32_t m0 = B.mstart[i], m1 = B.mstart[i + 1];
+  for (uint32_t m = m0; m < m1; ++m) {
+    const d3 x = ld3(B.mpos, m);
+    const double l = B.mrad[m] + kContactThreshold;
+    const double a[6] = {x.x - l, x.y - l, x.z - l, x.x + l, x.y + l, x.z + l};
+    if (m == m0) {
+      for (int k = 0; k < 6; ++k) o[k] = a[k];
+    } else {
+      for (int k = 0; k < 3; ++k) o[k] = (a[k] < o[k]) ? a[k] : o[k];
+      for (int k = 3; k < 6; ++k) o[k] = (o[k] < a[k]) ? a[k] : o[k];
+    }
+  }
 }
 
 // AABB of one body (SphereBase.h:208-221, BoxBase.cpp:127-141, PlaneBase.cpp:132-148).
@@ -115,25 +159,24 @@ PE_HD void body_aabb(int shape, d3 gp, const m33& R, const double* prm, d3 nrm, 
 
 // collideSphereSphere, MaxContacts.h:547-568: body1 is the one with the smaller SID.
 template <class Emit>
-__device__ void np_sphere_sphere(const Bodies& B, uint32_t s1, uint32_t s2, Emit& emit) {
-  if (B.sid[s2] < B.sid[s1]) { uint32_t t = s1; s1 = s2; s2 = t; }
-  const d3 p1 = ld3(B.pos, s1), p2 = ld3(B.pos, s2);
-  const double r1 = B.param[3 * s1], r2 = B.param[3 * s2];
-  d3 normal = p1 - p2;
-  const double dist = len3(normal) - r1 - r2;
+__device__ void np_sphere_sphere(SphereRef s1, SphereRef s2, Emit& emit) {
+  if (s2.sid < s1.sid) { const SphereRef t = s1; s1 = s2; s2 = t; }
+  d3 normal = s1.x - s2.x;
+  const double dist = len3(normal) - s1.r - s2.r;
   if (dist < kContactThreshold) {
     normal = nrm3(normal);
-    const double k = r2 + 0.5 * dist;
-    emit(s1, s2, p2 + normal * k, normal, dist);
+    const double k = s2.r + 0.5 * dist;
+    emit(s1.g, s2.g, s2.x + normal * k, normal, dist);
   }
 }
 
 // collideSphereBox, MaxContacts.h:589-667: the sphere is body1.
 template <class Emit>
-__device__ void np_sphere_box(const Bodies& B, uint32_t s, uint32_t b, Emit& emit) {
+__device__ void np_sphere_box(const Bodies& B, const SphereRef& S, uint32_t b, Emit& emit) {
   const double* L = B.param + 3 * (size_t)b;
   const double l0 = 0.5 * L[0], l1 = 0.5 * L[1], l2 = 0.5 * L[2];
-  const d3 spos = ld3(B.pos, s), bpos = ld3(B.pos, b);
+  const d3 spos = S.x, bpos = ld3(B.pos, b);
+  const uint32_t s = S.g;
   const m33 R = ld9(B.R, b);
   const double* r = R.a;
   bool outside = false;
@@ -146,7 +189,7 @@ __device__ void np_sphere_box(const Bodies& B, uint32_t s, uint32_t b, Emit& emi
     if (p[i] < -l[i]) { p[i] = -l[i]; outside = true; }
     else if (p[i] > l[i]) { p[i] = l[i]; outside = true; }
   }
-  const double radius = B.param[3 * s];
+  const double radius = S.r;
   if (!outside) {
     double dist = fabs(p[0]) - l[0];
     int face = 0;
@@ -169,13 +212,13 @@ __device__ void np_sphere_box(const Bodies& B, uint32_t s, uint32_t b, Emit& emi
 
 // collideSpherePlane, MaxContacts.h:777-792: the sphere is body1.
 template <class Emit>
-__device__ void np_sphere_plane(const Bodies& B, uint32_t s, uint32_t p, Emit& emit) {
+__device__ void np_sphere_plane(const Bodies& B, const SphereRef& S, uint32_t p, Emit& emit) {
   const d3 n = ld3(B.pnormal, p);
-  const d3 x = ld3(B.pos, s);
-  const double r = B.param[3 * s];
+  const d3 x = S.x;
+  const double r = S.r;
   const double k = dot3(n, x);
   const double dist = k - r - B.pd[p];
-  if (dist < kContactThreshold) emit(s, p, x - (r + dist) * n, n, dist);
+  if (dist < kContactThreshold) emit(S.g, p, x - (r + dist) * n, n, dist);
 }
 
 // collideBoxPlane, MaxContacts.h:2380-2528: corners in pe's order, each pre-filtered.
@@ -572,21 +615,21 @@ __device__ inline bool nearly_equal3(d3 a, d3 b) {
 // collideSphereCapsule, MaxContacts.h:690-725: the sphere (body1) against the closest point of
 // the capsule's axis.
 template <class Emit>
-__device__ void np_sphere_capsule(const Bodies& B, uint32_t s, uint32_t c, Emit& emit) {
+__device__ void np_sphere_capsule(const Bodies& B, const SphereRef& S, uint32_t c, Emit& emit) {
   const double hl = 0.5 * B.param[3 * (size_t)c + 1];
   const double rc = B.param[3 * (size_t)c];
-  const d3 spos = ld3(B.pos, s), cpos = ld3(B.pos, c);
+  const d3 spos = S.x, cpos = ld3(B.pos, c);
   const double* R = B.R + 9 * (size_t)c;
   double sx = R[0] * (spos.x - cpos.x) + R[3] * (spos.y - cpos.y) + R[6] * (spos.z - cpos.z);
   if (sx > hl) sx = hl;
   else if (sx < -hl) sx = -hl;
   const d3 spos2 = mk3(sx * R[0] + cpos.x, sx * R[3] + cpos.y, sx * R[6] + cpos.z);
   d3 normal = spos - spos2;
-  const double dist = len3(normal) - B.param[3 * (size_t)s] - rc;
+  const double dist = len3(normal) - S.r - rc;
   if (dist < kContactThreshold) {
     normal = nrm3(normal);
     const double k = rc + 0.5 * dist;
-    emit(s, c, spos2 + normal * k, normal, dist);
+    emit(S.g, c, spos2 + normal * k, normal, dist);
   }
 }
 
@@ -669,27 +712,50 @@ __device__ void np_capsule_plane(const Bodies& B, uint32_t c, uint32_t p, Emit& 
   if (distDn < kContactThreshold) emit(c, p, posDn - rc * n, n, distDn);
 }
 
-// MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/capsule/plane).
+// A sphere (body or union member) against body b, which is not a union.
+template <class Emit>
+__device__ void np_sphere_any(const Bodies& B, const SphereRef& s, uint32_t b, Emit& emit) {
+  const int tb = B.shape[b];
+  if (tb == kShapeSphere) np_sphere_sphere(s, body_sphere(B, b), emit);
+  else if (tb == kShapeBox) np_sphere_box(B, s, b, emit);
+  else if (tb == kShapeCapsule) np_sphere_capsule(B, s, b, emit);
+  else if (tb == kShapePlane) np_sphere_plane(B, s, b, emit);
+}
+
+// MaxContacts::collide dispatch, MaxContacts.h:242-526 (sphere/box/capsule/plane and unions of
+// spheres).  A union meets another body member by member in the union's order
+// (collide{Sphere,Box,Capsule,Plane}Union, MaxContacts.h:843-848, 2576-2581, 2828-2833, 3453-3457);
+// two unions pair every member of the one with the smaller SID with every member of the other
+// (collideUnionUnion, MaxContacts.h:3569-3581).
 template <class Emit>
 __device__ void narrowphase(const Bodies& B, uint32_t a, uint32_t b, Emit& emit) {
   const int ta = B.shape[a], tb = B.shape[b];
+  if (ta == kShapeUnion || tb == kShapeUnion) {
+    if (ta == kShapeUnion && tb == kShapeUnion) {
+      if (B.sid[b] < B.sid[a]) { const uint32_t t = a; a = b; b = t; }
+      for (uint32_t m1 = B.mstart[a]; m1 < B.mstart[a + 1]; ++m1)
+        for (uint32_t m2 = B.mstart[b]; m2 < B.mstart[b + 1]; ++m2)
+          np_sphere_sphere(member_sphere(B, m1), member_sphere(B, m2), emit);
+    } else {
+      const uint32_t u = (ta == kShapeUnion) ? a : b, o = (ta == kShapeUnion) ? b : a;
+      for (uint32_t m = B.mstart[u]; m < B.mstart[u + 1]; ++m) np_sphere_any(B, member_sphere(B, m), o, emit);
+    }
+    return;
+  }
   if (ta == kShapeSphere) {
-    if (tb == kShapeSphere) np_sphere_sphere(B, a, b, emit);
-    else if (tb == kShapeBox) np_sphere_box(B, a, b, emit);
-    else if (tb == kShapeCapsule) np_sphere_capsule(B, a, b, emit);
-    else np_sphere_plane(B, a, b, emit);
+    np_sphere_any(B, body_sphere(B, a), b, emit);
   } else if (ta == kShapeBox) {
-    if (tb == kShapeSphere) np_sphere_box(B, b, a, emit);
+    if (tb == kShapeSphere) np_sphere_box(B, body_sphere(B, b), a, emit);
     else if (tb == kShapeBox) np_box_box(B, a, b, emit);
     else if (tb == kShapeCapsule) np_box_capsule(B, a, b, emit);
     else if (tb == kShapePlane) np_box_plane(B, a, b, emit);
   } else if (ta == kShapeCapsule) {
-    if (tb == kShapeSphere) np_sphere_capsule(B, b, a, emit);
+    if (tb == kShapeSphere) np_sphere_capsule(B, body_sphere(B, b), a, emit);
     else if (tb == kShapeBox) np_box_capsule(B, b, a, emit);
     else if (tb == kShapeCapsule) np_capsule_capsule(B, a, b, emit);
     else if (tb == kShapePlane) np_capsule_plane(B, a, b, emit);
   } else {
-    if (tb == kShapeSphere) np_sphere_plane(B, b, a, emit);
+    if (tb == kShapeSphere) np_sphere_plane(B, body_sphere(B, b), a, emit);
     else if (tb == kShapeBox) np_box_plane(B, b, a, emit);
     else if (tb == kShapeCapsule) np_capsule_plane(B, b, a, emit);
   }
@@ -707,11 +773,12 @@ __device__ inline bool contact_is_dynamic(const Bodies& B, uint32_t b1, uint32_t
   return fabs(tvel) > kFrictionThreshold;
 }
 
-// Contact friction from the material pair tables (Materials.h:500-537).
+// Contact friction from the material pair tables (Materials.h:500-537) of the colliding
+// geometries g1, g2; the relative velocity is the top-level bodies' (Contact.cpp:81-105).
 __device__ inline double contact_friction(const Bodies& B, const double* csf, const double* cdf,
-                                          uint32_t nmat, uint32_t b1, uint32_t b2, d3 gpos, d3 n) {
-  const size_t idx = (size_t)B.mat[b1] * nmat + B.mat[b2];
-  return contact_is_dynamic(B, b1, b2, gpos, n) ? cdf[idx] : csf[idx];
+                                          uint32_t nmat, uint32_t g1, uint32_t g2, d3 gpos, d3 n) {
+  const size_t idx = (size_t)geom_mat(B, g1) * nmat + geom_mat(B, g2);
+  return contact_is_dynamic(B, geom_body(B, g1), geom_body(B, g2), gpos, n) ? cdf[idx] : csf[idx];
 }
 
 // std::max(a, b) as the reference evaluates it: (a < b) ? b : a.

@@ -11,7 +11,8 @@ namespace pe_amd {
 __device__ __forceinline__ void update_aabb(const Bodies& B, uint32_t i) {
   const m33 R = ld9(B.R, i);
   double o[6];
-  body_aabb(B.shape[i], ld3(B.pos, i), R, B.param + 3 * (size_t)i, ld3(B.pnormal, i), B.pd[i], o);
+  if (B.shape[i] == kShapeUnion) union_aabb(B, i, o);
+  else body_aabb(B.shape[i], ld3(B.pos, i), R, B.param + 3 * (size_t)i, ld3(B.pnormal, i), B.pd[i], o);
 #pragma unroll
   for (int k = 0; k < 6; ++k) B.aabb[6 * (size_t)i + k] = o[k];
 }
@@ -75,14 +76,21 @@ __device__ __forceinline__ void integrate_body(const Bodies& B, uint32_t i, doub
   d3 nrm = ld3(B.pnormal, i);
   double pd = B.pd[i];
   const int shape = B.shape[i];
-  if (shape == kShapePlane) {
-    nrm = quat_rotate(qn, mk3(0, 0, 1));
-    pd = dot3(nrm, gp);
-    st3(B.pnormal, i, nrm);
-    B.pd[i] = pd;
-  }
   double o[6];
-  body_aabb(shape, gp, R, B.param + 3 * (size_t)i, nrm, pd, o);
+  if (shape == kShapeUnion) {
+    // SuperBody::updateBody -> Sphere::update(dq), Sphere.cpp:1052-1070: every member at the
+    // union's new position plus its rotated relative position; then the merged box.
+    for (uint32_t m = B.mstart[i]; m < B.mstart[i + 1]; ++m) st3(B.mpos, m, gp + mat_vec(R, ld3(B.mrpos, m)));
+    union_aabb(B, i, o);
+  } else {
+    if (shape == kShapePlane) {
+      nrm = quat_rotate(qn, mk3(0, 0, 1));
+      pd = dot3(nrm, gp);
+      st3(B.pnormal, i, nrm);
+      B.pd[i] = pd;
+    }
+    body_aabb(shape, gp, R, B.param + 3 * (size_t)i, nrm, pd, o);
+  }
 #pragma unroll
   for (int k = 0; k < 6; ++k) B.aabb[6 * (size_t)i + k] = o[k];
 }

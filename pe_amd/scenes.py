@@ -31,7 +31,8 @@ class Scene:
     capsule_len: np.ndarray = None       # (nc,) length of the cylinder part (body x-axis)
     capsule_q: np.ndarray = None         # (nc, 4) orientation
     capsule_mat: np.ndarray = None
-    v0: np.ndarray = None                # (n_total, 3) initial linear velocities
+    v0: np.ndarray = None                # (n_bodies, 3) initial linear velocities
+    unions: list = None                  # created last, at rest: (gpos, [(offset, radius, material), ...])
     gravity: tuple = (0.0, 0.0, -9.81)
     dt: float = 1e-3
     steps: int = 1
@@ -144,6 +145,8 @@ class Scene:
         world.setMaxIterations(self.max_iterations)
         world.setRelaxationModel(self.model)
         world.add_bodies(self.arrays())
+        for gpos, spheres in (self.unions or []):
+            world.createUnion(0, gpos, spheres)
         return world
 
     def to_oracle(self, ow):
@@ -170,6 +173,8 @@ class Scene:
             if self.n_capsules:
                 st["q"][b0 + self.n_boxes:] = self.capsule_q
             ow.set_state(st)
+        for gpos, spheres in (self.unions or []):
+            ow.create_union(gpos, [o for o, _, _ in spheres], [r for _, r, _ in spheres], [m for _, _, m in spheres])
         return ow
 
 

@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from . import _abi
-from ._abi import BOX, CAPSULE, PLANE, SPHERE, check, ptr
+from ._abi import BOX, CAPSULE, PLANE, SPHERE, UNION, check, ptr
 
 CONTACT_THRESHOLD = 1e-8   # pe/core/Thresholds.h:151
 ACCURACY = 1e-8            # pe/util/Limits.h:385
@@ -42,6 +42,11 @@ def mat3_inverse(m):
                     det * ((v[3] * v[7]) - (v[4] * v[6])), det * ((v[6] * v[1]) - (v[7] * v[0])),
                     det * ((v[0] * v[4]) - (v[1] * v[3]))], axis=-1)
     return out
+
+
+def _mat3_mul(l, r):
+    """Row-major 3x3 product in pe's evaluation order (Matrix3x3.h:2241-2253)."""
+    return [l[3 * i] * r[j] + l[3 * i + 1] * r[3 + j] + l[3 * i + 2] * r[6 + j] for i in range(3) for j in range(3)]
 
 
 def quat_axis_angle(axis, angle):
@@ -94,6 +99,8 @@ class World:
         self._param, self._pos, self._q, self._v, self._w = [], [], [], [], []
         self._inv_mass, self._I, self._Iinv, self._normal, self._d = [], [], [], [], []
         self._uid = []
+        # union members (spheres), grouped by union in creation order
+        self._m_owner, self._m_sid, self._m_mat, self._m_rad, self._m_rpos, self._m_gpos = [], [], [], [], [], []
         self._arrays = None      # host SoA snapshot (numpy) when built
         self._ctx = None
         self._bodies_dirty = True
@@ -210,7 +217,10 @@ class World:
     def _sync_from_device(self):
         if self._state_on_device and self._ctx is not None:
             st = self.getState()
+            mg = st.pop("mgpos", None)
             self._arrays.update(st)
+            if mg is not None:
+                self._m_gpos = list(mg)
             self._state_on_device = False
 
     def _materialise_lists(self):
@@ -281,6 +291,107 @@ class World:
         return self._add(CAPSULE, material, False, (r, L, 0.0), gpos, (1.0, 0.0, 0.0, 0.0), 1.0 / m, I,
                          mat3_inverse(I), (0.0, 0.0, 0.0), 0.0, uid)
 
+    def createUnion(self, uid, gpos, spheres):
+        """A union of spheres, as pe builds one inside pe_CREATE_UNION and places it with
+        Union::setPosition (Union.cpp:251-272, 1482-1545; UnionBase.cpp:240-360).
+
+        spheres: (offset, radius, material) per sphere, offsets before placement.  The union takes
+        the next system ID, then each sphere its own.  After the last sphere: total mass and centre
+        of mass, the spheres' positions relative to it, the moment of inertia about it (parallel
+        axes), all with the identity orientation; then the centre of mass moves to gpos and every
+        sphere by the same displacement."""
+        spheres = [(np.asarray(o, float).reshape(3), float(r), int(m)) for o, r, m in spheres]
+        if not spheres:
+            raise ValueError("Invalid number of spheres")
+        for _, r, m in spheres:
+            if not r > 0:
+                raise ValueError("Invalid sphere radius")
+            if not 0 <= m < len(self.materials):
+                raise ValueError("Invalid material ID")
+        disp = [o for o, _, _ in spheres]
+        mass_k, inert_k = [], []
+        for _, r, m in spheres:
+            mk = 4.0 / 3.0 * math.pi * r * r * r * self.materials[m].density
+            mass_k.append(mk)
+            inert_k.append(0.4 * mk * r * r)
+        n = len(spheres)
+        if n == 1:
+            com = [float(disp[0][0]), float(disp[0][1]), float(disp[0][2])]
+            mass = mass_k[0]
+        else:
+            com, mass = [0.0, 0.0, 0.0], 0.0
+            for k in range(n):
+                mass += mass_k[k]
+                for j in range(3):
+                    com[j] += float(disp[k][j]) * mass_k[k]
+            tmp = 1.0 / mass          # Vector3::operator/=, Vector3.h:836-848
+            com = [c * tmp for c in com]
+        inv_mass = 1.0 / mass
+        eye = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+        rpos = []
+        for k in range(n):
+            d = [float(disp[k][j]) - com[j] for j in range(3)]
+            rpos.append([eye[3 * i] * d[0] + eye[3 * i + 1] * d[1] + eye[3 * i + 2] * d[2] for i in range(3)])
+        I = [0.0] * 9
+        for k in range(n):
+            mk = mass_k[k]
+            pos = [float(disp[k][j]) - com[j] for j in range(3)]
+            I[0] += inert_k[k]; I[4] += inert_k[k]; I[8] += inert_k[k]
+            I[0] += mk * (pos[1] * pos[1] + pos[2] * pos[2])
+            I[1] -= mk * pos[0] * pos[1]
+            I[2] -= mk * pos[0] * pos[2]
+            I[3] -= mk * pos[0] * pos[1]
+            I[4] += mk * (pos[0] * pos[0] + pos[2] * pos[2])
+            I[5] -= mk * pos[1] * pos[2]
+            I[6] -= mk * pos[0] * pos[2]
+            I[7] -= mk * pos[1] * pos[2]
+            I[8] += mk * (pos[0] * pos[0] + pos[1] * pos[1])
+        I = _mat3_mul(_mat3_mul(eye, I), eye)   # trans(R) * I * R with R = identity
+        Iinv = mat3_inverse(np.asarray(I))
+        g = [float(x) for x in np.asarray(gpos, float).reshape(3)]
+        dp = [g[j] - com[j] for j in range(3)]
+        bound = max(math.sqrt(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]) + r
+                    for rp, (_, r, _) in zip(rpos, spheres))
+        rmin = min(r for _, r, _ in spheres)
+        idx = self._add(UNION, spheres[0][2], False, (bound * (1.0 + 1e-12), rmin, 0.0), g, (1.0, 0.0, 0.0, 0.0),
+                        inv_mass, np.asarray(I), Iinv, (0.0, 0.0, 0.0), 0.0, uid)
+        for k in range(n):
+            self._m_owner.append(idx)
+            self._m_sid.append(self._next_sid)
+            self._next_sid += 1
+            self._m_mat.append(spheres[k][2])
+            self._m_rad.append(spheres[k][1])
+            self._m_rpos.append(np.asarray(rpos[k], float))
+            self._m_gpos.append(np.asarray([float(disp[k][j]) + dp[j] for j in range(3)]))
+        return idx
+
+    def createGranularParticle(self, uid, gpos, radius, material, rng=None):
+        """createGranularParticle (Union.cpp:2376-2402): a union of two to four spheres of radius
+        U(0.6, 0.8) * radius, each displaced from the centre along a normalised U(-1, 1)^3 direction
+        by radius minus its own radius.  The draws come from rng (a numpy Generator), not from pe's
+        Boost generator, so a particle equals pe's only in distribution."""
+        if not radius > 0:
+            raise ValueError("Invalid sphere radius")
+        rng = np.random.default_rng() if rng is None else rng
+        number = int(rng.integers(2, 5))
+        rmin, rmax = 0.6 * radius, 0.8 * radius
+        spheres = []
+        for _ in range(number):
+            rad = float(rng.uniform(rmin, rmax))
+            d = rng.uniform(-1.0, 1.0, size=3)
+            ilen = 1.0 / math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])   # Vector3::normalize
+            d = [float(x) * ilen * (radius - rad) for x in d]                  # Vector3::scale
+            spheres.append((d, rad, material))
+        return self.createUnion(uid, gpos, spheres)
+
+    def members(self):
+        """Union members: owner body, system ID, material, radius, relative and global position."""
+        self._sync_from_device()
+        k = len(self._m_owner)
+        return {"owner": np.asarray(self._m_owner, np.uint32), "sid": np.asarray(self._m_sid, np.uint64),
+                "mat": np.asarray(self._m_mat, np.uint32), "radius": np.asarray(self._m_rad, float),
+                "rpos": np.asarray(self._m_rpos, float).reshape(k, 3), "gpos": np.asarray(self._m_gpos, float).reshape(k, 3)}
+
     def createPlane(self, uid, normal, d, material):
         """createPlane (Plane.cpp:1202-1240) and the Plane ctor (Plane.cpp:77-102)."""
         nv = np.asarray(normal, float)
@@ -299,6 +410,8 @@ class World:
         self._materialise_lists()
         if self._shape[body] == PLANE or not fixed:
             return
+        if self._shape[body] == UNION:
+            raise ValueError("fixed unions are not supported")
         self._fixed[body] = 1
         self._inv_mass[body] = 0.0
         self._Iinv[body] = np.zeros(9)
@@ -314,6 +427,8 @@ class World:
         n = len(arrays["shape"])
         if base:
             raise ValueError("bulk creation only into an empty world")
+        if np.any(np.asarray(arrays["shape"]) == UNION):
+            raise ValueError("unions are created with createUnion")
         self._arrays = dict(arrays)
         if sids is None:
             self._arrays["sid"] = np.arange(self._next_sid, self._next_sid + n, dtype=np.uint64)
@@ -388,8 +503,22 @@ class World:
                                         ptr(c["mat"], _abi._U32), ptr(c["fixed"], _abi._U8), ptr(c["param"]),
                                         ptr(c["pos"]), ptr(c["q"]), ptr(c["v"]), ptr(c["w"]), ptr(c["inv_mass"]),
                                         ptr(c["I"]), ptr(c["Iinv"]), ptr(c["normal"]), ptr(c["d"])), self._ctx)
+            if self._m_owner:
+                self._upload_members()
             self._bodies_dirty = False
             self._state_on_device = False
+
+    def _upload_members(self):
+        """Union members to the device (after their union bodies)."""
+        k = len(self._m_owner)
+        mo = np.ascontiguousarray(self._m_owner, np.uint32)
+        ms = np.ascontiguousarray(self._m_sid, np.uint64)
+        mm = np.ascontiguousarray(self._m_mat, np.uint32)
+        mr = np.ascontiguousarray(self._m_rad, float)
+        mrp = np.ascontiguousarray(np.asarray(self._m_rpos, float).reshape(k, 3))
+        mg = np.ascontiguousarray(np.asarray(self._m_gpos, float).reshape(k, 3))
+        check(_abi.load().pehip_upload_members(self._ctx, k, ptr(mo, _abi._U32), ptr(ms, _abi._U64),
+                                               ptr(mm, _abi._U32), ptr(mr), ptr(mrp), ptr(mg)), self._ctx)
 
     def close(self):
         if self._ctx is not None:
@@ -445,6 +574,9 @@ class World:
               "normal": np.zeros((n, 3)), "d": np.zeros(n)}
         check(_abi.load().pehip_download_state(self._ctx, n, ptr(st["pos"]), ptr(st["q"]), ptr(st["v"]),
                                                ptr(st["w"]), ptr(st["normal"]), ptr(st["d"])), self._ctx)
+        if self._m_owner:   # union members' global positions
+            st["mgpos"] = np.zeros((len(self._m_owner), 3))
+            check(_abi.load().pehip_download_members(self._ctx, len(self._m_owner), ptr(st["mgpos"])), self._ctx)
         return st
 
     def setState(self, st):
@@ -456,6 +588,9 @@ class World:
         if self._arrays is not None:
             for k in c:
                 self._arrays[k] = c[k].copy()
+        if "mgpos" in st and self._m_owner:   # union members are placed as given
+            self._m_gpos = list(np.asarray(st["mgpos"], float).reshape(-1, 3))
+            self._upload_members()
         self._state_on_device = False
 
     def contacts(self):

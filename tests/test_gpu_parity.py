@@ -136,6 +136,77 @@ def capsule_scene():
                         v0=v0, dt=1e-3, max_iterations=30)
 
 
+def _granular(rng, R, mats):
+    """Offsets, radii and materials of a granular particle (createGranularParticle's recipe,
+    Union.cpp:2376-2402), and its spheres' offsets from its centre of mass."""
+    k = int(rng.integers(2, 5))
+    r = rng.uniform(0.6 * R, 0.8 * R, size=k)
+    d = rng.uniform(-1.0, 1.0, size=(k, 3))
+    d = d / np.linalg.norm(d, axis=1)[:, None] * (R - r)[:, None]
+    mat = [int(mats[j % len(mats)]) for j in range(k)]
+    m = r ** 3 * np.array([(2500.0, 1500.0)[x] for x in mat])
+    com = (d * m[:, None]).sum(0) / m.sum()
+    return [(d[j], float(r[j]), mat[j]) for j in range(k)], d - com, r
+
+
+def _drop(xy, rel, rad, placed, e, top=0.0):
+    """Height of a rigid cluster of spheres (offsets rel, radii rad) lowered at xy onto a flat
+    support at height top and the spheres already placed (centre, radius), penetrating its
+    support by e."""
+    z = max(float(np.max(rad - rel[:, 2])) + top - e, 0.0)
+    for o, ra in zip(rel, rad):
+        for c, rb in placed:
+            dxy = np.hypot(xy[0] + o[0] - c[0], xy[1] + o[1] - c[1])
+            reach = ra + rb - e
+            if dxy < reach:
+                z = max(z, c[2] - o[2] + np.sqrt(reach * reach - dxy * dxy))
+    return z
+
+
+def union_scene():
+    # granular particles (unions of 2-4 spheres) resting on the floor and on each other
+    # (union-plane, union-union), spheres resting on them (sphere-union), particles on a box and
+    # against a capsule (box-union, capsule-union); members of two materials
+    rng = np.random.default_rng(47)
+    e, R = 1e-6, 0.4
+    placed = []
+    bpos = np.array([[4.9, 1.5, 0.25]])
+    cpos = np.array([[4.9, 3.2, 0.2 - e]])
+    placed += [([4.3 + 0.005 * i, 3.2, 0.2 - e], 0.2) for i in range(241)]   # the capsule's axis
+    unions = []
+
+    def add(xy, mats, top=0.0):
+        sph, rel, rad = _granular(rng, R, mats)
+        z = _drop(xy, rel, rad, placed, e, top)
+        unions.append(([xy[0], xy[1], z], sph))
+        placed.extend((np.array([xy[0], xy[1], z]) + o, ra) for o, ra in zip(rel, rad))
+
+    for iy in range(4):
+        for ix in range(4):
+            add((0.9 + 0.79 * ix, 0.9 + 0.79 * iy), (0, 1))
+    for iy in range(3):
+        for ix in range(3):
+            add((1.3 + 0.79 * ix, 1.3 + 0.79 * iy), (1, 0))
+    add((4.9, 1.5), (0,), top=0.5)                      # on the box's top face
+    for k in range(2):
+        add((4.5 + 0.8 * k, 3.75), (1,))
+    spos = []
+    for xy in ((1.7, 1.7), (2.5, 2.1), (2.1, 2.9)):
+        z = _drop(xy, np.zeros((1, 3)), np.array([0.3]), placed, e)
+        spos.append([xy[0], xy[1], z])
+        placed.append((np.array(spos[-1]), 0.3))
+    spos = np.array(spos)
+    ns = len(spos)
+    v0 = np.concatenate([np.zeros((6, 3)), rng.uniform(-0.05, 0.05, size=(ns, 3)) * [1, 1, 0], np.zeros((2, 3))])
+    return scenes.Scene("unions", [{"name": "a", "density": 2500.0, "cor": 0.0, "csf": 0.2, "cdf": 0.2},
+                                   {"name": "b", "density": 1500.0, "cor": 0.0, "csf": 0.1, "cdf": 0.1}],
+                        scenes.box_planes(6.0, 6.0, 20.0), spos, np.full(ns, 0.3), np.ones(ns, np.uint32),
+                        box_pos=bpos, box_len=np.array([[1.0, 1.0, 0.5]]), box_q=np.array([[1.0, 0, 0, 0]]),
+                        box_mat=np.zeros(1, np.uint32), capsule_pos=cpos, capsule_r=np.full(1, 0.2),
+                        capsule_len=np.full(1, 1.2), capsule_q=np.array([[1.0, 0, 0, 0]]),
+                        capsule_mat=np.zeros(1, np.uint32), v0=v0, unions=unions, dt=1e-3, max_iterations=30)
+
+
 def stack_scene():
     # box-box contacts of every kind collideBoxBox distinguishes (MaxContacts.h:867-2268):
     # face-face between boxes turned about z (incident-face clipping), an edge resting on a
@@ -222,7 +293,7 @@ def hopper_scene():
 
 
 SCENES = {"dense": dense_scene, "poly": poly_scene, "boxes": box_scene, "stack": stack_scene,
-          "hopper": hopper_scene, "capsules": capsule_scene}
+          "hopper": hopper_scene, "capsules": capsule_scene, "unions": union_scene}
 
 
 def oracle_state_after(sc, steps, iters=30, cr_trig=False):

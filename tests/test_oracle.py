@@ -851,3 +851,218 @@ def test_capsule_host_mass_properties_equal_oracle():
     assert np.array_equal(a["inv_mass"], info["inv_mass"])
     assert np.array_equal(a["Iinv"], info["inv_inertia"])
     assert np.array_equal(a["param"][:, :2], info["param"][:, :2])
+
+
+# --------------------------------------------------------------------------- unions of spheres
+def test_union_two_spheres_mass_properties_closed_form():
+    # two equal spheres at +-a on the x axis: M = 2m, centre of mass at the midpoint,
+    # I_xx = 2 (2/5) m r^2, I_yy = I_zz = 2 ((2/5) m r^2 + m a^2), no products of inertia
+    r, a, rho = 0.3, 0.45, 2500.0
+    w = make_world(density=rho)
+    u = w.create_union([1.0, 2.0, 3.0], [[-a, 0.2, 0.1], [a, 0.2, 0.1]], [r, r], 0)
+    info = w.body_info()
+    m = 4.0 / 3.0 * math.pi * r ** 3 * rho
+    assert info["shape"][u] == O.UNION
+    assert 1.0 / info["inv_mass"][u] == pytest.approx(2 * m, rel=1e-15)
+    Iinv = info["inv_inertia"][u].reshape(3, 3)
+    I0 = 0.4 * m * r * r
+    assert np.allclose(np.diag(Iinv), [1 / (2 * I0), 1 / (2 * (I0 + m * a * a)), 1 / (2 * (I0 + m * a * a))], rtol=1e-14)
+    assert np.count_nonzero(Iinv - np.diag(np.diag(Iinv))) == 0
+    mem = w.members()
+    assert np.allclose(mem["rpos"], [[-a, 0, 0], [a, 0, 0]], atol=1e-16)
+    assert np.allclose(mem["gpos"], [[1.0 - a, 2.0, 3.0], [1.0 + a, 2.0, 3.0]], atol=1e-15)
+    assert np.array_equal(w.get_state()["pos"][u], [1.0, 2.0, 3.0])
+
+
+def test_union_general_mass_properties_parallel_axis():
+    # three spheres of different sizes and densities: total mass, centre of mass and the
+    # inertia tensor sum_k (2/5 m_k r_k^2 E + m_k (|d_k|^2 E - d_k d_k^T)) by numpy, inverted by numpy
+    rng = np.random.default_rng(5)
+    w = O.OracleWorld(gravity=(0, 0, 0))
+    mats = [w.create_material(1000.0, 0, 0.1, 0.1), w.create_material(3000.0, 0, 0.1, 0.1)]
+    for trial in range(5):
+        k = 3
+        disp = rng.uniform(-0.5, 0.5, size=(k, 3))
+        rad = rng.uniform(0.1, 0.3, size=k)
+        mat = [mats[j % 2] for j in range(k)]
+        u = w.create_union(rng.uniform(0, 5, 3), disp, rad, mat)
+        m = 4.0 / 3.0 * np.pi * rad ** 3 * np.array([(1000.0, 3000.0)[j % 2] for j in range(k)])
+        com = (disp * m[:, None]).sum(0) / m.sum()
+        d = disp - com
+        I = sum(0.4 * m[j] * rad[j] ** 2 * np.eye(3) + m[j] * (d[j] @ d[j] * np.eye(3) - np.outer(d[j], d[j]))
+                for j in range(k))
+        info = w.body_info()
+        assert 1.0 / info["inv_mass"][u] == pytest.approx(m.sum(), rel=1e-14)
+        assert np.allclose(info["inv_inertia"][u].reshape(3, 3), np.linalg.inv(I), rtol=1e-12, atol=0)
+        mem = w.members()
+        sel = mem["owner"] == u
+        assert np.allclose(mem["rpos"][sel], d, atol=1e-15)
+        assert np.array_equal(mem["sid"][sel], info["sid"][u] + 1 + np.arange(k))   # union first, then members
+
+
+def test_union_aabb_is_merged_member_boxes():
+    w = make_world()
+    u = w.create_union([0.5, 0.5, 0.5], [[0, 0, 0], [0.6, -0.2, 0.1], [0.1, 0.5, -0.4]], [0.3, 0.2, 0.25], 0)
+    mem = w.members()
+    lo = (mem["gpos"] - (mem["radius"][:, None] + 1e-8)).min(0)       # SphereBase.h:208-221
+    hi = (mem["gpos"] + (mem["radius"][:, None] + 1e-8)).max(0)
+    box = w.aabb()[u]
+    assert np.array_equal(box[:3], lo) and np.array_equal(box[3:], hi)
+
+
+def test_union_rotates_rigidly():
+    # a spinning union without contacts: members stay at the union's position plus the rotated
+    # relative positions, pairwise distances are kept, and the centre of mass moves uniformly
+    w = make_world()
+    u = w.create_union([2.0, 2.0, 2.0], [[-0.3, 0, 0], [0.3, 0.1, 0], [0, -0.2, 0.35]], [0.2, 0.25, 0.15], 0)
+    st = w.get_state()
+    st["v"][u] = [0.1, -0.2, 0.3]
+    st["w"][u] = [1.5, -0.7, 2.0]
+    w.set_state(st)
+    d0 = np.linalg.norm(w.members()["gpos"][:, None] - w.members()["gpos"][None], axis=-1)
+    w.step(1e-3, 50)
+    st = w.get_state()
+    mem = w.members()
+    R = O.quat_to_rotation(st["q"][u])
+    assert np.allclose(mem["gpos"], st["pos"][u] + mem["rpos"] @ R.T, atol=1e-15)
+    d1 = np.linalg.norm(mem["gpos"][:, None] - mem["gpos"][None], axis=-1)
+    assert np.allclose(d0, d1, atol=1e-13)
+    assert np.allclose(st["pos"][u], [2.0 + 0.005, 2.0 - 0.01, 2.0 + 0.015], atol=1e-14)
+    assert np.linalg.norm(st["q"][u] - [1, 0, 0, 0]) > 0.05
+
+
+def test_union_plane_contacts_use_union_and_member_materials():
+    # a dumbbell lying on the floor touches it with both spheres: two contacts between the union
+    # (body 1) and the plane, at the spheres' lowest points, with the members' materials' friction
+    w = O.OracleWorld(gravity=(0, 0, -9.81))
+    m0 = w.create_material(2500.0, 0.0, 0.1, 0.1)
+    m1 = w.create_material(2500.0, 0.0, 0.3, 0.3)
+    p = w.create_plane([0, 0, 1], 0.0, m0)
+    u = w.create_union([1.0, 1.0, 0.3 - 1e-9], [[-0.4, 0, 0], [0.4, 0, 0]], [0.3, 0.3], [m0, m1])
+    c = w.detect(exhaustive=True)
+    assert list(c["b1"]) == [u, u] and list(c["b2"]) == [p, p]
+    assert np.allclose(c["pos"], [[0.6, 1.0, 0.0], [1.4, 1.0, 0.0]], atol=1e-15)   # on the plane (MaxContacts.h:789)
+    assert np.allclose(c["dist"], [-1e-9, -1e-9], rtol=1e-6)
+    assert np.allclose(c["normal"], [[0, 0, 1], [0, 0, 1]])
+    assert np.allclose(c["mu"], [0.2, 0.4], rtol=1e-15)
+
+
+def test_union_resting_symmetrically_stays_at_rest():
+    # the dumbbell resting on both spheres at dist 0: the two normal impulses cancel gravity and
+    # each other's torque, so the union ends the step at rest (a sphere's closed form, twice)
+    g, dt = 9.81, 1e-3
+    w = make_world(gravity=(0, 0, -g), iters=200)
+    w.create_plane([0, 0, 1], 0.0, 0)
+    u = w.create_union([1.0, 1.0, 0.3], [[-0.4, 0, 0], [0.4, 0, 0]], [0.3, 0.3], 0)
+    w.step(dt)
+    st = w.get_state()
+    assert np.all(np.abs(st["v"][u]) < 1e-12) and np.all(np.abs(st["w"][u]) < 1e-12)
+    m = 2 * 4.0 / 3.0 * math.pi * 0.027 * 2500.0
+    assert w.impulses()[:, 2].sum() == pytest.approx(m * g * dt, rel=1e-9)
+
+
+def test_union_union_contact_orders_by_member_ids():
+    # two single-sphere unions touching: one contact, body 1 the union whose member has the
+    # smaller system ID, the normal from body 2's sphere to body 1's, the point halfway into the
+    # overlap (collideSphereSphere on the members)
+    w = make_world()
+    u1 = w.create_union([0.0, 0.0, 0.0], [[0, 0, 0]], [0.5], 0)
+    u2 = w.create_union([0.9, 0.0, 0.0], [[0, 0, 0]], [0.5], 0)
+    c = w.detect(exhaustive=True)
+    assert list(c["b1"]) == [u1] and list(c["b2"]) == [u2]
+    assert np.allclose(c["normal"][0], [-1, 0, 0]) and c["dist"][0] == pytest.approx(-0.1, abs=1e-15)
+    assert np.allclose(c["pos"][0], [0.45, 0, 0], atol=1e-15)
+
+
+def test_union_contacts_set_equals_member_brute_force():
+    # HashGrids over unions and spheres: the contact set equals the brute-force set over member
+    # and sphere pairs of different top-level bodies with dist < 1e-8
+    rng = np.random.default_rng(9)
+    w = make_world()
+    spheres = []   # (centre, radius, top-level body)
+    for k in range(40):
+        pos = rng.uniform(0, 3, 3)
+        if k % 2:
+            n = int(rng.integers(1, 4))
+            disp, rad = rng.uniform(-0.2, 0.2, size=(n, 3)), rng.uniform(0.1, 0.25, n)
+            b = w.create_union(pos, disp, rad, 0)
+        else:
+            b = w.create_sphere(pos, rng.uniform(0.1, 0.3), 0)
+    info = w.body_info()
+    mem = w.members()
+    st = w.get_state()
+    for b in range(w.n):
+        if info["shape"][b] == O.SPHERE:
+            spheres.append((st["pos"][b], info["param"][b][0], b))
+    for k in range(len(mem["owner"])):
+        spheres.append((mem["gpos"][k], mem["radius"][k], int(mem["owner"][k])))
+    expect = 0
+    for i in range(len(spheres)):
+        for j in range(i + 1, len(spheres)):
+            if spheres[i][2] != spheres[j][2]:
+                d = np.linalg.norm(spheres[i][0] - spheres[j][0]) - spheres[i][1] - spheres[j][1]
+                expect += d < 1e-8
+    hg = w.detect(exhaustive=False)
+    ex = w.detect(exhaustive=True)
+    assert expect > 10 and len(hg["b1"]) == len(ex["b1"]) == expect
+
+
+def test_union_host_properties_equal_oracle():
+    # World.createUnion / createGranularParticle compute pe's union properties in the reference's
+    # operation order: bit for bit the oracle's (system IDs, mass, inertia, relative and global
+    # member positions)
+    from pe_amd.world import World
+    rng = np.random.default_rng(3)
+    hw = World()
+    m = hw.createMaterial("g", 2500.0, 0.0, 0.25, 0.25)
+    ow = O.OracleWorld()
+    om = ow.create_material(2500.0, 0.0, 0.25, 0.25)
+    for t in range(30):
+        n = int(rng.integers(1, 5))
+        disp = rng.uniform(-0.3, 0.3, size=(n, 3))
+        rad = rng.uniform(0.1, 0.4, size=n)
+        gp = rng.uniform(0, 5, 3)
+        hw.createUnion(t, gp, [(disp[k], rad[k], m) for k in range(n)])
+        ow.create_union(gp, disp, rad, om)
+    a, info = hw.host_arrays(), ow.body_info()
+    hm, omem = hw.members(), ow.members()
+    assert np.array_equal(a["sid"], info["sid"]) and np.array_equal(hm["sid"], omem["sid"])
+    assert np.array_equal(a["inv_mass"], info["inv_mass"]) and np.array_equal(a["Iinv"], info["inv_inertia"])
+    assert np.array_equal(a["pos"], ow.get_state()["pos"])
+    for k in ("owner", "radius", "rpos", "gpos"):
+        assert np.array_equal(hm[k], omem[k]), k
+
+
+def test_granular_particle_recipe():
+    # createGranularParticle: 2-4 spheres of radius in [0.6 R, 0.8 R], each offset from the
+    # particle centre along a random direction by R minus its radius, so that every sphere touches
+    # the particle's bounding sphere from inside.  The union stores positions relative to its
+    # centre of mass; the draws are replayed from a second generator with the same seed to
+    # recover the offsets.  A sphere offset by r instead of R - r, or an unnormalised direction,
+    # fails the comparison.
+    from pe_amd.world import World
+    rng, replay = np.random.default_rng(1), np.random.default_rng(1)
+    hw = World()
+    m = hw.createMaterial("g", 2500.0, 0.0, 0.25, 0.25)
+    R = 0.5
+    counts = set()
+    for t in range(40):
+        u = hw.createGranularParticle(t, (1.0, 2.0, 3.0), R, m, rng)
+        n = int(replay.integers(2, 5))
+        radii, disp = [], []
+        for _ in range(n):
+            radii.append(float(replay.uniform(0.6 * R, 0.8 * R)))
+            d = replay.uniform(-1.0, 1.0, size=3)
+            disp.append(d / np.linalg.norm(d) * (R - radii[-1]))
+        radii, disp = np.array(radii), np.array(disp)
+        mem = hw.members()
+        sel = mem["owner"] == u
+        counts.add(int(sel.sum()))
+        r = mem["radius"][sel]
+        assert np.array_equal(r, radii)
+        assert np.all((r >= 0.6 * R) & (r <= 0.8 * R))
+        assert np.allclose(np.linalg.norm(disp, axis=1) + r, R, rtol=1e-15)
+        com = (disp * (r ** 3)[:, None]).sum(0) / (r ** 3).sum()
+        assert np.allclose(mem["rpos"][sel], disp - com, atol=1e-15)
+        assert np.allclose(mem["gpos"][sel], np.array([1.0, 2.0, 3.0]) + mem["rpos"][sel], atol=1e-14)
+    assert counts == {2, 3, 4}
