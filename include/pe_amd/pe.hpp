@@ -8,15 +8,18 @@
 //   createMaterial                                                  (src/core/Materials.cpp:175-247)
 //   createSphere / createBox / createCapsule / createPlane          (Sphere.cpp:1187, Box.cpp:1312,
 //                                                                    Capsule.cpp:1077, Plane.cpp:1202)
+//   createUnion / createGranularParticle                            (Union.cpp:1960, 2376)
 // Body handles (SphereID, BoxID, PlaneID) read and write their state through the device;
 // the state is downloaded lazily after a step and uploaded before the next one.
 #ifndef PE_AMD_PE_HPP
 #define PE_AMD_PE_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -63,7 +66,8 @@ struct Quat {
 
 typedef size_t MaterialID;
 
-enum BodyType { sphereType = PEHIP_SPHERE, boxType = PEHIP_BOX, planeType = PEHIP_PLANE, capsuleType = PEHIP_CAPSULE };
+enum BodyType { sphereType = PEHIP_SPHERE, boxType = PEHIP_BOX, planeType = PEHIP_PLANE, capsuleType = PEHIP_CAPSULE,
+                unionType = PEHIP_UNION };
 
 namespace detail {
 
@@ -156,7 +160,23 @@ class Plane : public RigidBody {
 typedef RigidBody* BodyID;
 typedef Sphere* SphereID;
 typedef Box* BoxID;
+// A union of spheres (pe/core/rigidbody/Union.h): one rigid body whose contacts are its spheres'.
+class Union : public RigidBody {
+ public:
+  size_t size() const;  // number of contained spheres
+ private:
+  friend class detail::Engine;
+  using RigidBody::RigidBody;
+};
+// One sphere of a union: offset from the union's reference point before placement, radius, material.
+struct UnionSphere {
+  Vec3 offset;
+  real radius;
+  MaterialID material;
+};
+
 typedef Capsule* CapsuleID;
+typedef Union* UnionID;
 typedef Plane* PlaneID;
 
 namespace detail {
@@ -183,6 +203,11 @@ class Engine {
   std::vector<real> param, pos, q, v, w, inv_mass, mass, I, Iinv, normal, d, force, torque;
   std::vector<Vec3> vpos, vv, vw;
   std::vector<Quat> vq;
+  // union members (spheres), grouped by union in creation order
+  std::vector<uint32_t> m_owner, m_mat;
+  std::vector<uint64_t> m_sid;
+  std::vector<real> m_rad, m_rpos, m_gpos;
+  std::mt19937_64 rng{12345};  // draws of createGranularParticle
   bool forces_pending = false;
   bool bodies_dirty = true, state_dirty = false, state_stale = false, params_dirty = true;
   uint64_t next_sid = 1;
@@ -255,6 +280,7 @@ class Engine {
     if (!state_stale || !ctx_) return;
     const uint32_t n = (uint32_t)shape.size();
     check(pehip_download_state(ctx_, n, pos.data(), q.data(), v.data(), w.data(), normal.data(), d.data()), ctx_);
+    if (!m_owner.empty()) check(pehip_download_members(ctx_, (uint32_t)m_owner.size(), m_gpos.data()), ctx_);
     state_stale = false;
   }
 
@@ -268,6 +294,9 @@ class Engine {
       check(pehip_upload_bodies(c, n, shape.data(), sid.data(), mat.data(), fixed.data(), param.data(), pos.data(),
                                 q.data(), v.data(), w.data(), inv_mass.data(), I.data(), Iinv.data(), normal.data(),
                                 d.data()), c);
+      if (!m_owner.empty())
+        check(pehip_upload_members(c, (uint32_t)m_owner.size(), m_owner.data(), m_sid.data(), m_mat.data(),
+                                   m_rad.data(), m_rpos.data(), m_gpos.data()), c);
       bodies_dirty = state_dirty = false;
     } else if (state_dirty) {
       check(pehip_upload_state(c, n, pos.data(), q.data(), v.data(), w.data(), normal.data(), d.data()), c);
@@ -386,6 +415,10 @@ inline void RigidBody::addTorque(const Vec3& t) {
 inline real Sphere::getRadius() const { return detail::E().param[3 * index_]; }
 inline real Capsule::getRadius() const { return detail::E().param[3 * index_]; }
 inline real Capsule::getLength() const { return detail::E().param[3 * index_ + 1]; }
+inline size_t Union::size() const {
+  const detail::Engine& e = detail::E();
+  return (size_t)std::count(e.m_owner.begin(), e.m_owner.end(), (uint32_t)index_);
+}
 inline Vec3 Box::getLengths() const {
   const detail::Engine& e = detail::E();
   return Vec3(e.param[3 * index_], e.param[3 * index_ + 1], e.param[3 * index_ + 2]);
@@ -453,6 +486,100 @@ inline CapsuleID createCapsule(id_t uid, const Vec3& gpos, real radius, real len
                 sphereMass * (real(0.4) * radius * radius + real(0.375) * radius * length + real(0.25) * length * length);
   const real prm[3] = {radius, length, 0};
   return e.add<Capsule>(capsuleType, uid, material, false, prm, gpos, Quat(), m, real(1) / m, I, Vec3(), 0);
+}
+
+// A union of spheres as pe builds one inside pe_CREATE_UNION and places with Union::setPosition
+// (Union.cpp:251-272, 1482-1545; UnionBase.cpp:240-360): the union takes the next system ID, each
+// sphere its own; after the last sphere the total mass, centre of mass, relative positions and the
+// moment of inertia about the centre of mass (identity orientation); then the centre of mass moves
+// to gpos and every sphere by the same displacement.
+inline UnionID createUnion(id_t uid, const Vec3& gpos, const std::vector<UnionSphere>& spheres, bool visible = true) {
+  (void)visible;
+  if (spheres.empty()) throw std::invalid_argument("Invalid number of spheres");
+  detail::Engine& e = detail::E();
+  const size_t n = spheres.size();
+  std::vector<real> mk(n), Ik(n);
+  for (size_t k = 0; k < n; ++k) {
+    const real r = spheres[k].radius;
+    if (r <= real(0)) throw std::invalid_argument("Invalid sphere radius");
+    mk[k] = real(4.0) / real(3.0) * M_PI * r * r * r * e.materials.at(spheres[k].material).density;
+    Ik[k] = real(0.4) * mk[k] * r * r;
+  }
+  real com[3] = {0, 0, 0}, mass = 0;
+  if (n == 1) {
+    for (int j = 0; j < 3; ++j) com[j] = spheres[0].offset[j];
+    mass = mk[0];
+  } else {
+    for (size_t k = 0; k < n; ++k) {
+      mass += mk[k];
+      for (int j = 0; j < 3; ++j) com[j] += spheres[k].offset[j] * mk[k];
+    }
+    const real tmp = real(1) / mass;  // Vector3::operator/=, Vector3.h:836-848
+    for (int j = 0; j < 3; ++j) com[j] = com[j] * tmp;
+  }
+  const real eye[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  std::vector<real> rpos(3 * n);
+  real I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  real bound = 0, rmin = spheres[0].radius;
+  for (size_t k = 0; k < n; ++k) {
+    real p[3];
+    for (int j = 0; j < 3; ++j) p[j] = spheres[k].offset[j] - com[j];
+    for (int i = 0; i < 3; ++i) rpos[3 * k + i] = eye[3 * i] * p[0] + eye[3 * i + 1] * p[1] + eye[3 * i + 2] * p[2];
+    const real m = mk[k];
+    I[0] += Ik[k]; I[4] += Ik[k]; I[8] += Ik[k];
+    I[0] += m * (p[1] * p[1] + p[2] * p[2]);
+    I[1] -= m * p[0] * p[1];
+    I[2] -= m * p[0] * p[2];
+    I[3] -= m * p[0] * p[1];
+    I[4] += m * (p[0] * p[0] + p[2] * p[2]);
+    I[5] -= m * p[1] * p[2];
+    I[6] -= m * p[0] * p[2];
+    I[7] -= m * p[1] * p[2];
+    I[8] += m * (p[0] * p[0] + p[1] * p[1]);
+    bound = std::max(bound, std::sqrt(rpos[3 * k] * rpos[3 * k] + rpos[3 * k + 1] * rpos[3 * k + 1] +
+                                      rpos[3 * k + 2] * rpos[3 * k + 2]) + spheres[k].radius);
+    rmin = std::min(rmin, spheres[k].radius);
+  }
+  real t1[9], Ib[9];  // trans(R) * I * R with R = identity, in pe's product order
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      t1[3 * i + j] = eye[3 * i] * I[j] + eye[3 * i + 1] * I[3 + j] + eye[3 * i + 2] * I[6 + j];
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Ib[3 * i + j] = t1[3 * i] * eye[j] + t1[3 * i + 1] * eye[3 + j] + t1[3 * i + 2] * eye[6 + j];
+  const real prm[3] = {bound * (real(1) + real(1e-12)), rmin, 0};
+  UnionID u = e.add<Union>(unionType, uid, spheres[0].material, false, prm, gpos, Quat(), mass, real(1) / mass, Ib,
+                           Vec3(), 0);
+  const real dp[3] = {gpos[0] - com[0], gpos[1] - com[1], gpos[2] - com[2]};
+  for (size_t k = 0; k < n; ++k) {
+    e.m_owner.push_back((uint32_t)(e.shape.size() - 1));
+    e.m_sid.push_back(e.next_sid++);
+    e.m_mat.push_back((uint32_t)spheres[k].material);
+    e.m_rad.push_back(spheres[k].radius);
+    for (int j = 0; j < 3; ++j) {
+      e.m_rpos.push_back(rpos[3 * k + j]);
+      e.m_gpos.push_back(spheres[k].offset[j] + dp[j]);
+    }
+  }
+  return u;
+}
+
+// createGranularParticle (Union.cpp:2376-2402): two to four spheres of radius U(0.6, 0.8) * radius,
+// each offset from the centre along a normalised U(-1, 1)^3 direction by radius minus its own
+// radius.  The draws come from a std::mt19937_64 (seed 12345), not pe's Boost generator.
+inline UnionID createGranularParticle(id_t uid, const Vec3& gpos, real radius, MaterialID material, bool visible = true) {
+  if (radius <= real(0)) throw std::invalid_argument("Invalid sphere radius");
+  std::mt19937_64& g = detail::E().rng;
+  const size_t number = std::uniform_int_distribution<size_t>(2, 4)(g);
+  std::uniform_real_distribution<real> urad(real(0.6) * radius, real(0.8) * radius), udir(real(-1), real(1));
+  std::vector<UnionSphere> spheres;
+  for (size_t i = 0; i < number; ++i) {
+    const real rad = urad(g);
+    Vec3 disp(udir(g), udir(g), udir(g));
+    disp.normalize();
+    spheres.push_back(UnionSphere{disp * (radius - rad), rad, material});
+  }
+  return createUnion(uid, gpos, spheres, visible);
 }
 
 // createPlane (Plane.cpp:1202-1240) and the Plane ctor (Plane.cpp:77-102)

@@ -305,10 +305,11 @@ struct WriteEmit {
   const double* cdf;
   uint32_t nmat;
   uint32_t at;
-  __device__ void operator()(uint32_t b1, uint32_t b2, d3 gpos, d3 n, double dist) {
-    const double mu = contact_friction(*B, csf, cdf, nmat, b1, b2, gpos, n);
-    R->b1[at] = b1;
-    R->b2[at] = b2;
+  // g1, g2: geometry ids (a body, or a union member); the contact joins their top-level bodies
+  __device__ void operator()(uint32_t g1, uint32_t g2, d3 gpos, d3 n, double dist) {
+    const double mu = contact_friction(*B, csf, cdf, nmat, g1, g2, gpos, n);
+    R->b1[at] = geom_body(*B, g1);
+    R->b2[at] = geom_body(*B, g2);
     st3(R->pos, at, gpos);
     st3(R->nrm, at, n);
     R->dist[at] = dist;

@@ -16,6 +16,7 @@ static int run(int argc, char** argv) {
   const int nz = argc > 3 ? std::atoi(argv[3]) : 5;
   const unsigned steps = argc > 4 ? (unsigned)std::atoi(argv[4]) : 100;
   const int ncaps = argc > 5 ? std::atoi(argv[5]) : 0;
+  const int nunions = argc > 6 ? std::atoi(argv[6]) : 0;
   const real r = 0.5, spacing = 1.2, dt = 1e-3;
   const real lx = 2.0 + nx * spacing, ly = 2.0 + ny * spacing, lz = 40.0;
 
@@ -41,6 +42,10 @@ static int run(int argc, char** argv) {
       }
   for (int c = 0; c < ncaps; ++c)
     createCapsule(7, Vec3(1.6 + 1.1 * c, 0.5 * ly + 0.05 * c, 0.6 + spacing * (nz - 1) + 0.77), 0.25, 0.8, granular);
+  for (int u = 0; u < nunions; ++u)  // dumbbells and triplets of spheres dropped on the pile
+    createUnion(8, Vec3(1.7 + 1.0 * u, 0.5 * ly - 0.6 + 0.07 * u, 0.6 + spacing * (nz - 1) + 1.9),
+                {UnionSphere{Vec3(-0.25, 0.0, 0.02), 0.22, granular}, UnionSphere{Vec3(0.25, 0.03, 0.0), 0.2, granular},
+                 UnionSphere{Vec3(0.0, 0.2, -0.1 * (u % 2)), 0.18, granular}});
 
   world->run(steps, dt);
 

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "pe_amd", "_lib", "granular_dropin")
 
 
-def python_run(nx, ny, nz, steps, ncaps=0):
+def python_run(nx, ny, nz, steps, ncaps=0, nunions=0):
     w = World()
     w.setGravity(0, 0, -9.81)
     w.setMaxIterations(50)
@@ -31,17 +31,21 @@ def python_run(nx, ny, nz, steps, ncaps=0):
                 w.createSphere(6, (1.5 + 1.2 * i + jit, 1.5 + 1.2 * j - jit, 0.6 + 1.2 * k + jit), 0.5, m)
     for c in range(ncaps):
         w.createCapsule(7, (1.6 + 1.1 * c, 0.5 * ly + 0.05 * c, 0.6 + 1.2 * (nz - 1) + 0.77), 0.25, 0.8, m)
+    for u in range(nunions):
+        w.createUnion(8, (1.7 + 1.0 * u, 0.5 * ly - 0.6 + 0.07 * u, 0.6 + 1.2 * (nz - 1) + 1.9),
+                      [((-0.25, 0.0, 0.02), 0.22, m), ((0.25, 0.03, 0.0), 0.2, m), ((0.0, 0.2, -0.1 * (u % 2)), 0.18, m)])
     w.run(steps, 1e-3)
     return w
 
 
-@pytest.mark.parametrize("ncaps", [0, 3])
-def test_cpp_dropin_matches_python_world(ncaps):
-    # with capsules: pe::createCapsule's mass properties equal World.createCapsule's bit for bit
+@pytest.mark.parametrize("ncaps,nunions", [(0, 0), (3, 0), (2, 3)])
+def test_cpp_dropin_matches_python_world(ncaps, nunions):
+    # with capsules and unions: pe::createCapsule's and pe::createUnion's properties equal the
+    # Python World's bit for bit
     if not os.path.exists(EXE):
         pytest.fail("drop-in example not built")
     nx, ny, nz, steps = 5, 4, 3, 220
-    r = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps), str(ncaps)], capture_output=True, text=True,
+    r = subprocess.run([EXE, str(nx), str(ny), str(nz), str(steps), str(ncaps), str(nunions)], capture_output=True, text=True,
                        timeout=120)
     print(r.stderr)
     assert r.returncode == 0, r.stderr
@@ -50,7 +54,7 @@ def test_cpp_dropin_matches_python_world(ncaps):
     rows = [l.split() for l in out[1:] if l.strip()]
     pos = np.array([[float.fromhex(x) for x in r[1:4]] for r in rows])
     vel = np.array([[float.fromhex(x) for x in r[4:7]] for r in rows])
-    w = python_run(nx, ny, nz, steps, ncaps)
+    w = python_run(nx, ny, nz, steps, ncaps, nunions)
     st = w.getState()
     assert len(rows) == w.size
     assert int(head[1]) == w.getNumberOfContacts() and int(head[1]) > 0
