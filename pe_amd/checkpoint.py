@@ -32,7 +32,7 @@ import struct
 
 import numpy as np
 
-from ._abi import BOX, CAPSULE, PLANE, SPHERE
+from ._abi import BOX, CAPSULE, PLANE, SPHERE, UNION
 from .world import mat3_inverse
 
 # pe/core/GeomType.h:45-53
@@ -117,6 +117,8 @@ def write_body_file(world, filename, fp_size=0):
     n = len(a["shape"])
     uid = a.get("uid", np.zeros(n, np.uint64))
     shape = np.asarray(a["shape"])
+    if np.any(shape == UNION):
+        raise ValueError("unions are not written to body files")
     planes = np.flatnonzero(shape == PLANE)
     glob = _Out(real)
     glob.put("Q", len(planes))                     # global ID counter

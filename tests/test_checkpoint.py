@@ -237,3 +237,14 @@ def test_restart_from_file_on_gpu(tmp_path):
     k1 = sorted(zip(inv[c1["b1"]].tolist(), inv[c1["b2"]].tolist(), map(tuple, c1["pos"].tolist())))
     k2 = sorted(zip(c2["b1"].tolist(), c2["b2"].tolist(), map(tuple, c2["pos"].tolist())))
     assert len(k1) > 10 and k1 == k2
+
+
+def test_union_world_is_refused(tmp_path):
+    # the writer covers spheres, boxes, capsules and planes; a world with a union is refused
+    # instead of being written without it
+    w = World()
+    m = w.createMaterial("g", 2500.0, 0.0, 0.2, 0.2)
+    w.createSphere(1, (0.0, 0.0, 1.0), 0.5, m)
+    w.createUnion(2, (2.0, 0.0, 1.0), [((0.0, 0.0, 0.0), 0.3, m), ((0.4, 0.0, 0.0), 0.2, m)])
+    with pytest.raises(ValueError, match="unions"):
+        checkpoint.write_body_file(w, str(tmp_path / "u.dat"))
