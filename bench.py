@@ -266,7 +266,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--spheres", type=int, default=200_000)
-    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "c3", "c5", "granular"], default="c2",
                     help="c2: the polydisperse pile (default); c3: 950k spheres + 50k boxes in a V-channel hopper, "
                          "30 sweeps, dt 2e-4 (use --steps 1000 for the whole job)")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
@@ -305,7 +305,8 @@ def main():
                           "n_gpus": 1, "dtype": "f64", "data": "synthetic", **adjoint_c5(args)}))
         return
 
-    sc = scenes.config2(args.spheres) if args.workload == "c2" else scenes.config3()
+    sc = (scenes.config2(args.spheres) if args.workload == "c2" else
+          scenes.config_granular() if args.workload == "granular" else scenes.config3())
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -366,6 +367,9 @@ def main():
         "config": {"workload": (("polydisperse granular pile, %d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, "
                                  "v0~U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), "
                                  "dt 5e-4" % args.spheres) if args.workload == "c2" else
+                                ("50000 granular particles (unions of 2-4 spheres, r 0.3-0.4 in a 0.5 bounding "
+                                 "sphere) on a 57x57 jittered lattice in a 60x60x200 m plane box, released at rest, "
+                                 "mu 0.5, 50 NBGS sweeps, dt 5e-4") if args.workload == "granular" else
                                 ("mixed shapes, 950k spheres r~U[0.2,0.4] + 50k boxes edges U[0.3,0.8] (Shoemake "
                                  "orientations) filling a V-channel hopper (planes at +-30 deg, 0.8 m gap) on a "
                                  "0.85 m jittered lattice, poured at v0 = (0,0,-10) + U[-1,1]^3 m/s, mu 0.6, 30 NBGS sweeps, "

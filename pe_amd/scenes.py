@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._abi import BOX, CAPSULE, PLANE, SPHERE
-from .world import mat3_inverse, normalize, quat_axis_angle, ACCURACY
+from .world import granular_particle_spheres, mat3_inverse, normalize, quat_axis_angle, ACCURACY
 
 
 @dataclass
@@ -218,6 +218,20 @@ def config2(n_spheres=200_000):
                  [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.25, "cdf": 0.25}],
                  box_planes(60.0, 60.0, 200.0), pos, r, np.zeros(n_spheres, np.uint32), v0=v0,
                  dt=5e-4, steps=2000, max_iterations=50)
+
+
+def config_granular(n_particles=50_000, radius=0.5):
+    """Pile of pe's granular particles (unions of 2-4 spheres, createGranularParticle) on a
+    jittered lattice (spacing 1.05, jitter 0.02, 57x57 per layer) in a 60x60x200 box, released
+    at rest; mu = 0.5, 50 sweeps, dt = 5e-4."""
+    rng = np.random.default_rng(12345 + 6)
+    per = 57
+    nz = (n_particles + per * per - 1) // (per * per)
+    pos = lattice(per, per, nz, 1.05, (0.525, 0.525, 0.525), 0.02, rng, count=n_particles)
+    unions = [(pos[k], granular_particle_spheres(rng, radius, 0)) for k in range(n_particles)]
+    return Scene("granular %d particles" % n_particles,
+                 [{"name": "granular", "density": 2500.0, "cor": 0.0, "csf": 0.25, "cdf": 0.25}],
+                 box_planes(60.0, 60.0, 200.0), unions=unions, dt=5e-4, steps=2000, max_iterations=50)
 
 
 def drop_scene(nx, ny, nz, r=0.5, spacing=1.2, jitter=0.05, box=(10.0, 10.0, 40.0), mu=0.4, seed=7,

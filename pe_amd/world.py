@@ -44,6 +44,26 @@ def mat3_inverse(m):
     return out
 
 
+def granular_particle_spheres(rng, radius, material):
+    """The spheres of pe's granular particle (createGranularParticle, Union.cpp:2376-2402): two to
+    four spheres of radius U(0.6, 0.8) * radius, each offset from the particle centre along a
+    normalised U(-1, 1)^3 direction by radius minus its own radius.  The draws come from rng (a
+    numpy Generator), not from pe's Boost generator, so a particle equals pe's only in
+    distribution.  Returns [(offset, radius, material), ...] for World.createUnion."""
+    if not radius > 0:
+        raise ValueError("Invalid sphere radius")
+    number = int(rng.integers(2, 5))
+    rmin, rmax = 0.6 * radius, 0.8 * radius
+    spheres = []
+    for _ in range(number):
+        rad = float(rng.uniform(rmin, rmax))
+        d = rng.uniform(-1.0, 1.0, size=3)
+        ilen = 1.0 / math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])   # Vector3::normalize
+        d = [float(x) * ilen * (radius - rad) for x in d]                  # Vector3::scale
+        spheres.append((d, rad, material))
+    return spheres
+
+
 def _mat3_mul(l, r):
     """Row-major 3x3 product in pe's evaluation order (Matrix3x3.h:2241-2253)."""
     return [l[3 * i] * r[j] + l[3 * i + 1] * r[3 + j] + l[3 * i + 2] * r[6 + j] for i in range(3) for j in range(3)]
@@ -366,23 +386,9 @@ class World:
         return idx
 
     def createGranularParticle(self, uid, gpos, radius, material, rng=None):
-        """createGranularParticle (Union.cpp:2376-2402): a union of two to four spheres of radius
-        U(0.6, 0.8) * radius, each displaced from the centre along a normalised U(-1, 1)^3 direction
-        by radius minus its own radius.  The draws come from rng (a numpy Generator), not from pe's
-        Boost generator, so a particle equals pe's only in distribution."""
-        if not radius > 0:
-            raise ValueError("Invalid sphere radius")
+        """createGranularParticle (Union.cpp:2376-2402); see granular_particle_spheres."""
         rng = np.random.default_rng() if rng is None else rng
-        number = int(rng.integers(2, 5))
-        rmin, rmax = 0.6 * radius, 0.8 * radius
-        spheres = []
-        for _ in range(number):
-            rad = float(rng.uniform(rmin, rmax))
-            d = rng.uniform(-1.0, 1.0, size=3)
-            ilen = 1.0 / math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])   # Vector3::normalize
-            d = [float(x) * ilen * (radius - rad) for x in d]                  # Vector3::scale
-            spheres.append((d, rad, material))
-        return self.createUnion(uid, gpos, spheres)
+        return self.createUnion(uid, gpos, granular_particle_spheres(rng, radius, material))
 
     def members(self):
         """Union members: owner body, system ID, material, radius, relative and global position."""
