@@ -220,12 +220,11 @@ def config2(n_spheres=200_000):
                  dt=5e-4, steps=2000, max_iterations=50)
 
 
-def config_granular(n_particles=50_000, radius=0.5):
+def config_granular(n_particles=50_000, radius=0.5, per=57):
     """Pile of pe's granular particles (unions of 2-4 spheres, createGranularParticle) on a
-    jittered lattice (spacing 1.05, jitter 0.02, 57x57 per layer) in a 60x60x200 box, released
-    at rest; mu = 0.5, 50 sweeps, dt = 5e-4."""
+    jittered lattice (spacing 1.05, jitter 0.02, per x per sites a layer) in a 60x60x200 box,
+    released at rest; mu = 0.5, 50 sweeps, dt = 5e-4."""
     rng = np.random.default_rng(12345 + 6)
-    per = 57
     nz = (n_particles + per * per - 1) // (per * per)
     pos = lattice(per, per, nz, 1.05, (0.525, 0.525, 0.525), 0.02, rng, count=n_particles)
     unions = [(pos[k], granular_particle_spheres(rng, radius, 0)) for k in range(n_particles)]

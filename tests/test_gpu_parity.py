@@ -707,3 +707,31 @@ def test_invalid_time_step_and_order():
     w.setContactOrder([0], [1])
     with pytest.raises(PeHipError):
         w.simulationStep(sc.dt)
+
+
+def test_granular_pile_contacts_and_colour_step():
+    # 1,500 granular particles (unions of 2-4 spheres, 12x12 a layer) after 1,200 oracle steps of
+    # falling and piling: the GPU's contact set equals pe's HashGrids and ExhaustiveSearch bit for
+    # bit, and one coloured step equals Gauss-Seidel in the same order on the oracle (correctly
+    # rounded trig for that step) to 1e-12
+    sc = scenes.config_granular(1500, per=12)
+    ow = oracle_for(sc, iters=20)
+    ow.step(sc.dt, 1200)
+    ow.L.orc_set_trig_mode(ow.h, 1)
+    st0 = ow.get_state()
+    w = gpu_for(sc, iters=20, exact_trig=True)
+    w.setState(st0)
+    g = contact_map(w.detect())
+    ex = contact_map(ow.detect(exhaustive=True))
+    assert sum(len(v) for v in ex.values()) > 150
+    assert g == ex and g == contact_map(ow.detect(exhaustive=False))
+    w.simulationStep(sc.dt)
+    g1 = w.getState()
+    idx, offs = w.sweepOrder()
+    gc = w.contacts()
+    ow.set_state(st0)
+    ow.set_sweep_order_ranked(gc["b1"][idx], gc["b2"][idx], pair_ranks(gc)[idx])
+    ow.step(sc.dt)
+    o1 = ow.get_state()
+    for k in ("v", "w", "pos", "q", "mgpos"):
+        assert rel_err(g1[k], o1[k]) <= 1e-12, k
