@@ -281,6 +281,8 @@ def main():
                          "error / 2h, so the solves run close to the floating-point fixed point")
     ap.add_argument("--c5-adjoint-sweeps", type=int, default=20000, help="c5: adjoint sweep cap per step")
     ap.add_argument("--c5-adjoint-tol", type=float, default=1e-15, help="c5: adjoint sweep threshold")
+    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="sweep schedule: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
@@ -320,6 +322,7 @@ def main():
         sc.to_world(w)
         w.setMaxIterations(sc.max_iterations)
         w.setRelaxationModel(sc.model)
+        w.setSweepSchedule(args.schedule)
         return w
 
     if args.warmup:
