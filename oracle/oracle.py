@@ -41,15 +41,17 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def lib(path=None):
+    """The oracle library (oracle/_build/liboracle.so), or a variant built from the same source
+    at `path` (the tests' planted-mistake builds)."""
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         sig = {
             "orc_create": (vp, []), "orc_destroy": (None, [vp]),
@@ -75,6 +77,7 @@ def lib():
             "orc_get_state": (None, [vp, _D, _D, _D, _D, _D, _D]),
             "orc_set_state": (None, [vp, _D, _D, _D, _D, _D, _D]),
             "orc_get_body_info": (None, [vp, _I32, _U64, _I32, _I32, _D, _D, _D]),
+            "orc_set_forces": (None, [vp, _D, _D]),
             "orc_get_aabb": (None, [vp, _D]),
             "orc_step": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int]),
             "orc_num_contacts": (ctypes.c_size_t, [vp]),
@@ -95,8 +98,8 @@ def lib():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a, typ=_D):
@@ -107,8 +110,8 @@ class OracleWorld:
     """One pe world restated on the CPU (HashGrids + MaxContacts + HCSITS)."""
 
     def __init__(self, gravity=(0.0, 0.0, -9.81), max_iterations=100,
-                 model=INELASTIC_COULOMB_DECOUPLING, erp=0.7, abort_threshold=0.0, cr_trig=False):
-        self.L = lib()
+                 model=INELASTIC_COULOMB_DECOUPLING, erp=0.7, abort_threshold=0.0, cr_trig=False, lib_path=None):
+        self.L = lib(lib_path)
         self.h = ctypes.c_void_p(self.L.orc_create())
         self.L.orc_set_gravity(self.h, *gravity)
         self.L.orc_set_max_iterations(self.h, max_iterations)
@@ -187,6 +190,13 @@ class OracleWorld:
         self.L.orc_set_state(self.h, *[_p(a) for a in arrs])
         if "mgpos" in st and self.L.orc_num_members(self.h):
             self.set_members(st["mgpos"])
+
+    def set_forces(self, force, torque):
+        """Force and torque per body (world frame) for the next step only."""
+        f = np.ascontiguousarray(force, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(torque, np.float64).reshape(-1, 3)
+        assert f.shape[0] == self.n and t.shape[0] == self.n
+        self.L.orc_set_forces(self.h, _p(f), _p(t))
 
     def body_info(self):
         n = self.n

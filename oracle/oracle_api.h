@@ -84,6 +84,9 @@ void orc_get_state(const orc_world* w, double* pos, double* q, double* v, double
  * planes normal/d are taken as given (they carry the libm history of the reference). */
 void orc_set_state(orc_world* w, const double* pos, const double* q, const double* v,
                    const double* om, const double* normal, const double* d);
+/* Force and torque on every body (world frame, 3 per body) for the next step only
+ * (RigidBody::addForce/addTorque; reset after the step as Settings::forceReset says). */
+void orc_set_forces(orc_world* w, const double* force, const double* torque);
 void orc_get_body_info(const orc_world* w, int* shape, uint64_t* sid, int* mat, int* fixed,
                        double* param /* 3 per body: radius | lengths | 0 */,
                        double* inv_mass, double* inv_inertia_body /* 9 per body */);

@@ -20,6 +20,14 @@
 
 #include <quadmath.h>
 
+// ORC_PLANT selects a deliberate mistake, compiled only by the tests that show their checks catch
+// it (tests/test_oracle.py builds such variants into a temporary directory): 0 = none (always the
+// case for oracle/_build), 1 = gyroscopic term with the cross product's operands swapped,
+// 2 = world-frame inertia as R^T I R instead of R I R^T, 3 = gyroscopic term left out.
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------------------------------------
@@ -226,8 +234,13 @@ struct Body {
   bool finite() const { return shape != ORC_PLANE; }
 
   // RigidBody::getInvInertia / getInertia, RigidBody.h:842-869
+#if ORC_PLANT == 2
+  M3 invInertiaWF() const { return trans(R) * Iinv * R; }
+  M3 inertiaWF() const { return trans(R) * I * R; }
+#else
   M3 invInertiaWF() const { return R * Iinv * trans(R); }
   M3 inertiaWF() const { return R * I * trans(R); }
+#endif
   // RigidBody::velFromWF, RigidBody.h:1080-1085
   V3 velFromWF(const V3& p) const { return v + cross(w, p - gpos); }
 
@@ -1379,7 +1392,13 @@ void orc_world::resolveContacts(double dt) {
     initializeVelocityCorrections(b, dv_[j], dw_[j], dt);
     if (b.awake && !b.fixed) {
       v_[j] = b.v + gravity * dt;
+#if ORC_PLANT == 1
+      w_[j] = b.w + dt * (b.invInertiaWF() * cross(b.w, b.inertiaWF() * b.w));
+#elif ORC_PLANT == 3
+      w_[j] = b.w;
+#else
       w_[j] = b.w + dt * (b.invInertiaWF() * cross(b.inertiaWF() * b.w, b.w));
+#endif
     } else {
       v_[j] = b.v;
       w_[j] = b.w;
@@ -2080,6 +2099,18 @@ void orc_set_state(orc_world* w, const double* pos, const double* q, const doubl
       b.d = d[i];
     }
     w->calcBoundingBox(b);  // a union's members keep their positions (orc_set_members moves them)
+  }
+}
+
+// RigidBody::addForce / addTorque (RigidBody.h:262-274): applied at the next step's body caching, then
+// reset (Settings::forceReset, Settings.cpp:46).
+void orc_set_forces(orc_world* w, const double* force, const double* torque) {
+  for (size_t i = 0; i < w->bodies.size(); ++i) {
+    Body& b = w->bodies[i];
+    for (int k = 0; k < 3; ++k) {
+      b.force[k] = force[3 * i + k];
+      b.torque[k] = torque[3 * i + k];
+    }
   }
 }
 
