@@ -2,13 +2,15 @@
 
 Workload (default): the polydisperse granular pile -- 200,000 spheres, radius U[0.25,0.5],
 jittered lattice (spacing 1.05) in a 60x60x200 m plane box, v0 ~ U[-0.5,0.5]^3, mu 0.5, 50 NBGS
-sweeps of InelasticCoulombContactByDecoupling, dt 5e-4, FP64 -- run as the whole job: steps
-0..2000 from the initial state.  A "step" is one full time step (detection, colouring, contact
-caching, 50 sweeps, integration) with all state resident in HBM.  Prints one JSON line.
+sweeps of InelasticCoulombContactByDecoupling, dt 5e-4, FP64, a 2,000-step job.  A "step" is one
+full time step (detection, colouring, contact caching, 50 sweeps, integration) with all state
+resident in HBM.  Prints one JSON line.
 
-Warmup steps run on a separate copy of the scene (same shapes) before the timed job.  The CPU
-baseline times the oracle (single-threaded restatement of pe's step) from the GPU states at the
-start, middle and end of the timed job and integrates the per-step times over the job.
+Timed window: the job's last --steps steps (default: the whole job).  The steps before the
+window run untimed, the last --warmup of them as warmup; the window ends with the job, where the
+pile holds the most contacts.  A whole-job window warms up on a separate copy of the scene.  The
+CPU baseline times the oracle (single-threaded restatement of pe's step) from the GPU states at
+the window's start and end (and middle, for the whole job).
 """
 import argparse
 import json
@@ -50,9 +52,10 @@ def cpu_step_time(sc, state, seconds, min_steps=1):
     return (time.perf_counter() - t0) / n, n
 
 
-def cpu_baseline(sc, states, steps, seconds):
-    """Oracle steps/s over the whole job: per-step times at the start, middle and end of the job
-    (from the GPU states there), integrated with the trapezoid rule over the job's steps."""
+def cpu_baseline(sc, states, steps, seconds, whole, first, job):
+    """Oracle steps/s over the timed steps.  Whole job: per-step times at its start, middle and end
+    (from the GPU states there) integrated with the trapezoid rule.  A window: the mean of the
+    per-step times from the GPU states at its start and end."""
     per = seconds / len(states)
     ts, ns = [], []
     for st in states:
@@ -60,35 +63,77 @@ def cpu_baseline(sc, states, steps, seconds):
         ts.append(t)
         ns.append(n)
         progress("cpu baseline: %d oracle steps, %.3f s/step" % (n, t))
-    total = steps * (ts[0] + 2 * ts[1] + ts[2]) / 4.0
+    if whole:
+        total = steps * (ts[0] + 2 * ts[1] + ts[2]) / 4.0
+        where = ("from the GPU states at steps 0, %d and %d of the job (%.3f, %.3f, %.3f s/step), integrated over the "
+                 "%d-step job (trapezoid rule)" % (steps // 2, steps, ts[0], ts[1], ts[2], steps))
+    else:
+        total = steps * (ts[0] + ts[1]) / 2.0
+        where = ("from the GPU states at steps %d and %d, the start and end of the timed window (%.3f, %.3f s/step), "
+                 "mean over the window's %d steps" % (first, job, ts[0], ts[1], steps))
     return {"value": steps / total, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": "oracle/pe_oracle.cpp (g++ -O3, no FMA, 1 thread) timed for %s steps from the GPU states at "
-                      "steps 0, %d and %d of the job (%.3f, %.3f, %.3f s/step), integrated over the %d-step job "
-                      "(trapezoid rule)" % ("/".join(str(n) for n in ns), steps // 2, steps, ts[0], ts[1], ts[2],
-                                           steps)}
+            "sample": "oracle/pe_oracle.cpp (g++ -O3, no FMA, 1 thread) timed for %s steps %s"
+                      % ("/".join(str(n) for n in ns), where)}
 
 
-def batched_c4(torch):
+def scene_state(scs, st, k):
+    """Slice of a whole-batch state that belongs to scene k (scenes stored back to back)."""
+    off = np.concatenate([[0], np.cumsum([sc.n_bodies for sc in scs])])
+    return {key: v[off[k]:off[k + 1]].copy() for key, v in st.items()}
+
+
+def _oracle_scene_steps(job):
+    """CPU worker: oracle time per step of the given batched scenes from their states (first
+    step untimed), until the time budget is spent.  Returns (seconds, steps) per scene."""
+    scs_idx, states, seconds = job
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from pe_amd import scenes
+    scs, _ = scenes.config4(4096)
+    out = []
+    per = seconds / len(scs_idx)
+    for k, st in zip(scs_idx, states):
+        sc = scs[k]
+        ow = O.OracleWorld(gravity=sc.gravity, max_iterations=40, model=3)
+        sc.to_oracle(ow)
+        ow.set_state(st)
+        ow.step(sc.dt, 1)
+        n, t0 = 0, time.perf_counter()
+        while n < 2 or time.perf_counter() - t0 < per:
+            ow.step(sc.dt, 1)
+            n += 1
+        out.append(((time.perf_counter() - t0) / n, n))
+    return out
+
+
+def batched_c4(torch, args):
     """Batched scenes (4,096 independent sphere scenes of 50-2,000 spheres, 40 sweeps, 300 steps,
     per-scene friction): the persistent batch kernel replayed from a hipGraph, against the
     single-world path stepping one scene at a time (sampled: 8 random scenes, 10 steps each from
-    the batch's state at step 150, extrapolated to 300 steps x 4,096 scenes)."""
+    the batch's state at step 150, extrapolated to 300 steps x 4,096 scenes), and against the
+    oracle on the host's cores (a seeded sample of scenes from the same state, 1 process and one
+    process per available CPU up to 16)."""
     from pe_amd import World, scenes
     from pe_amd.batch import BatchWorld
     scs, mus = scenes.config4(4096)
     bw = BatchWorld(scs, scene_friction=(mus, mus))
     tun = bw.autotune(trial_steps=3)
     bw.setTuning(use_graph=1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    bw.run(150)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    mid = bw.getState()
-    t2 = time.perf_counter()
-    bw.run(150)
-    torch.cuda.synchronize()
-    tb = (t1 - t0) + (time.perf_counter() - t2)
+    nb_all = int(sum(sc.n_bodies for sc in scs))
+    # 300 steps in chunks of 10: the last step of each chunk's contact count and sweep bytes
+    # stand for the chunk's steps in the algorithmic-bytes sum
+    chunk, tb, mid, sum_bytes, sum_contacts = 10, 0.0, None, 0.0, 0.0
+    for k in range(0, 300, chunk):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bw.run(chunk)
+        torch.cuda.synchronize()
+        tb += time.perf_counter() - t0
+        s = bw.stats()
+        sum_bytes += chunk * (s["sweep_bytes"] + 776.0 * nb_all + 616.0 * s["num_contacts"])
+        sum_contacts += chunk * s["num_contacts"]
+        if k + chunk == 150:
+            mid = bw.getState()
     progress("batched: 4096 scenes x 300 steps in %.2f s" % tb)
     st = bw.stats()
     bw.close()
@@ -100,7 +145,7 @@ def batched_c4(torch):
         sc.to_world(w)
         w.setMaxIterations(40)
         w.setRelaxationModel(3)
-        w.setState(bw.sceneState(mid, int(k)))
+        w.setState(scene_state(scs, mid, int(k)))
         w.run(2, sc.dt)
         torch.cuda.synchronize()
         a = time.perf_counter()
@@ -109,13 +154,58 @@ def batched_c4(torch):
         per.append((time.perf_counter() - a) / 10)
         w.close()
     unb = 1.0 / (float(np.mean(per)) * 300)
-    return {"workload": "4096 independent scenes, log-uniform 50-2000 spheres r 0.5 in 10x10x40 m plane boxes, "
-                        "per-scene mu U[0.2,0.8], 40 NBGS sweeps, dt 1e-3, 300 steps",
-            "scenes_per_s": 4096 / tb, "scene_steps_per_s": 4096 * 300 / tb, "seconds": tb,
-            "unbatched_scenes_per_s": unb, "batched_over_unbatched": (4096 / tb) / unb,
-            "tuning": tun, "contacts_last_step": int(st["num_contacts"]),
-            "sample": "unbatched: single-world path, one scene at a time, 8 random scenes x 10 steps from the "
-                      "batch state at step 150 (%.3f ms/step mean), extrapolated" % (1e3 * float(np.mean(per)))}
+    out = {"workload": "4096 independent scenes, log-uniform 50-2000 spheres r 0.5 in 10x10x40 m plane boxes, "
+                       "per-scene mu U[0.2,0.8], 40 NBGS sweeps, dt 1e-3, 300 steps",
+           "scenes_per_s": 4096 / tb, "scene_steps_per_s": 4096 * 300 / tb, "seconds": tb,
+           "unbatched_scenes_per_s": unb, "batched_over_unbatched": (4096 / tb) / unb,
+           "tuning": tun, "contacts_last_step": int(st["num_contacts"]),
+           "mean_contacts": sum_contacts / 300,
+           "roofline": {"bound": "hbm", "achieved": sum_bytes / tb / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": sum_bytes / tb / 1e9 / HBM_PEAK_GBS, "bytes_per_step": sum_bytes / 300,
+                        "formula": "per scene and step K*(296*Nc + 200*(2*Nc - Nfixed)) + 776*Nb + 616*Nc, contact "
+                                   "counts sampled every 10 steps, / wall time of the 300 steps"},
+           "sample": "unbatched: single-world path, one scene at a time, 8 random scenes x 10 steps from the "
+                     "batch state at step 150 (%.3f ms/step mean), extrapolated" % (1e3 * float(np.mean(per)))}
+    if not args.no_cpu:
+        out["cpu_baseline"] = batched_cpu(scs, mid, args.batched_cpu_seconds)
+    return out
+
+
+def batched_cpu(scs, mid, seconds):
+    """Oracle scenes/s over the 4,096 scenes: 32 seeded scenes (the smallest and the largest among
+    them) timed from the batch state at step 150; per-scene step times scaled by body count to the
+    whole batch.  One process, then one process per available CPU (up to 16) running the sample
+    concurrently, each on its own share of the scenes."""
+    import multiprocessing as mp
+    rng = np.random.default_rng(4096)
+    sizes = np.array([sc.n_bodies for sc in scs])
+    idx = sorted(set(rng.choice(len(scs), 30, replace=False).tolist()) | {int(sizes.argmin()), int(sizes.argmax())})
+    states = [scene_state(scs, mid, k) for k in idx]
+    res = _oracle_scene_steps((idx, states, seconds))
+    # per-step time per body of the sample, scaled to the batch's bodies: seconds per batch step
+    t_sample = sum(t for t, _ in res)
+    b_sample = sum(int(sizes[k]) for k in idx)
+    t_batch_step = t_sample * float(sizes.sum()) / b_sample
+    one = 1.0 / (300 * t_batch_step)
+    model, ncpu = host_cpu()
+    P = max(1, min(16, ncpu or 1))
+    parts = [(idx[j::P], states[j::P], seconds) for j in range(P)]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(P) as pool:
+        outs = pool.map(_oracle_scene_steps, parts)
+    wall = time.perf_counter() - t0
+    # each process steps its share of the sample; the processes ran concurrently, so the
+    # all-cores time per batch step is the slowest share's, scaled by the share's body count
+    t_share = max(sum(t for t, _ in o) * float(sizes.sum()) / max(1, sum(int(sizes[k]) for k in part[0]))
+                  for o, part in zip(outs, parts) if part[0]) / P
+    allc = 1.0 / (300 * t_share)
+    progress("batched cpu baseline: 1 core %.4f scenes/s, %d processes %.4f scenes/s (%.1f s)" % (one, P, allc, wall))
+    return {"value": one, "unit": "scenes/s", "cores": 1, "kind": "port", "all_cores_value": allc,
+            "all_cores": P, "host_cpu": model, "host_cpus_available": ncpu,
+            "sample": "oracle/pe_oracle.cpp (1 thread per process) on %d seeded scenes (%d-%d bodies) from the batch "
+                      "state at step 150, about %.0f s each way, per-step time scaled by body count to the 4,096 "
+                      "scenes x 300 steps; all cores: %d processes, each on its share of the sample"
+                      % (len(idx), int(sizes[idx].min()), int(sizes[idx].max()), seconds, P)}
 
 
 def step_signature(w):
@@ -260,15 +350,57 @@ def load_state(path):
         return {k: z[k] for k in z.files}
 
 
+def host_cpu():
+    """Model name of the host CPU and the number of CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return model, ncpu
+
+
+def step_bytes(nb, nc, sweep_bytes):
+    """Algorithmic HBM bytes of one time step: the sweeps' bytes (296 per contact + 200 per
+    non-fixed contact end, per sweep) + 776 per body (predictor 256 read + 144 written,
+    integration 152 + 152, broadphase 72) + 616 per contact (pair 8, narrowphase body reads 64,
+    raw contact 56 + 56, setup body reads 160, contact record 272)."""
+    return sweep_bytes + 776.0 * nb + 616.0 * nc
+
+
+WORKLOAD_TEXT = {
+    "c2": "polydisperse granular pile, %(n)d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, v0~U[-0.5,0.5]^3, "
+          "mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), dt 5e-4, %(job)d-step job",
+    "granular": "50000 granular particles (unions of 2-4 spheres, r 0.3-0.4 in a 0.5 bounding sphere) on a 57x57 "
+                "jittered lattice in a 60x60x200 m plane box, released at rest, mu 0.5, 50 NBGS sweeps, dt 5e-4, "
+                "%(job)d-step job",
+    "c3": "mixed shapes, 950k spheres r~U[0.2,0.4] + 50k boxes edges U[0.3,0.8] (Shoemake orientations) filling a "
+          "V-channel hopper (planes at +-30 deg, 0.8 m gap) on a 0.85 m jittered lattice, poured at v0 = (0,0,-10) + "
+          "U[-1,1]^3 m/s, mu 0.6, 30 NBGS sweeps, dt 2e-4, %(job)d-step job",
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps: the last STEPS steps of the job (default: the whole job)")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps just before the timed window (on a separate copy of the scene when the "
+                         "window is the whole job)")
+    ap.add_argument("--job-steps", type=int, default=None, help="length of the job (default: the workload's)")
     ap.add_argument("--spheres", type=int, default=200_000)
     ap.add_argument("--workload", choices=["c2", "c3", "c5", "granular"], default="c2",
                     help="c2: the polydisperse pile (default); c3: 950k spheres + 50k boxes in a V-channel hopper, "
-                         "30 sweeps, dt 2e-4 (use --steps 1000 for the whole job)")
+                         "30 sweeps, dt 2e-4; granular: 50k unions of spheres; c5: the differentiable step")
     ap.add_argument("--cpu-seconds", type=float, default=24.0)
     ap.add_argument("--fd-coords", type=int, default=64, help="c5: random v0 coordinates checked by central differences")
     ap.add_argument("--fd-contact-coords", type=int, default=16,
@@ -286,8 +418,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
-    ap.add_argument("--load-state", default=None, help="start the timed steps from this saved state (.npz)")
-    ap.add_argument("--save-state", default=None, help="save the state after the timed steps (.npz)")
+    ap.add_argument("--batched-cpu-seconds", type=float, default=12.0,
+                    help="oracle seconds per CPU worker for the batched scenes' CPU baseline")
+    ap.add_argument("--save-state", default=None, help="save the state at the end of the timed window (.npz)")
+    ap.add_argument("--save-window-start", default=None, help="save the state at the start of the timed window (.npz)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="before the run, 4 launches of a plain elementwise kernel that reads and writes 256 MiB "
                          "(scale reference for hardware byte counters)")
@@ -309,6 +443,10 @@ def main():
 
     sc = (scenes.config2(args.spheres) if args.workload == "c2" else
           scenes.config_granular() if args.workload == "granular" else scenes.config3())
+    job = args.job_steps or sc.steps
+    K = job if args.steps is None else args.steps
+    W = args.warmup
+    whole = K >= job
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -325,88 +463,115 @@ def main():
         w.setSweepSchedule(args.schedule)
         return w
 
-    if args.warmup:
-        ww = make_world()
-        ww.run(args.warmup, sc.dt)
-        torch.cuda.synchronize()
-        ww.close()
-        progress("%d warmup steps done" % args.warmup)
     w = make_world()
-    if args.load_state:
-        w.setState(load_state(args.load_state))
+    if whole:
+        # the window is the whole job from the initial state: warm up on a separate copy
+        first = 0
+        if W:
+            ww = make_world()
+            ww.run(W, sc.dt)
+            torch.cuda.synchronize()
+            ww.close()
+            progress("%d warmup steps done (separate copy of the scene)" % W)
+    else:
+        # the window is the job's last K steps: advance the job untimed to its start
+        first = job - K
+        pre = max(0, first - W)
+        t0 = time.perf_counter()
+        for k in range(0, pre, 250):
+            w.run(min(250, pre - k), sc.dt)
+            progress("advanced to step %d of %d (untimed)" % (min(k + 250, pre), job))
+        if first - pre:
+            w.run(first - pre, sc.dt)
+        torch.cuda.synchronize()
+        progress("steps 0..%d advanced untimed in %.1f s (the last %d as warmup)"
+                 % (first, time.perf_counter() - t0, first - pre))
     state0 = w.getState()
+    if args.save_window_start:
+        save_state(args.save_window_start, state0)
     if not args.no_profile:
         w.setProfiling(True)
-    contacts, colours, solve_ms = [], [], []
-    half = args.steps // 2
+    contacts, colours, solve_ms, sweep_bytes = [], [], [], []
+    half = K // 2
     t = 0.0
     state_mid = None
-    for seg in (half, args.steps - half):
+    for seg in (half, K - half):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(seg):
             w.run(1, sc.dt)
-            s = w.stats()
+            s = w.stats(kinetic=False)      # host-side record of the step: no launch, no sync
             contacts.append(s["num_contacts"])
             colours.append(s["num_colours"])
             solve_ms.append(s["t_solve_ms"])
+            sweep_bytes.append(s["sweep_bytes"])
         torch.cuda.synchronize()
         t += time.perf_counter() - t0
         if state_mid is None:
             state_mid = w.getState()           # between the two timed halves
-        progress("%d of %d timed steps done" % (len(contacts), args.steps))
+        progress("%d of %d timed steps done" % (len(contacts), K))
     st = w.stats()
     state_end = w.getState()
+    nbodies = int(w.size)
     if args.save_state:
         save_state(args.save_state, state_end)
+    w.close()                                  # every HIP object released before the next phase
+    torch.cuda.synchronize()
 
-    steps_per_s = args.steps / t
+    steps_per_s = K / t
     ci = float(np.sum(contacts)) * sc.max_iterations
     solve_s = float(np.sum(solve_ms)) * 1e-3
+    window = ("whole job: steps 0..%d from the initial state" % K if whole else
+              "timed window: steps %d..%d, the last %d steps of the %d-step job (steps 0..%d advanced untimed, "
+              "the last %d of them as warmup)" % (first, job, K, job, first, min(W, first)))
     out = {
-        "metric": METRIC, "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": steps_per_s, "unit": "steps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": 1e3 * t / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": (("polydisperse granular pile, %d spheres r~U[0.25,0.5] in a 60x60x200 m plane box, "
-                                 "v0~U[-0.5,0.5]^3, mu 0.5, 50 NBGS sweeps (InelasticCoulombContactByDecoupling), "
-                                 "dt 5e-4" % args.spheres) if args.workload == "c2" else
-                                ("50000 granular particles (unions of 2-4 spheres, r 0.3-0.4 in a 0.5 bounding "
-                                 "sphere) on a 57x57 jittered lattice in a 60x60x200 m plane box, released at rest, "
-                                 "mu 0.5, 50 NBGS sweeps, dt 5e-4") if args.workload == "granular" else
-                                ("mixed shapes, 950k spheres r~U[0.2,0.4] + 50k boxes edges U[0.3,0.8] (Shoemake "
-                                 "orientations) filling a V-channel hopper (planes at +-30 deg, 0.8 m gap) on a "
-                                 "0.85 m jittered lattice, poured at v0 = (0,0,-10) + U[-1,1]^3 m/s, mu 0.6, 30 NBGS sweeps, "
-                                 "dt 2e-4")) +
-                               ", whole job: steps 0..%d%s" % (args.steps, " from a saved state" if args.load_state else ""),
-                   "bodies": int(w.size), "sweeps_per_step": sc.max_iterations, "dt": sc.dt,
+        "config": {"workload": (WORKLOAD_TEXT[args.workload] % {"n": args.spheres, "job": job}) + "; " + window,
+                   "bodies": nbodies, "sweeps_per_step": sc.max_iterations, "dt": sc.dt,
+                   "window_first_step": first, "window_steps": K, "job_steps": job,
                    "parallelism": "single GPU"},
-        "contact_iterations_per_s": ci / solve_s if solve_s > 0 else None,
+        "contact_iterations_per_s": ci / solve_s if (solve_s > 0 and ci > 0) else None,
         "mean_contacts": float(np.mean(contacts)), "max_contacts": int(np.max(contacts)),
         "mean_colours": float(np.mean(colours)),
     }
+    if not any(contacts):
+        out["value_note"] = "no contacts in the timed window: the steps did no NBGS work"
     if not args.no_profile and st["prof_sweep_launches"]:
         avg_ms = st["prof_sweep_kernel_ms"] / st["prof_sweep_launches"]
         bytes_per_launch = st["prof_sweep_bytes"] / st["prof_sweep_launches"]
         achieved = st["prof_sweep_bytes"] / (st["prof_sweep_kernel_ms"] * 1e-3) / 1e9
         kernel = ("k_sweep_dataflow<3, TrigFast, 6>" if st["schedule_used"] == SCHEDULE_DATAFLOW
                   else "k_sweep<3> (one colour of one NBGS sweep)")
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE) and args.workload == "c2" and not args.load_state:
+        traffic, traffic_src = None, None
+        if os.path.exists(TRAFFIC_FILE) and args.workload == "c2" and args.spheres == 200_000:
             with open(TRAFFIC_FILE) as f:
                 tf = json.load(f)
-            if tf.get("kernel") == kernel:
+            if tf.get("kernel") == kernel and tf.get("window") == [first, job]:
                 traffic = tf.get("hbm_bytes_per_launch")
+                traffic_src = tf.get("source")
         out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                           "traffic_source": traffic_src or "not measured for this window",
                            "kernel": kernel + (" (all %d NBGS sweeps of one step)" % sc.max_iterations
                                                if st["schedule_used"] == SCHEDULE_DATAFLOW else ""),
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
-                           "launches": int(st["prof_sweep_launches"])}
-    if not args.no_batched and args.workload == "c2" and not args.load_state:
-        w.close()
-        out["batched"] = batched_c4(torch)
+                           "launches": int(st["prof_sweep_launches"]),
+                           "timing": "HIP events around every sweep launch on the context stream"}
+    total_bytes = sum(step_bytes(nbodies, c, b) for c, b in zip(contacts, sweep_bytes))
+    out["step_roofline"] = {"bound": "hbm", "achieved": total_bytes / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": total_bytes / t / 1e9 / HBM_PEAK_GBS, "bytes_per_step": total_bytes / K,
+                            "formula": "sum over the timed steps of K*(296*Nc + 200*(2*Nc - Nfixed)) + 776*Nb + 616*Nc "
+                                       "bytes / wall time of the window"}
+    if not args.no_batched and args.workload == "c2" and args.spheres == 200_000:
+        out["batched"] = batched_c4(torch, args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, [state0, state_mid, state_end], args.steps, args.cpu_seconds)
+        model, ncpu = host_cpu()
+        states = [state0, state_mid, state_end] if whole else [state0, state_end]
+        out["cpu_baseline"] = cpu_baseline(sc, states, K, args.cpu_seconds, whole, first, job)
+        out["cpu_baseline"]["host_cpu"] = model
+        out["cpu_baseline"]["host_cpus_available"] = ncpu
     print(json.dumps(out))
 
 

@@ -62,6 +62,18 @@ struct Quat {
     v[0] = c; v[1] = s * axis[0]; v[2] = s * axis[1]; v[3] = s * axis[2];
   }
   real operator[](size_t i) const { return v[i]; }
+  // Quaternion::toRotationMatrix (Quaternion.h:591-602), row-major
+  void toRotationMatrix(real* m) const {
+    m[0] = real(1) - real(2) * v[2] * v[2] - real(2) * v[3] * v[3];
+    m[1] = real(2) * (v[1] * v[2] - v[0] * v[3]);
+    m[2] = real(2) * (v[1] * v[3] + v[0] * v[2]);
+    m[3] = real(2) * (v[1] * v[2] + v[0] * v[3]);
+    m[4] = real(1) - real(2) * v[1] * v[1] - real(2) * v[3] * v[3];
+    m[5] = real(2) * (v[2] * v[3] - v[0] * v[1]);
+    m[6] = real(2) * (v[1] * v[3] - v[0] * v[2]);
+    m[7] = real(2) * (v[2] * v[3] + v[0] * v[1]);
+    m[8] = real(1) - real(2) * v[1] * v[1] - real(2) * v[2] * v[2];
+  }
 };
 
 typedef size_t MaterialID;
@@ -365,9 +377,17 @@ inline const Quat& RigidBody::getQuaternion() const {
 }
 inline real RigidBody::getMass() const { return detail::E().mass[index_]; }
 inline real RigidBody::getInvMass() const { return detail::E().inv_mass[index_]; }
+// Union::setPosition (Union.cpp:251-272): every member moves by the union's displacement
+// (Sphere::update(dp), Sphere.cpp:1031).
 inline void RigidBody::setPosition(const Vec3& p) {
   detail::Engine& e = detail::E();
   e.sync_host();
+  if (e.shape[index_] == unionType) {
+    const real dp[3] = {p[0] - e.pos[3 * index_], p[1] - e.pos[3 * index_ + 1], p[2] - e.pos[3 * index_ + 2]};
+    for (size_t m = 0; m < e.m_owner.size(); ++m)
+      if (e.m_owner[m] == index_)
+        for (int k = 0; k < 3; ++k) e.m_gpos[3 * m + k] = e.m_gpos[3 * m + k] + dp[k];
+  }
   for (int k = 0; k < 3; ++k) e.pos[3 * index_ + k] = p[k];
   e.state_dirty = true;
 }
@@ -385,10 +405,22 @@ inline void RigidBody::setAngularVel(const Vec3& om) {
   for (int k = 0; k < 3; ++k) e.w[3 * index_ + k] = om[k];
   e.state_dirty = true;
 }
+// Union::setOrientation (Union.cpp:306-322): every member goes to pos + R rpos
+// (Sphere::update(dq), Sphere.cpp:1059).
 inline void RigidBody::setOrientation(const Quat& qq) {
   detail::Engine& e = detail::E();
   e.sync_host();
   for (int k = 0; k < 4; ++k) e.q[4 * index_ + k] = qq[k];
+  if (e.shape[index_] == unionType) {
+    real R[9];
+    qq.toRotationMatrix(R);
+    const real* x = &e.pos[3 * index_];
+    for (size_t m = 0; m < e.m_owner.size(); ++m)
+      if (e.m_owner[m] == index_) {
+        const real* r = &e.m_rpos[3 * m];
+        for (int k = 0; k < 3; ++k) e.m_gpos[3 * m + k] = x[k] + (R[3 * k] * r[0] + R[3 * k + 1] * r[1] + R[3 * k + 2] * r[2]);
+      }
+  }
   e.state_dirty = true;
 }
 // RigidBody::fix (src/core/rigidbody/RigidBody.cpp:402-418)

@@ -131,8 +131,6 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
                                  const double* inv_inertia_body, const double* plane_normal,
                                  const double* plane_d);
 
-/* Overwrites the dynamic state of all n bodies (checkpoint restore).  Union members keep their
- * positions (pehip_upload_members places them). */
 /* The spheres contained in the union bodies (pe's Union of Spheres, pe/core/rigidbody/Union.h;
  * createGranularParticle, Union.cpp:2376-2402), replacing Union::add (Union.cpp:1482-1545).  Called
  * after pehip_upload_bodies whenever that call holds a union; until then pehip_step and
@@ -152,6 +150,11 @@ pehip_status pehip_upload_members(pehip_ctx* ctx, uint32_t nm, const uint32_t* o
 /* Global positions of the union members (they follow their unions each step, Sphere.cpp:1052). */
 pehip_status pehip_download_members(pehip_ctx* ctx, uint32_t nm, double* gpos);
 
+/* Overwrites the dynamic state of all n bodies (checkpoint restore; RigidBody::setPosition,
+ * setOrientation, setLinearVel, setAngularVel).  The members of a union whose pose changes follow
+ * it as pe's setters move them: a new orientation places them at pos + R rpos (Union.cpp:306-322,
+ * Sphere.cpp:1059), a new position alone translates them (Union.cpp:251-272, Sphere.cpp:1031);
+ * pehip_upload_members afterwards places them explicitly. */
 pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, const double* q,
                                 const double* v, const double* w, const double* plane_normal,
                                 const double* plane_d);
@@ -219,6 +222,10 @@ typedef struct {
   int32_t reserved;
 } pehip_stats;
 pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);
+/* The same record without the kinetic-energy reduction (no kernel launch, no synchronisation):
+ * kinetic_energy keeps the value of the last pehip_get_stats.  For per-step readout in loops
+ * (CollisionSystem::getNumberOfContacts / getMaximumPenetration, HCSITS.h:715-737). */
+pehip_status pehip_get_step_stats(pehip_ctx* ctx, pehip_stats* out);
 
 /* Phase timing and per-sweep-launch timing with HIP events on the context's stream
  * (off by default); enabling resets the profiling totals. */

@@ -22,7 +22,7 @@ EXPORTED = [
     "pehip_default_params", "pehip_create", "pehip_destroy", "pehip_set_params", "pehip_get_params",
     "pehip_set_materials", "pehip_upload_bodies", "pehip_upload_members", "pehip_download_members", "pehip_upload_state", "pehip_set_forces",
     "pehip_set_contact_order", "pehip_step", "pehip_detect", "pehip_download_state", "pehip_get_contacts",
-    "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_set_profiling", "pehip_eval_trig",
+    "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_get_step_stats", "pehip_set_profiling", "pehip_eval_trig",
     "pehip_stream", "pehip_last_error", "pehip_version",
     "pehip_batch_create", "pehip_batch_destroy", "pehip_batch_set_params", "pehip_batch_set_materials",
     "pehip_batch_upload", "pehip_batch_set_scene_friction", "pehip_batch_upload_state", "pehip_batch_step",
@@ -119,6 +119,7 @@ def load():
         "pehip_get_impulses": (st, [_VP, _SZ, ctypes.POINTER(_SZ), _D]),
         "pehip_get_sweep_order": (st, [_VP, _SZ, ctypes.POINTER(_SZ), _U32, _U32, _U32]),
         "pehip_get_stats": (st, [_VP, ctypes.POINTER(Stats)]),
+        "pehip_get_step_stats": (st, [_VP, ctypes.POINTER(Stats)]),
         "pehip_set_profiling": (st, [_VP, ctypes.c_int]),
         "pehip_eval_trig": (st, [_VP, ctypes.c_int, ctypes.c_int, _SZ, _D, _D, _D]),
         "pehip_stream": (_VP, [_VP]),

@@ -124,6 +124,43 @@ static void plan_grid(pehip_ctx* ctx, const uint8_t* shape, const double* param,
   ctx->key_bits = bits;
 }
 
+// The members of every union whose pose a state upload changes follow it as pe's setters move
+// them: a new orientation places each member at pos + R rpos (Union::setOrientation ->
+// Sphere::update(dq), Union.cpp:306-322, Sphere.cpp:1059); a new position alone translates each
+// member by the union's displacement (Union::setPosition -> Sphere::update(dp), Union.cpp:251-272,
+// Sphere.cpp:1031).  Unions whose pose is unchanged keep their members' positions bit for bit.
+static pehip_status move_members(pehip_ctx* ctx, const double* pos, const double* q, const std::vector<double>& hR) {
+  const uint32_t n = ctx->n, nm = ctx->nm;
+  const size_t D = sizeof(double);
+  std::vector<double> opos(3 * (size_t)n), oq(4 * (size_t)n), mp(3 * (size_t)nm), mr(3 * (size_t)nm);
+  std::vector<uint32_t> start((size_t)n + 1);
+  CK(hipMemcpy(opos.data(), ctx->pos, 3 * D * n, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(oq.data(), ctx->q, 4 * D * n, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(mp.data(), ctx->mpos, 3 * D * nm, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(mr.data(), ctx->mrpos, 3 * D * nm, hipMemcpyDeviceToHost));
+  CK(hipMemcpy(start.data(), ctx->mstart, 4 * start.size(), hipMemcpyDeviceToHost));
+  bool moved = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (ctx->h_shape[i] != PEHIP_UNION) continue;
+    const bool rot = std::memcmp(&oq[4 * (size_t)i], &q[4 * (size_t)i], 4 * D) != 0;
+    const bool tra = std::memcmp(&opos[3 * (size_t)i], &pos[3 * (size_t)i], 3 * D) != 0;
+    if (!rot && !tra) continue;
+    moved = true;
+    m33 R;
+    for (int k = 0; k < 9; ++k) R.a[k] = hR[9 * (size_t)i + k];
+    const d3 gp = mk3(pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]);
+    const d3 dp = gp - mk3(opos[3 * (size_t)i], opos[3 * (size_t)i + 1], opos[3 * (size_t)i + 2]);
+    for (uint32_t m = start[i]; m < start[i + 1]; ++m) {
+      double* x = &mp[3 * (size_t)m];
+      const d3 y = rot ? gp + mat_vec(R, mk3(mr[3 * (size_t)m], mr[3 * (size_t)m + 1], mr[3 * (size_t)m + 2]))
+                       : mk3(x[0], x[1], x[2]) + dp;
+      x[0] = y.x; x[1] = y.y; x[2] = y.z;
+    }
+  }
+  if (moved) CK(hipMemcpy(ctx->mpos, mp.data(), 3 * D * nm, hipMemcpyHostToDevice));
+  return PEHIP_OK;
+}
+
 pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
   if (p->relaxation_model != PEHIP_INELASTIC_FRICTIONLESS &&
       p->relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
@@ -427,6 +464,7 @@ pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, c
     for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
   }
   const size_t D = sizeof(double);
+  if (ctx->nm) CKS(move_members(ctx, pos, q, hR));
   CK(hipMemcpy(ctx->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
   CK(hipMemcpy(ctx->q, q, 4 * D * n, hipMemcpyHostToDevice));
   CK(hipMemcpy(ctx->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
@@ -845,6 +883,12 @@ pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out) {
     for (double x : part) e += x;
     ctx->stats.kinetic_energy = e;
   }
+  *out = ctx->stats;
+  return PEHIP_OK;
+}
+
+pehip_status pehip_get_step_stats(pehip_ctx* ctx, pehip_stats* out) {
+  if (!ctx || !out) return PEHIP_INVALID_ARG;
   *out = ctx->stats;
   return PEHIP_OK;
 }

@@ -336,6 +336,11 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
       if (lane == 0) __hip_atomic_fetch_or(flags, ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
     }
+    // Compiler barrier: the relaxation's loads of the shared data are relaxed atomics of other
+    // addresses, which the language would let the compiler hoist above the wait loop.  The fence
+    // emits no instruction; what orders the loads at run time is the wait loop's exit, which
+    // depends on the counter's loaded value (in-order issue, sc1 loads coherent per access).
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const double d = relax_contact<MODEL, Trig, SPEC, true>(C, s, e1, e2, sb, dvw, dtinv, maxSub);
     if (lane == 0) {
       if (d > 0) atomicMax(&deltas[q], (unsigned long long)__double_as_longlong(d));

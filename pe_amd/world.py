@@ -630,11 +630,14 @@ class World:
         check(L.pehip_get_impulses(self._ctx, n.value, ctypes.byref(n), ptr(p)), self._ctx)
         return p
 
-    def stats(self):
+    def stats(self, kinetic=True):
+        """Statistics of the last step; kinetic=False skips the kinetic-energy reduction (no
+        kernel launch or synchronisation; kinetic_energy keeps its last computed value)."""
         s = _abi.Stats()
         if self._ctx is None:         # never stepped (e.g. a world without bodies): all zero
             return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
-        check(_abi.load().pehip_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
+        fn = _abi.load().pehip_get_stats if kinetic else _abi.load().pehip_get_step_stats
+        check(fn(self._ctx, ctypes.byref(s)), self._ctx)
         return {f: getattr(s, f) for f, _ in _abi.Stats._fields_}
 
     def setProfiling(self, on=True):

@@ -352,7 +352,7 @@ def test_reference_order_step_matches_oracle(name, model):
     # Model 3 searches with sin/cos/atan2/acos: against the oracle on the host libm, whose results
     # are not always correctly rounded, the union scene's velocities differ by ~1e-12; against the
     # correctly rounded oracle (the next test) every scene holds 1e-12.
-    assert max(errs.values()) <= (1e-12 if model != 3 else 5e-12), errs
+    assert max(errs.values()) <= (5e-12 if (model == 3 and name == "unions") else 1e-12), errs
     # Model 3's friction of a box standing on four corners is statically indeterminate: the
     # split of the impulse among the corners moves by ~1e-8 with the host libm's rare non
     # correctly rounded trig results, while the velocities (the bar) do not.
