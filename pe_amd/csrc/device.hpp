@@ -1018,62 +1018,45 @@ __device__ __forceinline__ void sweep_st(double* p, double v) {
   else *p = v;
 }
 
-// e1, e2: the contact's body entries C.b1[i], C.b2[i] (the caller has them already).
-// COHERENT: the whole wavefront owns contact i (the dataflow kernel).
-template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
-__device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32_t e1, uint32_t e2,
-                                       const double* __restrict__ sb, double* __restrict__ dvw, double dtinv,
-                                       int maxSub);
-template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
-__device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
-                                       double* __restrict__ dvw, double dtinv, int maxSub) {
-  return relax_contact<MODEL, Trig, SPEC, COHERENT>(C, i, C.b1[i], C.b2[i], sb, dvw, dtinv, maxSub);
-}
-template <int MODEL, class Trig, int SPEC, bool COHERENT>
-__device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32_t e1, uint32_t e2,
-                                       const double* __restrict__ sb, double* __restrict__ dvw, double dtinv,
-                                       int maxSub) {
-  const size_t cap = C.cap;
-  double* f = C.f;
-  const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
-  const bool fx1 = (e1 & kFixedBit) != 0, fx2 = (e2 & kFixedBit) != 0;
-
-  const double* s1 = sb + (size_t)kSB * i1;
-  const double* s2 = sb + (size_t)kSB * i2;
-  const d3 v1 = mk3(s1[SB_V], s1[SB_V + 1], s1[SB_V + 2]);
-  const d3 w1 = mk3(s1[SB_W], s1[SB_W + 1], s1[SB_W + 2]);
-  const d3 v2 = mk3(s2[SB_V], s2[SB_V + 1], s2[SB_V + 2]);
-  const d3 w2 = mk3(s2[SB_W], s2[SB_W + 1], s2[SB_W + 2]);
-  const double im1 = s1[SB_IM], im2 = s2[SB_IM];
-  m33 I1, I2;
+// Per-body inputs of a relaxation: predicted velocities, inverse mass, world-frame inverse inertia
+// (a solver body record, kSB doubles).
+struct RelaxBody {
+  d3 v, w;
+  double im;
+  m33 I;
+};
+__device__ __forceinline__ RelaxBody relax_body(const double* s) {
+  RelaxBody b;
+  b.v = mk3(s[SB_V], s[SB_V + 1], s[SB_V + 2]);
+  b.w = mk3(s[SB_W], s[SB_W + 1], s[SB_W + 2]);
+  b.im = s[SB_IM];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) { I1.a[k] = s1[SB_IW + k]; I2.a[k] = s2[SB_IW + k]; }
-  double* q1 = dvw + (size_t)kDVW * i1;
-  double* q2 = dvw + (size_t)kDVW * i2;
-  d3 dv1 = mk3(0, 0, 0), dw1 = mk3(0, 0, 0), dv2 = mk3(0, 0, 0), dw2 = mk3(0, 0, 0);
-  if (!fx1) {
-    dv1 = mk3(sweep_ld<COHERENT>(q1), sweep_ld<COHERENT>(q1 + 1), sweep_ld<COHERENT>(q1 + 2));
-    dw1 = mk3(sweep_ld<COHERENT>(q1 + 3), sweep_ld<COHERENT>(q1 + 4), sweep_ld<COHERENT>(q1 + 5));
-  }
-  if (!fx2) {
-    dv2 = mk3(sweep_ld<COHERENT>(q2), sweep_ld<COHERENT>(q2 + 1), sweep_ld<COHERENT>(q2 + 2));
-    dw2 = mk3(sweep_ld<COHERENT>(q2 + 3), sweep_ld<COHERENT>(q2 + 4), sweep_ld<COHERENT>(q2 + 5));
-  }
+  for (int k = 0; k < 9; ++k) b.I.a[k] = s[SB_IW + k];
+  return b;
+}
 
-  const d3 r1 = cf_ld3(f, cap, CF_R1, i), r2 = cf_ld3(f, cap, CF_R2, i);
-  const d3 n = cf_ld3(f, cap, CF_N, i), t = cf_ld3(f, cap, CF_T, i), o = cf_ld3(f, cap, CF_O, i);
-  const double ddt = f[CF_DDT * cap + i];
-  d3 p;
-  if constexpr (COHERENT) {
-    // one load instruction for the three components (lanes 0-2), then broadcast: issued one
-    // after another, the three agent-scope loads cost three memory round trips
-    const uint32_t lane = threadIdx.x & 63u;
-    double pl = 0.0;
-    if (lane < 3) pl = sweep_ld<true>(f + (CF_P + lane) * cap + i);
-    p = mk3(wave_read_lane(pl, 0), wave_read_lane(pl, 1), wave_read_lane(pl, 2));
-  } else {
-    p = mk3(f[CF_P * cap + i], f[(CF_P + 1) * cap + i], f[(CF_P + 2) * cap + i]);
-  }
+// Reader of contact i's fields in the field-major solver arrays.
+struct GlobalFields {
+  const double* f;
+  size_t cap, i;
+  __device__ __forceinline__ double operator()(int k) const { return f[(size_t)k * cap + i]; }
+};
+
+// The arithmetic of one contact's NBGS update, wherever its data live: cf(field) reads the
+// contact's fields (CF_*), b1 / b2 are its bodies, dv / dw their velocity corrections (updated in
+// place) and p its world-frame impulse (updated in place).  Returns |p_old - p_new|_inf (or
+// |p_old|_inf for a separating contact).
+template <int MODEL, class Trig, int SPEC, class CF>
+__device__ inline double relax_update(const CF& cf, const RelaxBody& b1, const RelaxBody& b2, d3& dv1, d3& dw1,
+                                      d3& dv2, d3& dw2, d3& p, int maxSub) {
+  const d3 v1 = b1.v, w1 = b1.w, v2 = b2.v, w2 = b2.w;
+  const double im1 = b1.im, im2 = b2.im;
+  const m33& I1 = b1.I;
+  const m33& I2 = b2.I;
+  const d3 r1 = mk3(cf(CF_R1), cf(CF_R1 + 1), cf(CF_R1 + 2)), r2 = mk3(cf(CF_R2), cf(CF_R2 + 1), cf(CF_R2 + 2));
+  const d3 n = mk3(cf(CF_N), cf(CF_N + 1), cf(CF_N + 2)), t = mk3(cf(CF_T), cf(CF_T + 1), cf(CF_T + 2));
+  const d3 o = mk3(cf(CF_O), cf(CF_O + 1), cf(CF_O + 2));
+  const double ddt = cf(CF_DDT);
 
   // Remove this contact's reaction (HCSITS.h:2343-2346).
   dv1 = dv1 - im1 * p;
@@ -1093,11 +1076,11 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32
     // current impulse, p_cf - (D p_cf + gdot_nto), Anitescu/Tasora (2); De Saxce/Feng (4) adds
     // mu |(D p_cf + gdot_nto)_to| to the normal component first.  pe's branches are kept as
     // written: its separating test compares fsq with p_n / mu^2 < 0 and so never holds.
-    const double mu = f[CF_MU * cap + i];
-    const double D00 = f[CF_D0 * cap + i], D01 = f[(CF_D0 + 1) * cap + i], D02 = f[(CF_D0 + 2) * cap + i];
-    const double D10 = f[(CF_D0 + 3) * cap + i], D20 = f[(CF_D0 + 4) * cap + i];
-    const double D11 = f[CF_DTO * cap + i], D12 = f[(CF_DTO + 1) * cap + i];
-    const double D21 = f[(CF_DTO + 2) * cap + i], D22 = f[(CF_DTO + 3) * cap + i];
+    const double mu = cf(CF_MU);
+    const double D00 = cf(CF_D0), D01 = cf(CF_D0 + 1), D02 = cf(CF_D0 + 2);
+    const double D10 = cf(CF_D0 + 3), D20 = cf(CF_D0 + 4);
+    const double D11 = cf(CF_DTO), D12 = cf(CF_DTO + 1);
+    const double D21 = cf(CF_DTO + 2), D22 = cf(CF_DTO + 3);
     const d3 pc = mat_tvec(cfm, p);
     d3 tmp = mk3((D00 * pc.x + D01 * pc.y + D02 * pc.z) + g.x, (D10 * pc.x + D11 * pc.y + D12 * pc.z) + g.y,
                  (D20 * pc.x + D21 * pc.y + D22 * pc.z) + g.z);
@@ -1130,12 +1113,12 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32
   } else {
     d3 pwf;
     if (MODEL == 0) {
-      pwf = n * (-f[CF_DNI * cap + i] * g.x);
+      pwf = n * (-cf(CF_DNI) * g.x);
     } else {
       m33 Dinv;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Dinv.a[k] = f[(CF_DINV + k) * cap + i];
-      const double mu = f[CF_MU * cap + i];
+      for (int k = 0; k < 9; ++k) Dinv.a[k] = cf(CF_DINV + k);
+      const double mu = cf(CF_MU);
       double pc[3];
       {
         const d3 s = -mat_vec(Dinv, g);
@@ -1144,9 +1127,9 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32
       double flimit = mu * pc[0];
       double fsq = pc[1] * pc[1] + pc[2] * pc[2];
       if (fsq > flimit * flimit || pc[0] < 0) {
-        const double dni = f[CF_DNI * cap + i];
-        const double T00 = f[(CF_DTOI + 0) * cap + i], T01 = f[(CF_DTOI + 1) * cap + i];
-        const double T10 = f[(CF_DTOI + 2) * cap + i], T11 = f[(CF_DTOI + 3) * cap + i];
+        const double dni = cf(CF_DNI);
+        const double T00 = cf(CF_DTOI + 0), T01 = cf(CF_DTOI + 1);
+        const double T10 = cf(CF_DTOI + 2), T11 = cf(CF_DTOI + 3);
         if (MODEL == 1) {
           // ApproximateInelasticCoulombContactByDecoupling, HCSITS.h:2186-2226
           d3 ptmp = dot3(t, p) * t + dot3(o, p) * o;
@@ -1173,9 +1156,9 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32
           // boundary's radius is positive (angleI) and cheaper than the unconstrained direction
           // (angleJ); the unconstrained direction replaces the result if cheaper.
           constexpr double kTwoPi = 2.0 * 3.141592653589793;   // 2 * M_PI, pe/math/Constants.h:163
-          const double D00 = f[CF_D0 * cap + i], D01 = f[(CF_D0 + 1) * cap + i], D02 = f[(CF_D0 + 2) * cap + i];
-          const double D11 = f[CF_DTO * cap + i], D12 = f[(CF_DTO + 1) * cap + i];
-          const double D22 = f[(CF_DTO + 3) * cap + i];
+          const double D00 = cf(CF_D0), D01 = cf(CF_D0 + 1), D02 = cf(CF_D0 + 2);
+          const double D11 = cf(CF_DTO), D12 = cf(CF_DTO + 1);
+          const double D22 = cf(CF_DTO + 3);
           const double invA_nn = 1.0 / D00;
           const double offdiag = D12 - invA_nn * D01 * D02;
           const double A00 = D11 - invA_nn * (D01 * D01), A11 = D22 - invA_nn * (D02 * D02);
@@ -1241,8 +1224,8 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32
           // Once c_j equals c_{j-P} bit for bit (P = 1, 2, 3) the passes repeat with period P
           // from pass j-P on, the result of pass 19 is that of pass j-P+((19-j+P) mod P), and
           // the remaining passes are skipped: the same bits as running all 20.
-          const double A00 = f[(CF_DTO + 0) * cap + i], A01 = f[(CF_DTO + 1) * cap + i];
-          const double A10 = f[(CF_DTO + 2) * cap + i], A11 = f[(CF_DTO + 3) * cap + i];
+          const double A00 = cf(CF_DTO + 0), A01 = cf(CF_DTO + 1);
+          const double A10 = cf(CF_DTO + 2), A11 = cf(CF_DTO + 3);
           double c1 = 0.0, c2 = 0.0, c3 = 0.0;  // c_{j-1}, c_{j-2}, c_{j-3}
           double u1 = 0.0, u2 = 0.0;            // t_{j-2}
           double w1 = 0.0, w2 = 0.0;            // t_{j-3}
@@ -1309,6 +1292,55 @@ __device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32
     dv2 = dv2 - im2 * p;
     dw2 = dw2 - mat_vec(I2, cross3(r2, p));
   }
+  return delta;
+}
+
+// e1, e2: the contact's body entries C.b1[i], C.b2[i] (the caller has them already).
+// COHERENT: the whole wavefront owns contact i (the dataflow kernel).
+template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
+__device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32_t e1, uint32_t e2,
+                                       const double* __restrict__ sb, double* __restrict__ dvw, double dtinv,
+                                       int maxSub);
+template <int MODEL, class Trig, int SPEC = 0, bool COHERENT = false>
+__device__ inline double relax_contact(const SolverContacts& C, size_t i, const double* __restrict__ sb,
+                                       double* __restrict__ dvw, double dtinv, int maxSub) {
+  return relax_contact<MODEL, Trig, SPEC, COHERENT>(C, i, C.b1[i], C.b2[i], sb, dvw, dtinv, maxSub);
+}
+template <int MODEL, class Trig, int SPEC, bool COHERENT>
+__device__ inline double relax_contact(const SolverContacts& C, size_t i, uint32_t e1, uint32_t e2,
+                                       const double* __restrict__ sb, double* __restrict__ dvw, double dtinv,
+                                       int maxSub) {
+  (void)dtinv;  // the contact record holds dist / dt
+  const size_t cap = C.cap;
+  double* f = C.f;
+  const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
+  const bool fx1 = (e1 & kFixedBit) != 0, fx2 = (e2 & kFixedBit) != 0;
+  const RelaxBody B1 = relax_body(sb + (size_t)kSB * i1);
+  const RelaxBody B2 = relax_body(sb + (size_t)kSB * i2);
+  double* q1 = dvw + (size_t)kDVW * i1;
+  double* q2 = dvw + (size_t)kDVW * i2;
+  d3 dv1 = mk3(0, 0, 0), dw1 = mk3(0, 0, 0), dv2 = mk3(0, 0, 0), dw2 = mk3(0, 0, 0);
+  if (!fx1) {
+    dv1 = mk3(sweep_ld<COHERENT>(q1), sweep_ld<COHERENT>(q1 + 1), sweep_ld<COHERENT>(q1 + 2));
+    dw1 = mk3(sweep_ld<COHERENT>(q1 + 3), sweep_ld<COHERENT>(q1 + 4), sweep_ld<COHERENT>(q1 + 5));
+  }
+  if (!fx2) {
+    dv2 = mk3(sweep_ld<COHERENT>(q2), sweep_ld<COHERENT>(q2 + 1), sweep_ld<COHERENT>(q2 + 2));
+    dw2 = mk3(sweep_ld<COHERENT>(q2 + 3), sweep_ld<COHERENT>(q2 + 4), sweep_ld<COHERENT>(q2 + 5));
+  }
+  d3 p;
+  if constexpr (COHERENT) {
+    // one load instruction for the three components (lanes 0-2), then broadcast: issued one
+    // after another, the three agent-scope loads cost three memory round trips
+    const uint32_t lane = threadIdx.x & 63u;
+    double pl = 0.0;
+    if (lane < 3) pl = sweep_ld<true>(f + (CF_P + lane) * cap + i);
+    p = mk3(wave_read_lane(pl, 0), wave_read_lane(pl, 1), wave_read_lane(pl, 2));
+  } else {
+    p = mk3(f[CF_P * cap + i], f[(CF_P + 1) * cap + i], f[(CF_P + 2) * cap + i]);
+  }
+  const GlobalFields cf{f, cap, i};
+  const double delta = relax_update<MODEL, Trig, SPEC>(cf, B1, B2, dv1, dw1, dv2, dw2, p, maxSub);
   // one lane of the contact's group stores (COHERENT: the whole wavefront owns the contact)
   const uint32_t group_mask = COHERENT ? 63u : ((1u << SPEC) - 1u);
   if ((threadIdx.x & group_mask) == 0) {
