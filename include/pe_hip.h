@@ -176,6 +176,19 @@ pehip_status pehip_set_contact_order(pehip_ctx* ctx, uint32_t n, const uint32_t*
  * dt <= 0 -> PEHIP_INVALID_ARG (World.cpp:333-334). */
 pehip_status pehip_step(pehip_ctx* ctx, double dt, uint32_t nsteps);
 
+/* pehip_step that allocates nothing (no counterpart in pe): for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation.  The step's per-contact arrays (raw and solver contacts, colouring, sweep order,
+ * dataflow ranks) live in one static device array of the library (12 MiB per device, held until
+ * the process ends) instead of the context's own buffers, so a step may hold at most 16,384
+ * contacts: a step that detects more returns PEHIP_INVALID_ARG before changing the state.
+ * Coloured sweep mode only, profiling off (PEHIP_NOT_SUPPORTED otherwise).  Every context on a
+ * device shares that device's array: the contact, sweep-order and impulse readbacks of a context
+ * show the last pehip_step_noalloc on the device until the context's next pehip_step, and calls
+ * that overlap on one device (from several host threads or streams) race.  Bodies, materials and
+ * all other per-body buffers are allocated beforehand by the upload calls, as for pehip_step. */
+pehip_status pehip_step_noalloc(pehip_ctx* ctx, double dt, uint32_t nsteps);
+
 /* Detection only (HashGrids::findContacts, HashGrids.h:1121-1314) on the current state,
  * without resolving or integrating; the contacts are then read with pehip_get_contacts. */
 pehip_status pehip_detect(pehip_ctx* ctx);

@@ -21,7 +21,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "CAPACITY_RETR
 EXPORTED = [
     "pehip_default_params", "pehip_create", "pehip_destroy", "pehip_set_params", "pehip_get_params",
     "pehip_set_materials", "pehip_upload_bodies", "pehip_upload_members", "pehip_download_members", "pehip_upload_state", "pehip_set_forces",
-    "pehip_set_contact_order", "pehip_step", "pehip_detect", "pehip_download_state", "pehip_get_contacts",
+    "pehip_set_contact_order", "pehip_step", "pehip_step_noalloc", "pehip_detect", "pehip_download_state", "pehip_get_contacts",
     "pehip_get_sweep_order", "pehip_get_impulses", "pehip_get_stats", "pehip_get_step_stats", "pehip_set_profiling", "pehip_eval_trig",
     "pehip_stream", "pehip_last_error", "pehip_version",
     "pehip_batch_create", "pehip_batch_destroy", "pehip_batch_set_params", "pehip_batch_set_materials",
@@ -113,6 +113,7 @@ def load():
         "pehip_set_forces": (st, [_VP, ctypes.c_uint32, _D, _D]),
         "pehip_set_contact_order": (st, [_VP, ctypes.c_uint32, _U32, _U32]),
         "pehip_step": (st, [_VP, ctypes.c_double, ctypes.c_uint32]),
+        "pehip_step_noalloc": (st, [_VP, ctypes.c_double, ctypes.c_uint32]),
         "pehip_detect": (st, [_VP]),
         "pehip_download_state": (st, [_VP, ctypes.c_uint32, _D, _D, _D, _D, _D, _D]),
         "pehip_get_contacts": (st, [_VP, _SZ, ctypes.POINTER(_SZ), _U32, _U32, _D, _D, _D, _D]),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -39,24 +40,69 @@ uint32_t next_pow2(uint64_t x) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------
+// Static scratch of pehip_step_noalloc: the per-contact arrays of a step for at most
+// kScratchContacts contacts, in a file-scope device array (one copy per device, held until the
+// process ends).  Host code never passes `scratch` itself to a kernel or a copy (in host code the
+// name is a stand-in, not a device address): pehip_create looks up the array's address on the
+// context's device once with hipGetSymbolAddress and the context uses that pointer.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kScratchContacts = 16384;
+constexpr size_t kScratchBytes = 12u << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+}  // namespace
+
+// Device address of the static scratch on `device` (the current device), looked up once.
+static pehip_status scratch_address(int device, void** out, std::string& why) {
+  if (device < 0 || device >= kMaxDevices) { why = "device ordinal beyond the scratch table"; return PEHIP_INVALID_ARG; }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return PEHIP_HIP_ERROR;
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return PEHIP_OK;
+}
+
+// The context's own contact arrays bound again after steps on the static scratch.
+static pehip_status use_own_contacts(pehip_ctx* ctx) {
+  if (!ctx->contacts_in_scratch) return PEHIP_OK;
+  CK(hipStreamSynchronize(ctx->stream));
+  ctx->unbind_contacts();
+  ctx->contacts_in_scratch = false;
+  if (ctx->cblock) {
+    ctx->carve_contacts(static_cast<char*>(ctx->cblock), ctx->own_ccap);
+    ctx->ccap = ctx->own_ccap;
+  }
+  return PEHIP_OK;
+}
+
 static pehip_status ensure_contact_capacity(pehip_ctx* ctx, size_t need) {
+  if (ctx->no_alloc) {
+    if (need <= ctx->ccap) return PEHIP_OK;
+    return fail(ctx, PEHIP_INVALID_ARG, "pehip_step_noalloc: " + std::to_string(need) +
+                                            " contacts exceed the static scratch's maximum of " +
+                                            std::to_string(kScratchContacts));
+  }
+  CKS(use_own_contacts(ctx));
   if (need <= ctx->ccap) return PEHIP_OK;
   CK(hipStreamSynchronize(ctx->stream));
   ctx->free_contacts();
-  size_t cap = std::max<size_t>(need + need / 2, 4096);
-  CK(dalloc(ctx->raw.b1, cap)); CK(dalloc(ctx->raw.b2, cap)); CK(dalloc(ctx->raw.pos, 3 * cap));
-  CK(dalloc(ctx->raw.nrm, 3 * cap)); CK(dalloc(ctx->raw.dist, cap)); CK(dalloc(ctx->raw.mu, cap));
-  CK(dalloc(ctx->sol.b1, cap)); CK(dalloc(ctx->sol.b2, cap)); CK(dalloc(ctx->sol.f, (size_t)CF_COUNT * cap));
-  CK(dalloc(ctx->col.adj, 2 * cap)); CK(dalloc(ctx->col.colour, cap)); CK(dalloc(ctx->col.prio, cap));
-  CK(dalloc(ctx->col.cround, cap));
-  CK(dalloc(ctx->perm, cap)); CK(dalloc(ctx->sort_keys, cap)); CK(dalloc(ctx->sort_keys_tmp, cap));
-  CK(dalloc(ctx->sort_vals_tmp, cap));
-  CK(dalloc(ctx->df_inv, cap)); CK(dalloc(ctx->df_rank1, cap)); CK(dalloc(ctx->df_rank2, cap));
-  CK(dalloc(ctx->cscan_scratch, scan_scratch_words(cap)));
-  CK(dalloc(ctx->csort_scratch, sort_scratch_words(cap)));
-  ctx->raw.cap = cap;
-  ctx->sol.cap = cap;
-  ctx->ccap = cap;
+  const size_t cap = std::max<size_t>(need + need / 2, 4096);
+  const size_t bytes = ctx->carve_contacts(nullptr, cap);
+  CK(hipMalloc(&ctx->cblock, bytes));
+  ctx->carve_contacts(static_cast<char*>(ctx->cblock), cap);
+  ctx->ccap = ctx->own_ccap = cap;
   return PEHIP_OK;
 }
 
@@ -236,6 +282,21 @@ pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** ou
     return PEHIP_OUT_OF_MEMORY;
   }
   for (auto& e : ctx->ev) hipEventCreate(&e);
+  ctx->colour_off.reserve(kMaxColours + 1);
+  {
+    std::string why;
+    if (ctx->carve_contacts(nullptr, kScratchContacts) > kScratchBytes) {
+      g_create_error = "static scratch smaller than the contact layout";
+      pehip_destroy(ctx);
+      return PEHIP_HIP_ERROR;
+    }
+    const pehip_status ss = scratch_address(device, &ctx->scratch, why);
+    if (ss != PEHIP_OK) {
+      g_create_error = why;
+      pehip_destroy(ctx);
+      return ss;
+    }
+  }
   *out = ctx;
   return PEHIP_OK;
 }
@@ -791,6 +852,29 @@ pehip_status pehip_step(pehip_ctx* ctx, double dt, uint32_t nsteps) {
   hipSetDevice(ctx->device);
   for (uint32_t k = 0; k < nsteps; ++k) CKS(pe_one_step(ctx, dt));
   return PEHIP_OK;
+}
+
+pehip_status pehip_step_noalloc(pehip_ctx* ctx, double dt, uint32_t nsteps) {
+  if (!ctx) return PEHIP_INVALID_ARG;
+  if (!(dt > 0)) return fail(ctx, PEHIP_INVALID_ARG, "Invalid time step size");
+  if (ctx->n == 0) return fail(ctx, PEHIP_INVALID_ARG, "no bodies uploaded");
+  if (ctx->members_missing) return fail(ctx, PEHIP_INVALID_ARG, "union members not uploaded");
+  if (ctx->prm.sweep_mode != PEHIP_SWEEP_COLOURED)
+    return fail(ctx, PEHIP_NOT_SUPPORTED, "pehip_step_noalloc runs the coloured sweep mode only");
+  if (ctx->profiling) return fail(ctx, PEHIP_NOT_SUPPORTED, "pehip_step_noalloc does not profile");
+  hipSetDevice(ctx->device);
+  if (!ctx->contacts_in_scratch) {
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->unbind_contacts();
+    ctx->carve_contacts(static_cast<char*>(ctx->scratch), kScratchContacts);
+    ctx->ccap = kScratchContacts;
+    ctx->contacts_in_scratch = true;
+  }
+  ctx->no_alloc = true;
+  pehip_status st = PEHIP_OK;
+  for (uint32_t k = 0; k < nsteps && st == PEHIP_OK; ++k) st = pe_one_step(ctx, dt);
+  ctx->no_alloc = false;
+  return st;
 }
 
 pehip_status pehip_detect(pehip_ctx* ctx) {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pe_hip.h"
@@ -65,8 +66,15 @@ struct pehip_ctx {
   uint32_t *planes = nullptr, *scan_scratch = nullptr, *sort_scratch = nullptr;
   uint32_t *flags = nullptr, *total = nullptr;
 
-  // contacts
+  // contacts: the per-contact arrays below are carved out of one device block, the context's own
+  // (cblock, allocated with capacity own_ccap) or, during pehip_step_noalloc, the library's static
+  // scratch on this device (scratch; capacity kScratchContacts)
   size_t ccap = 0;
+  void* cblock = nullptr;
+  size_t own_ccap = 0;
+  bool contacts_in_scratch = false;
+  bool no_alloc = false;        // inside pehip_step_noalloc: outgrowing the scratch is an error
+  void* scratch = nullptr;      // device address of the static scratch on this context's device
   uint32_t nc = 0;
   RawContacts raw{};
   SolverContacts sol{};
@@ -118,14 +126,39 @@ struct pehip_ctx {
     dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial); dfree(df_done);
     n = 0;
   }
-  void free_contacts() {
-    dfree(raw.b1); dfree(raw.b2); dfree(raw.pos); dfree(raw.nrm); dfree(raw.dist); dfree(raw.mu);
-    dfree(sol.b1); dfree(sol.b2); dfree(sol.f);
-    dfree(col.adj); dfree(col.colour); dfree(col.prio); dfree(col.cround);
-    dfree(perm); dfree(sort_keys); dfree(sort_keys_tmp); dfree(sort_vals_tmp);
-    dfree(df_inv); dfree(df_rank1); dfree(df_rank2);
-    dfree(cscan_scratch); dfree(csort_scratch);
+  // Points the per-contact arrays into `base` laid out for `cap` contacts and returns the bytes
+  // the layout needs (every array starts on a 256-byte boundary).  base = nullptr only measures;
+  // clear = true sets every array pointer to nullptr instead.
+  size_t carve_contacts(char* base, size_t cap, bool clear = false) {
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t count) {
+      using T = std::remove_reference_t<decltype(*p)>;
+      off = (off + 255) & ~(size_t)255;
+      if (clear) p = nullptr;
+      else if (base) p = reinterpret_cast<T*>(base + off);
+      off += (count ? count : 1) * sizeof(T);
+    };
+    take(raw.b1, cap); take(raw.b2, cap); take(raw.pos, 3 * cap); take(raw.nrm, 3 * cap);
+    take(raw.dist, cap); take(raw.mu, cap);
+    take(sol.b1, cap); take(sol.b2, cap); take(sol.f, (size_t)CF_COUNT * cap);
+    take(col.adj, 2 * cap); take(col.colour, cap); take(col.prio, cap); take(col.cround, cap);
+    take(perm, cap); take(sort_keys, cap); take(sort_keys_tmp, cap); take(sort_vals_tmp, cap);
+    take(df_inv, cap); take(df_rank1, cap); take(df_rank2, cap);
+    take(cscan_scratch, scan_scratch_words(cap)); take(csort_scratch, sort_scratch_words(cap));
+    if (clear) raw.cap = sol.cap = 0;
+    else if (base) raw.cap = sol.cap = cap;
+    return (off + 255) & ~(size_t)255;
+  }
+  void unbind_contacts() {
+    carve_contacts(nullptr, 0, true);
     ccap = 0;
+  }
+  void free_contacts() {
+    unbind_contacts();
+    if (cblock) (void)hipFree(cblock);
+    cblock = nullptr;
+    own_ccap = 0;
+    contacts_in_scratch = false;
   }
 };
 
