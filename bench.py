@@ -413,8 +413,8 @@ def main():
                          "error / 2h, so the solves run close to the floating-point fixed point")
     ap.add_argument("--c5-adjoint-sweeps", type=int, default=20000, help="c5: adjoint sweep cap per step")
     ap.add_argument("--c5-adjoint-tol", type=float, default=1e-15, help="c5: adjoint sweep threshold")
-    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3],
-                    help="sweep schedule: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup")
+    ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="sweep schedule: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup, 4 islands")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
@@ -434,7 +434,7 @@ def main():
 
     import torch
     from pe_amd import World, scenes
-    from pe_amd._abi import SCHEDULE_DATAFLOW
+    from pe_amd._abi import SCHEDULE_DATAFLOW, SCHEDULE_ISLANDS
 
     if args.workload == "c5":
         print(json.dumps({"metric": "adjoint of the time step: forward + reverse seconds, FD agreement",
@@ -542,8 +542,9 @@ def main():
         avg_ms = st["prof_sweep_kernel_ms"] / st["prof_sweep_launches"]
         bytes_per_launch = st["prof_sweep_bytes"] / st["prof_sweep_launches"]
         achieved = st["prof_sweep_bytes"] / (st["prof_sweep_kernel_ms"] * 1e-3) / 1e9
-        kernel = ("k_sweep_dataflow<3, TrigFast, 6>" if st["schedule_used"] == SCHEDULE_DATAFLOW
-                  else "k_sweep<3> (one colour of one NBGS sweep)")
+        kernel = {SCHEDULE_DATAFLOW: "k_sweep_dataflow<3, TrigFast, 6>",
+                  SCHEDULE_ISLANDS: "k_sweep_islands<3, TrigFast>"}.get(st["schedule_used"],
+                                                                        "k_sweep<3> (one colour of one NBGS sweep)")
         traffic, traffic_src = None, None
         if os.path.exists(TRAFFIC_FILE) and args.workload == "c2" and args.spheres == 200_000:
             with open(TRAFFIC_FILE) as f:
@@ -555,7 +556,8 @@ def main():
                            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                            "traffic_source": traffic_src or "not measured for this window",
                            "kernel": kernel + (" (all %d NBGS sweeps of one step)" % sc.max_iterations
-                                               if st["schedule_used"] == SCHEDULE_DATAFLOW else ""),
+                                               if st["schedule_used"] in (SCHEDULE_DATAFLOW, SCHEDULE_ISLANDS)
+                                               else ""),
                            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch,
                            "launches": int(st["prof_sweep_launches"]),
                            "timing": "HIP events around every sweep launch on the context stream"}

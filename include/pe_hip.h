@@ -90,7 +90,12 @@ typedef enum {
   PEHIP_SCHEDULE_DATAFLOW = 2,
   /* all sweeps in one launch of one workgroup, a barrier between colours and the abort test after
    * every sweep on the device (same results as the colour launches, bit for bit); for small steps */
-  PEHIP_SCHEDULE_SINGLE_BLOCK = 3
+  PEHIP_SCHEDULE_SINGLE_BLOCK = 3,
+  /* all sweeps of each island of the contact graph (the non-fixed bodies joined by contacts) in
+   * one wavefront (up to 64 contacts, several islands per wavefront) or one workgroup (up to 256),
+   * bodies in LDS, one lane per contact; larger islands by dataflow (same results as the colour
+   * launches, bit for bit); abort_threshold must be <= 0 */
+  PEHIP_SCHEDULE_ISLANDS = 4
 } pehip_sweep_schedule;
 
 /* Fills p with pe's defaults (HCSITS.h:408-431, Settings.cpp:46-48). */
@@ -231,7 +236,8 @@ typedef struct {
   double prof_sweep_kernel_ms;
   double prof_sweep_bytes;
   int32_t schedule_used;      /* schedule of the last step's sweeps: PEHIP_SCHEDULE_COLOUR_LAUNCHES,
-                                 _DATAFLOW or _SINGLE_BLOCK; -1 for reference order or no sweeps */
+                                 _DATAFLOW, _SINGLE_BLOCK or _ISLANDS; -1 for reference order or no
+                                 sweeps */
   int32_t reserved;
 } pehip_stats;
 pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);

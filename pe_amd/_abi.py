@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libpehip.so")
 OK, INVALID_ARG, OUT_OF_MEMORY, CAPACITY_RETRY, HIP_ERROR, NONFINITE, NOT_SUPPORTED = range(7)
 SPHERE, BOX, PLANE, CAPSULE, UNION = 1, 2, 3, 4, 5
 SWEEP_COLOURED, SWEEP_REFERENCE_ORDER = 0, 1
-SCHEDULE_AUTO, SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW, SCHEDULE_SINGLE_BLOCK = 0, 1, 2, 3
+SCHEDULE_AUTO, SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW, SCHEDULE_SINGLE_BLOCK, SCHEDULE_ISLANDS = 0, 1, 2, 3, 4
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "CAPACITY_RETRY", 4: "HIP_ERROR",
                 5: "NONFINITE", 6: "NOT_SUPPORTED"}

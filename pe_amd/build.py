@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "_lib")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libpehip.so")
 
-SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_solver.hip", "k_batch.hip", "k_adjoint.hip", "context.hip",
+SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_solver.hip", "k_islands.hip", "k_batch.hip", "k_adjoint.hip", "context.hip",
            "batch.hip", "adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -223,7 +223,7 @@ pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
     return PEHIP_INVALID_ARG;
   }
   if (p->damping > 1 || p->damping < 0) { why = "Invalid damping."; return PEHIP_INVALID_ARG; }
-  if (p->sweep_schedule < PEHIP_SCHEDULE_AUTO || p->sweep_schedule > PEHIP_SCHEDULE_SINGLE_BLOCK) {
+  if (p->sweep_schedule < PEHIP_SCHEDULE_AUTO || p->sweep_schedule > PEHIP_SCHEDULE_ISLANDS) {
     why = "Invalid sweep schedule.";
     return PEHIP_INVALID_ARG;
   }
@@ -447,6 +447,8 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   CK(dalloc(ctx->col.deg, n)); CK(dalloc(ctx->col.adj_start, (size_t)n + 1)); CK(dalloc(ctx->col.adj_cursor, n));
   CK(dalloc(ctx->ke_partial, (n + 255) / 256));
   CK(dalloc(ctx->df_done, n));
+  CK(dalloc(ctx->isl.parent, n)); CK(dalloc(ctx->isl.ncont, n)); CK(dalloc(ctx->isl.cmask, 4 * (size_t)n));
+  CK(dalloc(ctx->isl.cursor, n)); CK(dalloc(ctx->isl.place, n)); CK(dalloc(ctx->isl.blocal, n));
   P.planes = ctx->planes;
   P.nplanes = (uint32_t)hplanes.size();
   P.flags = ctx->flags;
@@ -667,6 +669,63 @@ static pehip_status reference_perm(pehip_ctx* ctx) {
   return PEHIP_OK;
 }
 
+// Island sweeps (PEHIP_SCHEDULE_ISLANDS): the contact graph's islands are found and packed into
+// units on the device; one launch runs every small and large island's sweeps, and the islands over
+// kIslandLarge contacts go to the dataflow sweeps (over their slots only) first.
+static pehip_status island_sweeps(pehip_ctx* ctx, const Bodies& B, double dtinv, bool exact, bool* swept) {
+  const pehip_params& P = ctx->prm;
+  const hipStream_t s = ctx->stream;
+  const uint32_t nc = ctx->nc;
+  *swept = false;
+  ColourOffsets O;
+  O.ncol = (uint32_t)ctx->colour_off.size() - 1;
+  for (uint32_t k = 0; k <= O.ncol; ++k) O.off[k] = ctx->colour_off[k];
+  launch_island_prepare(ctx->sol, nc, O, B, ctx->col.deg, ctx->isl, s);
+  CK(hipMemcpyAsync(ctx->hstage, ctx->isl.hdr + ISL_SMALL_UNITS, 12, hipMemcpyDeviceToHost, s));
+  CKS(sync_check(ctx));
+  const uint32_t su = ctx->hstage[0], lu = ctx->hstage[1], dfc = ctx->hstage[2];
+  if (dfc && (dfc > kDataflowMaxContacts || ctx->dataflow_refused))
+    return fail(ctx, PEHIP_NOT_SUPPORTED, "island sweeps: " + std::to_string(dfc) +
+                                              " contacts in islands over 256 contacts need the dataflow sweeps, " +
+                                              (ctx->dataflow_refused ? ctx->dataflow_refusal
+                                                                     : std::string("which take at most 65536")));
+  launch_island_assign(ctx->sol, nc, O, B, ctx->col.deg, ctx->isl, su, lu, dfc, ctx->cscan_scratch, s);
+  size_t nev = 0;
+  if (dfc) {
+    DataflowBuffers D;
+    D.perm = ctx->perm; D.adj_start = ctx->col.adj_start; D.adj = ctx->col.adj; D.deg = ctx->col.deg;
+    D.inv = ctx->df_inv; D.rank1 = ctx->df_rank1; D.rank2 = ctx->df_rank2; D.done = ctx->df_done;
+    D.deltas = ctx->deltas; D.flags = ctx->flags; D.spin_limit = kDataflowSpinLimit;
+    fill_u32(ctx->flags, 0u, 1, s);
+    launch_dataflow_prepare(ctx->sol, nc, B, D, s);
+    if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+    DataflowLaunchInfo info;
+    if (launch_sweep_dataflow(P.relaxation_model, exact, ctx->sol, dfc, P.max_iterations, B, dtinv,
+                              (int)P.max_sub_iterations, D, ctx->ncu, s, &info, ctx->isl.dslots) != SWEEP_OK) {
+      const std::string why = std::string("dataflow sweep launch refused: occupancy query ") +
+                              hipGetErrorName(info.occupancy_err) + ", " + std::to_string(info.per_cu) +
+                              " workgroups per CU, " + std::to_string(ctx->ncu) + " CUs, grid " +
+                              std::to_string(info.grid) + ", cooperative launch " + hipGetErrorName(info.launch_err);
+      (void)hipGetLastError();  // the refusal is not a sticky error
+      return fail(ctx, PEHIP_HIP_ERROR, why);
+    }
+    if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+  }
+  if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+  launch_sweep_islands(P.relaxation_model, exact, ctx->sol, ctx->isl, su, lu, P.max_iterations, B,
+                       (int)P.max_sub_iterations, ctx->deltas, s);
+  if (ctx->profiling) CKS(record_launch_event(ctx, nev++));
+  ctx->n_launch_events = ctx->profiling ? nev : 0;
+  if (dfc) {
+    CK(hipMemcpyAsync(ctx->hstage, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+    CKS(sync_check(ctx));
+    if (ctx->hstage[0] & ERR_TIMEOUT) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
+    ++ctx->dataflow_steps;
+  }
+  *swept = true;
+  return PEHIP_OK;
+}
+
 pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
   const hipStream_t s = ctx->stream;
   const double dtinv = 1.0 / dt;
@@ -726,7 +785,14 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
       // cooperative launch, e.g. while another process holds the device's cooperative-launch
       // resources; nothing has run then.  An explicit DATAFLOW schedule reports the refusal.
       bool swept = false;
-      if (P.abort_threshold <= 0 &&
+      if (P.abort_threshold <= 0 && P.sweep_schedule == PEHIP_SCHEDULE_ISLANDS) {
+        CKS(island_sweeps(ctx, B, dtinv, exact, &swept));
+        if (swept) {
+          iters = P.max_iterations;
+          schedule_used = PEHIP_SCHEDULE_ISLANDS;
+        }
+      }
+      if (!swept && P.abort_threshold <= 0 &&
           (P.sweep_schedule == PEHIP_SCHEDULE_DATAFLOW ||
            (P.sweep_schedule == PEHIP_SCHEDULE_AUTO && nc <= kDataflowMaxContacts && !ctx->dataflow_refused))) {
         DataflowBuffers D;

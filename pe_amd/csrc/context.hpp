@@ -85,6 +85,8 @@ struct pehip_ctx {
   bool dataflow_refused = false;  // the device refused a cooperative launch: AUTO uses colour launches
   std::string dataflow_refusal;   // what the refused launch saw
   uint32_t *cscan_scratch = nullptr, *csort_scratch = nullptr;
+  IslandBuffers isl{};           // island sweeps: per-body arrays allocated with the bodies, the
+                                 // per-contact and per-unit ones carved with the contacts
   std::vector<uint32_t> colour_off;
   unsigned long long *deltas = nullptr, *maxpen = nullptr;
   double* ke_partial = nullptr;
@@ -124,6 +126,7 @@ struct pehip_ctx {
     dfree(bp.keys); dfree(bp.vals); dfree(bp.keys_tmp); dfree(bp.vals_tmp); dfree(bp.cell_start);
     dfree(bp.cell_end); dfree(bp.counts); dfree(bp.offsets); dfree(scan_scratch); dfree(sort_scratch);
     dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial); dfree(df_done);
+    dfree(isl.parent); dfree(isl.ncont); dfree(isl.cmask); dfree(isl.cursor); dfree(isl.place); dfree(isl.blocal);
     n = 0;
   }
   // Points the per-contact arrays into `base` laid out for `cap` contacts and returns the bytes
@@ -145,6 +148,13 @@ struct pehip_ctx {
     take(perm, cap); take(sort_keys, cap); take(sort_keys_tmp, cap); take(sort_vals_tmp, cap);
     take(df_inv, cap); take(df_rank1, cap); take(df_rank2, cap);
     take(cscan_scratch, scan_scratch_words(cap)); take(csort_scratch, sort_scratch_words(cap));
+    const size_t su = island_small_units_bound(cap), lu = island_large_units_bound(cap);
+    take(isl.sroot, cap); take(isl.dflag, cap); take(isl.dpos, cap); take(isl.dslots, cap);
+    take(isl.sslot, su * kIslandSmall); take(isl.sround, su * kIslandSmall); take(isl.sbody, su * kIslandSmallBodies);
+    take(isl.snb, su); take(isl.srounds, su);
+    take(isl.lslot, lu * kIslandLarge); take(isl.lround, lu * kIslandLarge); take(isl.lbody, lu * kIslandLargeBodies);
+    take(isl.lnb, lu); take(isl.lrounds, lu);
+    take(isl.hdr, ISL_HDR_WORDS);
     if (clear) raw.cap = sol.cap = 0;
     else if (base) raw.cap = sol.cap = cap;
     return (off + 255) & ~(size_t)255;

@@ -315,14 +315,15 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
                                                         const uint32_t* deg, const uint32_t* rank1,
                                                         const uint32_t* rank2, uint32_t* done,
                                                         unsigned long long* deltas, uint32_t* flags,
-                                                        uint32_t spin_limit) {
+                                                        uint32_t spin_limit, const uint32_t* slots) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   const uint64_t total = (uint64_t)iters * nc;
   for (uint64_t item = wave; item < total; item += nwaves) {
     const uint32_t q = (uint32_t)(item / nc);
-    const uint32_t s = __builtin_amdgcn_readfirstlane((uint32_t)(item - (uint64_t)q * nc));
+    const uint32_t k = (uint32_t)(item - (uint64_t)q * nc);
+    const uint32_t s = __builtin_amdgcn_readfirstlane(slots ? slots[k] : k);
     const uint32_t e1 = C.b1[s], e2 = C.b2[s];
     const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
     const bool f1 = (e1 & kFixedBit) != 0, f2 = (e2 & kFixedBit) != 0;
@@ -416,7 +417,8 @@ static void sweep_dispatch(int model, const SolverContacts& C, uint32_t begin, u
 template <class Trig, int SPEC>
 static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, uint32_t nc, uint32_t iters,
                                            const Bodies& B, double dtinv, int maxSub, const DataflowBuffers& D,
-                                           uint32_t ncu, hipStream_t s, DataflowLaunchInfo* info) {
+                                           uint32_t ncu, hipStream_t s, DataflowLaunchInfo* info,
+                                           const uint32_t* slots) {
   void* args_fn = nullptr;
   switch (model) {
     case 0: args_fn = (void*)k_sweep_dataflow<0, Trig, 0>; break;
@@ -438,7 +440,7 @@ static pehip_sweep_status dataflow_dispatch(int model, const SolverContacts& C, 
   double* dvw = B.dvw;
   uint32_t spin = D.spin_limit;
   void* args[] = {&Cc, &nc, &iters, &sb, &dvw, &dtinv, &maxSub, (void*)&D.deg, (void*)&D.rank1, (void*)&D.rank2,
-                  (void*)&D.done, (void*)&D.deltas, (void*)&D.flags, &spin};
+                  (void*)&D.done, (void*)&D.deltas, (void*)&D.flags, &spin, (void*)&slots};
   info->launch_err = hipLaunchCooperativeKernel(args_fn, dim3(grid), dim3(256), args, 0, s);
   return info->launch_err == hipSuccess ? SWEEP_OK : SWEEP_LAUNCH_FAILED;
 }
@@ -455,11 +457,11 @@ void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies&
 pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
                                          uint32_t iters, const Bodies& B, double dtinv, int maxSub,
                                          const DataflowBuffers& D, uint32_t ncu, hipStream_t s,
-                                         DataflowLaunchInfo* info) {
+                                         DataflowLaunchInfo* info, const uint32_t* slots) {
   if (!nc || !iters) return SWEEP_OK;
   // a whole wavefront owns each item, so the speculated search costs no parallelism here
-  if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s, info);
-  return dataflow_dispatch<TrigFast, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s, info);
+  if (exact_trig) return dataflow_dispatch<TrigExact, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s, info, slots);
+  return dataflow_dispatch<TrigFast, 6>(model, C, nc, iters, B, dtinv, maxSub, D, ncu, s, info, slots);
 }
 
 void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t begin, uint32_t end,

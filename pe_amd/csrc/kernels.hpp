@@ -103,6 +103,12 @@ void fill_u64(unsigned long long* p, unsigned long long value, size_t n, hipStre
 void iota_u32(uint32_t* p, size_t n, hipStream_t s);
 void launch_colour_hist(uint32_t nc, ColourBuffers& C, hipStream_t s);
 
+// Colour boundaries of the solver order, by value (a kernel argument).
+struct ColourOffsets {
+  uint32_t ncol;
+  uint32_t off[kMaxColours + 1];
+};
+
 // Contact caching (HCSITS.h:1741-1818) in solver order perm[0..nc).
 void launch_setup(const Bodies& B, const RawContacts& R, const uint32_t* perm, uint32_t nc,
                   SolverContacts& C, double erp, double dtinv, unsigned long long* maxpen,
@@ -134,10 +140,78 @@ struct DataflowLaunchInfo {
 void launch_dataflow_prepare(const SolverContacts& C, uint32_t nc, const Bodies& B, const DataflowBuffers& D,
                              hipStream_t s);
 // ... then all `iters` sweeps in one cooperative launch.
+// slots: the slots each sweep relaxes, ascending (nullptr: all nc slots); nc: their number.  A slot
+// list must hold every slot of each body it touches (whole islands).
 pehip_sweep_status launch_sweep_dataflow(int model, bool exact_trig, const SolverContacts& C, uint32_t nc,
                                          uint32_t iters, const Bodies& B, double dtinv, int maxSub,
                                          const DataflowBuffers& D, uint32_t ncu, hipStream_t s,
-                                         DataflowLaunchInfo* info);
+                                         DataflowLaunchInfo* info, const uint32_t* slots = nullptr);
+
+// Island sweeps.  The non-fixed bodies joined by contacts form islands (connected components of
+// the contact graph without the fixed bodies); no body takes part in two islands, so each island's
+// Gauss-Seidel sweeps are independent of every other's.  An island of up to kIslandSmall contacts
+// is relaxed by one wavefront with one lane per contact (several small islands share a wavefront),
+// one of up to kIslandLarge contacts by one workgroup, all K sweeps in a row with the bodies'
+// records and velocity corrections in LDS; larger islands go to the dataflow sweeps.  Within an
+// island the contacts are relaxed colour by colour in the coloured order, so every body sees its
+// updates in the same order as with the colour launches: the same bits.
+constexpr uint32_t kIslandSmall = 64;          // contacts per small unit (one wavefront)
+constexpr uint32_t kIslandSmallBodies = 128;   // non-fixed bodies per small unit
+constexpr uint32_t kIslandLarge = 256;         // contacts per large unit (one workgroup)
+constexpr uint32_t kIslandLargeBodies = 512;   // non-fixed bodies per large unit
+constexpr uint32_t kIslandClasses = 7;         // small-island size classes: 1, 2, 3-4, ..., 33-64
+constexpr uint32_t kIslandNone = 0xFFFFFFFFu;
+// header words (IslandBuffers::hdr)
+enum : uint32_t {
+  ISL_CLASS_COUNT = 0,     // [7] small islands per size class
+  ISL_CLASS_BASE = 8,      // [7] first small unit of each class
+  ISL_SMALL_UNITS = 16,
+  ISL_LARGE_UNITS = 17,
+  ISL_DATAFLOW_CONTACTS = 18,
+  ISL_HDR_WORDS = 32
+};
+struct IslandBuffers {
+  // per body
+  uint32_t* parent;   // union-find forest over the non-fixed bodies (a root is its set's smallest index)
+  uint32_t* ncont;    // per root: the island's contacts
+  uint32_t* cmask;    // per root: 4 words, the colours present in the island
+  uint32_t* cursor;   // per root: lanes handed out
+  uint32_t* place;    // per root: kind << 30 | class << 24 | position (kind 0 small, 1 large, 2 dataflow)
+  uint32_t* blocal;   // per non-fixed body with contacts: its index in its unit's body table
+  // per contact slot, and per unit (bounds from island_unit_bounds)
+  uint32_t* sroot;    // per slot: root of its island
+  uint32_t* dflag;    // per slot: 1 if its island goes to the dataflow sweeps
+  uint32_t* dpos;     // per slot: exclusive scan of dflag
+  uint32_t* dslots;   // the dataflow islands' slots, ascending
+  uint32_t* sslot;    // small units: kIslandSmall lanes each, slot or kIslandNone
+  uint32_t* sround;   // small units: each lane's round (rank of its colour within its island)
+  uint32_t* sbody;    // small units: kIslandSmallBodies bodies each
+  uint32_t* snb;      // small units: bodies
+  uint32_t* srounds;  // small units: rounds per sweep
+  uint32_t* lslot;    // large units: kIslandLarge lanes each
+  uint32_t* lround;
+  uint32_t* lbody;    // large units: kIslandLargeBodies bodies each
+  uint32_t* lnb;
+  uint32_t* lrounds;
+  uint32_t* hdr;      // ISL_HDR_WORDS words
+};
+// Upper bounds of the unit counts for nc contacts: small units (every small island of size m takes
+// at most 2m - 1 lanes of its class, plus one partly filled unit per class) and large units.
+inline size_t island_small_units_bound(size_t nc) { return (2 * nc) / kIslandSmall + kIslandClasses + 1; }
+inline size_t island_large_units_bound(size_t nc) { return nc / (kIslandSmall + 1) + 1; }
+// Islands of the coloured order: union-find, sizes, colour masks, units, bodies.  After it,
+// hdr[ISL_SMALL_UNITS], hdr[ISL_LARGE_UNITS] and hdr[ISL_DATAFLOW_CONTACTS] hold the counts (the
+// host reads them before launch_island_assign).
+void launch_island_prepare(const SolverContacts& C, uint32_t nc, const ColourOffsets& O, const Bodies& B,
+                           const uint32_t* deg, IslandBuffers& I, hipStream_t s);
+// Slots and bodies into their units, and the dataflow islands' slot list.
+void launch_island_assign(const SolverContacts& C, uint32_t nc, const ColourOffsets& O, const Bodies& B,
+                          const uint32_t* deg, IslandBuffers& I, uint32_t small_units, uint32_t large_units,
+                          uint32_t dataflow_contacts, uint32_t* scan_scratch, hipStream_t s);
+// All `iters` sweeps of the small and large units (one launch).
+void launch_sweep_islands(int model, bool exact_trig, const SolverContacts& C, const IslandBuffers& I,
+                          uint32_t small_units, uint32_t large_units, uint32_t iters, const Bodies& B, int maxSub,
+                          unsigned long long* deltas, hipStream_t s);
 
 // One colour of one sweep; ncu: compute units of the device (sizes the lanes per contact).  A
 // non-null stop makes the launch return at once once stop[0] is set (see launch_abort_check).
@@ -147,11 +221,6 @@ void launch_sweep(int model, bool exact_trig, const SolverContacts& C, uint32_t 
 // pe's abort test on the device: sets stop[0] = 1 and stop[1] = it + 1 if sweep it changed no
 // impulse by thr or more and no earlier sweep did (stop[0..1] cleared before the first sweep).
 void launch_abort_check(const unsigned long long* deltas, uint32_t it, double thr, uint32_t* stop, hipStream_t s);
-// Colour boundaries of the solver order, by value (a kernel argument).
-struct ColourOffsets {
-  uint32_t ncol;
-  uint32_t off[kMaxColours + 1];
-};
 // All `iters` sweeps in one workgroup, colour by colour with a barrier between colours; after
 // each sweep deltas[it] = its L-inf impulse change, and the sweeps stop after the first one
 // below thr > 0.  deltas[iters] receives the number of sweeps run.

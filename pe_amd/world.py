@@ -176,9 +176,9 @@ class World:
         self._params_dirty = True
 
     def setSweepSchedule(self, schedule):
-        """Coloured mode: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup
+        """Coloured mode: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup, 4 islands
         (pehip_sweep_schedule)."""
-        if schedule not in (0, 1, 2, 3):
+        if schedule not in (0, 1, 2, 3, 4):
             raise ValueError("Invalid sweep schedule.")
         self._schedule = int(schedule)
         self._params_dirty = True

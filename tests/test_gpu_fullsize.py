@@ -18,7 +18,7 @@ import pytest
 
 import oracle as O
 from pe_amd import World, scenes, SWEEP_COLOURED
-from pe_amd._abi import SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW
+from pe_amd._abi import SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW, SCHEDULE_ISLANDS
 
 pytestmark = pytest.mark.gpu
 
@@ -140,6 +140,35 @@ def test_c2_dataflow_equals_colour_launches(c2_state, exact):
         assert np.array_equal(a[0][k], b[0][k]), k
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_c2_islands_equal_colour_launches(c2_state, exact):
+    # each island of the pile's contact graph swept on its own, colour by colour: the colour
+    # launches' per-body update order, so the same bits
+    sc, st = c2_state
+    a = one_gpu_step(sc, st, sc.max_iterations, exact, SCHEDULE_ISLANDS)
+    b = one_gpu_step(sc, st, sc.max_iterations, exact, SCHEDULE_COLOUR_LAUNCHES)
+    assert a[4]["schedule_used"] == SCHEDULE_ISLANDS and b[4]["schedule_used"] == SCHEDULE_COLOUR_LAUNCHES
+    for k in ("v", "w", "pos", "q"):
+        assert np.array_equal(a[0][k], b[0][k]), k
+
+
+def test_c2_islands_several_steps_equal_colour_launches(c2_state):
+    # 20 steps in a row (every step re-finds and re-packs the islands)
+    sc, st = c2_state
+    out = []
+    for schedule in (SCHEDULE_ISLANDS, SCHEDULE_COLOUR_LAUNCHES):
+        w = gpu_world(sc, sc.max_iterations, False, schedule)
+        w.setState(st)
+        for _ in range(20):
+            w.simulationStep(sc.dt)
+            assert w.stats(kinetic=False)["schedule_used"] == schedule
+        out.append((w.getState(), w.impulses()))
+        w.close()
+    for k in ("v", "w", "pos", "q"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 # --------------------------------------------------------------------------- 1M-body hopper
 @pytest.fixture(scope="module")
 def c3_state():
@@ -222,6 +251,15 @@ def test_c3_coloured_step_exact_trig_equals_gs_in_colour_order(c3_state):
     o1 = oracle_step_in_order(sc, st, 3, True, gc, idx)
     for k in ("v", "w", "pos", "q"):
         assert rel_err(g1[k], o1[k]) <= 1e-12, k
+
+
+def test_c3_islands_equal_colour_launches(c3_state):
+    sc, st = c3_state
+    a = one_gpu_step(sc, st, sc.max_iterations, False, SCHEDULE_ISLANDS)
+    b = one_gpu_step(sc, st, sc.max_iterations, False, SCHEDULE_COLOUR_LAUNCHES)
+    assert a[4]["schedule_used"] == SCHEDULE_ISLANDS
+    for k in ("v", "w", "pos", "q"):
+        assert np.array_equal(a[0][k], b[0][k]), k
 
 
 def test_c3_production_step_close_to_gs_in_colour_order(c3_state):

@@ -495,6 +495,77 @@ def test_dataflow_sweeps_equal_colour_launches(name, exact, model):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("model", [3, 5, 1, 0, 2, 4])
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_island_sweeps_equal_colour_launches(name, exact, model):
+    # Each island of the contact graph swept on its own (small ones packed into wavefronts, large
+    # ones in a workgroup, the largest by dataflow), colour by colour within the island: the colour
+    # launches' per-body update order, so the same bits.
+    sc = SCENES[name]()
+    _, st0 = oracle_state_after(sc, 10)
+    out = []
+    for schedule in (1, 4):
+        w = gpu_for(sc, iters=30, model=model, exact_trig=exact)
+        w.setSweepSchedule(schedule)
+        w.setState(st0)
+        for _ in range(3):
+            w.simulationStep(sc.dt)
+            assert w.stats(kinetic=False)["schedule_used"] == schedule
+        out.append((w.getState(), w.impulses(), w.stats(kinetic=False)["last_delta"]))
+        w.close()
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+
+
+def test_islands_of_every_kind():
+    # one scene with islands of every unit kind: single contacts and small clusters (wavefront
+    # units of several size classes), a 150-contact cluster (a workgroup unit) and a 400-contact
+    # packing (dataflow), against the colour launches bit for bit
+    rng = np.random.default_rng(5)
+    pos, rad = [], []
+    # separate clusters of touching spheres on the floor: sizes 1..40 spheres in lines
+    x = 1.0
+    for m in (1, 1, 2, 3, 5, 8, 12, 20, 33, 40):
+        for k in range(m):
+            pos.append([x + k * (1.0 - 2e-6), 2.0, 0.5 - 1e-6])
+            rad.append(0.5)
+        x += m + 1.5
+    # an 8x8 touching layer (112 contacts + 64 with the floor: a workgroup unit)
+    for i in range(8):
+        for j in range(8):
+            pos.append([2.0 + i * (1.0 - 2e-6), 6.0 + j * (1.0 - 2e-6), 0.5 - 1e-6])
+            rad.append(0.5)
+    # a 6x6x4 touching block (348 contacts + 36 with the floor: dataflow)
+    for i in range(6):
+        for j in range(6):
+            for k in range(4):
+                pos.append([20.0 + i * (1.0 - 2e-6), 8.0 + j * (1.0 - 2e-6), 0.5 - 1e-6 + k * (1.0 - 2e-6)])
+                rad.append(0.5)
+    pos = np.array(pos) + rng.uniform(-1e-7, 1e-7, size=(len(pos), 3))
+    v0 = np.concatenate([np.zeros((6, 3)), rng.uniform(-0.05, 0.05, size=(len(pos), 3))])
+    sc = scenes.Scene("islands", [{"name": "g", "density": 2500.0, "cor": 0.0, "csf": 0.2, "cdf": 0.2}],
+                      scenes.box_planes(x + 2.0, 16.0, 20.0), pos, np.array(rad), np.zeros(len(pos), np.uint32),
+                      v0=v0, dt=1e-3, max_iterations=30)
+    out = []
+    for schedule in (1, 4):
+        w = gpu_for(sc, iters=30, model=3)
+        w.setSweepSchedule(schedule)
+        counts = []
+        for _ in range(4):
+            w.simulationStep(sc.dt)
+            assert w.stats(kinetic=False)["schedule_used"] == schedule
+            counts.append(w.stats(kinetic=False)["num_contacts"])
+        out.append((w.getState(), w.impulses(), counts))
+        w.close()
+    assert out[0][2][0] == 800 and out[0][2] == out[1][2]   # the first step holds every kind
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("model,abort", [(3, 0.0), (1, 0.0), (0, 0.0), (1, 1e-3), (3, 1e-3), (2, 0.0), (4, 1e-3),
                                          (5, 0.0), (5, 1e-3)])
 @pytest.mark.parametrize("name", sorted(SCENES))
