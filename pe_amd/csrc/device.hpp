@@ -866,16 +866,16 @@ __device__ __forceinline__ double wave_read_lane(double x, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Value of x in lane `node` of the caller's group of 2^SPEC lanes (node uniform in the group;
-// SPEC 6: the whole wavefront, read by the scalar unit).
-template <int SPEC>
+// Value of x in lane `node` of the caller's group of 2^SPEC lanes (node uniform in the group).
+// UNIFORM: node is the same in the whole wavefront, which reads it with the scalar unit.
+template <int SPEC, bool UNIFORM>
 __device__ __forceinline__ double group_read(double x, uint32_t node) {
-  if constexpr (SPEC == 6) return wave_read_lane(x, (int)node);
+  if constexpr (UNIFORM) return wave_read_lane(x, (int)node);
   else return __shfl(x, (int)node, 1 << SPEC);
 }
-template <int SPEC>
+template <int SPEC, bool UNIFORM>
 __device__ __forceinline__ bool group_read(bool x, uint32_t node) {
-  if constexpr (SPEC == 6) return __builtin_amdgcn_readlane(x ? 1 : 0, (int)node) != 0;
+  if constexpr (UNIFORM) return __builtin_amdgcn_readlane(x ? 1 : 0, (int)node) != 0;
   else return __shfl(x ? 1 : 0, (int)node, 1 << SPEC) != 0;
 }
 
@@ -924,8 +924,15 @@ struct ConeEval {
 // the bracket after the round is one golden_step from the state at the last node on the walk.
 // Every evaluated point and every comparison is the one the sequential search makes, bit for
 // bit.  In the first round the group's spare lane evaluates S.am.
-template <int SPEC, class Eval>
+//
+// SPEC = 6 (a whole wavefront owns the contact) runs the 5-level tree of 32 lanes in both halves
+// of the wavefront alike, so the taken node is wave-uniform and read by the scalar unit: rounds of
+// a 6-level tree cost enough more (cross-lane reads over 64 lanes) that 20 steps in 4 rounds of 5
+// levels take 6.0k cycles against 8.2k in rounds of 6, 6, 6 and 2 (one wavefront, device libm).
+template <int SPEC_, class Eval>
 __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const Eval& eval) {
+  constexpr bool UNIFORM = SPEC_ == 6;
+  constexpr int SPEC = SPEC_ == 6 ? 5 : SPEC_;
   if constexpr (SPEC == 0) {
     for (int k = 0; k < maxSub; ++k) {
       const double an = S.al + (S.ar - S.am);
@@ -951,9 +958,9 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const
       double xn0, xn1;
       const double fn = eval(an, xn0, xn1);
       if (k == 0) {
-        S.fm = group_read<SPEC>(fn, L - 1);
-        S.xm0 = group_read<SPEC>(xn0, L - 1);
-        S.xm1 = group_read<SPEC>(xn1, L - 1);
+        S.fm = group_read<SPEC, UNIFORM>(fn, L - 1);
+        S.xm0 = group_read<SPEC, UNIFORM>(xn0, L - 1);
+        S.xm1 = group_read<SPEC, UNIFORM>(xn1, L - 1);
       }
       // best value held on reaching this node: fold of the ancestors' values, root first (all
       // the ancestors' values are read before the fold, so the cross-lane reads overlap)
@@ -967,7 +974,8 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const
 #pragma unroll
       for (int l = 0; l < SPEC - 1; ++l) m = (l < depth && fanc[l] < m) ? fanc[l] : m;
       const unsigned long long all = __ballot(fn < m);
-      const uint64_t bits = SPEC == 6 ? all : (all >> base) & ((1ull << L) - 1ull);
+      // (UNIFORM: both halves hold the same tree; the lower half's bits are wave-uniform)
+      const uint64_t bits = UNIFORM ? (all & ((1ull << L) - 1ull)) : (all >> base) & ((1ull << L) - 1ull);
       // walk the taken path: fin = the last node compared, last = the last improving one
       uint32_t node = 0, fin = 0;
       int last = -1;
@@ -978,9 +986,9 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const
         if (ltf) last = (int)node;
         node = 2 * node + (ltf ? 2u : 1u);
       }
-      const double fal = group_read<SPEC>(al, fin), far = group_read<SPEC>(ar, fin);
-      const double fam = group_read<SPEC>(am, fin), fa = group_read<SPEC>(an, fin);
-      const bool fleft = group_read<SPEC>(left, fin);
+      const double fal = group_read<SPEC, UNIFORM>(al, fin), far = group_read<SPEC, UNIFORM>(ar, fin);
+      const double fam = group_read<SPEC, UNIFORM>(am, fin), fa = group_read<SPEC, UNIFORM>(an, fin);
+      const bool fleft = group_read<SPEC, UNIFORM>(left, fin);
       if (fleft) {
         if (ltf) { S.al = fal; S.ar = fam; S.am = fa; S.left = true; }
         else { S.al = fa; S.ar = far; S.am = fam; S.left = false; }
@@ -989,9 +997,9 @@ __device__ __forceinline__ void golden_section(GoldenState& S, int maxSub, const
         else { S.al = fal; S.ar = fa; S.am = fam; S.left = true; }
       }
       if (last >= 0) {
-        S.fm = group_read<SPEC>(fn, (uint32_t)last);
-        S.xm0 = group_read<SPEC>(xn0, (uint32_t)last);
-        S.xm1 = group_read<SPEC>(xn1, (uint32_t)last);
+        S.fm = group_read<SPEC, UNIFORM>(fn, (uint32_t)last);
+        S.xm0 = group_read<SPEC, UNIFORM>(xn0, (uint32_t)last);
+        S.xm1 = group_read<SPEC, UNIFORM>(xn1, (uint32_t)last);
       }
     }
   }
