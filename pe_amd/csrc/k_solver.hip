@@ -309,6 +309,53 @@ __device__ __forceinline__ bool wait_count(const uint32_t* ctr, uint32_t target,
   return true;
 }
 
+// A contact's fields held in registers (the impulse excluded: it is loaded after the wait).
+struct PrefetchedFields {
+  double v[CF_COUNT];
+  __device__ __forceinline__ double operator()(int k) const { return v[k]; }
+};
+
+// relax_contact<..., COHERENT = true> with the read-only inputs already loaded: the velocity
+// corrections and the impulse by agent-scope loads, the new ones stored by lane 0 likewise.
+template <int MODEL, class Trig, int SPEC>
+__device__ __forceinline__ double relax_coherent(const SolverContacts& C, uint32_t i, const PrefetchedFields& cf,
+                                                 const RelaxBody& B1, const RelaxBody& B2, uint32_t i1,
+                                                 uint32_t i2, bool fx1, bool fx2, double* dvw, int maxSub) {
+  const size_t cap = C.cap;
+  double* f = C.f;
+  double* q1 = dvw + (size_t)kDVW * i1;
+  double* q2 = dvw + (size_t)kDVW * i2;
+  d3 dv1 = mk3(0, 0, 0), dw1 = mk3(0, 0, 0), dv2 = mk3(0, 0, 0), dw2 = mk3(0, 0, 0);
+  if (!fx1) {
+    dv1 = mk3(sweep_ld<true>(q1), sweep_ld<true>(q1 + 1), sweep_ld<true>(q1 + 2));
+    dw1 = mk3(sweep_ld<true>(q1 + 3), sweep_ld<true>(q1 + 4), sweep_ld<true>(q1 + 5));
+  }
+  if (!fx2) {
+    dv2 = mk3(sweep_ld<true>(q2), sweep_ld<true>(q2 + 1), sweep_ld<true>(q2 + 2));
+    dw2 = mk3(sweep_ld<true>(q2 + 3), sweep_ld<true>(q2 + 4), sweep_ld<true>(q2 + 5));
+  }
+  // the impulse's three components with one load instruction (lanes 0-2), then broadcast
+  const uint32_t lane = threadIdx.x & 63u;
+  double pl = 0.0;
+  if (lane < 3) pl = sweep_ld<true>(f + (CF_P + lane) * cap + i);
+  d3 p = mk3(wave_read_lane(pl, 0), wave_read_lane(pl, 1), wave_read_lane(pl, 2));
+  const double delta = relax_update<MODEL, Trig, SPEC>(cf, B1, B2, dv1, dw1, dv2, dw2, p, maxSub);
+  if (lane == 0) {
+    sweep_st<true>(f + CF_P * cap + i, p.x);
+    sweep_st<true>(f + (CF_P + 1) * cap + i, p.y);
+    sweep_st<true>(f + (CF_P + 2) * cap + i, p.z);
+    if (!fx1) {
+      sweep_st<true>(q1, dv1.x); sweep_st<true>(q1 + 1, dv1.y); sweep_st<true>(q1 + 2, dv1.z);
+      sweep_st<true>(q1 + 3, dw1.x); sweep_st<true>(q1 + 4, dw1.y); sweep_st<true>(q1 + 5, dw1.z);
+    }
+    if (!fx2) {
+      sweep_st<true>(q2, dv2.x); sweep_st<true>(q2 + 1, dv2.y); sweep_st<true>(q2 + 2, dv2.z);
+      sweep_st<true>(q2 + 3, dw2.x); sweep_st<true>(q2 + 4, dw2.y); sweep_st<true>(q2 + 5, dw2.z);
+    }
+  }
+  return delta;
+}
+
 template <int MODEL, class Trig, int SPEC>
 __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32_t nc, uint32_t iters,
                                                         const double* sb, double* dvw, double dtinv, int maxSub,
@@ -332,6 +379,16 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
     // (the loads are unconditional, so they issue together: indices of fixed bodies are valid too)
     const uint32_t d1 = deg[i1], k1 = rank1[s], d2 = deg[i2], k2 = rank2[s];
     const uint32_t t1 = q * d1 + k1, t2 = q * d2 + k2;
+    // The relaxation's read-only inputs -- the contact record and the bodies' records -- are
+    // loaded before the wait, whose latency they then overlap; only the velocity corrections and
+    // the impulse, which earlier items write, are loaded after it.
+    PrefetchedFields cf;
+#pragma unroll
+    for (int k = 0; k < CF_P; ++k) cf.v[k] = C.f[(size_t)k * C.cap + s];
+#pragma unroll
+    for (int k = CF_P + 3; k < CF_COUNT; ++k) cf.v[k] = C.f[(size_t)k * C.cap + s];
+    const RelaxBody B1 = relax_body(sb + (size_t)kSB * i1);
+    const RelaxBody B2 = relax_body(sb + (size_t)kSB * i2);
     if ((!f1 && !wait_count(done + i1, t1, flags, spin_limit)) || (!f2 && !wait_count(done + i2, t2, flags, spin_limit))) {
       // give up and raise the flag, so that the wavefronts waiting for this one stop as well
       if (lane == 0) __hip_atomic_fetch_or(flags, ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -342,7 +399,7 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
     // emits no instruction; what orders the loads at run time is the wait loop's exit, which
     // depends on the counter's loaded value (in-order issue, sc1 loads coherent per access).
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const double d = relax_contact<MODEL, Trig, SPEC, true>(C, s, e1, e2, sb, dvw, dtinv, maxSub);
+    const double d = relax_coherent<MODEL, Trig, SPEC>(C, s, cf, B1, B2, i1, i2, f1, f2, dvw, maxSub);
     if (lane == 0) {
       if (d > 0) atomicMax(&deltas[q], (unsigned long long)__double_as_longlong(d));
       // Release without an agent-scope release: that writes back the XCD's whole L2 (measured
