@@ -209,14 +209,16 @@ def batched_cpu(scs, mid, seconds):
 
 
 def step_signature(w):
-    """Hash of the last step's active set: every contact's body pair and mode (separating,
-    static inside the friction cone, sliding on it)."""
+    """Hash of the last step's active set: every contact's body pair, mode (separating, static
+    inside the friction cone, sliding on it) and the sign of its distance (pe scales only negative
+    distances by the error-reduction parameter, HCSITS.h:1795-1798)."""
     import hashlib
     c = w.contacts()
     p = w.impulses()
     pn = np.einsum("ij,ij->i", p, c["normal"])
     pt = np.linalg.norm(p - pn[:, None] * c["normal"], axis=1)
     mode = np.where(np.all(p == 0, axis=1), 0, np.where(pt < c["mu"] * pn * (1 - 1e-9), 1, 2)).astype(np.uint8)
+    mode = mode | (np.asarray(c["dist"] < 0, np.uint8) << 2)
     key = (np.asarray(c["b1"], np.uint64) << np.uint64(32)) | np.asarray(c["b2"], np.uint64)
     i = np.argsort(key, kind="stable")
     return hashlib.sha1(key[i].tobytes() + mode[i].tobytes()).hexdigest(), len(key)
@@ -412,7 +414,8 @@ def main():
                     help="c5: sweep threshold on |dp|; differences of runs stopped early carry the truncation "
                          "error / 2h, so the solves run close to the floating-point fixed point")
     ap.add_argument("--c5-adjoint-sweeps", type=int, default=20000, help="c5: adjoint sweep cap per step")
-    ap.add_argument("--c5-adjoint-tol", type=float, default=1e-15, help="c5: adjoint sweep threshold")
+    ap.add_argument("--c5-adjoint-tol", type=float, default=1e-20,
+                    help="c5: adjoint sweep threshold (absolute change of the adjoint impulses, whose scale is ~1e-5)")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="sweep schedule: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup, 4 islands")
     ap.add_argument("--no-cpu", action="store_true")
