@@ -657,6 +657,55 @@ def test_coloured_converged_matches_reference_converged():
         assert rel_err(g1[k], o1[k]) <= 1e-8, k
 
 
+def test_converged_settled_c1_pile():
+    # The 1k-sphere run continued to 4,000 steps settles into a pile (about 1,800 contacts).  Run
+    # to pe's abort threshold (1e-7, HCSITS.h:424), the coloured solve and the oracle's solve in
+    # pe's own order converge to the same impulses: velocities agree to 1e-8.  Only the
+    # frictionless model converges there: on this state the oracle's decoupled and projected
+    # friction models (1-4) stall at changes of 3e-3 to 0.4 after 20,000 sweeps (3: 4e-3 after
+    # 100,000), so no converged solution exists to compare with.
+    sc = scenes.config1()
+    w = gpu_for(sc, iters=100, model=3)
+    w.run(4000, sc.dt)
+    st = w.getState()
+    w.close()
+    g = gpu_for(sc, iters=20000, model=0, abort=1e-7)
+    g.setState(st)
+    g.simulationStep(sc.dt)
+    gs, gstats = g.getState(), g.stats()
+    g.close()
+    ow = oracle_for(sc, iters=20000, model=0, abort=1e-7)
+    ow.set_state(st)
+    ow.step(sc.dt)
+    os_, ostats = ow.get_state(), ow.stats()
+    assert gstats["num_contacts"] == ostats["num_contacts"] > 1500
+    assert gstats["iterations"] < 20000 and ostats["iterations"] < 20000
+    assert gstats["last_delta"] < 1e-7 and ostats["last_delta"] < 1e-7
+    for k in ("v", "w"):
+        assert rel_err(gs[k], os_[k]) <= 1e-8, (k, rel_err(gs[k], os_[k]))
+
+
+def test_dataflow_at_the_automatic_limit():
+    # about 60,000 contacts (a touching 30x30x22 packing), just under the automatic schedule's
+    # dataflow limit of 65,536: the automatic choice is the dataflow launch, equal to the colour
+    # launches bit for bit
+    sc = scenes.drop_scene(30, 30, 22, r=0.5, spacing=1.0 - 2e-6, jitter=2e-7, box=(31.5, 31.5, 60.0), mu=0.4,
+                           seed=21, v_amp=0.05, max_iterations=30, z0=0.5 - 1e-6)
+    out = []
+    for schedule in (0, 1):
+        w = gpu_for(sc, iters=30, model=3)
+        w.setSweepSchedule(schedule)
+        w.simulationStep(sc.dt)
+        s = w.stats(kinetic=False)
+        out.append((w.getState(), w.impulses(), s))
+        w.close()
+    assert 50000 < out[0][2]["num_contacts"] <= 65536
+    assert out[0][2]["schedule_used"] == 2 and out[1][2]["schedule_used"] == 1
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_trajectory_observables_c1():
     # C1 over 500 steps: kinetic energy and maximum penetration of the coloured GPU run track
     # pe's sequential run within 1% (once contacts exist).
