@@ -494,7 +494,7 @@ def main():
         save_state(args.save_window_start, state0)
     if not args.no_profile:
         w.setProfiling(True)
-    contacts, colours, solve_ms, sweep_bytes = [], [], [], []
+    contacts, colours, solve_ms, sweep_bytes, phases = [], [], [], [], []
     half = K // 2
     t = 0.0
     state_mid = None
@@ -508,6 +508,8 @@ def main():
             colours.append(s["num_colours"])
             solve_ms.append(s["t_solve_ms"])
             sweep_bytes.append(s["sweep_bytes"])
+            phases.append([s["t_detection_ms"], s["t_colouring_ms"], s["t_setup_ms"], s["t_solve_ms"],
+                           s["t_integrate_ms"]])
         torch.cuda.synchronize()
         t += time.perf_counter() - t0
         if state_mid is None:
@@ -541,6 +543,10 @@ def main():
     }
     if not any(contacts):
         out["value_note"] = "no contacts in the timed window: the steps did no NBGS work"
+    if not args.no_profile:
+        ph = np.mean(np.array(phases), axis=0)
+        out["phase_ms"] = {"detection": ph[0], "colouring": ph[1], "setup": ph[2], "solve": ph[3], "integration": ph[4],
+                           "timing": "HIP events on the context stream, mean over the timed steps"}
     if not args.no_profile and st["prof_sweep_launches"]:
         avg_ms = st["prof_sweep_kernel_ms"] / st["prof_sweep_launches"]
         bytes_per_launch = st["prof_sweep_bytes"] / st["prof_sweep_launches"]

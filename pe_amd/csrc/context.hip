@@ -442,6 +442,9 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   CK(dalloc(P.keys, n)); CK(dalloc(P.vals, n)); CK(dalloc(P.keys_tmp, n)); CK(dalloc(P.vals_tmp, n));
   CK(dalloc(P.cell_start, cells)); CK(dalloc(P.cell_end, cells));
   CK(dalloc(P.counts, (size_t)n + 1)); CK(dalloc(P.offsets, (size_t)n + 1));
+  CK(dalloc(P.saabb, 6 * (size_t)n)); CK(dalloc(P.sfixed, n));
+  P.stride = n;
+  ctx->spheres_only = std::all_of(shape, shape + n, [](uint8_t t) { return t == PEHIP_SPHERE || t == PEHIP_PLANE; });
   CK(dalloc(ctx->scan_scratch, scan_scratch_words((size_t)n + 1)));
   CK(dalloc(ctx->sort_scratch, sort_scratch_words(n)));
   CK(dalloc(ctx->col.deg, n)); CK(dalloc(ctx->col.adj_start, (size_t)n + 1)); CK(dalloc(ctx->col.adj_cursor, n));
@@ -586,7 +589,8 @@ static pehip_status detect(pehip_ctx* ctx) {
   radix_sort_pairs(ctx->bp.keys, ctx->bp.vals, ctx->bp.keys_tmp, ctx->bp.vals_tmp, ctx->n, ctx->key_bits,
                    ctx->sort_scratch, s);
   launch_cell_ranges(ctx->n, ctx->grid, ctx->bp, s);
-  launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, false, s);
+  launch_sorted_boxes(B, ctx->bp, s);
+  launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, false, ctx->spheres_only, s);
   const uint32_t ngrid = ctx->n - ctx->bp.nplanes;
   exclusive_scan_u32(ctx->bp.counts, ctx->bp.offsets, ngrid, ctx->total, ctx->scan_scratch, s);
   CK(hipMemcpyAsync(ctx->hstage, ctx->total, 4, hipMemcpyDeviceToHost, s));
@@ -596,7 +600,7 @@ static pehip_status detect(pehip_ctx* ctx) {
   if (fl & ERR_NONFINITE) return fail(ctx, PEHIP_NONFINITE, "non-finite body position or bounding box");
   if (fl & ERR_LEVEL_OVERFLOW) return fail(ctx, PEHIP_INVALID_ARG, "body larger than the coarsest grid cell");
   CKS(ensure_contact_capacity(ctx, nc));
-  if (nc) launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, true, s);
+  if (nc) launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, true, ctx->spheres_only, s);
   ctx->nc = nc;
   return PEHIP_OK;
 }

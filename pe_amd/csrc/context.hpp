@@ -48,6 +48,7 @@ struct pehip_ctx {
   double *pnormal = nullptr, *pd = nullptr, *aabb = nullptr, *inv_mass = nullptr, *Ib = nullptr, *Ibinv = nullptr;
   double *force = nullptr, *torque = nullptr, *sb = nullptr, *dvw = nullptr;
   std::vector<uint8_t> h_fixed, h_shape;
+  bool spheres_only = false;  // every body a sphere or a plane (the pair kernels' narrowphase)
   // union members (spheres), grouped by union; mstart has n + 1 entries when nm > 0
   uint32_t nm = 0;
   uint32_t *mstart = nullptr, *mowner = nullptr, *mmat = nullptr;
@@ -125,6 +126,7 @@ struct pehip_ctx {
     dfree(force); dfree(torque); dfree(sb); dfree(dvw); dfree(planes);
     dfree(bp.keys); dfree(bp.vals); dfree(bp.keys_tmp); dfree(bp.vals_tmp); dfree(bp.cell_start);
     dfree(bp.cell_end); dfree(bp.counts); dfree(bp.offsets); dfree(scan_scratch); dfree(sort_scratch);
+    dfree(bp.saabb); dfree(bp.sfixed);
     dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial); dfree(df_done);
     dfree(isl.parent); dfree(isl.ncont); dfree(isl.cmask); dfree(isl.cursor); dfree(isl.place); dfree(isl.blocal);
     n = 0;

@@ -322,16 +322,27 @@ struct WriteEmit {
 // 27-cell neighbourhood and emitted by the body with the smaller index; a body also
 // scans the 27-cell neighbourhood of its min-corner cell in every coarser level; then it
 // is tested against every plane.  The filter is pe's: not both fixed, overlapping AABBs.
+// Candidates are read from the sorted-order copies (P.saabb, P.sfixed): the bodies of one cell
+// are consecutive there, so a cell's boxes are contiguous instead of gathered by body index.
+// The narrowphase of a scene holding spheres and planes only: the calls narrowphase() makes for
+// a grid sphere a against a sphere or a plane b, without the other shapes' routines, whose
+// registers and stack would otherwise size the pair kernels.
 template <class Emit>
+__device__ __forceinline__ void narrowphase_spheres(const Bodies& B, uint32_t a, uint32_t b, Emit& emit) {
+  if (B.shape[b] == kShapePlane) np_sphere_plane(B, body_sphere(B, a), b, emit);
+  else np_sphere_sphere(body_sphere(B, a), body_sphere(B, b), emit);
+}
+
+template <bool SPHERES, class Emit>
 __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const BroadphaseBuffers& P,
                                 uint32_t s, Emit& emit, uint32_t* flags) {
   const uint32_t a = P.vals[s];
   const uint32_t la = P.keys[s] / G.table;
-  const double* aa = B.aabb + 6 * (size_t)a;
+  const size_t W = P.stride;
   double abox[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) abox[k] = aa[k];
-  const bool fa = B.fixed[a] != 0;
+  for (int k = 0; k < 6; ++k) abox[k] = P.saabb[k * W + s];
+  const bool fa = P.sfixed[s] != 0;
   for (int l = (int)la; l < G.nlevels; ++l) {
     long long c[3];
     bucket_of(G, l, abox, c);
@@ -358,9 +369,13 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
       for (uint32_t t = st; t < en; ++t) {
         const uint32_t b = P.vals[t];
         if (l == (int)la && b <= a) continue;
-        if (fa && B.fixed[b]) continue;
-        if (!aabb_overlap(abox, B.aabb + 6 * (size_t)b)) continue;
-        narrowphase(B, a, b, emit);
+        if (fa && P.sfixed[t]) continue;
+        double bbox[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) bbox[k] = P.saabb[k * W + t];
+        if (!aabb_overlap(abox, bbox)) continue;
+        if constexpr (SPHERES) narrowphase_spheres(B, a, b, emit);
+        else narrowphase(B, a, b, emit);
       }
     }
   }
@@ -368,26 +383,41 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
     const uint32_t p = P.planes[k];
     if (fa) continue;  // planes are fixed
     if (!aabb_overlap(abox, B.aabb + 6 * (size_t)p)) continue;
-    narrowphase(B, a, p, emit);
+    if constexpr (SPHERES) narrowphase_spheres(B, a, p, emit);
+    else narrowphase(B, a, p, emit);
   }
 }
 
+template <bool SPHERES>
 __global__ void k_pairs_count(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ngrid) return;
   CountEmit e;
-  enumerate_pairs(B, G, P, s, e, P.flags);
+  enumerate_pairs<SPHERES>(B, G, P, s, e, P.flags);
   P.counts[s] = e.n;
 }
 
+template <bool SPHERES>
 __global__ void k_pairs_write(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid, RawContacts R,
                               const double* csf, const double* cdf, uint32_t nmat) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   // the count pass found this body's contacts; most bodies have none (the enumeration is
-  // deterministic, so the walk would emit nothing) and their wavefronts end at once
+  // deterministic, so the walk would emit nothing) and their lanes end at once.  (Measured:
+  // walking a compacted list of the bodies with contacts instead takes longer, 286 against
+  // 163 us on the 200k pile: too few wavefronts are left to hide the walk's memory latency.)
   if (s >= ngrid || P.counts[s] == 0) return;
   WriteEmit e{&B, &R, csf, cdf, nmat, P.offsets[s]};
-  enumerate_pairs(B, G, P, s, e, P.flags);
+  enumerate_pairs<SPHERES>(B, G, P, s, e, P.flags);
+}
+
+__global__ void k_sorted_boxes(Bodies B, BroadphaseBuffers P, uint32_t ngrid) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ngrid) return;
+  const uint32_t b = P.vals[s];
+  const double* a = B.aabb + 6 * (size_t)b;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) P.saabb[(size_t)k * P.stride + s] = a[k];
+  P.sfixed[s] = B.fixed[b];
 }
 
 void launch_cell_keys(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, hipStream_t s) {
@@ -400,14 +430,23 @@ void launch_cell_ranges(uint32_t n, const GridParams& G, BroadphaseBuffers& P, h
   if (ngrid) hipLaunchKernelGGL(k_cell_ranges, dim3((ngrid + 255) / 256), dim3(256), 0, s, P.keys, ngrid, P.cell_start, P.cell_end);
 }
 
+void launch_sorted_boxes(const Bodies& B, BroadphaseBuffers& P, hipStream_t s) {
+  const uint32_t ngrid = B.n - P.nplanes;
+  if (ngrid) hipLaunchKernelGGL(k_sorted_boxes, dim3((ngrid + 255) / 256), dim3(256), 0, s, B, P, ngrid);
+}
+
 void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, const RawContacts& R,
-                  const double* csf, const double* cdf, uint32_t nmat, bool write, hipStream_t s) {
+                  const double* csf, const double* cdf, uint32_t nmat, bool write, bool spheres_only, hipStream_t s) {
   const uint32_t ngrid = B.n - P.nplanes;
   if (!ngrid) return;
-  if (!write)
-    hipLaunchKernelGGL(k_pairs_count, dim3((ngrid + 127) / 128), dim3(128), 0, s, B, G, P, ngrid);
-  else
-    hipLaunchKernelGGL(k_pairs_write, dim3((ngrid + 127) / 128), dim3(128), 0, s, B, G, P, ngrid, R, csf, cdf, nmat);
+  const dim3 g((ngrid + 127) / 128), t(128);
+  if (!write) {
+    if (spheres_only) hipLaunchKernelGGL(k_pairs_count<true>, g, t, 0, s, B, G, P, ngrid);
+    else hipLaunchKernelGGL(k_pairs_count<false>, g, t, 0, s, B, G, P, ngrid);
+  } else {
+    if (spheres_only) hipLaunchKernelGGL(k_pairs_write<true>, g, t, 0, s, B, G, P, ngrid, R, csf, cdf, nmat);
+    else hipLaunchKernelGGL(k_pairs_write<false>, g, t, 0, s, B, G, P, ngrid, R, csf, cdf, nmat);
+  }
 }
 
 }  // namespace pe_amd

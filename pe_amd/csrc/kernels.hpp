@@ -52,6 +52,9 @@ struct BroadphaseBuffers {
   uint32_t* counts;    // n+1 per sorted slot
   uint32_t* offsets;   // n+1
   uint32_t* flags;     // error flags
+  double* saabb;       // 6 * n: the grid bodies' AABBs in sorted-slot order, field-major (saabb[k * n + s])
+  uint8_t* sfixed;     // n: fixed flags in sorted-slot order
+  uint32_t stride;     // n (field stride of saabb)
 };
 
 // Scratch of the device-wide primitives: sized by scan_scratch_words / sort_scratch_words.
@@ -74,10 +77,13 @@ uint32_t launch_kinetic(const Bodies& B, double* partial, hipStream_t s);
 // Broadphase: cell keys + sort + cell ranges.
 void launch_cell_keys(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, hipStream_t s);
 void launch_cell_ranges(uint32_t n, const GridParams& G, BroadphaseBuffers& P, hipStream_t s);
-// Pair enumeration + narrowphase; count pass (write = false) fills P.counts,
-// write pass emits into R at P.offsets.
+// The grid bodies' AABBs and fixed flags gathered into sorted-slot order (after the sort).
+void launch_sorted_boxes(const Bodies& B, BroadphaseBuffers& P, hipStream_t s);
+// Pair enumeration + narrowphase; count pass (write = false) fills P.counts, write pass emits
+// into R at P.offsets.  spheres_only: the scene holds spheres and planes only (a narrowphase of
+// those two shapes, and smaller kernels).
 void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, const RawContacts& R,
-                  const double* csf, const double* cdf, uint32_t nmat, bool write, hipStream_t s);
+                  const double* csf, const double* cdf, uint32_t nmat, bool write, bool spheres_only, hipStream_t s);
 
 // Colouring of the contact conflict graph.
 struct ColourBuffers {
