@@ -418,6 +418,8 @@ def main():
                     help="c5: adjoint sweep threshold (absolute change of the adjoint impulses, whose scale is ~1e-5)")
     ap.add_argument("--schedule", type=int, default=0, choices=[0, 1, 2, 3, 4],
                     help="sweep schedule: 0 auto, 1 one launch per colour, 2 dataflow, 3 one workgroup, 4 islands")
+    ap.add_argument("--exact-trig", action="store_true",
+                    help="correctly rounded sin/cos/atan2/acos in the solver (pehip_params.exact_trig)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="no per-launch HIP events in the timed region")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-scenes measurement")
@@ -464,6 +466,7 @@ def main():
         w.setMaxIterations(sc.max_iterations)
         w.setRelaxationModel(sc.model)
         w.setSweepSchedule(args.schedule)
+        w.setExactTrig(args.exact_trig)
         return w
 
     w = make_world()
@@ -536,6 +539,7 @@ def main():
         "config": {"workload": (WORKLOAD_TEXT[args.workload] % {"n": args.spheres, "job": job}) + "; " + window,
                    "bodies": nbodies, "sweeps_per_step": sc.max_iterations, "dt": sc.dt,
                    "window_first_step": first, "window_steps": K, "job_steps": job,
+                   "trig": "correctly rounded" if args.exact_trig else "device libm",
                    "parallelism": "single GPU"},
         "contact_iterations_per_s": ci / solve_s if (solve_s > 0 and ci > 0) else None,
         "mean_contacts": float(np.mean(contacts)), "max_contacts": int(np.max(contacts)),

@@ -675,6 +675,8 @@ class CollisionSystem {
   }
   real getMaximumPenetration() const { return detail::E().stats().max_penetration; }
   size_t getNumberOfContacts() const { return (size_t)detail::E().stats().num_contacts; }
+  // No pe counterpart: the sweep schedule of the last step (pehip_sweep_schedule; -1 without sweeps).
+  int getLastSweepSchedule() const { return (int)detail::E().stats().schedule_used; }
   pehip_stats getStats() const { return detail::E().stats(); }
 
  private:

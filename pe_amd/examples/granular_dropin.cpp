@@ -49,7 +49,8 @@ static int run(int argc, char** argv) {
 
   world->run(steps, dt);
 
-  std::printf("contacts %zu maxpen %a\n", cs->getNumberOfContacts(), cs->getMaximumPenetration());
+  std::printf("contacts %zu maxpen %a schedule %d\n", cs->getNumberOfContacts(), cs->getMaximumPenetration(),
+              cs->getLastSweepSchedule());
   for (size_t b = 0; b < world->size(); ++b) {
     BodyID body = (*world)[b];
     const Vec3& p = body->getPosition();

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from pe_amd import World
+from pe_amd._abi import SCHEDULE_COLOUR_LAUNCHES, SCHEDULE_DATAFLOW
 
 pytestmark = pytest.mark.gpu
 
@@ -59,3 +60,10 @@ def test_cpp_dropin_matches_python_world(ncaps, nunions):
     assert len(rows) == w.size
     assert int(head[1]) == w.getNumberOfContacts() and int(head[1]) > 0
     assert np.array_equal(pos, st["pos"]) and np.array_equal(vel, st["v"])
+    # The automatic schedule: the dataflow sweeps, unless the device refuses the child process's
+    # cooperative launch (observed while this test process holds cooperative launches of its own),
+    # in which case the library says so once on stderr and sweeps with colour launches -- the same
+    # bits, as the comparison above shows for whichever ran.
+    refused = "dataflow sweep launch refused" in r.stderr
+    assert int(head[5]) == (SCHEDULE_COLOUR_LAUNCHES if refused else SCHEDULE_DATAFLOW), (head, r.stderr)
+    print("child schedule:", "colour launches (cooperative launch refused)" if refused else "dataflow")
