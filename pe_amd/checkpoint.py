@@ -21,10 +21,19 @@ shape's parameters: a sphere's radius, a box's three edge lengths, a capsule's r
 length, nothing for a plane (Marshalling.h:184-210, 290-300, 321-330, 352-356, 416-420, 498-501).  Reals are doubles unless the file
 was written with a smaller floating-point size (the header's first size byte: 4 = single).
 
+A union (Marshalling.h:523-570) is the common record without a material, then its total mass
+(real), its body-frame inertia (9 reals, row by row), its bounding box (xmin, ymin, zmin, xmax,
+ymax, zmax), its number of members (size_t) and per member a type byte and the member's record:
+a member carries its position relative to the union after its global one and no velocities
+(Marshalling.h:184-197), so a sphere member is sid, uid, visible, fixed, position (3), relative
+position (3), orientation (4), material, radius.
+
 The reader creates the local chunk's bodies first, then the global chunk's
 (BodyBinaryReader.cpp:265-268), keeping their system IDs; a plane's normal is its orientation
 applied to +z and its displacement the normal's dot product with its position
-(src/core/rigidbody/Plane.cpp:121-142).  Cylinders, meshes and unions are not shapes of this
+(src/core/rigidbody/Plane.cpp:121-142); a union keeps the file's mass, inertia and orientation and
+its members their positions (BodyBinaryReader.cpp:432-500, 594-603; Union.cpp:100-133).
+Cylinders, meshes, unions of anything but spheres and fixed unions are not bodies of this
 library: reading a file that holds any raises ``ValueError``.
 """
 import math
@@ -33,7 +42,7 @@ import struct
 import numpy as np
 
 from ._abi import BOX, CAPSULE, PLANE, SPHERE, UNION
-from .world import mat3_inverse
+from .world import CONTACT_THRESHOLD, mat3_inverse
 
 # pe/core/GeomType.h:45-53
 SPHERE_TYPE, BOX_TYPE, CAPSULE_TYPE, CYLINDER_TYPE, PLANE_TYPE, MESH_TYPE, UNION_TYPE = 1, 2, 3, 4, 5, 6, 7
@@ -106,10 +115,55 @@ def _marshal_bodies(out, a, idx, gtype, uid):
         out.reals(a["param"][i][:_NPARAM[gtype]])
 
 
+def union_aabb(gpos, radius):
+    """A union's bounding box: the merge of its spheres' boxes, each the sphere's centre -/+ its
+    radius plus the contact threshold (SphereBase.h:208-218, UnionBase.cpp:209-231)."""
+    lo = [math.inf] * 3
+    hi = [-math.inf] * 3
+    for g, r in zip(gpos, radius):
+        length = float(r) + CONTACT_THRESHOLD
+        for j in range(3):
+            lo[j] = min(lo[j], float(g[j]) - length)
+            hi[j] = max(hi[j], float(g[j]) + length)
+    return lo + hi
+
+
+def _marshal_unions(out, world, a, idx, uid):
+    """Unions of spheres as Marshalling.h:523-570 writes them: a member's orientation is its
+    union's (the spheres are created unrotated inside the union and turn with it)."""
+    out.put("B", UNION_TYPE)
+    out.put("I", len(idx))
+    if not len(idx):
+        return
+    m = world.members()
+    owner = m["owner"]
+    for i in idx:
+        fixed = int(a["fixed"][i])
+        out.put("QQBB", int(a["sid"][i]), int(uid[i]), 1, fixed)
+        out.reals(a["pos"][i])
+        out.reals(a["q"][i])
+        out.reals(a["v"][i])
+        out.reals(a["w"][i])
+        out.reals([world.unionMass(int(i))])
+        out.reals(a["I"][i])
+        sel = np.flatnonzero(owner == i)
+        out.reals(union_aabb(m["gpos"][sel], m["radius"][sel]))
+        out.put("Q", len(sel))
+        for k in sel:
+            out.put("B", SPHERE_TYPE)
+            out.put("QQBB", int(m["sid"][k]), int(m["uid"][k]), 1, fixed)
+            out.reals(m["gpos"][k])
+            out.reals(m["rpos"][k])
+            out.reals(a["q"][i])
+            out.put("Q", int(m["mat"][k]))
+            out.reals([m["radius"][k]])
+
+
 def write_body_file(world, filename, fp_size=0):
     """Writes the world's bodies as pe's BodyBinaryWriter::writeFile does for one process
-    (planes as global bodies, spheres and boxes as local ones).  fp_size: 0 = doubles (the
-    size of ``real``), 2 = single precision, 3 = double precision (setFloatingPointSize)."""
+    (planes as global bodies, spheres, boxes, capsules and unions as local ones).  fp_size: 0 =
+    doubles (the size of ``real``), 2 = single precision, 3 = double precision
+    (setFloatingPointSize)."""
     if fp_size not in (0, 2, 3):
         raise ValueError("Invalid floating point conversion size.")
     real = "f" if fp_size == 2 else "d"
@@ -117,8 +171,6 @@ def write_body_file(world, filename, fp_size=0):
     n = len(a["shape"])
     uid = a.get("uid", np.zeros(n, np.uint64))
     shape = np.asarray(a["shape"])
-    if np.any(shape == UNION):
-        raise ValueError("unions are not written to body files")
     planes = np.flatnonzero(shape == PLANE)
     glob = _Out(real)
     glob.put("Q", len(planes))                     # global ID counter
@@ -127,6 +179,9 @@ def write_body_file(world, filename, fp_size=0):
     loc = _Out(real)
     loc.put("Q", int(world._next_sid))              # ID counter
     for t in _LOCAL_ORDER:
+        if t == UNION_TYPE:
+            _marshal_unions(loc, world, a, np.flatnonzero(shape == UNION), uid)
+            continue
         idx = np.flatnonzero(shape == _SHAPE_OF[t]) if t in _SHAPE_OF else []
         _marshal_bodies(loc, a, idx, t, uid)
     head = _Out(real)
@@ -145,7 +200,9 @@ def write_body_file(world, filename, fp_size=0):
 def read_body_file(filename):
     """Parses a pe binary body file.  Returns {"counter", "global_counter", "bodies"} with the
     bodies in the order pe's reader creates them (local chunks, then the global chunk), each a
-    dict: type, sid, uid, visible, fixed, pos, q, v, w, material, param, global."""
+    dict: type, sid, uid, visible, fixed, pos, q, v, w, material, param, global; a union has
+    mass, I, aabb and members (each: sid, uid, visible, fixed, pos, rpos, q, material, radius)
+    in place of material and param."""
     with open(filename, "rb") as f:
         data = f.read()
     h = _In(data)
@@ -173,6 +230,9 @@ def read_body_file(filename):
         while r.pos < hi - lo:
             t, = r.get("B")
             cnt, = r.get("I")
+            if t == UNION_TYPE:
+                out += [dict(_unmarshal_union(r), **{"global": is_global}) for _ in range(cnt)]
+                continue
             if cnt and t not in _NPARAM:
                 raise ValueError("Unsupported geometry type %d in body file" % t)
             for _ in range(cnt):
@@ -195,13 +255,38 @@ def read_body_file(filename):
     return {"counter": counter, "global_counter": gcounter, "bodies": bodies}
 
 
+def _unmarshal_union(r):
+    """One union record (Marshalling.h:560-620), members restricted to spheres."""
+    sid, uid, vis, fixed = r.get("QQBB")
+    pos, q, v, w = r.reals(3), r.reals(4), r.reals(3), r.reals(3)
+    mass, = r.reals(1)
+    I, aabb = r.reals(9), r.reals(6)
+    size, = r.get("Q")
+    members = []
+    for _ in range(size):
+        mt, = r.get("B")
+        if mt != SPHERE_TYPE:
+            raise ValueError("Unsupported geometry type %d in a union" % mt)
+        msid, muid, mvis, mfixed = r.get("QQBB")
+        mpos, mrpos, mq = r.reals(3), r.reals(3), r.reals(4)
+        mmat, = r.get("Q")
+        mrad, = r.reals(1)
+        members.append({"sid": msid, "uid": muid, "visible": bool(mvis), "fixed": bool(mfixed), "pos": mpos,
+                        "rpos": mrpos, "q": mq, "material": mmat, "radius": mrad})
+    return {"type": UNION_TYPE, "sid": sid, "uid": uid, "visible": bool(vis), "fixed": bool(fixed), "pos": pos,
+            "q": q, "v": v, "w": w, "mass": mass, "I": I, "aabb": aabb, "members": members}
+
+
 def load_body_file(world, filename):
     """Creates the file's bodies in an empty world whose materials exist, as pe's
     BodyBinaryReader::readFile does (system IDs, orientations and velocities from the file;
-    masses and inertias from the shapes and materials, as the instantiate* functions do)."""
+    masses and inertias from the shapes and materials, as the instantiate* functions do; a
+    union's from the file, Union.cpp:100-133)."""
     info = read_body_file(filename)
     B = info["bodies"]
     n = len(B)
+    mem = {"owner": [], "sid": [], "uid": [], "mat": [], "radius": [], "rpos": [], "gpos": []}
+    union_mass = {}
     dens = np.array([m.density for m in world.materials])
     a = {"shape": np.zeros(n, np.uint8), "mat": np.zeros(n, np.uint32), "fixed": np.zeros(n, np.uint8),
          "param": np.zeros((n, 3)), "pos": np.zeros((n, 3)), "q": np.zeros((n, 4)), "v": np.zeros((n, 3)),
@@ -210,6 +295,31 @@ def load_body_file(world, filename):
     sids = np.zeros(n, np.uint64)
     for i, b in enumerate(B):
         t = b["type"]
+        if t == UNION_TYPE:
+            if b["fixed"]:
+                raise ValueError("union %d is fixed: fixed unions are not supported" % b["sid"])
+            if not b["members"]:
+                raise ValueError("union %d has no members" % b["sid"])
+            for m in b["members"]:
+                if m["material"] >= len(dens):
+                    raise ValueError("body %d uses material %d, which the world does not have"
+                                     % (m["sid"], m["material"]))
+                mem["owner"].append(i)
+                mem["sid"].append(m["sid"])
+                mem["uid"].append(m["uid"])
+                mem["mat"].append(m["material"])
+                mem["radius"].append(m["radius"])
+                mem["rpos"].append(m["rpos"])
+                mem["gpos"].append(m["pos"])
+            a["shape"][i] = UNION
+            a["mat"][i] = b["members"][0]["material"]
+            a["pos"][i], a["q"][i], a["v"][i], a["w"][i] = b["pos"], b["q"], b["v"], b["w"]
+            a["uid"][i] = b["uid"]
+            sids[i] = b["sid"]
+            a["I"][i] = b["I"]
+            a["inv_mass"][i] = 1.0 / b["mass"]
+            union_mass[i] = b["mass"]
+            continue
         if b["material"] >= len(dens):
             raise ValueError("body %d uses material %d, which the world does not have" % (b["sid"], b["material"]))
         a["shape"][i] = _SHAPE_OF[t]
@@ -261,5 +371,10 @@ def load_body_file(world, filename):
     a["Iinv"][fx] = 0.0
     a["v"][fx] = 0.0
     a["w"][fx] = 0.0
-    world.add_bodies(a, sids=sids, next_sid=max(int(info["counter"]), int(sids.max()) + 1 if n else 0))
+    top = max([int(sids.max()) if n else 0] + [int(x) for x in mem["sid"]])
+    members = None
+    if mem["owner"]:
+        members = {k: np.asarray(v) for k, v in mem.items()}
+    world.add_bodies(a, sids=sids, next_sid=max(int(info["counter"]), top + 1 if n else 0), members=members,
+                     union_mass=union_mass)
     return info

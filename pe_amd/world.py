@@ -95,6 +95,15 @@ class Material:
         self.poisson, self.young, self.stiffness, self.dampingN, self.dampingT = poisson, young, stiffness, dampingN, dampingT
 
 
+def union_param(rpos, radii):
+    """A union's radii for the broadphase and for the solver's body records: the radius of a
+    sphere about its centre of mass that holds every member (widened by 1e-12 so that rounding
+    never lets a member poke out) and the smallest member radius."""
+    bound = max(math.sqrt(float(rp[0]) * float(rp[0]) + float(rp[1]) * float(rp[1]) + float(rp[2]) * float(rp[2]))
+                + float(r) for rp, r in zip(rpos, radii))
+    return (bound * (1.0 + 1e-12), min(float(r) for r in radii), 0.0)
+
+
 class World:
     """One pe world (``theWorld()`` + ``theCollisionSystem()`` of the reference)."""
 
@@ -121,6 +130,8 @@ class World:
         self._uid = []
         # union members (spheres), grouped by union in creation order
         self._m_owner, self._m_sid, self._m_mat, self._m_rad, self._m_rpos, self._m_gpos = [], [], [], [], [], []
+        self._m_uid = []
+        self._u_mass = {}        # union body index -> total mass (Union::getMass)
         self._arrays = None      # host SoA snapshot (numpy) when built
         self._ctx = None
         self._bodies_dirty = True
@@ -315,12 +326,14 @@ class World:
         """A union of spheres, as pe builds one inside pe_CREATE_UNION and places it with
         Union::setPosition (Union.cpp:251-272, 1482-1545; UnionBase.cpp:240-360).
 
-        spheres: (offset, radius, material) per sphere, offsets before placement.  The union takes
-        the next system ID, then each sphere its own.  After the last sphere: total mass and centre
+        spheres: (offset, radius, material) or (offset, radius, material, uid) per sphere, offsets
+        before placement; a sphere without a user ID takes the union's (as createGranularParticle's
+        spheres do, Union.cpp:2376-2402).  The union takes the next system ID, then each sphere its own.  After the last sphere: total mass and centre
         of mass, the spheres' positions relative to it, the moment of inertia about it (parallel
         axes), all with the identity orientation; then the centre of mass moves to gpos and every
         sphere by the same displacement."""
-        spheres = [(np.asarray(o, float).reshape(3), float(r), int(m)) for o, r, m in spheres]
+        muid = [int(t[3]) if len(t) > 3 else int(uid) for t in spheres]
+        spheres = [(np.asarray(t[0], float).reshape(3), float(t[1]), int(t[2])) for t in spheres]
         if not spheres:
             raise ValueError("Invalid number of spheres")
         for _, r, m in spheres:
@@ -370,15 +383,14 @@ class World:
         Iinv = mat3_inverse(np.asarray(I))
         g = [float(x) for x in np.asarray(gpos, float).reshape(3)]
         dp = [g[j] - com[j] for j in range(3)]
-        bound = max(math.sqrt(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]) + r
-                    for rp, (_, r, _) in zip(rpos, spheres))
-        rmin = min(r for _, r, _ in spheres)
-        idx = self._add(UNION, spheres[0][2], False, (bound * (1.0 + 1e-12), rmin, 0.0), g, (1.0, 0.0, 0.0, 0.0),
-                        inv_mass, np.asarray(I), Iinv, (0.0, 0.0, 0.0), 0.0, uid)
+        idx = self._add(UNION, spheres[0][2], False, union_param(rpos, [r for _, r, _ in spheres]), g,
+                        (1.0, 0.0, 0.0, 0.0), inv_mass, np.asarray(I), Iinv, (0.0, 0.0, 0.0), 0.0, uid)
+        self._u_mass[idx] = mass
         for k in range(n):
             self._m_owner.append(idx)
             self._m_sid.append(self._next_sid)
             self._next_sid += 1
+            self._m_uid.append(muid[k])
             self._m_mat.append(spheres[k][2])
             self._m_rad.append(spheres[k][1])
             self._m_rpos.append(np.asarray(rpos[k], float))
@@ -391,10 +403,12 @@ class World:
         return self.createUnion(uid, gpos, granular_particle_spheres(rng, radius, material))
 
     def members(self):
-        """Union members: owner body, system ID, material, radius, relative and global position."""
+        """Union members: owner body, system ID, user ID, material, radius, relative and global
+        position."""
         self._sync_from_device()
         k = len(self._m_owner)
         return {"owner": np.asarray(self._m_owner, np.uint32), "sid": np.asarray(self._m_sid, np.uint64),
+                "uid": np.asarray(self._m_uid, np.uint64),
                 "mat": np.asarray(self._m_mat, np.uint32), "radius": np.asarray(self._m_rad, float),
                 "rpos": np.asarray(self._m_rpos, float).reshape(k, 3), "gpos": np.asarray(self._m_gpos, float).reshape(k, 3)}
 
@@ -425,17 +439,55 @@ class World:
         self._w[body] = np.zeros(3)
         self._bodies_dirty = True
 
-    def add_bodies(self, arrays, sids=None, next_sid=None):
+    def unionMass(self, body):
+        """Total mass of a union (Union::getMass), as created or as read from a body file."""
+        return self._u_mass[body]
+
+    def add_bodies(self, arrays, sids=None, next_sid=None, members=None, union_mass=None):
         """Bulk creation from SoA arrays in body order (as a sequence of create* calls; or with
-        given system IDs, as a body file reader instantiates bodies)."""
+        given system IDs, as a body file reader instantiates bodies).
+
+        Unions come with their stored mass properties, as pe's reader instantiates them
+        (Union.cpp:100-133, Union::add skipping the recomputation inside pe_INSTANTIATE_UNION,
+        Union.cpp:1522-1524): a union row's inv_mass, I and Iinv are taken as given, union_mass
+        maps its body index to its total mass, and members (a dict as ``members()`` returns, grouped
+        by union in body order) holds its spheres.  A union's bounding radii are its members'."""
         self._materialise_lists()
         base = len(self._shape)
         n = len(arrays["shape"])
         if base:
             raise ValueError("bulk creation only into an empty world")
-        if np.any(np.asarray(arrays["shape"]) == UNION):
-            raise ValueError("unions are created with createUnion")
-        self._arrays = dict(arrays)
+        shape = np.asarray(arrays["shape"])
+        unions = np.flatnonzero(shape == UNION)
+        arrays = dict(arrays)
+        if len(unions):
+            if members is None:
+                raise ValueError("unions need their members")
+            owner = np.asarray(members["owner"], np.int64)
+            if np.any(np.diff(owner) < 0) or np.any(shape[owner] != UNION):
+                raise ValueError("members must be grouped by union in body order")
+            if set(owner.tolist()) != set(unions.tolist()):
+                raise ValueError("union without members")
+            if np.any(np.asarray(arrays["fixed"])[unions]):
+                raise ValueError("fixed unions are not supported")
+            k = len(owner)
+            rad = np.asarray(members["radius"], float)
+            rpos = np.asarray(members["rpos"], float).reshape(k, 3)
+            arrays["param"] = np.array(arrays["param"], float)
+            for u in unions:
+                sel = np.flatnonzero(owner == u)
+                arrays["param"][u] = union_param(rpos[sel], rad[sel])
+            self._m_owner = owner.tolist()
+            self._m_sid = [int(x) for x in members["sid"]]
+            self._m_uid = [int(x) for x in members.get("uid", np.asarray(arrays["uid"])[owner])]
+            self._m_mat = [int(x) for x in members["mat"]]
+            self._m_rad = rad.tolist()
+            self._m_rpos = list(rpos)
+            self._m_gpos = list(np.asarray(members["gpos"], float).reshape(k, 3))
+            im = np.asarray(arrays["inv_mass"], float)
+            self._u_mass = {int(u): (float(union_mass[int(u)]) if union_mass is not None else 1.0 / float(im[u]))
+                            for u in unions}
+        self._arrays = arrays
         if sids is None:
             self._arrays["sid"] = np.arange(self._next_sid, self._next_sid + n, dtype=np.uint64)
             self._next_sid += n

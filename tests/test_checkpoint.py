@@ -8,7 +8,7 @@ import pytest
 
 import oracle as O
 from pe_amd import World, checkpoint, scenes
-from pe_amd._abi import BOX, CAPSULE, PLANE, SPHERE
+from pe_amd._abi import BOX, CAPSULE, PLANE, SPHERE, UNION
 
 
 def mixed_world():
@@ -239,12 +239,171 @@ def test_restart_from_file_on_gpu(tmp_path):
     assert len(k1) > 10 and k1 == k2
 
 
-def test_union_world_is_refused(tmp_path):
-    # the writer covers spheres, boxes, capsules and planes; a world with a union is refused
-    # instead of being written without it
+def union_world():
+    """Spheres, a box, unions (one with its own member user IDs) and planes, with velocities."""
+    rng = np.random.default_rng(21)
     w = World()
-    m = w.createMaterial("g", 2500.0, 0.0, 0.2, 0.2)
-    w.createSphere(1, (0.0, 0.0, 1.0), 0.5, m)
-    w.createUnion(2, (2.0, 0.0, 1.0), [((0.0, 0.0, 0.0), 0.3, m), ((0.4, 0.0, 0.0), 0.2, m)])
-    with pytest.raises(ValueError, match="unions"):
-        checkpoint.write_body_file(w, str(tmp_path / "u.dat"))
+    g = w.createMaterial("glass", 2500.0, 0.0, 0.2, 0.2)
+    s = w.createMaterial("steel", 7800.0, 0.0, 0.3, 0.3)
+    w.createPlane(1, (0, 0, 1), 0.0, g)
+    w.createSphere(2, (0.0, 0.0, 1.0), 0.5, g)
+    w.createBox(3, (3.0, 0.0, 1.0), (0.5, 0.6, 0.7), s)
+    w.createUnion(4, (2.0, 0.0, 1.0), [((0.0, 0.0, 0.0), 0.3, g, 40), ((0.4, 0.1, 0.0), 0.2, s, 41)])
+    for k in range(3):
+        w.createGranularParticle(10 + k, rng.uniform(1, 6, 3), 0.5, g, rng=rng)
+    w.createPlane(5, (1, 0, 0), -1.0, s)
+    w.createSphere(6, (5.0, 5.0, 1.0), 0.4, s)
+    a = w.host_arrays()
+    fin = a["shape"] != PLANE
+    a["v"][fin] = rng.uniform(-1, 1, (int(fin.sum()), 3))
+    a["w"][fin] = rng.uniform(-2, 2, (int(fin.sum()), 3))
+    return w
+
+
+def reader_order(a):
+    """pe's reader order with unions: spheres, boxes, capsules, unions (local), planes (global)."""
+    s = a["shape"]
+    return np.concatenate([np.flatnonzero(s == t) for t in (SPHERE, BOX, CAPSULE, UNION, PLANE)])
+
+
+def copy_materials(w):
+    w2 = World()
+    for m in w.materials:
+        w2.createMaterial(m.name, m.density, m.cor, m.csf, m.cdf)
+    return w2
+
+
+def test_union_round_trip(tmp_path):
+    # unions are local bodies after capsules; reading them back keeps the file's mass, body
+    # inertia, orientation, velocities and the members' IDs, materials, radii and positions
+    # (BodyBinaryReader.cpp:432-500, 594-603), and writing the read world gives the same bytes
+    w = union_world()
+    f = tmp_path / "u.bin"
+    checkpoint.write_body_file(w, str(f))
+    w2 = copy_materials(w)
+    checkpoint.load_body_file(w2, str(f))
+    a, b = w.host_arrays(), w2.host_arrays()
+    order = reader_order(a)
+    assert len(order) == len(b["shape"]) == 9
+    for k in ("shape", "sid", "uid", "mat", "fixed", "pos", "q", "v", "w", "param", "inv_mass", "I", "Iinv"):
+        assert np.array_equal(np.asarray(a[k])[order], np.asarray(b[k])), k
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    m1, m2 = w.members(), w2.members()
+    assert np.array_equal(inv[m1["owner"]], m2["owner"])   # unions stay in creation order
+    for k in ("sid", "uid", "mat", "radius", "rpos", "gpos"):
+        assert np.array_equal(m1[k], m2[k]), k
+    assert list(m1["uid"][:2]) == [40, 41] and set(m1["uid"][2:]) == {10, 11, 12}
+    for u in np.flatnonzero(a["shape"] == UNION):
+        assert w2.unionMass(int(inv[u])) == w.unionMass(int(u))
+    assert w2._next_sid == w._next_sid
+    f2 = tmp_path / "u2.bin"
+    checkpoint.write_body_file(w2, str(f2))
+    assert f2.read_bytes() == f.read_bytes()
+
+
+def test_union_byte_layout(tmp_path):
+    # one union of two spheres: its record (Marshalling.h:523-570) ends the local chunk
+    w = World()
+    m = w.createMaterial("m", 1000.0, 0.0, 0.1, 0.1)
+    w.createUnion(9, (1.0, 2.0, 3.0), [((0.0, 0.0, 0.0), 0.5, m), ((1.0, 0.0, 0.0), 0.25, m, 77)])
+    a = w.host_arrays()
+    a["v"][0] = (0.5, -0.25, 2.0)
+    f = tmp_path / "one_union.bin"
+    checkpoint.write_body_file(w, str(f))
+    data = f.read_bytes()
+    off_l = struct.unpack(">I", data[17:21])[0]
+    l = data[off_l:]
+    assert struct.unpack(">Q", l[:8])[0] == 4                 # the union and its two spheres: IDs 1-3
+    # sphere, box, capsule, cylinder and mesh blocks are empty; then one union
+    assert l[8:33] == bytes([1, 0, 0, 0, 0, 2, 0, 0, 0, 0, 3, 0, 0, 0, 0, 4, 0, 0, 0, 0, 6, 0, 0, 0, 0])
+    assert l[33:38] == bytes([7, 0, 0, 0, 1])
+    r = l[38:]
+    assert struct.unpack(">QQBB", r[:18]) == (1, 9, 1, 0)
+    x = struct.unpack(">13d", r[18:18 + 104])
+    assert x[:3] == (1.0, 2.0, 3.0) and x[3:7] == (1.0, 0.0, 0.0, 0.0) and x[7:10] == (0.5, -0.25, 2.0)
+    o = 18 + 104
+    # closed forms: masses 4/3 pi r^3 rho; the centre of mass a ninth of the way to the small
+    # sphere (mass ratio 8:1); parallel axes about it
+    m1, m2 = 4.0 / 3.0 * math.pi * 0.125 * 1000.0, 4.0 / 3.0 * math.pi * 0.015625 * 1000.0
+    mass, = struct.unpack(">d", r[o:o + 8])
+    assert mass == w.unionMass(0) and abs(mass - (m1 + m2)) < 1e-12 * mass
+    I = struct.unpack(">9d", r[o + 8:o + 80])
+    d1, d2 = 1.0 / 9.0, 8.0 / 9.0
+    ixx = 0.4 * m1 * 0.25 + 0.4 * m2 * 0.0625
+    iyy = ixx + m1 * d1 * d1 + m2 * d2 * d2
+    assert abs(I[0] - ixx) < 1e-12 * ixx and abs(I[4] - iyy) < 1e-12 * iyy and abs(I[8] - iyy) < 1e-12 * iyy
+    assert I[1] == I[2] == I[3] == I[5] == I[6] == I[7] == 0.0
+    aabb = struct.unpack(">6d", r[o + 80:o + 128])
+    g1 = 1.0 - d1                                           # the big sphere's x after placement
+    exp = (g1 - 0.5 - 1e-8, 2.0 - 0.5 - 1e-8, 3.0 - 0.5 - 1e-8, g1 + 1.0 + 0.25 + 1e-8, 2.0 + 0.5 + 1e-8,
+           3.0 + 0.5 + 1e-8)
+    assert np.abs(np.array(aabb) - exp).max() < 1e-14
+    assert struct.unpack(">Q", r[o + 128:o + 136])[0] == 2
+    s = r[o + 136:]
+    rec = 1 + 18 + 10 * 8 + 8 + 8                           # type, ids and flags, 3+3+4 reals, material, radius
+    assert len(s) == 2 * rec
+    mem = w.members()
+    for k in range(2):
+        t = s[k * rec:(k + 1) * rec]
+        assert t[0] == 1
+        assert struct.unpack(">QQBB", t[1:19]) == (2 + k, (9, 77)[k], 1, 0)
+        v = struct.unpack(">10d", t[19:99])
+        assert v[:3] == tuple(mem["gpos"][k]) and v[3:6] == tuple(mem["rpos"][k]) and v[6:] == (1.0, 0.0, 0.0, 0.0)
+        assert struct.unpack(">Qd", t[99:115]) == (m, (0.5, 0.25)[k])
+    assert abs(mem["rpos"][0][0] + d1) < 1e-15 and abs(mem["rpos"][1][0] - d2) < 1e-15
+
+
+def test_union_records_rejected(tmp_path):
+    w = World()
+    m = w.createMaterial("m", 1000.0, 0.0, 0.1, 0.1)
+    w.createUnion(9, (1.0, 2.0, 3.0), [((0.0, 0.0, 0.0), 0.5, m), ((1.0, 0.0, 0.0), 0.25, m)])
+    f = tmp_path / "u.bin"
+    checkpoint.write_body_file(w, str(f))
+    data = bytearray(f.read_bytes())
+    off_l = struct.unpack(">I", bytes(data[17:21]))[0]
+    urec = off_l + 38
+    first_member = urec + 18 + 104 + 136
+    bad = tmp_path / "bad.bin"
+    v = bytearray(data)
+    v[first_member] = 2                                      # a box member
+    bad.write_bytes(bytes(v))
+    with pytest.raises(ValueError, match="Unsupported geometry type 2 in a union"):
+        checkpoint.read_body_file(str(bad))
+    v = bytearray(data)
+    v[urec + 17] = 1                                         # a fixed union
+    bad.write_bytes(bytes(v))
+    with pytest.raises(ValueError, match="fixed unions"):
+        checkpoint.load_body_file(copy_materials(w), str(bad))
+    w3 = World()                                             # a member material the world lacks
+    with pytest.raises(ValueError, match="material 0"):
+        checkpoint.load_body_file(w3, str(f))
+
+
+@pytest.mark.gpu
+def test_union_restart_continues_bit_for_bit(tmp_path):
+    # granular particles above a floor, created in pe's reader order (unions, then planes): a
+    # world read back from a body file written mid-run continues exactly as the original
+    rng = np.random.default_rng(5)
+    w = World()
+    g = w.createMaterial("granular", 2500.0, 0.0, 0.3, 0.3)
+    w.setGravity(0.0, 0.0, -9.81)
+    for k in range(60):
+        i, j, l = k % 5, (k // 5) % 4, k // 20
+        w.createGranularParticle(k, (0.6 + 1.02 * i, 0.6 + 1.02 * j, 0.55 + 1.02 * l), 0.5, g, rng=rng)
+    for nrm, d in (((0, 0, 1), 0.0), ((1, 0, 0), 0.0), ((-1, 0, 0), -5.2), ((0, 1, 0), 0.0), ((0, -1, 0), -4.2)):
+        w.createPlane(100, nrm, d, g)
+    w.run(80, 2e-3)
+    f = tmp_path / "mid.bin"
+    checkpoint.write_body_file(w, str(f))
+    w2 = copy_materials(w)
+    w2.setGravity(0.0, 0.0, -9.81)
+    checkpoint.load_body_file(w2, str(f))
+    a = w.host_arrays()
+    assert np.array_equal(reader_order(a), np.arange(len(a["shape"])))
+    assert w.getNumberOfContacts() > 15
+    w.run(40, 2e-3)
+    w2.run(40, 2e-3)
+    s1, s2 = w.getState(), w2.getState()
+    for k in ("pos", "q", "v", "w", "mgpos"):
+        assert np.array_equal(s1[k], s2[k]), k
