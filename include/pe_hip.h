@@ -273,8 +273,10 @@ const char* pehip_version(void);
  * step through its converged contact impulses at fixed active set (each contact keeps its mode:
  * separating, static or sliding), solving the transposed linearised fixed-point system with
  * Gauss-Seidel sweeps in the forward's colouring.  Supported: spheres and planes, relaxation
- * models 0 and 1, coloured mode, no damping.  Derivatives are of the converged solve, so the
- * forward should run to a tight abort_threshold with enough max_iterations.
+ * models 0, 1 and 3, coloured mode, no damping.  Derivatives are of the converged solve, so the
+ * forward should run to a tight abort_threshold with enough max_iterations.  Model 3 is
+ * differentiated at the exact minimiser on the friction circle, which its golden-section search
+ * approximates: raise max_sub_iterations (about 80) for a forward run that meets it to rounding.
  * ------------------------------------------------------------------------------------------ */
 typedef struct {
   uint64_t steps;          /* steps differentiated */

@@ -19,7 +19,8 @@ from .world import World, ApproximateInelasticCoulombContactByDecoupling
 
 class DifferentiableRun:
     def __init__(self, scene, steps, dt=None, checkpoint_every=20, model=ApproximateInelasticCoulombContactByDecoupling,
-                 max_iterations=4000, abort_threshold=1e-13, adjoint_sweeps=4000, adjoint_tol=1e-13, device=0):
+                 max_iterations=4000, abort_threshold=1e-13, adjoint_sweeps=4000, adjoint_tol=1e-13, device=0,
+                 max_sub_iterations=20):
         self.scene = scene
         self.steps = int(steps)
         self.dt = scene.dt if dt is None else dt
@@ -30,6 +31,7 @@ class DifferentiableRun:
         self.adjoint_sweeps = adjoint_sweeps
         self.adjoint_tol = adjoint_tol
         self.device = device
+        self.max_sub_iterations = int(max_sub_iterations)
         self.world = None
         self.last_stats = None
 
@@ -40,6 +42,7 @@ class DifferentiableRun:
         w.setMaxIterations(self.max_iterations)
         w.setRelaxationModel(self.model)
         w.setAbortThreshold(self.abort_threshold)
+        w.setMaxSubIterations(self.max_sub_iterations)
         nm = len(self.scene.materials)
         for i in range(nm):
             for j in range(nm):

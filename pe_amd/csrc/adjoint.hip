@@ -88,10 +88,9 @@ static void restore_state(pehip_ctx* ctx, const double* src) {
 
 static pehip_status check_supported(pehip_ctx* ctx, const pehip_params& P) {
   if (P.relaxation_model != PEHIP_INELASTIC_FRICTIONLESS &&
-      P.relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING)
-    return fail(ctx, PEHIP_NOT_SUPPORTED,
-                "the adjoint supports relaxation models 0 and 1 (model 3's golden-section search is a discrete "
-                "search, not a differentiable map)");
+      P.relaxation_model != PEHIP_APPROX_INELASTIC_COULOMB_DECOUPLING &&
+      P.relaxation_model != PEHIP_INELASTIC_COULOMB_DECOUPLING)
+    return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint supports relaxation models 0, 1 and 3");
   if (P.sweep_mode != PEHIP_SWEEP_COLOURED) return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint runs in coloured mode");
   if (P.damping < 1.0) return fail(ctx, PEHIP_NOT_SUPPORTED, "the adjoint does not support damping");
   for (uint32_t i = 0; i < ctx->n; ++i)

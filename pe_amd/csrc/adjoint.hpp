@@ -1,4 +1,4 @@
-// Adjoint of the time step (spheres and planes; relaxation models 0 and 1): the per-contact map
+// Adjoint of the time step (spheres and planes; relaxation models 0, 1 and 3): the per-contact map
 // whose Jacobian the reverse pass needs, written once for any scalar type so that dual numbers
 // give its exact directional derivatives (internal to the library).
 //
@@ -32,8 +32,22 @@ PE_HD Dual sqrt(Dual a) {
   const double s = ::sqrt(a.v);
   return mkd(s, s > 0.0 ? 0.5 * a.d / s : 0.0);
 }
+PE_HD Dual cos(Dual a) { return mkd(::cos(a.v), -::sin(a.v) * a.d); }
+PE_HD Dual sin(Dual a) { return mkd(::sin(a.v), ::cos(a.v) * a.d); }
 PE_HD double val(double a) { return a; }
 PE_HD double val(Dual a) { return a.v; }
+PE_HD double der(double) { return 0.0; }
+PE_HD double der(Dual a) { return a.d; }
+// a value of type T with the given derivative (dropped for plain doubles)
+template <class T>
+PE_HD T with_d(double v, double d);
+template <>
+PE_HD double with_d<double>(double v, double) { return v; }
+template <>
+PE_HD Dual with_d<Dual>(double v, double d) { return mkd(v, d); }
+// the value as a dual number with zero derivative
+PE_HD Dual zd(double a) { return mkd(a); }
+PE_HD Dual zd(Dual a) { return mkd(a.v); }
 template <class T>
 PE_HD T cst(double v) { return T(v); }
 template <>
@@ -133,6 +147,99 @@ PE_HD M3<T> inv(const M3<T>& m) {
   return r;
 }
 
+// The quantities that pe's decoupled passes of InelasticCoulombContactByDecoupling read
+// (HCSITS.h:2395-2520): the contact frame, lever arms, world inverse inertias, relative velocity
+// with the contact's own impulse removed, gap rate, 1 / D_nn, the tangential block of D and mu.
+template <class S>
+struct SlideQ {
+  V3<S> n, t, o, r1, r2, gdot;
+  M3<S> I1, I2;
+  S ddt, dni, A00, A01, A10, A11, mu;
+};
+
+template <class S>
+PE_HD SlideQ<Dual> strip(const SlideQ<S>& Q) {
+  SlideQ<Dual> R;
+  R.n = v3(zd(Q.n.x), zd(Q.n.y), zd(Q.n.z)); R.t = v3(zd(Q.t.x), zd(Q.t.y), zd(Q.t.z));
+  R.o = v3(zd(Q.o.x), zd(Q.o.y), zd(Q.o.z)); R.r1 = v3(zd(Q.r1.x), zd(Q.r1.y), zd(Q.r1.z));
+  R.r2 = v3(zd(Q.r2.x), zd(Q.r2.y), zd(Q.r2.z)); R.gdot = v3(zd(Q.gdot.x), zd(Q.gdot.y), zd(Q.gdot.z));
+  for (int k = 0; k < 9; ++k) { R.I1.a[k] = zd(Q.I1.a[k]); R.I2.a[k] = zd(Q.I2.a[k]); }
+  R.ddt = zd(Q.ddt); R.dni = zd(Q.dni);
+  R.A00 = zd(Q.A00); R.A01 = zd(Q.A01); R.A10 = zd(Q.A10); R.A11 = zd(Q.A11);
+  R.mu = zd(Q.mu);
+  return R;
+}
+
+// Residuals of the passes' fixed point with the normal impulse c > 0 and the tangential impulse
+// on the cone's circle at angle a, x = mu c (cos a, sin a):
+//   F1 = c + dni (n . (gdot + coupling(t x0 + o x1)) + ddt)    (the normal pass, HCSITS.h:2406-2414)
+//   F2 = x'(a) . (As x + g_to(c)) / (mu c)                     (x minimises 0.5 x.A x + x.g_to on the
+//                                                              circle, the golden-section search's
+//                                                              target, HCSITS.h:2416-2509)
+// where coupling(p) = (I1 (r1 x p)) x r1 + (I2 (r2 x p)) x r2, g_to = (t, o) . (gdot + coupling(n c))
+// and As = (A + A^T) / 2, the part of A the objective sees (A is symmetric up to rounding).
+template <class S>
+PE_HD void slide_residual(const SlideQ<S>& Q, S c, S a, S& F1, S& F2) {
+  const S ca = cos(a), sa = sin(a);
+  const S x0 = Q.mu * c * ca, x1 = Q.mu * c * sa;
+  const V3<S> pt = scale(Q.t, x0) + scale(Q.o, x1);
+  const V3<S> gc = Q.gdot + cross(mv(Q.I1, cross(Q.r1, pt)), Q.r1) + cross(mv(Q.I2, cross(Q.r2, pt)), Q.r2);
+  F1 = c + Q.dni * (dot(Q.n, gc) + Q.ddt);
+  const V3<S> pn = scale(Q.n, c);
+  const V3<S> g2 = Q.gdot + cross(mv(Q.I1, cross(Q.r1, pn)), Q.r1) + cross(mv(Q.I2, cross(Q.r2, pn)), Q.r2);
+  const S as = cst<S>(0.5) * (Q.A01 + Q.A10);
+  const S y0 = Q.A00 * x0 + as * x1 + dot(Q.t, g2);
+  const S y1 = as * x0 + Q.A11 * x1 + dot(Q.o, g2);
+  F2 = -sa * y0 + ca * y1;
+}
+
+// Jacobian of (F1, F2) with respect to (c, a) at values, with the residual.
+PE_HD void slide_jacobian(const SlideQ<Dual>& Q0, double c, double a, double J[4], double F[2]) {
+  Dual F1, F2;
+  slide_residual(Q0, mkd(c, 1.0), mkd(a), F1, F2);
+  J[0] = F1.d; J[2] = F2.d;
+  F[0] = F1.v; F[1] = F2.v;
+  slide_residual(Q0, mkd(c), mkd(a, 1.0), F1, F2);
+  J[1] = F1.d; J[3] = F2.d;
+}
+
+// The sliding contact's impulse (c, x0, x1) in the contact frame as a function of the contact's
+// inputs: the fixed point (c, a) of the decoupled passes with the exact minimiser on the circle,
+// found by Newton's method from the forward run's converged impulse (c0, a0), and differentiated
+// by the implicit function theorem: d(c, a) = -J^-1 dF.  The golden-section search of the forward
+// run approximates this minimiser; with enough steps it meets it to rounding.
+template <class T>
+PE_HD void slide_fixed_point(const SlideQ<T>& Q, double c0, double a0, T& pc0, T& pc1, T& pc2) {
+  const SlideQ<Dual> Q0 = strip(Q);
+  double c = c0, a = a0, J[4], F[2];
+  slide_jacobian(Q0, c, a, J, F);
+  for (int it = 0; it < 4; ++it) {
+    const double det = J[0] * J[3] - J[1] * J[2];
+    if (!(det != 0.0) || !isfinite(det)) break;
+    const double dc = -(J[3] * F[0] - J[1] * F[1]) / det;
+    const double da = -(-J[2] * F[0] + J[0] * F[1]) / det;
+    double Jn[4], Fn[2];
+    slide_jacobian(Q0, c + dc, a + da, Jn, Fn);
+    if (!(fabs(Fn[0]) + fabs(Fn[1]) < fabs(F[0]) + fabs(F[1]))) break;   // no further progress
+    c += dc; a += da;
+    for (int k = 0; k < 4; ++k) J[k] = Jn[k];
+    F[0] = Fn[0]; F[1] = Fn[1];
+  }
+  // derivative of F with respect to the inputs at fixed (c, a)
+  T F1, F2;
+  slide_residual(Q, cst<T>(c), cst<T>(a), F1, F2);
+  const double det = J[0] * J[3] - J[1] * J[2];
+  double dc = 0.0, da = 0.0;
+  if (det != 0.0 && isfinite(det)) {
+    dc = -(J[3] * der(F1) - J[1] * der(F2)) / det;
+    da = -(-J[2] * der(F1) + J[0] * der(F2)) / det;
+  }
+  const T cT = with_d<T>(c, dc), aT = with_d<T>(a, da);
+  pc0 = cT;
+  pc1 = Q.mu * cT * cos(aT);
+  pc2 = Q.mu * cT * sin(aT);
+}
+
 // Constants of one contact (at the evaluation point).
 struct ContactConst {
   double rad1, rad2;         // radii (rad2 unused when body 2 is a plane)
@@ -147,9 +254,11 @@ struct ContactConst {
 constexpr int kAdjIn = 22;   // x1(3) x2(3) U1(6) U2(6) p(3) mu
 constexpr int kAdjOut = 15;  // E2 (3), -contrib (12)
 
-// One contact's part of E1/E2 (see the top of this file) for relaxation model MODEL (0 or 1).
+// One contact's part of E1/E2 (see the top of this file) for relaxation model MODEL (0, 1 or 3).
 // pe's branch structure is followed on the values, so the derivative is the one of the branch
-// (active mode) the contact is in.
+// (active mode) the contact is in.  Model 3's sliding branch is the fixed point of its decoupled
+// passes with the exact minimiser on the friction circle (slide_fixed_point), located from the
+// converged impulse p; a sliding contact whose normal impulse is zero has zero impulse.
 template <class T, int MODEL>
 PE_HD void contact_map(const ContactConst& K, const T* z, T* out) {
   const V3<T> x1 = v3(z[0], z[1], z[2]);
@@ -211,7 +320,7 @@ PE_HD void contact_map(const ContactConst& K, const T* z, T* out) {
     const T dni = cst<T>(1.0) / D.a[0];
     if (MODEL == 0) {
       pn = scale(n, -(dni * g.x));
-    } else {
+    } else if (MODEL == 1) {
       const M3<T> Di = inv(D);
       const V3<T> s = mv(Di, g);
       T pc0 = -s.x, pc1 = -s.y, pc2 = -s.z;
@@ -237,6 +346,32 @@ PE_HD void contact_map(const ContactConst& K, const T* z, T* out) {
           const T fr = flimit / sqrt(fsq);
           pc1 = pc1 * fr;
           pc2 = pc2 * fr;
+        }
+      }
+      pn = mv(C, v3(pc0, pc1, pc2));
+    } else if (MODEL == 3) {
+      const M3<T> Di = inv(D);
+      const V3<T> s = mv(Di, g);
+      T pc0 = -s.x, pc1 = -s.y, pc2 = -s.z;
+      const T flimit = mu * pc0;
+      const T fsq = pc1 * pc1 + pc2 * pc2;
+      if (val(fsq) > val(flimit) * val(flimit) || val(pc0) < 0.0) {
+        // dynamic (HCSITS.h:2395-2520): located from the converged impulse's contact-frame parts
+        const double c0 = val(n.x) * val(p.x) + val(n.y) * val(p.y) + val(n.z) * val(p.z);
+        const double q1 = val(t.x) * val(p.x) + val(t.y) * val(p.y) + val(t.z) * val(p.z);
+        const double q2 = val(o.x) * val(p.x) + val(o.y) * val(p.y) + val(o.z) * val(p.z);
+        if (!(c0 > 0.0)) {
+          pc0 = pc1 = pc2 = cst<T>(0.0);
+        } else if (val(mu) == 0.0) {
+          // no friction: the tangential impulse is zero and the normal pass alone remains
+          pc0 = -(dni * g.x);
+          pc1 = pc2 = cst<T>(0.0);
+        } else {
+          SlideQ<T> Q;
+          Q.n = n; Q.t = t; Q.o = o; Q.r1 = r1; Q.r2 = r2; Q.gdot = gdot; Q.I1 = I1; Q.I2 = I2;
+          Q.ddt = ddt; Q.dni = dni; Q.mu = mu;
+          Q.A00 = D.a[4]; Q.A01 = D.a[5]; Q.A10 = D.a[7]; Q.A11 = D.a[8];
+          slide_fixed_point(Q, c0, atan2(q2, q1), pc0, pc1, pc2);
         }
       }
       pn = mv(C, v3(pc0, pc1, pc2));

@@ -1,4 +1,4 @@
-// Kernels of the adjoint time step (spheres and planes, relaxation models 0 and 1).
+// Kernels of the adjoint time step (spheres and planes, relaxation models 0, 1 and 3).
 //
 // For one recorded step (contacts in sweep order with their converged impulses, positions at
 // the start of the step, solver body records, post-solve velocities):
@@ -263,7 +263,8 @@ void adj_jacobian(int model, const AdjTape& T, const AdjConsts& A, double* jac, 
   if (!T.nc) return;
   const size_t n = (size_t)T.nc * kAdjIn;
   if (model == 0) hipLaunchKernelGGL(k_adj_jacobian<0>, blocks(n), dim3(256), 0, s, T, A, jac);
-  else hipLaunchKernelGGL(k_adj_jacobian<1>, blocks(n), dim3(256), 0, s, T, A, jac);
+  else if (model == 1) hipLaunchKernelGGL(k_adj_jacobian<1>, blocks(n), dim3(256), 0, s, T, A, jac);
+  else hipLaunchKernelGGL(k_adj_jacobian<3>, blocks(n), dim3(256), 0, s, T, A, jac);
 }
 void adj_prepare(uint32_t nc, const double* jac, double* minv, hipStream_t s) {
   if (nc) hipLaunchKernelGGL(k_adj_prepare, blocks(nc), dim3(256), 0, s, nc, jac, minv);

@@ -194,6 +194,14 @@ class World:
         self._schedule = int(schedule)
         self._params_dirty = True
 
+    def setMaxSubIterations(self, n):
+        """Golden-section steps of relaxation models 3 and 5 (pe's maxSubIterations_,
+        HCSITS.h:305, 423: fixed at 20 there, with no setter)."""
+        if int(n) < 0:
+            raise ValueError("Invalid number of sub-iterations")
+        self._max_sub = int(n)
+        self._params_dirty = True
+
     def setExactTrig(self, on=True):
         """Correctly rounded sin/cos/atan2/acos in the solver (reproduces the host libm)."""
         self._exact_trig = 1 if on else 0
