@@ -393,7 +393,7 @@ def test_union_restart_continues_bit_for_bit(tmp_path):
         w.createGranularParticle(k, (0.6 + 1.02 * i, 0.6 + 1.02 * j, 0.55 + 1.02 * l), 0.5, g, rng=rng)
     for nrm, d in (((0, 0, 1), 0.0), ((1, 0, 0), 0.0), ((-1, 0, 0), -5.2), ((0, 1, 0), 0.0), ((0, -1, 0), -4.2)):
         w.createPlane(100, nrm, d, g)
-    w.run(80, 2e-3)
+    w.run(200, 2e-3)                   # landed on the floor and on each other
     f = tmp_path / "mid.bin"
     checkpoint.write_body_file(w, str(f))
     w2 = copy_materials(w)
