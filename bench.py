@@ -174,8 +174,10 @@ def batched_c4(torch, args):
 def batched_cpu(scs, mid, seconds):
     """Oracle scenes/s over the 4,096 scenes: 32 seeded scenes (the smallest and the largest among
     them) timed from the batch state at step 150; per-scene step times scaled by body count to the
-    whole batch.  One process, then one process per available CPU (up to 16) running the sample
-    concurrently, each on its own share of the scenes."""
+    whole batch.  One process, then one process per available CPU (up to 16), each running the
+    whole sample at the same time: the all-cores rate is the sum of the processes' rates under
+    that load (a share of the sample per process would scale a few scenes' body-count ratio up to
+    the batch, which made the estimate depend on which scenes a share got)."""
     import multiprocessing as mp
     rng = np.random.default_rng(4096)
     sizes = np.array([sc.n_bodies for sc in scs])
@@ -189,22 +191,18 @@ def batched_cpu(scs, mid, seconds):
     one = 1.0 / (300 * t_batch_step)
     model, ncpu = host_cpu()
     P = max(1, min(16, ncpu or 1))
-    parts = [(idx[j::P], states[j::P], seconds) for j in range(P)]
     t0 = time.perf_counter()
     with mp.get_context("spawn").Pool(P) as pool:
-        outs = pool.map(_oracle_scene_steps, parts)
+        outs = pool.map(_oracle_scene_steps, [(idx, states, seconds)] * P)
     wall = time.perf_counter() - t0
-    # each process steps its share of the sample; the processes ran concurrently, so the
-    # all-cores time per batch step is the slowest share's, scaled by the share's body count
-    t_share = max(sum(t for t, _ in o) * float(sizes.sum()) / max(1, sum(int(sizes[k]) for k in part[0]))
-                  for o, part in zip(outs, parts) if part[0]) / P
-    allc = 1.0 / (300 * t_share)
+    # every process stepped the whole sample concurrently: each one's rate under load, summed
+    allc = sum(1.0 / (300 * sum(t for t, _ in o) * float(sizes.sum()) / b_sample) for o in outs)
     progress("batched cpu baseline: 1 core %.4f scenes/s, %d processes %.4f scenes/s (%.1f s)" % (one, P, allc, wall))
     return {"value": one, "unit": "scenes/s", "cores": 1, "kind": "port", "all_cores_value": allc,
             "all_cores": P, "host_cpu": model, "host_cpus_available": ncpu,
             "sample": "oracle/pe_oracle.cpp (1 thread per process) on %d seeded scenes (%d-%d bodies) from the batch "
                       "state at step 150, about %.0f s each way, per-step time scaled by body count to the 4,096 "
-                      "scenes x 300 steps; all cores: %d processes, each on its share of the sample"
+                      "scenes x 300 steps; all cores: %d processes, each on the whole sample at once, rates summed"
                       % (len(idx), int(sizes[idx].min()), int(sizes[idx].max()), seconds, P)}
 
 
