@@ -109,7 +109,8 @@ void orc_set_sweep_order_ranked(orc_world* w, size_t n, const uint32_t* b1, cons
 size_t orc_num_contacts(const orc_world* w);
 void orc_get_contacts(const orc_world* w, uint32_t* b1, uint32_t* b2, double* pos,
                       double* normal, double* dist, double* mu);
-/* Solved impulses p (world frame, 3 per contact) of the last step. */
+/* Solved impulses p (world frame, 3 per contact) of the last step; zero for the contacts of an
+ * orc_detect call, which has solved nothing (one entry per contact orc_num_contacts counts). */
 void orc_get_impulses(const orc_world* w, double* p);
 
 /* Runs only the detection phase on the current state (no state change) with

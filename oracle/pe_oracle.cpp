@@ -2181,6 +2181,7 @@ void orc_get_impulses(const orc_world* w, double* p) {
 
 size_t orc_detect(orc_world* w, int exhaustive) {
   w->contacts.clear();
+  w->p_.clear();   // no solve has run on the contacts found here: their impulses are zero
   Fine fine{w, &w->contacts};
   try {
     if (exhaustive) w->exhaustiveFindContacts(fine);
@@ -2188,6 +2189,7 @@ size_t orc_detect(orc_world* w, int exhaustive) {
   } catch (const std::exception&) {
     return (size_t)-1;
   }
+  w->p_.assign(w->contacts.size(), V3{});
   return w->contacts.size();
 }
 

@@ -1,6 +1,7 @@
 """Pins the CPU oracle (restatement of pe's time step) with closed forms, brute force and
 invariants.  The reference ships no tests or golden vectors, so these are the anchors."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -481,6 +482,47 @@ def test_orthogonal_projection_resting_unit_sphere(model):
     p = w.impulses()[0]
     assert p[2] == pytest.approx(g * dt, rel=1e-14) and p[0] == 0 and p[1] == 0
     assert np.all(np.abs(w.get_state()["v"][1]) < 1e-15)
+
+
+@pytest.mark.parametrize("model", [O.APPROX_INELASTIC_COULOMB_ORTHO, O.INELASTIC_COULOMB_ORTHO])
+def test_orthogonal_projection_keeps_pe_division_by_zero(model):
+    # pe's dual-cone test (fsq < p_n / mu^2, HCSITS.h:2683) cannot hold for p_n < 0, so a separating
+    # contact (p_n < 0 after the step) with exactly zero tangential velocity reaches the dynamic
+    # branch and divides by |p_t| = 0 (HCSITS.h:2690-2695): mu p_n / 0 = -inf, 0 * -inf = NaN.  The
+    # restatement keeps pe's arithmetic: a sphere in contact but moving straight up off a plane
+    # gets NaN impulses; with a sideways velocity the same contact takes the dynamic branch with a
+    # finite impulse whose normal part is negative (pulling the sphere back), also pe's arithmetic.
+    for vel, nan in (((0.0, 0.0, 0.5), True), ((0.2, 0.0, 0.5), False)):
+        w = make_world(gravity=(0, 0, -9.81), model=model, iters=1)
+        w.create_plane([0, 0, 1], 0.0, 0)
+        w.create_sphere([0, 0, 0.5 - 1e-4], 0.5, 0)
+        st = w.get_state()
+        st["v"][1] = vel
+        w.set_state(st)
+        w.step(1e-3)
+        p = w.impulses()[0]
+        if nan:
+            assert np.all(np.isnan(p))         # NaN tangential parts spread through the frame
+        else:
+            assert np.all(np.isfinite(p)) and p[2] < 0.0
+
+
+def test_sanitized_oracle_run(tmp_path):
+    # the oracle and a driver built with AddressSanitizer and UndefinedBehaviorSanitizer
+    # (oracle/Makefile, target sanitize): a scene of spheres, boxes, capsules, unions and planes
+    # stepped with every relaxation model and both trig modes, a reversed sweep order, an abort
+    # threshold, and HashGrids against exhaustive detection; the sanitizers abort on any finding
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle")
+    out = str(tmp_path / "san")
+    r = subprocess.run(["make", "-s", "-C", here, "sanitize", "OUT=" + out], capture_output=True, text=True,
+                       timeout=900)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([os.path.join(out, "sanitize_check")], capture_output=True, text=True, timeout=900,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1",
+                                UBSAN_OPTIONS="print_stacktrace=1"))
+    assert r.returncode == 0, (r.stdout[-1000:], r.stderr[-3000:])
+    assert "sanitized oracle runs" in r.stdout
 
 
 # ------------------------------------------- generalized maximum dissipation (model 5)
