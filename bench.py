@@ -425,6 +425,9 @@ def main():
                     help="oracle seconds per CPU worker for the batched scenes' CPU baseline")
     ap.add_argument("--save-state", default=None, help="save the state at the end of the timed window (.npz)")
     ap.add_argument("--save-window-start", default=None, help="save the state at the start of the timed window (.npz)")
+    ap.add_argument("--dump-maps", default=None,
+                    help="write the process's memory map (/proc/self/maps) to this file after the result line, to "
+                         "place addresses of a stack trace printed at exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="before the run, 4 launches of a plain elementwise kernel that reads and writes 256 MiB "
                          "(scale reference for hardware byte counters)")
@@ -586,6 +589,10 @@ def main():
         out["cpu_baseline"]["host_cpu"] = model
         out["cpu_baseline"]["host_cpus_available"] = ncpu
     print(json.dumps(out))
+    sys.stdout.flush()
+    if args.dump_maps:
+        with open("/proc/self/maps") as f, open(args.dump_maps, "w") as g:
+            g.write(f.read())
 
 
 if __name__ == "__main__":

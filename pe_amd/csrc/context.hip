@@ -612,18 +612,23 @@ static pehip_status colour(pehip_ctx* ctx) {
   Bodies B = ctx->bodies();
   launch_adjacency(B, ctx->raw, nc, ctx->col, ctx->scan_scratch, s);
   launch_colour_init(ctx->raw, B, nc, ctx->col, s);
-  uint32_t round = 0;
+  // Rounds in batches, the first as long as the last step's colouring, each batch followed by the
+  // histogram and one read-back of (remaining, histogram, flags): a step whose colouring needs no
+  // more rounds than the last one's synchronises once.  A round after the last is a no-op (every
+  // contact is coloured), so the colours do not depend on the batching.
+  uint32_t round = 0, batch = std::max(ctx->colour_rounds, 4u);
   for (;;) {
-    for (int k = 0; k < 4; ++k) launch_colour_round(ctx->raw, B, nc, ctx->col, round++, ctx->flags, s);
-    CK(hipMemcpyAsync(ctx->hstage, ctx->col.remaining, 4, hipMemcpyDeviceToHost, s));
+    for (uint32_t k = 0; k < batch; ++k) launch_colour_round(ctx->raw, B, nc, ctx->col, round++, ctx->flags, s);
+    launch_colour_hist(nc, ctx->col, s);
+    CK(hipMemcpyAsync(ctx->hstage, ctx->col.hist, 4 * kMaxColours, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(ctx->hstage + kMaxColours, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(ctx->hstage + kMaxColours + 1, ctx->col.remaining, 4, hipMemcpyDeviceToHost, s));
     CKS(sync_check(ctx));
-    if (ctx->hstage[0] == 0) break;
+    if (ctx->hstage[kMaxColours + 1] == 0) break;
     if (round > 4u * 4096u) return fail(ctx, PEHIP_HIP_ERROR, "colouring did not terminate");
+    batch = 4;
   }
-  launch_colour_hist(nc, ctx->col, s);
-  CK(hipMemcpyAsync(ctx->hstage, ctx->col.hist, 4 * kMaxColours, hipMemcpyDeviceToHost, s));
-  CK(hipMemcpyAsync(ctx->hstage + kMaxColours, ctx->flags, 4, hipMemcpyDeviceToHost, s));
-  CKS(sync_check(ctx));
+  ctx->colour_rounds = round;
   if (ctx->hstage[kMaxColours] & ERR_COLOURS) return fail(ctx, PEHIP_NOT_SUPPORTED, "more than 128 colours needed");
   uint32_t ncol = 0;
   for (uint32_t k = 0; k < kMaxColours; ++k)

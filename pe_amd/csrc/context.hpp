@@ -89,6 +89,7 @@ struct pehip_ctx {
   IslandBuffers isl{};           // island sweeps: per-body arrays allocated with the bodies, the
                                  // per-contact and per-unit ones carved with the contacts
   std::vector<uint32_t> colour_off;
+  uint32_t colour_rounds = 4;   // rounds the last colouring needed (the next one's first batch)
   unsigned long long *deltas = nullptr, *maxpen = nullptr;
   double* ke_partial = nullptr;
   uint32_t* hstage = nullptr;  // pinned
