@@ -622,13 +622,14 @@ static pehip_status colour(pehip_ctx* ctx) {
     launch_colour_hist(nc, ctx->col, s);
     CK(hipMemcpyAsync(ctx->hstage, ctx->col.hist, 4 * kMaxColours, hipMemcpyDeviceToHost, s));
     CK(hipMemcpyAsync(ctx->hstage + kMaxColours, ctx->flags, 4, hipMemcpyDeviceToHost, s));
-    CK(hipMemcpyAsync(ctx->hstage + kMaxColours + 1, ctx->col.remaining, 4, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(ctx->hstage + kMaxColours + 1, ctx->col.remaining, 8, hipMemcpyDeviceToHost, s));
     CKS(sync_check(ctx));
     if (ctx->hstage[kMaxColours + 1] == 0) break;
     if (round > 4u * 4096u) return fail(ctx, PEHIP_HIP_ERROR, "colouring did not terminate");
     batch = 4;
   }
-  ctx->colour_rounds = round;
+  // the rounds this colouring needed, and one more: the next step's first batch
+  ctx->colour_rounds = ctx->hstage[kMaxColours + 2] + 2;
   if (ctx->hstage[kMaxColours] & ERR_COLOURS) return fail(ctx, PEHIP_NOT_SUPPORTED, "more than 128 colours needed");
   uint32_t ncol = 0;
   for (uint32_t k = 0; k < kMaxColours; ++k)

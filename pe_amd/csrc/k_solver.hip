@@ -99,14 +99,26 @@ __global__ void k_colour_round(RawContacts R, const uint8_t* fixed, uint32_t nc,
   atomicSub(remaining, 1u);
 }
 
-__global__ void k_colour_hist(const uint32_t* colour, uint32_t nc, uint32_t* hist) {
+// Colour histogram, and the last round that coloured a contact (max of cround over the coloured
+// contacts; uncoloured ones hold kUncoloured and are skipped).
+__global__ void k_colour_hist(const uint32_t* colour, const uint32_t* cround, uint32_t nc, uint32_t* hist,
+                              uint32_t* last_round) {
   __shared__ uint32_t h[kMaxColours];
+  __shared__ uint32_t lr;
   if (threadIdx.x < kMaxColours) h[threadIdx.x] = 0;
+  if (threadIdx.x == 0) lr = 0;
   __syncthreads();
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
-    atomicAdd(&h[colour[c]], 1u);
+  uint32_t mr = 0;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    const uint32_t k = colour[c];
+    if (k == kUncoloured) continue;
+    atomicAdd(&h[k], 1u);
+    mr = max(mr, cround[c]);
+  }
+  atomicMax(&lr, mr);
   __syncthreads();
   if (threadIdx.x < kMaxColours && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+  if (threadIdx.x == 0 && lr) atomicMax(last_round, lr);
 }
 
 void launch_colour_init(const RawContacts& R, const Bodies& B, uint32_t nc, ColourBuffers& C, hipStream_t s) {
@@ -122,10 +134,11 @@ void launch_colour_round(const RawContacts& R, const Bodies& B, uint32_t nc, Col
 
 void launch_colour_hist(uint32_t nc, ColourBuffers& C, hipStream_t s) {
   fill_u32(C.hist, 0u, kMaxColours, s);
+  fill_u32(C.remaining + 1, 0u, 1, s);
   if (nc) {
     uint32_t nb = (nc + 255) / 256;
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_colour_hist, dim3(nb), dim3(256), 0, s, C.colour, nc, C.hist);
+    hipLaunchKernelGGL(k_colour_hist, dim3(nb), dim3(256), 0, s, C.colour, C.cround, nc, C.hist, C.remaining + 1);
   }
 }
 

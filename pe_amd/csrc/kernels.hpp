@@ -94,7 +94,7 @@ struct ColourBuffers {
   uint32_t* colour;     // nc
   uint32_t* prio;       // nc
   uint32_t* cround;     // nc: round in which the contact was coloured
-  uint32_t* remaining;  // 1
+  uint32_t* remaining;  // 2: contacts still uncoloured, the last round that coloured one
   uint32_t* hist;       // 128
 };
 constexpr uint32_t kMaxColours = 128;
