@@ -298,6 +298,29 @@ struct CountEmit {
   __device__ void operator()(uint32_t, uint32_t, d3, d3, double) { ++n; }
 };
 
+// Counts a body's contacts and stages the first kStage of them (as WriteEmit writes them).
+struct StageEmit {
+  const Bodies* B;
+  const BroadphaseBuffers* P;
+  const double* csf;
+  const double* cdf;
+  uint32_t nmat;
+  uint32_t base;   // slot * kStage
+  uint32_t n = 0;
+  __device__ void operator()(uint32_t g1, uint32_t g2, d3 gpos, d3 nrm, double dist) {
+    if (n < kStage) {
+      const uint32_t at = base + n;
+      P->st_b1[at] = geom_body(*B, g1);
+      P->st_b2[at] = geom_body(*B, g2);
+      st3(P->st_pos, at, gpos);
+      st3(P->st_nrm, at, nrm);
+      P->st_dist[at] = dist;
+      P->st_mu[at] = contact_friction(*B, csf, cdf, nmat, g1, g2, gpos, nrm);
+    }
+    ++n;
+  }
+};
+
 struct WriteEmit {
   const Bodies* B;
   const RawContacts* R;
@@ -389,25 +412,46 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
 }
 
 template <bool SPHERES>
-__global__ void k_pairs_count(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid) {
+__global__ void k_pairs_count(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid, const double* csf,
+                              const double* cdf, uint32_t nmat) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ngrid) return;
-  CountEmit e;
+  StageEmit e{&B, &P, csf, cdf, nmat, s * kStage};
   enumerate_pairs<SPHERES>(B, G, P, s, e, P.flags);
   P.counts[s] = e.n;
+  if (e.n > kStage) atomicOr(P.flags, (uint32_t)ERR_STAGE_FULL);
 }
 
 template <bool SPHERES>
 __global__ void k_pairs_write(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid, RawContacts R,
                               const double* csf, const double* cdf, uint32_t nmat) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  // the count pass found this body's contacts; most bodies have none (the enumeration is
-  // deterministic, so the walk would emit nothing) and their lanes end at once.  (Measured:
-  // walking a compacted list of the bodies with contacts instead takes longer, 286 against
-  // 163 us on the 200k pile: too few wavefronts are left to hide the walk's memory latency.)
-  if (s >= ngrid || P.counts[s] == 0) return;
+  // the bodies whose contacts the count pass could not stage: the walk again, now writing (the
+  // enumeration is deterministic, so it emits the same contacts in the same order).  (Measured
+  // when this pass wrote every body's contacts: walking a compacted list of the bodies with
+  // contacts took longer than skipping the others, 286 against 163 us on the 200k pile.)
+  if (s >= ngrid || P.counts[s] <= kStage) return;
   WriteEmit e{&B, &R, csf, cdf, nmat, P.offsets[s]};
   enumerate_pairs<SPHERES>(B, G, P, s, e, P.flags);
+}
+
+__global__ void k_pairs_place(uint32_t ngrid, BroadphaseBuffers P, RawContacts R) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ngrid) return;
+  const uint32_t n = P.counts[s];
+  if (n == 0 || n > kStage) return;
+  const uint32_t at = P.offsets[s], from = s * kStage;
+  for (uint32_t k = 0; k < n; ++k) {
+    R.b1[at + k] = P.st_b1[from + k];
+    R.b2[at + k] = P.st_b2[from + k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R.pos[3 * (size_t)(at + k) + c] = P.st_pos[3 * (size_t)(from + k) + c];
+      R.nrm[3 * (size_t)(at + k) + c] = P.st_nrm[3 * (size_t)(from + k) + c];
+    }
+    R.dist[at + k] = P.st_dist[from + k];
+    R.mu[at + k] = P.st_mu[from + k];
+  }
 }
 
 __global__ void k_sorted_boxes(Bodies B, BroadphaseBuffers P, uint32_t ngrid) {
@@ -418,6 +462,10 @@ __global__ void k_sorted_boxes(Bodies B, BroadphaseBuffers P, uint32_t ngrid) {
 #pragma unroll
   for (int k = 0; k < 6; ++k) P.saabb[(size_t)k * P.stride + s] = a[k];
   P.sfixed[s] = B.fixed[b];
+}
+
+void launch_pairs_place(uint32_t ngrid, const BroadphaseBuffers& P, const RawContacts& R, hipStream_t s) {
+  if (ngrid) hipLaunchKernelGGL(k_pairs_place, dim3((ngrid + 255) / 256), dim3(256), 0, s, ngrid, P, R);
 }
 
 void launch_cell_keys(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, hipStream_t s) {
@@ -441,8 +489,8 @@ void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, co
   if (!ngrid) return;
   const dim3 g((ngrid + 127) / 128), t(128);
   if (!write) {
-    if (spheres_only) hipLaunchKernelGGL(k_pairs_count<true>, g, t, 0, s, B, G, P, ngrid);
-    else hipLaunchKernelGGL(k_pairs_count<false>, g, t, 0, s, B, G, P, ngrid);
+    if (spheres_only) hipLaunchKernelGGL(k_pairs_count<true>, g, t, 0, s, B, G, P, ngrid, csf, cdf, nmat);
+    else hipLaunchKernelGGL(k_pairs_count<false>, g, t, 0, s, B, G, P, ngrid, csf, cdf, nmat);
   } else {
     if (spheres_only) hipLaunchKernelGGL(k_pairs_write<true>, g, t, 0, s, B, G, P, ngrid, R, csf, cdf, nmat);
     else hipLaunchKernelGGL(k_pairs_write<false>, g, t, 0, s, B, G, P, ngrid, R, csf, cdf, nmat);

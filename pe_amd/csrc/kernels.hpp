@@ -37,7 +37,8 @@ enum : uint32_t {
   ERR_CAPACITY = 4u,
   ERR_COLOURS = 8u,
   ERR_INTERNAL = 16u,
-  ERR_TIMEOUT = 32u
+  ERR_TIMEOUT = 32u,
+  ERR_STAGE_FULL = 64u   // a body emitted more contacts than the count pass stages (not an error)
 };
 
 struct BroadphaseBuffers {
@@ -55,7 +56,17 @@ struct BroadphaseBuffers {
   double* saabb;       // 6 * n: the grid bodies' AABBs in sorted-slot order, field-major (saabb[k * n + s])
   uint8_t* sfixed;     // n: fixed flags in sorted-slot order
   uint32_t stride;     // n (field stride of saabb)
+  // The count pass keeps the first kStage contacts a body emits, in emission order, at
+  // slot * kStage + k (the raw-contact fields); a body that emits more raises ERR_STAGE_FULL and
+  // is written by the write pass instead.
+  uint32_t* st_b1;     // kStage * n
+  uint32_t* st_b2;     // kStage * n
+  double* st_pos;      // 3 * kStage * n
+  double* st_nrm;      // 3 * kStage * n
+  double* st_dist;     // kStage * n
+  double* st_mu;       // kStage * n
 };
+constexpr uint32_t kStage = 8;
 
 // Scratch of the device-wide primitives: sized by scan_scratch_words / sort_scratch_words.
 size_t scan_scratch_words(size_t n);
@@ -84,6 +95,9 @@ void launch_sorted_boxes(const Bodies& B, BroadphaseBuffers& P, hipStream_t s);
 // those two shapes, and smaller kernels).
 void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, const RawContacts& R,
                   const double* csf, const double* cdf, uint32_t nmat, bool write, bool spheres_only, hipStream_t s);
+// After the count pass and the scan: the staged contacts of bodies with at most kStage of them
+// copied to R at P.offsets (the write pass then covers the bodies with more).
+void launch_pairs_place(uint32_t ngrid, const BroadphaseBuffers& P, const RawContacts& R, hipStream_t s);
 
 // Colouring of the contact conflict graph.
 struct ColourBuffers {
