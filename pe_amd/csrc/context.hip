@@ -269,13 +269,15 @@ pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** ou
     ctx->ncu = (uint32_t)ncu;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&ctx->hstage, 4096, hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void**)&ctx->flags, 64) != hipSuccess ||
-      hipMalloc((void**)&ctx->total, 64) != hipSuccess || hipMalloc((void**)&ctx->maxpen, 64) != hipSuccess ||
-      hipMalloc((void**)&ctx->col.remaining, 64) != hipSuccess || hipMalloc((void**)&ctx->col.hist, 4 * kMaxColours) != hipSuccess) {
+      hipMalloc((void**)&ctx->status, 4 * kStWords) != hipSuccess || hipMalloc((void**)&ctx->maxpen, 64) != hipSuccess) {
     g_create_error = "HIP initialisation failed";
     pehip_destroy(ctx);
     return PEHIP_HIP_ERROR;
   }
+  ctx->flags = ctx->status + kStFlags;
+  ctx->total = ctx->status + kStTotal;
+  ctx->col.remaining = ctx->status + kStRemaining;
+  ctx->col.hist = ctx->status + kStHist;
   if (hipMalloc((void**)&ctx->deltas, sizeof(unsigned long long) * (p.max_iterations + 1)) != hipSuccess) {
     g_create_error = "HIP allocation failed";
     pehip_destroy(ctx);
@@ -308,8 +310,8 @@ pehip_status pehip_destroy(pehip_ctx* ctx) {
   pe_adjoint_free(ctx);
   ctx->free_bodies();
   ctx->free_contacts();
-  dfree(ctx->csf); dfree(ctx->cdf); dfree(ctx->flags); dfree(ctx->total); dfree(ctx->maxpen);
-  dfree(ctx->col.remaining); dfree(ctx->col.hist); dfree(ctx->deltas);
+  dfree(ctx->csf); dfree(ctx->cdf); dfree(ctx->status); dfree(ctx->maxpen); dfree(ctx->deltas);
+  ctx->flags = ctx->total = ctx->col.remaining = ctx->col.hist = nullptr;
   if (ctx->hstage) hipHostFree(ctx->hstage);
   for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
   for (auto& e : ctx->launch_ev) hipEventDestroy(e);
@@ -596,10 +598,9 @@ static pehip_status detect(pehip_ctx* ctx) {
   launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, false, ctx->spheres_only, s);
   const uint32_t ngrid = ctx->n - ctx->bp.nplanes;
   exclusive_scan_u32(ctx->bp.counts, ctx->bp.offsets, ngrid, ctx->total, ctx->scan_scratch, s);
-  CK(hipMemcpyAsync(ctx->hstage, ctx->total, 4, hipMemcpyDeviceToHost, s));
-  CK(hipMemcpyAsync(ctx->hstage + 1, ctx->flags, 4, hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(ctx->hstage, ctx->status, 4 * (kStTotal + 1), hipMemcpyDeviceToHost, s));
   CKS(sync_check(ctx));
-  const uint32_t nc = ctx->hstage[0], fl = ctx->hstage[1];
+  const uint32_t nc = ctx->hstage[kStTotal], fl = ctx->hstage[kStFlags];
   if (fl & ERR_NONFINITE) return fail(ctx, PEHIP_NONFINITE, "non-finite body position or bounding box");
   if (fl & ERR_LEVEL_OVERFLOW) return fail(ctx, PEHIP_INVALID_ARG, "body larger than the coarsest grid cell");
   CKS(ensure_contact_capacity(ctx, nc));
@@ -627,26 +628,24 @@ static pehip_status colour(pehip_ctx* ctx) {
   for (;;) {
     for (uint32_t k = 0; k < batch; ++k) launch_colour_round(ctx->raw, B, nc, ctx->col, round++, ctx->flags, s);
     launch_colour_hist(nc, ctx->col, s);
-    CK(hipMemcpyAsync(ctx->hstage, ctx->col.hist, 4 * kMaxColours, hipMemcpyDeviceToHost, s));
-    CK(hipMemcpyAsync(ctx->hstage + kMaxColours, ctx->flags, 4, hipMemcpyDeviceToHost, s));
-    CK(hipMemcpyAsync(ctx->hstage + kMaxColours + 1, ctx->col.remaining, 8, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(ctx->hstage, ctx->status, 4 * kStWords, hipMemcpyDeviceToHost, s));
     CKS(sync_check(ctx));
-    if (ctx->hstage[kMaxColours + 1] == 0) break;
+    if (ctx->hstage[kStRemaining] == 0) break;
     if (round > 4u * 4096u) return fail(ctx, PEHIP_HIP_ERROR, "colouring did not terminate");
     batch = 4;
   }
   // the rounds this colouring needed, and one more: the next step's first batch
-  ctx->colour_rounds = ctx->hstage[kMaxColours + 2] + 2;
-  if (ctx->hstage[kMaxColours] & ERR_COLOURS) return fail(ctx, PEHIP_NOT_SUPPORTED, "more than 128 colours needed");
+  ctx->colour_rounds = ctx->hstage[kStRemaining + 1] + 2;
+  if (ctx->hstage[kStFlags] & ERR_COLOURS) return fail(ctx, PEHIP_NOT_SUPPORTED, "more than 128 colours needed");
+  const uint32_t* hist = ctx->hstage + kStHist;
   uint32_t ncol = 0;
   for (uint32_t k = 0; k < kMaxColours; ++k)
-    if (ctx->hstage[k]) ncol = k + 1;
+    if (hist[k]) ncol = k + 1;
   ctx->colour_off.assign(ncol + 1, 0);
-  for (uint32_t k = 0; k < ncol; ++k) ctx->colour_off[k + 1] = ctx->colour_off[k] + ctx->hstage[k];
-  // stable sort of contact indices by colour (8-bit keys: one pass)
-  CK(hipMemcpyAsync(ctx->sort_keys, ctx->col.colour, 4 * (size_t)nc, hipMemcpyDeviceToDevice, s));
-  iota_u32(ctx->perm, nc, s);
-  radix_sort_pairs(ctx->sort_keys, ctx->perm, ctx->sort_keys_tmp, ctx->sort_vals_tmp, nc, 8, ctx->csort_scratch, s);
+  for (uint32_t k = 0; k < ncol; ++k) ctx->colour_off[k + 1] = ctx->colour_off[k] + hist[k];
+  // stable sort of contact indices by colour (colours < 128: one 8-bit pass, straight into perm)
+  iota_u32(ctx->sort_vals_tmp, nc, s);
+  radix_sort_pairs_8bit(ctx->col.colour, ctx->sort_vals_tmp, ctx->sort_keys, ctx->perm, nc, ctx->csort_scratch, s);
   ctx->stats.num_colours = ncol;
   return PEHIP_OK;
 }

@@ -65,7 +65,11 @@ struct pehip_ctx {
   int key_bits = 0;
   BroadphaseBuffers bp{};
   uint32_t *planes = nullptr, *scan_scratch = nullptr, *sort_scratch = nullptr;
-  uint32_t *flags = nullptr, *total = nullptr;
+  // One device block for the words the host reads back during a step, so that each read-back is
+  // one copy: kStFlags (error flags; the abort flags at +4), kStTotal (contact count),
+  // kStRemaining (colouring: uncoloured contacts, last round), kStHist (colour histogram).
+  uint32_t* status = nullptr;
+  uint32_t *flags = nullptr, *total = nullptr;   // into status
 
   // contacts: the per-contact arrays below are carved out of one device block, the context's own
   // (cblock, allocated with capacity own_ccap) or, during pehip_step_noalloc, the library's static

@@ -250,6 +250,20 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32
   }
 }
 
+void radix_sort_pairs_8bit(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                           size_t n, uint32_t* scratch, hipStream_t s) {
+  if (n == 0) return;
+  const uint32_t ntiles = (uint32_t)((n + kTile - 1) / kTile);
+  const size_t h = 256 * (size_t)ntiles;
+  uint32_t* hist = scratch;
+  uint32_t* hs = scratch + h;
+  uint32_t* sc = scratch + 2 * h;
+  hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kThreads), 0, s, keys_in, n, 0, hist, ntiles);
+  exclusive_scan_u32(hist, hs, h, nullptr, sc, s);
+  hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kThreads), 0, s, keys_in, vals_in, keys_out, vals_out, n, 0, hs,
+                     ntiles);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Hash-grid keys: a body goes to the finest level whose cell edge is at least its AABB
 // size, and to the cell of its AABB's min corner (floor, periodic wrap into the table).

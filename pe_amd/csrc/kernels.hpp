@@ -74,6 +74,10 @@ size_t sort_scratch_words(size_t n);
 // Exclusive prefix sum of n u32 values; writes the total to *total (device pointer).
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total,
                         uint32_t* scratch, hipStream_t s);
+// One stable pass on the low 8 key bits from (keys_in, vals_in) into (keys_out, vals_out)
+// (keys below 256: a complete sort; scratch as for radix_sort_pairs).
+void radix_sort_pairs_8bit(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                           size_t n, uint32_t* scratch, hipStream_t s);
 // Stable LSD radix sort of (key, value) pairs on the low `bits` key bits; the result
 // ends in keys/vals (tmp arrays are ping-pong space).
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
@@ -112,6 +116,8 @@ struct ColourBuffers {
   uint32_t* hist;       // 128
 };
 constexpr uint32_t kMaxColours = 128;
+// word offsets in the context's status block
+constexpr uint32_t kStFlags = 0, kStTotal = 16, kStRemaining = 18, kStHist = 32, kStWords = kStHist + kMaxColours;
 constexpr uint32_t kUncoloured = 0xFFFFFFFFu;
 void launch_adjacency(const Bodies& B, const RawContacts& R, uint32_t nc, ColourBuffers& C,
                       uint32_t* scan_scratch, hipStream_t s);
