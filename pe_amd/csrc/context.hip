@@ -756,6 +756,7 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
   const uint32_t nc = ctx->nc;
   uint32_t iters = 0;
   int32_t schedule_used = -1;
+  bool dataflow_ran = false;
   double last_delta = 0;
   fill_u64(ctx->maxpen, 0ull, 2, s);
   if (nc) {
@@ -834,9 +835,9 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
         } else {
           if (ctx->profiling) CKS(record_launch_event(ctx, 1));
           ctx->n_launch_events = ctx->profiling ? 2 : 0;
-          CK(hipMemcpyAsync(ctx->hstage, ctx->flags, 4, hipMemcpyDeviceToHost, s));
-          CKS(sync_check(ctx));
-          if (ctx->hstage[0] & ERR_TIMEOUT) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
+          // its timeout flag is read with the step's statistics (one synchronisation per step
+          // fewer): a timed-out step still fails, after the integration it no longer matters for
+          dataflow_ran = true;
           iters = P.max_iterations;
           schedule_used = PEHIP_SCHEDULE_DATAFLOW;
           ++ctx->dataflow_steps;
@@ -883,7 +884,9 @@ pehip_status pe_one_step(pehip_ctx* ctx, double dt) {
   // stats of this step
   CK(hipMemcpyAsync(ctx->hstage, ctx->maxpen, 16, hipMemcpyDeviceToHost, s));
   if (iters) CK(hipMemcpyAsync(ctx->hstage + 4, ctx->deltas + (iters - 1), 8, hipMemcpyDeviceToHost, s));
+  if (dataflow_ran) CK(hipMemcpyAsync(ctx->hstage + 6, ctx->flags, 4, hipMemcpyDeviceToHost, s));
   CKS(sync_check(ctx));
+  if (dataflow_ran && (ctx->hstage[6] & ERR_TIMEOUT)) return fail(ctx, PEHIP_HIP_ERROR, "dataflow sweep timed out");
   double mp;
   std::memcpy(&mp, ctx->hstage, 8);
   unsigned long long nfx;
