@@ -447,7 +447,7 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   CK(dalloc(P.saabb, 6 * (size_t)n)); CK(dalloc(P.sfixed, n));
   CK(dalloc(P.st_b1, (size_t)kStage * n)); CK(dalloc(P.st_b2, (size_t)kStage * n));
   CK(dalloc(P.st_pos, 3 * (size_t)kStage * n)); CK(dalloc(P.st_nrm, 3 * (size_t)kStage * n));
-  CK(dalloc(P.st_dist, (size_t)kStage * n)); CK(dalloc(P.st_mu, (size_t)kStage * n));
+  CK(dalloc(P.st_dist, (size_t)kStage * n));
   P.stride = n;
   ctx->spheres_only = std::all_of(shape, shape + n, [](uint8_t t) { return t == PEHIP_SPHERE || t == PEHIP_PLANE; });
   CK(dalloc(ctx->scan_scratch, scan_scratch_words((size_t)n + 1)));
@@ -605,7 +605,7 @@ static pehip_status detect(pehip_ctx* ctx) {
   if (fl & ERR_LEVEL_OVERFLOW) return fail(ctx, PEHIP_INVALID_ARG, "body larger than the coarsest grid cell");
   CKS(ensure_contact_capacity(ctx, nc));
   if (nc) {
-    launch_pairs_place(ngrid, ctx->bp, ctx->raw, s);
+    launch_pairs_place(B, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, s);
     if (fl & ERR_STAGE_FULL)
       launch_pairs(B, ctx->grid, ctx->bp, ctx->raw, ctx->csf, ctx->cdf, ctx->nmat, true, ctx->spheres_only, s);
   }

@@ -132,7 +132,7 @@ struct pehip_ctx {
     dfree(bp.keys); dfree(bp.vals); dfree(bp.keys_tmp); dfree(bp.vals_tmp); dfree(bp.cell_start);
     dfree(bp.cell_end); dfree(bp.counts); dfree(bp.offsets); dfree(scan_scratch); dfree(sort_scratch);
     dfree(bp.saabb); dfree(bp.sfixed);
-    dfree(bp.st_b1); dfree(bp.st_b2); dfree(bp.st_pos); dfree(bp.st_nrm); dfree(bp.st_dist); dfree(bp.st_mu);
+    dfree(bp.st_b1); dfree(bp.st_b2); dfree(bp.st_pos); dfree(bp.st_nrm); dfree(bp.st_dist);
     dfree(col.deg); dfree(col.adj_start); dfree(col.adj_cursor); dfree(ke_partial); dfree(df_done);
     dfree(isl.parent); dfree(isl.ncont); dfree(isl.cmask); dfree(isl.cursor); dfree(isl.place); dfree(isl.blocal);
     n = 0;

@@ -312,24 +312,20 @@ struct CountEmit {
   __device__ void operator()(uint32_t, uint32_t, d3, d3, double) { ++n; }
 };
 
-// Counts a body's contacts and stages the first kStage of them (as WriteEmit writes them).
+// Counts a body's contacts and stages the first kStage of them: the geometry ids, point, normal
+// and distance (k_pairs_place adds the top-level bodies and the friction, as WriteEmit does).
 struct StageEmit {
-  const Bodies* B;
   const BroadphaseBuffers* P;
-  const double* csf;
-  const double* cdf;
-  uint32_t nmat;
   uint32_t base;   // slot * kStage
   uint32_t n = 0;
   __device__ void operator()(uint32_t g1, uint32_t g2, d3 gpos, d3 nrm, double dist) {
     if (n < kStage) {
       const uint32_t at = base + n;
-      P->st_b1[at] = geom_body(*B, g1);
-      P->st_b2[at] = geom_body(*B, g2);
+      P->st_b1[at] = g1;
+      P->st_b2[at] = g2;
       st3(P->st_pos, at, gpos);
       st3(P->st_nrm, at, nrm);
       P->st_dist[at] = dist;
-      P->st_mu[at] = contact_friction(*B, csf, cdf, nmat, g1, g2, gpos, nrm);
     }
     ++n;
   }
@@ -426,11 +422,10 @@ __device__ void enumerate_pairs(const Bodies& B, const GridParams& G, const Broa
 }
 
 template <bool SPHERES>
-__global__ void k_pairs_count(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid, const double* csf,
-                              const double* cdf, uint32_t nmat) {
+__global__ void k_pairs_count(Bodies B, GridParams G, BroadphaseBuffers P, uint32_t ngrid) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ngrid) return;
-  StageEmit e{&B, &P, csf, cdf, nmat, s * kStage};
+  StageEmit e{&P, s * kStage};
   enumerate_pairs<SPHERES>(B, G, P, s, e, P.flags);
   P.counts[s] = e.n;
   if (e.n > kStage) atomicOr(P.flags, (uint32_t)ERR_STAGE_FULL);
@@ -449,22 +444,25 @@ __global__ void k_pairs_write(Bodies B, GridParams G, BroadphaseBuffers P, uint3
   enumerate_pairs<SPHERES>(B, G, P, s, e, P.flags);
 }
 
-__global__ void k_pairs_place(uint32_t ngrid, BroadphaseBuffers P, RawContacts R) {
+__global__ void k_pairs_place(Bodies B, uint32_t ngrid, BroadphaseBuffers P, RawContacts R, const double* csf,
+                              const double* cdf, uint32_t nmat) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ngrid) return;
   const uint32_t n = P.counts[s];
   if (n == 0 || n > kStage) return;
   const uint32_t at = P.offsets[s], from = s * kStage;
   for (uint32_t k = 0; k < n; ++k) {
-    R.b1[at + k] = P.st_b1[from + k];
-    R.b2[at + k] = P.st_b2[from + k];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      R.pos[3 * (size_t)(at + k) + c] = P.st_pos[3 * (size_t)(from + k) + c];
-      R.nrm[3 * (size_t)(at + k) + c] = P.st_nrm[3 * (size_t)(from + k) + c];
-    }
+    const uint32_t g1 = P.st_b1[from + k], g2 = P.st_b2[from + k];
+    const d3 gpos = mk3(P.st_pos[3 * (size_t)(from + k)], P.st_pos[3 * (size_t)(from + k) + 1],
+                        P.st_pos[3 * (size_t)(from + k) + 2]);
+    const d3 nrm = mk3(P.st_nrm[3 * (size_t)(from + k)], P.st_nrm[3 * (size_t)(from + k) + 1],
+                       P.st_nrm[3 * (size_t)(from + k) + 2]);
+    R.b1[at + k] = geom_body(B, g1);
+    R.b2[at + k] = geom_body(B, g2);
+    st3(R.pos, at + k, gpos);
+    st3(R.nrm, at + k, nrm);
     R.dist[at + k] = P.st_dist[from + k];
-    R.mu[at + k] = P.st_mu[from + k];
+    R.mu[at + k] = contact_friction(B, csf, cdf, nmat, g1, g2, gpos, nrm);
   }
 }
 
@@ -478,8 +476,11 @@ __global__ void k_sorted_boxes(Bodies B, BroadphaseBuffers P, uint32_t ngrid) {
   P.sfixed[s] = B.fixed[b];
 }
 
-void launch_pairs_place(uint32_t ngrid, const BroadphaseBuffers& P, const RawContacts& R, hipStream_t s) {
-  if (ngrid) hipLaunchKernelGGL(k_pairs_place, dim3((ngrid + 255) / 256), dim3(256), 0, s, ngrid, P, R);
+void launch_pairs_place(const Bodies& B, const BroadphaseBuffers& P, const RawContacts& R, const double* csf,
+                        const double* cdf, uint32_t nmat, hipStream_t s) {
+  const uint32_t ngrid = B.n - P.nplanes;
+  if (ngrid)
+    hipLaunchKernelGGL(k_pairs_place, dim3((ngrid + 255) / 256), dim3(256), 0, s, B, ngrid, P, R, csf, cdf, nmat);
 }
 
 void launch_cell_keys(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, hipStream_t s) {
@@ -503,8 +504,8 @@ void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, co
   if (!ngrid) return;
   const dim3 g((ngrid + 127) / 128), t(128);
   if (!write) {
-    if (spheres_only) hipLaunchKernelGGL(k_pairs_count<true>, g, t, 0, s, B, G, P, ngrid, csf, cdf, nmat);
-    else hipLaunchKernelGGL(k_pairs_count<false>, g, t, 0, s, B, G, P, ngrid, csf, cdf, nmat);
+    if (spheres_only) hipLaunchKernelGGL(k_pairs_count<true>, g, t, 0, s, B, G, P, ngrid);
+    else hipLaunchKernelGGL(k_pairs_count<false>, g, t, 0, s, B, G, P, ngrid);
   } else {
     if (spheres_only) hipLaunchKernelGGL(k_pairs_write<true>, g, t, 0, s, B, G, P, ngrid, R, csf, cdf, nmat);
     else hipLaunchKernelGGL(k_pairs_write<false>, g, t, 0, s, B, G, P, ngrid, R, csf, cdf, nmat);

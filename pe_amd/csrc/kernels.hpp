@@ -57,14 +57,13 @@ struct BroadphaseBuffers {
   uint8_t* sfixed;     // n: fixed flags in sorted-slot order
   uint32_t stride;     // n (field stride of saabb)
   // The count pass keeps the first kStage contacts a body emits, in emission order, at
-  // slot * kStage + k (the raw-contact fields); a body that emits more raises ERR_STAGE_FULL and
-  // is written by the write pass instead.
+  // slot * kStage + k (geometry ids, point, normal, distance); a body that emits more raises
+  // ERR_STAGE_FULL and is written by the write pass instead.
   uint32_t* st_b1;     // kStage * n
   uint32_t* st_b2;     // kStage * n
   double* st_pos;      // 3 * kStage * n
   double* st_nrm;      // 3 * kStage * n
   double* st_dist;     // kStage * n
-  double* st_mu;       // kStage * n
 };
 constexpr uint32_t kStage = 8;
 
@@ -100,8 +99,10 @@ void launch_sorted_boxes(const Bodies& B, BroadphaseBuffers& P, hipStream_t s);
 void launch_pairs(const Bodies& B, const GridParams& G, BroadphaseBuffers& P, const RawContacts& R,
                   const double* csf, const double* cdf, uint32_t nmat, bool write, bool spheres_only, hipStream_t s);
 // After the count pass and the scan: the staged contacts of bodies with at most kStage of them
-// copied to R at P.offsets (the write pass then covers the bodies with more).
-void launch_pairs_place(uint32_t ngrid, const BroadphaseBuffers& P, const RawContacts& R, hipStream_t s);
+// placed in R at P.offsets with their bodies and friction (the write pass then covers the bodies
+// with more).
+void launch_pairs_place(const Bodies& B, const BroadphaseBuffers& P, const RawContacts& R, const double* csf,
+                        const double* cdf, uint32_t nmat, hipStream_t s);
 
 // Colouring of the contact conflict graph.
 struct ColourBuffers {
