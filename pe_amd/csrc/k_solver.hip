@@ -380,9 +380,19 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   const uint64_t total = (uint64_t)iters * nc;
+  // the largest impulse change of this wavefront's items of sweep qd, added to deltas[qd] once
+  // per sweep (a wavefront's items are in increasing order, so its sweeps come one after the
+  // other) instead of by every item on its release path
+  double dmax = 0.0;
+  uint32_t qd = (uint32_t)(wave / nc);
   for (uint64_t item = wave; item < total; item += nwaves) {
     const uint32_t q = (uint32_t)(item / nc);
     const uint32_t k = (uint32_t)(item - (uint64_t)q * nc);
+    if (q != qd) {
+      if (lane == 0 && dmax > 0) atomicMax(&deltas[qd], (unsigned long long)__double_as_longlong(dmax));
+      dmax = 0.0;
+      qd = q;
+    }
     const uint32_t s = __builtin_amdgcn_readfirstlane(slots ? slots[k] : k);
     const uint32_t e1 = C.b1[s], e2 = C.b2[s];
     const uint32_t i1 = e1 & ~kFixedBit, i2 = e2 & ~kFixedBit;
@@ -413,8 +423,8 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
     // depends on the counter's loaded value (in-order issue, sc1 loads coherent per access).
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const double d = relax_coherent<MODEL, Trig, SPEC>(C, s, cf, B1, B2, i1, i2, f1, f2, dvw, maxSub);
+    dmax = fmax(dmax, d);
     if (lane == 0) {
-      if (d > 0) atomicMax(&deltas[q], (unsigned long long)__double_as_longlong(d));
       // Release without an agent-scope release: that writes back the XCD's whole L2 (measured
       // 215 -> 248 steps/s without it).  The item's stores are all agent-scope atomics (device
       // scope bit set), which complete only once coherent at device scope, so waiting for them
@@ -427,6 +437,7 @@ __global__ void __launch_bounds__(256) k_sweep_dataflow(SolverContacts C, uint32
       if (!f2) __hip_atomic_fetch_add(done + i2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+  if (lane == 0 && dmax > 0) atomicMax(&deltas[qd], (unsigned long long)__double_as_longlong(dmax));
 }
 
 // Lanes per contact.  A wavefront issues one instruction every four cycles whatever its lanes
