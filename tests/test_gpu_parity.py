@@ -11,6 +11,7 @@ import pytest
 
 import oracle as O
 from pe_amd import World, scenes, SWEEP_REFERENCE_ORDER, SWEEP_COLOURED, PeHipError
+from pe_amd._abi import SPHERE
 
 pytestmark = pytest.mark.gpu
 
@@ -329,6 +330,31 @@ def test_contact_set_bitexact_c1_late_state():
     g = contact_map(w.detect())
     ex = contact_map(ow.detect(exhaustive=True))
     assert len(ex) > 50 and g == ex
+
+
+def test_contact_set_bitexact_beyond_the_staged_contacts():
+    # The count pass stages up to 8 contacts per emitting body; a body that emits more is written
+    # by a second walk.  A sphere with its 12 kissing neighbours (all one size, the centre created
+    # first, so it emits every pair) and, nearby, spheres with fewer contacts: both paths in one
+    # detection, against pe's ExhaustiveSearch and HashGrids.
+    c0 = np.array([5.0, 5.0, 3.0])
+    dirs = [(1, 1, 0), (1, -1, 0), (-1, 1, 0), (-1, -1, 0), (1, 0, 1), (1, 0, -1), (-1, 0, 1), (-1, 0, -1),
+            (0, 1, 1), (0, 1, -1), (0, -1, 1), (0, -1, -1)]
+    pos = [c0] + [c0 + 0.99999 * np.array(d, float) / np.sqrt(2.0) for d in dirs]
+    pos += [np.array([2.0 + 0.99999 * k, 2.0, 0.5 - 1e-6]) for k in range(4)]     # a row on the floor
+    pos = np.array(pos)
+    n = len(pos)
+    sc = scenes.Scene("kissing", [{"name": "m", "density": 2500.0, "cor": 0.0, "csf": 0.3, "cdf": 0.3}],
+                      scenes.box_planes(10.0, 10.0, 20.0), pos, np.full(n, 0.5), np.zeros(n, np.uint32),
+                      dt=1e-3, max_iterations=20)
+    ow = oracle_for(sc)
+    w = gpu_for(sc)
+    g = contact_map(w.detect())
+    ex = contact_map(ow.detect(exhaustive=True))
+    hg = contact_map(ow.detect(exhaustive=False))
+    centre = int(np.flatnonzero(w.host_arrays()["shape"] == SPHERE)[0])
+    assert sum(len(v) for (a, b), v in ex.items() if centre in (a, b)) == 12
+    assert g == ex and g == hg
 
 
 # --------------------------------------------------------------------------- one step, pe's order
