@@ -309,17 +309,35 @@ __global__ void k_body_ranks(const SolverContacts C, const uint8_t* fixed, uint3
 // after the wait loop has seen the counter (its exit depends on the loaded value); and the
 // producer waits for its stores' completion before its counter update.  Everything else the
 // relaxation reads was written by earlier kernels.
+__device__ __forceinline__ bool poll_step(uint32_t seen, uint32_t target, uint32_t& spins, uint32_t spin_limit,
+                                          const uint32_t* abort_flag, bool& stop) {
+  if (seen >= target) return true;
+  stop = ++spins > spin_limit ||
+         ((spins & 63u) == 0 &&
+          __hip_atomic_load(const_cast<uint32_t*>(abort_flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  return false;
+}
+
 __device__ __forceinline__ bool wait_count(const uint32_t* ctr, uint32_t target, const uint32_t* abort_flag,
                                            uint32_t spin_limit) {
+  uint32_t* c = const_cast<uint32_t*>(ctr);
+  uint32_t a = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (a >= target) return true;
   uint32_t spins = 0;
-  while (__hip_atomic_load(const_cast<uint32_t*>(ctr), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-    if (++spins > spin_limit) return false;
-    if ((spins & 63u) == 0 &&
-        __hip_atomic_load(const_cast<uint32_t*>(abort_flag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      return false;
+  bool stop = false;
+  // two polls in flight: the next one is issued before the previous one's value is looked at, so
+  // a counter update is seen about half a round trip sooner (the loop alternates two registers:
+  // a copy from one to the other would wait for the newer poll)
+  for (;;) {
     __builtin_amdgcn_s_sleep(1);
+    const uint32_t b = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (poll_step(a, target, spins, spin_limit, abort_flag, stop)) return true;
+    if (stop) return false;
+    __builtin_amdgcn_s_sleep(1);
+    a = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (poll_step(b, target, spins, spin_limit, abort_flag, stop)) return true;
+    if (stop) return false;
   }
-  return true;
 }
 
 // A contact's fields held in registers (the impulse excluded: it is loaded after the wait).
