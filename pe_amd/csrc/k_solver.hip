@@ -300,15 +300,8 @@ __global__ void k_body_ranks(const SolverContacts C, const uint8_t* fixed, uint3
   }
 }
 
-// Polls with relaxed agent-scope loads.  No acquire: on gfx950 an agent-scope acquire
-// invalidates the XCD's whole L2, which made every wavefront's reads miss (measured: 121 steps/s
-// with an acquire per poll, 187 with one acquire per item, 215 with none).  None is needed for
-// what an item reads after its wait: everything the sweeps write and other wavefronts read (the
-// velocity corrections, the impulses) is accessed with agent-scope atomics, which are coherent
-// across the XCDs per access (sc1 loads and stores); the item's loads of them are issued only
-// after the wait loop has seen the counter (its exit depends on the loaded value); and the
-// producer waits for its stores' completion before its counter update.  Everything else the
-// relaxation reads was written by earlier kernels.
+// One look at a polled counter value: reached the target, or not yet (stop: the spin limit is
+// exhausted or another wavefront raised the abort flag, checked every 64 polls).
 __device__ __forceinline__ bool poll_step(uint32_t seen, uint32_t target, uint32_t& spins, uint32_t spin_limit,
                                           const uint32_t* abort_flag, bool& stop) {
   if (seen >= target) return true;
@@ -318,6 +311,15 @@ __device__ __forceinline__ bool poll_step(uint32_t seen, uint32_t target, uint32
   return false;
 }
 
+// Polls with relaxed agent-scope loads.  No acquire: on gfx950 an agent-scope acquire
+// invalidates the XCD's whole L2, which made every wavefront's reads miss (measured: 121 steps/s
+// with an acquire per poll, 187 with one acquire per item, 215 with none).  None is needed for
+// what an item reads after its wait: everything the sweeps write and other wavefronts read (the
+// velocity corrections, the impulses) is accessed with agent-scope atomics, which are coherent
+// across the XCDs per access (sc1 loads and stores); the item's loads of them are issued only
+// after the wait loop has seen the counter (its exit depends on the loaded value); and the
+// producer waits for its stores' completion before its counter update.  Everything else the
+// relaxation reads was written by earlier kernels.
 __device__ __forceinline__ bool wait_count(const uint32_t* ctr, uint32_t target, const uint32_t* abort_flag,
                                            uint32_t spin_limit) {
   uint32_t* c = const_cast<uint32_t*>(ctr);
