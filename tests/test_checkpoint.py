@@ -380,6 +380,40 @@ def test_union_records_rejected(tmp_path):
         checkpoint.load_body_file(w3, str(f))
 
 
+def test_union_single_precision_file(tmp_path):
+    # reals of 4 bytes (setFloatingPointSize(2)) in union records too: the mass, inertia, bounding
+    # box and the members' positions read back as the float32 roundings of the world's values
+    w = union_world()
+    f = tmp_path / "u32.bin"
+    checkpoint.write_body_file(w, str(f), fp_size=2)
+    info = checkpoint.read_body_file(str(f))
+    unions = [b for b in info["bodies"] if b["type"] == checkpoint.UNION_TYPE]
+    a = w.host_arrays()
+    ui = np.flatnonzero(a["shape"] == UNION)
+    m = w.members()
+    assert len(unions) == len(ui) == 4
+    for b, i in zip(unions, ui):
+        f32 = lambda x: np.asarray(x, np.float64).astype(np.float32).astype(np.float64)
+        assert np.array_equal(b["pos"], f32(a["pos"][i])) and np.array_equal(b["I"], f32(a["I"][i]))
+        assert b["mass"] == f32(w.unionMass(int(i)))
+        sel = np.flatnonzero(m["owner"] == i)
+        assert [mm["sid"] for mm in b["members"]] == [int(x) for x in m["sid"][sel]]
+        assert np.array_equal(np.array([mm["pos"] for mm in b["members"]]), f32(m["gpos"][sel]))
+
+
+def test_bulk_unions_are_checked():
+    # add_bodies with union rows needs their members, grouped by union in body order
+    w = union_world()
+    a = {k: np.array(v, copy=True) for k, v in w.host_arrays().items()}
+    m = w.members()
+    for bad, msg in ((None, "members"),
+                     (dict(m, owner=m["owner"][::-1].copy()), "grouped"),
+                     ({k: v[m["owner"] != m["owner"][0]] for k, v in m.items()}, "without members")):
+        w2 = copy_materials(w)
+        with pytest.raises(ValueError, match=msg):
+            w2.add_bodies(a, sids=a["sid"], members=bad)
+
+
 @pytest.mark.gpu
 def test_union_restart_continues_bit_for_bit(tmp_path):
     # granular particles above a floor, created in pe's reader order (unions, then planes): a
