@@ -1,8 +1,9 @@
 // Host sanitizer run of the oracle (test infrastructure only): built by `make sanitize` with
 // AddressSanitizer and UndefinedBehaviorSanitizer, it builds a mixed scene through the oracle's C
 // API -- spheres, boxes, capsules, unions of spheres and planes -- and steps it with every
-// relaxation model, both trig modes, the HashGrids and exhaustive detectors, a forced sweep order
-// and an abort threshold, so that every allocation, index and table of the oracle is exercised.
+// relaxation model, both trig modes, the HashGrids and exhaustive detectors, forced sweep orders
+// (plain and ranked), an abort threshold, forces, damping and the member round trip, so that
+// every allocation, index and table of the oracle is exercised.
 // Exit status 0 when the sanitizers report nothing (they abort the process on a finding).
 #include <cmath>
 #include <cstdio>
@@ -86,6 +87,35 @@ int main() {
         std::printf("model %d: HashGrids %zu contacts, exhaustive %zu\n", model, a, b);
         return 3;
       }
+      // the impulses of a detection are zero, one per contact
+      std::vector<double> pd(3 * b + 3, -1.0);
+      orc_get_impulses(w, pd.data());
+      for (size_t k = 0; k < 3 * b; ++k)
+        if (pd[k] != 0.0) return 5;
+      // forces for one step, members read and written back, a damped step, the ranked order
+      const size_t nb = orc_num_bodies(w), nm = orc_num_members(w);
+      std::vector<double> force(3 * nb, 0.0), torque(3 * nb, 0.0);
+      for (size_t i = 0; i < nb; ++i) { force[3 * i] = 0.5; torque[3 * i + 2] = 0.1; }
+      orc_set_forces(w, force.data(), torque.data());
+      std::vector<uint32_t> owner(nm);
+      std::vector<uint64_t> msid(nm);
+      std::vector<int> mmat(nm);
+      std::vector<double> mrad(nm), mrpos(3 * nm), mgpos(3 * nm);
+      orc_get_members(w, owner.data(), msid.data(), mmat.data(), mrad.data(), mrpos.data(), mgpos.data());
+      orc_set_members(w, mgpos.data());
+      orc_set_damping(w, 0.99);
+      orc_set_erp(w, 0.5);
+      const size_t nc2 = orc_detect(w, 0);
+      std::vector<uint32_t> c1(nc2), c2(nc2), rank(nc2, 0);
+      std::vector<double> cp(3 * nc2), cn(3 * nc2), cd(nc2), cm(nc2);
+      orc_get_contacts(w, c1.data(), c2.data(), cp.data(), cn.data(), cd.data(), cm.data());
+      for (size_t k = 1; k < nc2; ++k)   // the rank of each contact among its pair's, in emission order
+        for (size_t j = 0; j < k; ++j)
+          if (c1[j] == c1[k] && c2[j] == c2[k]) ++rank[k];
+      orc_set_sweep_order_ranked(w, nc2, c1.data(), c2.data(), rank.data());
+      if (orc_step(w, 2e-3, 1) != 0) return 2;
+      std::vector<uint32_t> pa(nb * nb), pb(nb * nb);
+      orc_candidate_pairs(w, pa.data(), pb.data(), nb * nb);
       orc_destroy(w);
     }
   std::printf("sanitized oracle runs: %zu contacts over 12 runs\n", total);
