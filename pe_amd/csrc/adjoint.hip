@@ -104,13 +104,17 @@ static pehip_status record(pehip_ctx* ctx, AdjointState* a, StepTape& t, const d
   const hipStream_t s = ctx->stream;
   const size_t n = ctx->n, D = sizeof(double);
   const uint32_t nc = ctx->nc;
-  if (!t.pos) {
-    CK(dalloc(t.pos, 3 * n)); CK(dalloc(t.sb, (size_t)kSB * n)); CK(dalloc(t.vel, 6 * n));
-  }
+  // each array is allocated on its own and a capacity is recorded only once its arrays all exist,
+  // so a failed allocation is retried by the next call instead of leaving a null array in use
+  if (!t.pos) CK(dalloc(t.pos, 3 * n));
+  if (!t.sb) CK(dalloc(t.sb, (size_t)kSB * n));
+  if (!t.vel) CK(dalloc(t.vel, 6 * n));
   if (nc > t.cap) {
     dfree(t.b1); dfree(t.b2); dfree(t.p); dfree(t.mu);
-    t.cap = std::max<uint32_t>(nc + nc / 4, 1024);
-    CK(dalloc(t.b1, t.cap)); CK(dalloc(t.b2, t.cap)); CK(dalloc(t.p, 3 * (size_t)t.cap)); CK(dalloc(t.mu, t.cap));
+    t.cap = 0;
+    const uint32_t cap = std::max<uint32_t>(nc + nc / 4, 1024);
+    CK(dalloc(t.b1, cap)); CK(dalloc(t.b2, cap)); CK(dalloc(t.p, 3 * (size_t)cap)); CK(dalloc(t.mu, cap));
+    t.cap = cap;
   }
   t.nc = nc;
   t.colour_off = ctx->colour_off;
@@ -124,10 +128,12 @@ static pehip_status record(pehip_ctx* ctx, AdjointState* a, StepTape& t, const d
   }
   if (nc > a->ccap) {
     dfree(a->jac); dfree(a->minv); dfree(a->lambda);
-    a->ccap = std::max<uint32_t>(nc + nc / 4, 1024);
-    CK(dalloc(a->jac, (size_t)a->ccap * ad::kAdjIn * ad::kAdjOut));
-    CK(dalloc(a->minv, 9 * (size_t)a->ccap));
-    CK(dalloc(a->lambda, 3 * (size_t)a->ccap));
+    a->ccap = 0;
+    const uint32_t cap = std::max<uint32_t>(nc + nc / 4, 1024);
+    CK(dalloc(a->jac, (size_t)cap * ad::kAdjIn * ad::kAdjOut));
+    CK(dalloc(a->minv, 9 * (size_t)cap));
+    CK(dalloc(a->lambda, 3 * (size_t)cap));
+    a->ccap = cap;
   }
   return PEHIP_OK;
 }
@@ -188,18 +194,23 @@ pehip_status pehip_adjoint_forward(pehip_ctx* ctx, double dt, uint32_t nsteps, u
   a->every = checkpoint_every;
   a->prm = ctx->prm;
   const size_t n = ctx->n;
-  for (uint32_t st = 0; st < nsteps; ++st) {
-    if (st % checkpoint_every == 0) {
-      double* c = nullptr;
-      CK(dalloc(c, kStateDoubles * n));
-      a->checkpoints.push_back(c);
-      save_state(ctx, c);
+  const pehip_status result = [&]() -> pehip_status {
+    for (uint32_t st = 0; st < nsteps; ++st) {
+      if (st % checkpoint_every == 0) {
+        double* c = nullptr;
+        CK(dalloc(c, kStateDoubles * n));
+        a->checkpoints.push_back(c);
+        save_state(ctx, c);
+      }
+      CKS(pe_one_step(ctx, dt));
     }
-    CKS(pe_one_step(ctx, dt));
-  }
-  CK(dalloc(a->final_state, kStateDoubles * n));
-  save_state(ctx, a->final_state);
-  return sync_check(ctx);
+    CK(dalloc(a->final_state, kStateDoubles * n));
+    save_state(ctx, a->final_state);
+    return sync_check(ctx);
+  }();
+  // a run that did not complete leaves nothing to differentiate
+  if (result != PEHIP_OK) pe_adjoint_free(ctx);
+  return result;
 }
 
 pehip_status pehip_adjoint_backward(pehip_ctx* ctx, const double* dL_dpos, const double* dL_dv, const double* dL_dw,
@@ -212,10 +223,12 @@ pehip_status pehip_adjoint_backward(pehip_ctx* ctx, const double* dL_dpos, const
   hipSetDevice(ctx->device);
   const hipStream_t s = ctx->stream;
   const size_t n = ctx->n, D = sizeof(double);
-  if (!a->xbar) {
-    CK(dalloc(a->xbar, 3 * n)); CK(dalloc(a->vbar, 3 * n)); CK(dalloc(a->wbar, 3 * n));
-    CK(dalloc(a->alpha, 6 * n)); CK(dalloc(a->mubar, 1)); CK(dalloc(a->delta, 1));
-  }
+  if (!a->xbar) CK(dalloc(a->xbar, 3 * n));
+  if (!a->vbar) CK(dalloc(a->vbar, 3 * n));
+  if (!a->wbar) CK(dalloc(a->wbar, 3 * n));
+  if (!a->alpha) CK(dalloc(a->alpha, 6 * n));
+  if (!a->mubar) CK(dalloc(a->mubar, 1));
+  if (!a->delta) CK(dalloc(a->delta, 1));
   fill_u64((unsigned long long*)a->xbar, 0ull, 3 * n, s);
   fill_u64((unsigned long long*)a->vbar, 0ull, 3 * n, s);
   fill_u64((unsigned long long*)a->wbar, 0ull, 3 * n, s);
@@ -224,45 +237,52 @@ pehip_status pehip_adjoint_backward(pehip_ctx* ctx, const double* dL_dpos, const
   if (dL_dpos) CK(hipMemcpy(a->xbar, dL_dpos, 3 * D * n, hipMemcpyHostToDevice));
   if (dL_dv) CK(hipMemcpy(a->vbar, dL_dv, 3 * D * n, hipMemcpyHostToDevice));
   if (dL_dw) CK(hipMemcpy(a->wbar, dL_dw, 3 * D * n, hipMemcpyHostToDevice));
+  // the re-runs use the forward run's parameters; the caller's come back however the pass ends
   const pehip_params saved = ctx->prm;
   ctx->prm = a->prm;
   const bool prof = ctx->profiling;
   ctx->profiling = false;
   pehip_adjoint_stats st{};
   double* pos_before = nullptr;
-  CK(dalloc(pos_before, 3 * n));
-  pehip_status result = PEHIP_OK;
-  const uint32_t nseg = (uint32_t)a->checkpoints.size();
-  if (a->tape.size() < a->every) a->tape.resize(a->every);
-  for (int seg = (int)nseg - 1; seg >= 0 && result == PEHIP_OK; --seg) {
-    const uint32_t s0 = (uint32_t)seg * a->every;
-    const uint32_t m = std::min(a->every, a->nsteps - s0);
-    restore_state(ctx, a->checkpoints[seg]);
-    for (uint32_t j = 0; j < m && result == PEHIP_OK; ++j) {
-      hipMemcpyAsync(pos_before, ctx->pos, 3 * D * n, hipMemcpyDeviceToDevice, s);
-      result = pe_one_step(ctx, a->dt);
-      if (result == PEHIP_OK) result = record(ctx, a, a->tape[j], pos_before);
-      st.max_contacts = std::max<uint64_t>(st.max_contacts, ctx->nc);
+  pehip_status result = [&]() -> pehip_status {
+    CK(dalloc(pos_before, 3 * n));
+    const uint32_t nseg = (uint32_t)a->checkpoints.size();
+    if (a->tape.size() < a->every) a->tape.resize(a->every);
+    for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+      const uint32_t s0 = (uint32_t)seg * a->every;
+      const uint32_t m = std::min(a->every, a->nsteps - s0);
+      restore_state(ctx, a->checkpoints[seg]);
+      for (uint32_t j = 0; j < m; ++j) {
+        CK(hipMemcpyAsync(pos_before, ctx->pos, 3 * D * n, hipMemcpyDeviceToDevice, s));
+        CKS(pe_one_step(ctx, a->dt));
+        CKS(record(ctx, a, a->tape[j], pos_before));
+        st.max_contacts = std::max<uint64_t>(st.max_contacts, ctx->nc);
+      }
+      for (int j = (int)m - 1; j >= 0; --j) {
+        uint32_t used = 0;
+        double d = 0;
+        CKS(reverse_step(ctx, a, a->tape[j], adjoint_sweeps, adjoint_tol, &used, &d));
+        st.total_sweeps += used;
+        st.max_last_delta = std::max(st.max_last_delta, d);
+        ++st.steps;
+      }
     }
-    for (int j = (int)m - 1; j >= 0 && result == PEHIP_OK; --j) {
-      uint32_t used = 0;
-      double d = 0;
-      result = reverse_step(ctx, a, a->tape[j], adjoint_sweeps, adjoint_tol, &used, &d);
-      st.total_sweeps += used;
-      st.max_last_delta = std::max(st.max_last_delta, d);
-      ++st.steps;
-    }
-  }
+    return PEHIP_OK;
+  }();
+  // the world is left in the forward run's final state whether or not the pass succeeded
+  restore_state(ctx, a->final_state);
+  const hipError_t se = hipStreamSynchronize(s);
   dfree(pos_before);
   if (result == PEHIP_OK) {
-    restore_state(ctx, a->final_state);
-    CK(hipStreamSynchronize(s));
-    if (dL_dpos0) CK(hipMemcpy(dL_dpos0, a->xbar, 3 * D * n, hipMemcpyDeviceToHost));
-    if (dL_dv0) CK(hipMemcpy(dL_dv0, a->vbar, 3 * D * n, hipMemcpyDeviceToHost));
-    if (dL_dw0) CK(hipMemcpy(dL_dw0, a->wbar, 3 * D * n, hipMemcpyDeviceToHost));
-    if (dL_dmu) CK(hipMemcpy(dL_dmu, a->mubar, D, hipMemcpyDeviceToHost));
-    if (stats) *stats = st;
-    result = sync_check(ctx);
+    result = [&]() -> pehip_status {
+      CK(se);
+      if (dL_dpos0) CK(hipMemcpy(dL_dpos0, a->xbar, 3 * D * n, hipMemcpyDeviceToHost));
+      if (dL_dv0) CK(hipMemcpy(dL_dv0, a->vbar, 3 * D * n, hipMemcpyDeviceToHost));
+      if (dL_dw0) CK(hipMemcpy(dL_dw0, a->wbar, 3 * D * n, hipMemcpyDeviceToHost));
+      if (dL_dmu) CK(hipMemcpy(dL_dmu, a->mubar, D, hipMemcpyDeviceToHost));
+      if (stats) *stats = st;
+      return sync_check(ctx);
+    }();
   }
   ctx->prm = saved;
   ctx->profiling = prof;

@@ -126,9 +126,16 @@ static pehip_status bfail(pehip_batch* b, pehip_status st, const std::string& ms
 }
 
 // Contact slots of every scene from hinfo[].ccap; (re)allocates the contact arrays.
+static pehip_status layout_contacts_into(pehip_batch* b);
 static pehip_status layout_contacts(pehip_batch* b) {
   BK(hipStreamSynchronize(b->stream));
   b->free_contacts();
+  // a failure leaves the batch without scenes (never scenes over contact arrays that do not exist)
+  const pehip_status st = layout_contacts_into(b);
+  if (st != PEHIP_OK) b->free_all();
+  return st;
+}
+static pehip_status layout_contacts_into(pehip_batch* b) {
   uint64_t c = 0;
   for (auto& s : b->hinfo) {
     s.c0 = (uint32_t)c;
@@ -332,12 +339,23 @@ pehip_status pehip_batch_set_materials(pehip_batch* b, uint32_t n, const double*
   if (b->n && n < b->nmat) return bfail(b, PEHIP_INVALID_ARG, "material table cannot shrink below the bodies' materials");
   hipSetDevice(b->device);
   BK(hipStreamSynchronize(b->stream));
+  // the new tables replace the old ones only once both are on the device
+  const size_t nn = (size_t)n * n;
+  double *csf = nullptr, *cdf = nullptr;
+  hipError_t e = balloc(csf, nn);
+  if (e == hipSuccess) e = balloc(cdf, nn);
+  if (e == hipSuccess) e = hipMemcpy(csf, csf_pair, sizeof(double) * nn, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(cdf, cdf_pair, sizeof(double) * nn, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    bfree(csf);
+    bfree(cdf);
+    return bfail(b, e == hipErrorOutOfMemory ? PEHIP_OUT_OF_MEMORY : PEHIP_HIP_ERROR,
+                 std::string("material tables: ") + hipGetErrorString(e));
+  }
   bfree(b->csf);
   bfree(b->cdf);
-  BK(balloc(b->csf, (size_t)n * n));
-  BK(balloc(b->cdf, (size_t)n * n));
-  BK(hipMemcpy(b->csf, csf_pair, sizeof(double) * n * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->cdf, cdf_pair, sizeof(double) * n * n, hipMemcpyHostToDevice));
+  b->csf = csf;
+  b->cdf = cdf;
   b->nmat = n;
   b->drop_graph();
   return PEHIP_OK;
@@ -422,61 +440,67 @@ pehip_status pehip_batch_upload(pehip_batch* b, uint32_t nscenes, const uint32_t
   hipSetDevice(b->device);
   BK(hipStreamSynchronize(b->stream));
   b->free_all();
-  b->n = n;
-  b->nscenes = nscenes;
-  b->hinfo = info;
-  b->hplane_idx = pidx;
-  b->scene_mu = false;
-  b->capacity_retries = 0;
-  std::vector<double> hR(9 * (size_t)n);
-  for (uint64_t i = 0; i < n; ++i) {
-    qd qq;
-    for (int k = 0; k < 4; ++k) qq.v[k] = q[4 * (size_t)i + k];
-    const m33 m = quat_to_R(qq);
-    for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
-  }
-  // largest scenes first (ties by index): the persistent grid ends on the small ones
-  std::vector<uint32_t> ord(nscenes);
-  for (uint32_t s = 0; s < nscenes; ++s) ord[s] = s;
-  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return info[a].nbody > info[c].nbody; });
-  BK(balloc(b->shape, n)); BK(balloc(b->fixed, n)); BK(balloc(b->mat, n)); BK(balloc(b->sid, n));
-  BK(balloc(b->param, 3 * n)); BK(balloc(b->pos, 3 * n)); BK(balloc(b->q, 4 * n)); BK(balloc(b->R, 9 * n));
-  BK(balloc(b->v, 3 * n)); BK(balloc(b->w, 3 * n)); BK(balloc(b->pnormal, 3 * n)); BK(balloc(b->pd, n));
-  BK(balloc(b->aabb, 6 * n)); BK(balloc(b->inv_mass, n)); BK(balloc(b->Ib, 9 * n)); BK(balloc(b->Ibinv, 9 * n));
-  BK(balloc(b->force, 3 * n)); BK(balloc(b->torque, 3 * n)); BK(balloc(b->sb, (size_t)kSB * n));
-  BK(balloc(b->dvw, (size_t)kDVW * n)); BK(balloc(b->bucket, n)); BK(balloc(b->sorted, n)); BK(balloc(b->count, n));
-  BK(balloc(b->info, nscenes)); BK(balloc(b->order, nscenes)); BK(balloc(b->plane_idx, pidx.size()));
-  BK(balloc(b->colour_off, (size_t)nscenes * (kMaxColours + 1))); BK(balloc(b->out, nscenes));
-  const size_t D = sizeof(double);
-  BK(hipMemcpy(b->shape, shape, n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->fixed, fx.data(), n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->mat, material, 4 * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->sid, sid, 8 * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->param, param, 3 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->q, q, 4 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->v, v, 3 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->w, w, 3 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->pnormal, plane_normal, 3 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->pd, plane_d, D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->inv_mass, inv_mass, D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->Ib, inertia_body, 9 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->Ibinv, inv_inertia_body, 9 * D * n, hipMemcpyHostToDevice));
-  BK(hipMemcpy(b->order, ord.data(), 4 * (size_t)nscenes, hipMemcpyHostToDevice));
-  if (!pidx.empty()) BK(hipMemcpy(b->plane_idx, pidx.data(), 4 * pidx.size(), hipMemcpyHostToDevice));
-  BK(hipMemset(b->force, 0, 3 * D * n));
-  BK(hipMemset(b->torque, 0, 3 * D * n));
-  BK(hipMemset(b->dvw, 0, kDVW * D * n));
-  BK(hipMemset(b->sb, 0, kSB * D * n));
-  BK(hipMemset(b->out, 0, sizeof(SceneOut) * nscenes));
-  BK(hipMemset(b->colour_off, 0, 4 * (size_t)nscenes * (kMaxColours + 1)));
-  BK(hipMemset(b->queue, 0, 64));
-  BKS(layout_contacts(b));
-  launch_aabb(b->bodies(), b->stream);
-  BK(hipGetLastError());
-  BK(hipStreamSynchronize(b->stream));
-  return PEHIP_OK;
+  // a failure from here on leaves the batch without scenes (never a body count over arrays that
+  // do not exist)
+  const pehip_status st = [&]() -> pehip_status {
+    b->n = n;
+    b->nscenes = nscenes;
+    b->hinfo = info;
+    b->hplane_idx = pidx;
+    b->scene_mu = false;
+    b->capacity_retries = 0;
+    std::vector<double> hR(9 * (size_t)n);
+    for (uint64_t i = 0; i < n; ++i) {
+      qd qq;
+      for (int k = 0; k < 4; ++k) qq.v[k] = q[4 * (size_t)i + k];
+      const m33 m = quat_to_R(qq);
+      for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
+    }
+    // largest scenes first (ties by index): the persistent grid ends on the small ones
+    std::vector<uint32_t> ord(nscenes);
+    for (uint32_t s = 0; s < nscenes; ++s) ord[s] = s;
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) { return info[a].nbody > info[c].nbody; });
+    BK(balloc(b->shape, n)); BK(balloc(b->fixed, n)); BK(balloc(b->mat, n)); BK(balloc(b->sid, n));
+    BK(balloc(b->param, 3 * n)); BK(balloc(b->pos, 3 * n)); BK(balloc(b->q, 4 * n)); BK(balloc(b->R, 9 * n));
+    BK(balloc(b->v, 3 * n)); BK(balloc(b->w, 3 * n)); BK(balloc(b->pnormal, 3 * n)); BK(balloc(b->pd, n));
+    BK(balloc(b->aabb, 6 * n)); BK(balloc(b->inv_mass, n)); BK(balloc(b->Ib, 9 * n)); BK(balloc(b->Ibinv, 9 * n));
+    BK(balloc(b->force, 3 * n)); BK(balloc(b->torque, 3 * n)); BK(balloc(b->sb, (size_t)kSB * n));
+    BK(balloc(b->dvw, (size_t)kDVW * n)); BK(balloc(b->bucket, n)); BK(balloc(b->sorted, n)); BK(balloc(b->count, n));
+    BK(balloc(b->info, nscenes)); BK(balloc(b->order, nscenes)); BK(balloc(b->plane_idx, pidx.size()));
+    BK(balloc(b->colour_off, (size_t)nscenes * (kMaxColours + 1))); BK(balloc(b->out, nscenes));
+    const size_t D = sizeof(double);
+    BK(hipMemcpy(b->shape, shape, n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->fixed, fx.data(), n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->mat, material, 4 * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->sid, sid, 8 * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->param, param, 3 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->q, q, 4 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->v, v, 3 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->w, w, 3 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->pnormal, plane_normal, 3 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->pd, plane_d, D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->inv_mass, inv_mass, D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->Ib, inertia_body, 9 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->Ibinv, inv_inertia_body, 9 * D * n, hipMemcpyHostToDevice));
+    BK(hipMemcpy(b->order, ord.data(), 4 * (size_t)nscenes, hipMemcpyHostToDevice));
+    if (!pidx.empty()) BK(hipMemcpy(b->plane_idx, pidx.data(), 4 * pidx.size(), hipMemcpyHostToDevice));
+    BK(hipMemset(b->force, 0, 3 * D * n));
+    BK(hipMemset(b->torque, 0, 3 * D * n));
+    BK(hipMemset(b->dvw, 0, kDVW * D * n));
+    BK(hipMemset(b->sb, 0, kSB * D * n));
+    BK(hipMemset(b->out, 0, sizeof(SceneOut) * nscenes));
+    BK(hipMemset(b->colour_off, 0, 4 * (size_t)nscenes * (kMaxColours + 1)));
+    BK(hipMemset(b->queue, 0, 64));
+    BKS(layout_contacts(b));
+    launch_aabb(b->bodies(), b->stream);
+    BK(hipGetLastError());
+    BK(hipStreamSynchronize(b->stream));
+    return PEHIP_OK;
+  }();
+  if (st != PEHIP_OK) b->free_all();
+  return st;
 }
 
 pehip_status pehip_batch_set_scene_friction(pehip_batch* b, uint32_t nscenes, const double* static_mu,

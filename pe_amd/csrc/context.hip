@@ -98,6 +98,7 @@ static pehip_status ensure_contact_capacity(pehip_ctx* ctx, size_t need) {
   if (need <= ctx->ccap) return PEHIP_OK;
   CK(hipStreamSynchronize(ctx->stream));
   ctx->free_contacts();
+  ctx->nc = 0;  // the last step's contacts went with the arrays
   const size_t cap = std::max<size_t>(need + need / 2, 4096);
   const size_t bytes = ctx->carve_contacts(nullptr, cap);
   CK(hipMalloc(&ctx->cblock, bytes));
@@ -205,6 +206,85 @@ static pehip_status move_members(pehip_ctx* ctx, const double* pos, const double
   }
   if (moved) CK(hipMemcpy(ctx->mpos, mp.data(), 3 * D * nm, hipMemcpyHostToDevice));
   return PEHIP_OK;
+}
+
+// The arrays of pehip_upload_bodies' validated bodies on the device, the broadphase's and
+// colouring's per-body buffers, and the grid plan (the context holds no bodies on entry).
+static pehip_status upload_body_arrays(pehip_ctx* ctx, uint32_t n, const std::vector<uint8_t>& fx,
+                                       const std::vector<uint32_t>& hplanes, const uint8_t* shape,
+                                       const uint64_t* sid, const uint32_t* material, const double* param,
+                                       const double* pos, const double* q, const double* v, const double* w,
+                                       const double* inv_mass, const double* inertia_body,
+                                       const double* inv_inertia_body, const double* plane_normal,
+                                       const double* plane_d) {
+  ctx->n = n;
+  ctx->h_fixed = fx;
+  ctx->h_shape.assign(shape, shape + n);
+  // rotation matrices from the quaternions (Quaternion::toRotationMatrix)
+  std::vector<double> hR(9 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i) {
+    qd qq;
+    for (int k = 0; k < 4; ++k) qq.v[k] = q[4 * (size_t)i + k];
+    const m33 m = quat_to_R(qq);
+    for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
+  }
+  CK(dalloc(ctx->shape, n)); CK(dalloc(ctx->fixed, n)); CK(dalloc(ctx->mat, n)); CK(dalloc(ctx->sid, n));
+  CK(dalloc(ctx->param, 3 * (size_t)n)); CK(dalloc(ctx->pos, 3 * (size_t)n)); CK(dalloc(ctx->q, 4 * (size_t)n));
+  CK(dalloc(ctx->R, 9 * (size_t)n)); CK(dalloc(ctx->v, 3 * (size_t)n)); CK(dalloc(ctx->w, 3 * (size_t)n));
+  CK(dalloc(ctx->pnormal, 3 * (size_t)n)); CK(dalloc(ctx->pd, n)); CK(dalloc(ctx->aabb, 6 * (size_t)n));
+  CK(dalloc(ctx->inv_mass, n)); CK(dalloc(ctx->Ib, 9 * (size_t)n)); CK(dalloc(ctx->Ibinv, 9 * (size_t)n));
+  CK(dalloc(ctx->force, 3 * (size_t)n)); CK(dalloc(ctx->torque, 3 * (size_t)n));
+  CK(dalloc(ctx->sb, (size_t)kSB * n)); CK(dalloc(ctx->dvw, (size_t)kDVW * n));
+  CK(dalloc(ctx->planes, hplanes.size()));
+  const size_t D = sizeof(double);
+  CK(hipMemcpy(ctx->shape, shape, n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->fixed, fx.data(), n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->mat, material, 4 * (size_t)n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->sid, sid, 8 * (size_t)n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->param, param, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->q, q, 4 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->v, v, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->w, w, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pnormal, plane_normal, 3 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->pd, plane_d, D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->inv_mass, inv_mass, D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->Ib, inertia_body, 9 * D * n, hipMemcpyHostToDevice));
+  CK(hipMemcpy(ctx->Ibinv, inv_inertia_body, 9 * D * n, hipMemcpyHostToDevice));
+  if (!hplanes.empty()) CK(hipMemcpy(ctx->planes, hplanes.data(), 4 * hplanes.size(), hipMemcpyHostToDevice));
+  CK(hipMemset(ctx->force, 0, 3 * D * n));
+  CK(hipMemset(ctx->torque, 0, 3 * D * n));
+  CK(hipMemset(ctx->dvw, 0, kDVW * D * n));
+  CK(hipMemset(ctx->sb, 0, kSB * D * n));
+
+  plan_grid(ctx, shape, param, pos);
+  const size_t cells = (size_t)ctx->grid.nlevels * ctx->grid.table;
+  BroadphaseBuffers& P = ctx->bp;
+  CK(dalloc(P.keys, n)); CK(dalloc(P.vals, n)); CK(dalloc(P.keys_tmp, n)); CK(dalloc(P.vals_tmp, n));
+  CK(dalloc(P.cell_start, cells)); CK(dalloc(P.cell_end, cells));
+  CK(dalloc(P.counts, (size_t)n + 1)); CK(dalloc(P.offsets, (size_t)n + 1));
+  CK(dalloc(P.saabb, 6 * (size_t)n)); CK(dalloc(P.sfixed, n));
+  CK(dalloc(P.st_b1, (size_t)kStage * n)); CK(dalloc(P.st_b2, (size_t)kStage * n));
+  CK(dalloc(P.st_pos, 3 * (size_t)kStage * n)); CK(dalloc(P.st_nrm, 3 * (size_t)kStage * n));
+  CK(dalloc(P.st_dist, (size_t)kStage * n));
+  P.stride = n;
+  ctx->spheres_only = std::all_of(shape, shape + n, [](uint8_t t) { return t == PEHIP_SPHERE || t == PEHIP_PLANE; });
+  CK(dalloc(ctx->scan_scratch, scan_scratch_words((size_t)n + 1)));
+  CK(dalloc(ctx->sort_scratch, sort_scratch_words(n)));
+  CK(dalloc(ctx->col.deg, n)); CK(dalloc(ctx->col.adj_start, (size_t)n + 1)); CK(dalloc(ctx->col.adj_cursor, n));
+  CK(dalloc(ctx->ke_partial, (n + 255) / 256));
+  CK(dalloc(ctx->df_done, n));
+  CK(dalloc(ctx->isl.parent, n)); CK(dalloc(ctx->isl.ncont, n)); CK(dalloc(ctx->isl.cmask, 4 * (size_t)n));
+  CK(dalloc(ctx->isl.cursor, n)); CK(dalloc(ctx->isl.place, n)); CK(dalloc(ctx->isl.blocal, n));
+  P.planes = ctx->planes;
+  P.nplanes = (uint32_t)hplanes.size();
+  P.flags = ctx->flags;
+  ctx->members_missing = std::find(shape, shape + n, (uint8_t)PEHIP_UNION) != shape + n;
+  if (!ctx->members_missing) launch_aabb(ctx->bodies(), ctx->stream);  // unions: after their members
+  ctx->nc = 0;
+  ctx->order_pending = false;
+  return sync_check(ctx);
 }
 
 pehip_status pe_amd::check_params(const pehip_params* p, std::string& why) {
@@ -328,8 +408,12 @@ pehip_status pehip_set_params(pehip_ctx* ctx, const pehip_params* params) {
   if (params->max_iterations != ctx->prm.max_iterations) {
     hipSetDevice(ctx->device);
     CK(hipStreamSynchronize(ctx->stream));
+    // the sweep-delta array is replaced only once the new one exists (a failure keeps the old
+    // parameters and the array sized for them)
+    unsigned long long* deltas = nullptr;
+    CK(dalloc(deltas, (size_t)params->max_iterations + 1));
     dfree(ctx->deltas);
-    CK(dalloc(ctx->deltas, (size_t)params->max_iterations + 1));
+    ctx->deltas = deltas;
   }
   ctx->prm = *params;
   return PEHIP_OK;
@@ -347,13 +431,27 @@ pehip_status pehip_set_materials(pehip_ctx* ctx, uint32_t n, const double* csf_p
   for (size_t i = 0; i < (size_t)n * n; ++i)
     if (!(csf_pair[i] >= 0) || !(cdf_pair[i] >= 0))
       return fail(ctx, PEHIP_INVALID_ARG, "Invalid coefficient of friction!");  // Materials.cpp:203-209
+  // uploaded bodies and members index the tables by their material IDs, all below the current
+  // count (pe's materials are only ever added, Materials.cpp:175-247)
+  if (ctx->n && n < ctx->nmat)
+    return fail(ctx, PEHIP_INVALID_ARG, "material table cannot shrink below the bodies' materials");
   hipSetDevice(ctx->device);
   CK(hipStreamSynchronize(ctx->stream));
+  // the new tables replace the old ones only once both are on the device
+  const size_t nn = (size_t)n * n;
+  double *csf = nullptr, *cdf = nullptr;
+  hipError_t e = dalloc(csf, nn);
+  if (e == hipSuccess) e = dalloc(cdf, nn);
+  if (e == hipSuccess) e = hipMemcpy(csf, csf_pair, sizeof(double) * nn, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(cdf, cdf_pair, sizeof(double) * nn, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dfree(csf); dfree(cdf);
+    return fail(ctx, e == hipErrorOutOfMemory ? PEHIP_OUT_OF_MEMORY : PEHIP_HIP_ERROR,
+                std::string("material tables: ") + hipGetErrorString(e));
+  }
   dfree(ctx->csf); dfree(ctx->cdf);
-  CK(dalloc(ctx->csf, (size_t)n * n));
-  CK(dalloc(ctx->cdf, (size_t)n * n));
-  CK(hipMemcpy(ctx->csf, csf_pair, sizeof(double) * n * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->cdf, cdf_pair, sizeof(double) * n * n, hipMemcpyHostToDevice));
+  ctx->csf = csf;
+  ctx->cdf = cdf;
   ctx->nmat = n;
   return PEHIP_OK;
 }
@@ -397,74 +495,11 @@ pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shap
   CK(hipStreamSynchronize(ctx->stream));
   pe_adjoint_free(ctx);
   ctx->free_bodies();
-  ctx->n = n;
-  ctx->h_fixed = fx;
-  ctx->h_shape.assign(shape, shape + n);
-  // rotation matrices from the quaternions (Quaternion::toRotationMatrix)
-  std::vector<double> hR(9 * (size_t)n);
-  for (uint32_t i = 0; i < n; ++i) {
-    qd qq;
-    for (int k = 0; k < 4; ++k) qq.v[k] = q[4 * (size_t)i + k];
-    const m33 m = quat_to_R(qq);
-    for (int k = 0; k < 9; ++k) hR[9 * (size_t)i + k] = m.a[k];
-  }
-  CK(dalloc(ctx->shape, n)); CK(dalloc(ctx->fixed, n)); CK(dalloc(ctx->mat, n)); CK(dalloc(ctx->sid, n));
-  CK(dalloc(ctx->param, 3 * (size_t)n)); CK(dalloc(ctx->pos, 3 * (size_t)n)); CK(dalloc(ctx->q, 4 * (size_t)n));
-  CK(dalloc(ctx->R, 9 * (size_t)n)); CK(dalloc(ctx->v, 3 * (size_t)n)); CK(dalloc(ctx->w, 3 * (size_t)n));
-  CK(dalloc(ctx->pnormal, 3 * (size_t)n)); CK(dalloc(ctx->pd, n)); CK(dalloc(ctx->aabb, 6 * (size_t)n));
-  CK(dalloc(ctx->inv_mass, n)); CK(dalloc(ctx->Ib, 9 * (size_t)n)); CK(dalloc(ctx->Ibinv, 9 * (size_t)n));
-  CK(dalloc(ctx->force, 3 * (size_t)n)); CK(dalloc(ctx->torque, 3 * (size_t)n));
-  CK(dalloc(ctx->sb, (size_t)kSB * n)); CK(dalloc(ctx->dvw, (size_t)kDVW * n));
-  CK(dalloc(ctx->planes, hplanes.size()));
-  const size_t D = sizeof(double);
-  CK(hipMemcpy(ctx->shape, shape, n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->fixed, fx.data(), n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->mat, material, 4 * (size_t)n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->sid, sid, 8 * (size_t)n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->param, param, 3 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->pos, pos, 3 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->q, q, 4 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->R, hR.data(), 9 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->v, v, 3 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->w, w, 3 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->pnormal, plane_normal, 3 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->pd, plane_d, D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->inv_mass, inv_mass, D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->Ib, inertia_body, 9 * D * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->Ibinv, inv_inertia_body, 9 * D * n, hipMemcpyHostToDevice));
-  if (!hplanes.empty()) CK(hipMemcpy(ctx->planes, hplanes.data(), 4 * hplanes.size(), hipMemcpyHostToDevice));
-  CK(hipMemset(ctx->force, 0, 3 * D * n));
-  CK(hipMemset(ctx->torque, 0, 3 * D * n));
-  CK(hipMemset(ctx->dvw, 0, kDVW * D * n));
-  CK(hipMemset(ctx->sb, 0, kSB * D * n));
-
-  plan_grid(ctx, shape, param, pos);
-  const size_t cells = (size_t)ctx->grid.nlevels * ctx->grid.table;
-  BroadphaseBuffers& P = ctx->bp;
-  CK(dalloc(P.keys, n)); CK(dalloc(P.vals, n)); CK(dalloc(P.keys_tmp, n)); CK(dalloc(P.vals_tmp, n));
-  CK(dalloc(P.cell_start, cells)); CK(dalloc(P.cell_end, cells));
-  CK(dalloc(P.counts, (size_t)n + 1)); CK(dalloc(P.offsets, (size_t)n + 1));
-  CK(dalloc(P.saabb, 6 * (size_t)n)); CK(dalloc(P.sfixed, n));
-  CK(dalloc(P.st_b1, (size_t)kStage * n)); CK(dalloc(P.st_b2, (size_t)kStage * n));
-  CK(dalloc(P.st_pos, 3 * (size_t)kStage * n)); CK(dalloc(P.st_nrm, 3 * (size_t)kStage * n));
-  CK(dalloc(P.st_dist, (size_t)kStage * n));
-  P.stride = n;
-  ctx->spheres_only = std::all_of(shape, shape + n, [](uint8_t t) { return t == PEHIP_SPHERE || t == PEHIP_PLANE; });
-  CK(dalloc(ctx->scan_scratch, scan_scratch_words((size_t)n + 1)));
-  CK(dalloc(ctx->sort_scratch, sort_scratch_words(n)));
-  CK(dalloc(ctx->col.deg, n)); CK(dalloc(ctx->col.adj_start, (size_t)n + 1)); CK(dalloc(ctx->col.adj_cursor, n));
-  CK(dalloc(ctx->ke_partial, (n + 255) / 256));
-  CK(dalloc(ctx->df_done, n));
-  CK(dalloc(ctx->isl.parent, n)); CK(dalloc(ctx->isl.ncont, n)); CK(dalloc(ctx->isl.cmask, 4 * (size_t)n));
-  CK(dalloc(ctx->isl.cursor, n)); CK(dalloc(ctx->isl.place, n)); CK(dalloc(ctx->isl.blocal, n));
-  P.planes = ctx->planes;
-  P.nplanes = (uint32_t)hplanes.size();
-  P.flags = ctx->flags;
-  ctx->members_missing = std::find(shape, shape + n, (uint8_t)PEHIP_UNION) != shape + n;
-  if (!ctx->members_missing) launch_aabb(ctx->bodies(), ctx->stream);  // unions: after their members
-  ctx->nc = 0;
-  ctx->order_pending = false;
-  return sync_check(ctx);
+  // a failure from here on leaves no bodies (never a body count over arrays that do not exist)
+  const pehip_status st = upload_body_arrays(ctx, n, fx, hplanes, shape, sid, material, param, pos, q, v, w,
+                                             inv_mass, inertia_body, inv_inertia_body, plane_normal, plane_d);
+  if (st != PEHIP_OK) ctx->free_bodies();
+  return st;
 }
 
 pehip_status pehip_upload_members(pehip_ctx* ctx, uint32_t nm, const uint32_t* owner, const uint64_t* sid,
@@ -492,22 +527,30 @@ pehip_status pehip_upload_members(pehip_ctx* ctx, uint32_t nm, const uint32_t* o
   hipSetDevice(ctx->device);
   CK(hipStreamSynchronize(ctx->stream));
   ctx->free_members();
-  const size_t D = sizeof(double);
-  CK(dalloc(ctx->mstart, (size_t)ctx->n + 1)); CK(dalloc(ctx->mowner, nm)); CK(dalloc(ctx->mmat, nm));
-  CK(dalloc(ctx->msid, nm)); CK(dalloc(ctx->mrad, nm)); CK(dalloc(ctx->mrpos, 3 * (size_t)nm));
-  CK(dalloc(ctx->mpos, 3 * (size_t)nm));
-  CK(hipMemcpy(ctx->mstart, start.data(), 4 * start.size(), hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->mowner, owner, 4 * (size_t)nm, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->mmat, material, 4 * (size_t)nm, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->msid, sid, 8 * (size_t)nm, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->mrad, radius, D * nm, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->mrpos, rpos, 3 * D * nm, hipMemcpyHostToDevice));
-  CK(hipMemcpy(ctx->mpos, gpos, 3 * D * nm, hipMemcpyHostToDevice));
-  ctx->nm = nm;
-  ctx->members_missing = false;
-  launch_aabb(ctx->bodies(), ctx->stream);
-  ctx->nc = 0;
-  return sync_check(ctx);
+  // a failure below leaves the unions without members: steps refuse to run until an upload succeeds
+  const pehip_status st = [&]() -> pehip_status {
+    const size_t D = sizeof(double);
+    CK(dalloc(ctx->mstart, (size_t)ctx->n + 1)); CK(dalloc(ctx->mowner, nm)); CK(dalloc(ctx->mmat, nm));
+    CK(dalloc(ctx->msid, nm)); CK(dalloc(ctx->mrad, nm)); CK(dalloc(ctx->mrpos, 3 * (size_t)nm));
+    CK(dalloc(ctx->mpos, 3 * (size_t)nm));
+    CK(hipMemcpy(ctx->mstart, start.data(), 4 * start.size(), hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->mowner, owner, 4 * (size_t)nm, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->mmat, material, 4 * (size_t)nm, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->msid, sid, 8 * (size_t)nm, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->mrad, radius, D * nm, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->mrpos, rpos, 3 * D * nm, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->mpos, gpos, 3 * D * nm, hipMemcpyHostToDevice));
+    ctx->nm = nm;
+    ctx->members_missing = false;
+    launch_aabb(ctx->bodies(), ctx->stream);
+    ctx->nc = 0;
+    return sync_check(ctx);
+  }();
+  if (st != PEHIP_OK) {
+    ctx->free_members();
+    ctx->members_missing = true;
+  }
+  return st;
 }
 
 pehip_status pehip_download_members(pehip_ctx* ctx, uint32_t nm, double* gpos) {
@@ -1081,16 +1124,19 @@ pehip_status pehip_eval_trig(pehip_ctx* ctx, int fn, int exact, size_t n, const 
   if (!n) return PEHIP_OK;
   hipSetDevice(ctx->device);
   double *dx = nullptr, *dy = nullptr, *dout = nullptr;
-  CK(dalloc(dx, n));
-  CK(dalloc(dy, n));
-  CK(dalloc(dout, n));
-  CK(hipMemcpy(dx, x, 8 * n, hipMemcpyHostToDevice));
-  CK(hipMemcpy(dy, y ? y : x, 8 * n, hipMemcpyHostToDevice));
-  launch_eval_trig(fn, exact != 0, n, dx, dy, dout, ctx->stream);
-  CK(hipStreamSynchronize(ctx->stream));
-  CK(hipMemcpy(out, dout, 8 * n, hipMemcpyDeviceToHost));
+  const pehip_status st = [&]() -> pehip_status {
+    CK(dalloc(dx, n));
+    CK(dalloc(dy, n));
+    CK(dalloc(dout, n));
+    CK(hipMemcpy(dx, x, 8 * n, hipMemcpyHostToDevice));
+    CK(hipMemcpy(dy, y ? y : x, 8 * n, hipMemcpyHostToDevice));
+    launch_eval_trig(fn, exact != 0, n, dx, dy, dout, ctx->stream);
+    CK(hipStreamSynchronize(ctx->stream));
+    CK(hipMemcpy(out, dout, 8 * n, hipMemcpyDeviceToHost));
+    return sync_check(ctx);
+  }();
   dfree(dx); dfree(dy); dfree(dout);
-  return sync_check(ctx);
+  return st;
 }
 
 void* pehip_stream(pehip_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }

@@ -1,11 +1,13 @@
 """Edge cases of the single-world path on the GPU: a world without bodies, a world of planes
 only, and bodies created between steps (World::add after World::run re-uploads the scene and
-keeps the system IDs), checked against the oracle."""
+keeps the system IDs), checked against the oracle; material tables that grow between steps and
+refuse to shrink below the uploaded bodies' material IDs."""
 import numpy as np
 import pytest
 
 import oracle as O
-from pe_amd import World
+from pe_amd import World, scenes, _abi
+from pe_amd.batch import BatchWorld
 
 pytestmark = pytest.mark.gpu
 
@@ -78,3 +80,68 @@ def test_bodies_added_between_steps_match_oracle():
         assert np.array_equal(g[k], o[k]), k
     assert np.array_equal(w.host_arrays()["sid"], ow.body_info()["sid"])
     w.close()
+
+
+def _resting_lattice(seed):
+    # touching spheres on a cubic lattice resting on the floor, small random velocities
+    return scenes.drop_scene(4, 4, 2, r=0.5, spacing=1.0 - 2e-6, jitter=2e-7, box=(5.5, 5.5, 18.0), mu=0.4,
+                             seed=seed, v_amp=0.05, max_iterations=25, z0=0.5 - 1e-6)
+
+
+def _grown(t, nm):
+    g = np.zeros((nm + 1, nm + 1))
+    g[:nm, :nm] = np.asarray(t, float).reshape(nm, nm)
+    return g
+
+
+def test_material_tables_grow_and_never_shrink_below_the_bodies():
+    # The bodies index the friction tables by material ID (row stride = the table's size), so a
+    # table smaller than the one the bodies were uploaded with is refused: it would be read past
+    # its end.  A larger table whose leading block is the old one is taken, and the world then
+    # steps bit for bit as one that kept the old table.
+    L = _abi.load()
+    sc = _resting_lattice(seed=5)
+    ref, w = World(), World()
+    for x in (ref, w):
+        sc.to_world(x)
+        x.run(3, sc.dt)
+    nm = len(w.materials)
+    csf, cdf = _grown(w._csf, nm), _grown(w._cdf, nm)
+    _abi.check(L.pehip_set_materials(w._ctx, nm + 1, _abi.ptr(csf), _abi.ptr(cdf)), w._ctx)
+    small = np.ascontiguousarray(np.asarray(w._csf, float).reshape(nm, nm))
+    st = L.pehip_set_materials(w._ctx, nm, _abi.ptr(small), _abi.ptr(small))
+    assert st == _abi.INVALID_ARG and "shrink" in L.pehip_last_error(w._ctx).decode()
+    ref.run(5, sc.dt)
+    w.run(5, sc.dt)
+    assert w.getNumberOfContacts() > 20
+    a, b = ref.getState(), w.getState()
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(ref.impulses(), w.impulses())
+    ref.close()
+    w.close()
+
+    # the batched scenes follow the same rule
+    scs = [_resting_lattice(seed=6), _resting_lattice(seed=7)]
+    bref, bw = BatchWorld(scs), BatchWorld(scs)
+    nm = len(scs[0].materials)
+    csf = np.zeros((nm, nm))
+    for i, mi in enumerate(scs[0].materials):
+        for j, mj in enumerate(scs[0].materials):
+            csf[i, j] = mi["csf"] + mj["csf"]
+    cdf = np.zeros((nm, nm))
+    for i, mi in enumerate(scs[0].materials):
+        for j, mj in enumerate(scs[0].materials):
+            cdf[i, j] = mi["cdf"] + mj["cdf"]
+    g1, g2 = _grown(csf, nm), _grown(cdf, nm)
+    _abi.check_batch(L.pehip_batch_set_materials(bw._b, nm + 1, _abi.ptr(g1), _abi.ptr(g2)), bw._b)
+    st = L.pehip_batch_set_materials(bw._b, nm, _abi.ptr(csf), _abi.ptr(cdf))
+    assert st == _abi.INVALID_ARG and "shrink" in L.pehip_batch_last_error(bw._b).decode()
+    bref.run(5)
+    bw.run(5)
+    a, b = bref.getState(), bw.getState()
+    for k in ("pos", "q", "v", "w"):
+        assert np.array_equal(a[k], b[k]), k
+    assert len(bw.contacts(0)["b1"]) > 20
+    bref.close()
+    bw.close()
