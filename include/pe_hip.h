@@ -110,7 +110,9 @@ pehip_status pehip_get_params(const pehip_ctx* ctx, pehip_params* params);
 
 /* Material pair tables already combined on the host: n x n row-major static and
  * dynamic friction (Materials.cpp:237-243 combine csf_i + csf_j), replacing
- * Material::get{Static,Dynamic}Friction (pe/core/Materials.h:500-537). */
+ * Material::get{Static,Dynamic}Friction (pe/core/Materials.h:500-537).  The tables may grow
+ * between steps (pe only ever adds materials); with bodies uploaded, n below the current count is
+ * refused (the bodies' material IDs index the tables).  A failure keeps the previous tables. */
 pehip_status pehip_set_materials(pehip_ctx* ctx, uint32_t n, const double* csf_pair,
                                  const double* cdf_pair);
 
@@ -127,7 +129,8 @@ pehip_status pehip_set_materials(pehip_ctx* ctx, uint32_t n, const double* csf_p
  *   pos[3i], q[4i], v[3i], w[3i]   state (R is derived from q as R = q.toRotationMatrix())
  *   inv_mass[i], inertia_body[9i], inv_inertia_body[9i]   mass properties (body frame)
  *   plane_normal[3i], plane_d[i]   plane parameters, host-computed (meaningful for planes)
- * Any previous bodies are replaced. */
+ * Any previous bodies are replaced.  A failure after validation (out of device memory) leaves
+ * the context without bodies. */
 pehip_status pehip_upload_bodies(pehip_ctx* ctx, uint32_t n, const uint8_t* shape,
                                  const uint64_t* sid, const uint32_t* material,
                                  const uint8_t* fixed, const double* param, const double* pos,
@@ -286,7 +289,8 @@ typedef struct {
 } pehip_adjoint_stats;
 
 /* World::run(nsteps, dt) as pehip_step, keeping a checkpoint of the state every
- * checkpoint_every steps (the first at the start). */
+ * checkpoint_every steps (the first at the start).  A run that fails part-way is dropped:
+ * pehip_adjoint_backward then reports that there is no differentiable run. */
 pehip_status pehip_adjoint_forward(pehip_ctx* ctx, double dt, uint32_t nsteps, uint32_t checkpoint_every);
 
 /* Reverse pass of the last pehip_adjoint_forward.  Inputs: dL/d(final positions, linear and
@@ -295,7 +299,8 @@ pehip_status pehip_adjoint_forward(pehip_ctx* ctx, double dt, uint32_t nsteps, u
  * of dL/d(contact friction) -- the derivative with respect to the friction coefficient when every
  * contact's friction is that one coefficient.  adjoint_sweeps bounds the sweeps per step;
  * adjoint_tol > 0 stops a step's sweeps when the L-inf change of the adjoint impulses drops below
- * it.  The context is left at the forward run's final state. */
+ * it.  The context is left at the forward run's final state, and with the parameters it had
+ * before the call, also when the pass fails. */
 pehip_status pehip_adjoint_backward(pehip_ctx* ctx, const double* dL_dpos, const double* dL_dv,
                                     const double* dL_dw, double* dL_dpos0, double* dL_dv0,
                                     double* dL_dw0, double* dL_dmu, uint32_t adjoint_sweeps,
@@ -342,7 +347,8 @@ pehip_status pehip_batch_set_materials(pehip_batch* b, uint32_t n, const double*
  * arrays, which have the layout of pehip_upload_bodies.  A scene holds at most 4096 bodies and
  * 16 planes; its grid bodies must be of similar size (one hash-grid level per scene, cell edge
  * = the largest grid-body bounding box).  contacts_per_body sizes each scene's contact slots
- * (capacity = contacts_per_body * bodies + 64; grown on demand, see pehip_batch_step). */
+ * (capacity = contacts_per_body * bodies + 64; grown on demand, see pehip_batch_step).  A failure
+ * after validation (out of device memory) leaves the batch without scenes. */
 pehip_status pehip_batch_upload(pehip_batch* b, uint32_t nscenes, const uint32_t* scene_offset,
                                 const uint8_t* shape, const uint64_t* sid, const uint32_t* material,
                                 const uint8_t* fixed, const double* param, const double* pos,
@@ -361,7 +367,9 @@ pehip_status pehip_batch_upload_state(pehip_batch* b, uint64_t n, const double* 
                                       const double* plane_d);
 /* Advances every scene by nsteps time steps of dt.  A scene whose contacts outgrow its slots
  * stops where it is; the library then grows that scene's capacity and finishes its steps, so
- * the call returns with every scene advanced (PEHIP_CAPACITY_RETRY only if the growth fails). */
+ * the call returns with every scene advanced (PEHIP_CAPACITY_RETRY if the grown capacity still
+ * does not suffice; PEHIP_OUT_OF_MEMORY if the device cannot hold it, which leaves the batch
+ * without scenes). */
 pehip_status pehip_batch_step(pehip_batch* b, double dt, uint32_t nsteps);
 pehip_status pehip_batch_download_state(pehip_batch* b, uint64_t n, double* pos, double* q, double* v,
                                         double* w, double* plane_normal, double* plane_d);
