@@ -12,6 +12,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import check_batch, ptr
+from .world import _per_body
 
 
 class BatchWorld:
@@ -122,7 +123,8 @@ class BatchWorld:
         return st
 
     def setState(self, st):
-        c = {k: np.ascontiguousarray(st[k], float) for k in ("pos", "q", "v", "w", "normal", "d")}
+        c = {k: _per_body(st[k], self.n, w, k) for k, w in (("pos", 3), ("q", 4), ("v", 3), ("w", 3), ("normal", 3),
+                                                            ("d", 0))}
         check_batch(_abi.load().pehip_batch_upload_state(self._b, self.n, ptr(c["pos"]), ptr(c["q"]), ptr(c["v"]),
                                                          ptr(c["w"]), ptr(c["normal"]), ptr(c["d"])), self._b)
 

@@ -95,6 +95,16 @@ class Material:
         self.poisson, self.young, self.stiffness, self.dampingN, self.dampingT = poisson, young, stiffness, dampingN, dampingT
 
 
+def _per_body(a, n, k, name, dtype=float):
+    """a as a contiguous (n, k) array (n values when k == 0); the C-ABI reads exactly that many
+    values, so any other size is refused before it reaches the library."""
+    arr = np.ascontiguousarray(a, dtype)
+    want = n * k if k else n
+    if arr.size != want:
+        raise ValueError("%s: expected %d values (%d bodies), got %d" % (name, want, n, arr.size))
+    return arr
+
+
 def union_param(rpos, radii):
     """A union's radii for the broadphase and for the solver's body records: the radius of a
     sphere about its centre of mass that holds every member (widened by 1e-12 so that rounding
@@ -621,15 +631,17 @@ class World:
         return self.contacts()
 
     def setContactOrder(self, body1, body2):
+        b1 = np.ascontiguousarray(body1, np.uint32).ravel()
+        b2 = np.ascontiguousarray(body2, np.uint32).ravel()
+        if len(b1) != len(b2):
+            raise ValueError("contact order: %d first bodies but %d second bodies" % (len(b1), len(b2)))
         self._ensure_device()
-        b1 = np.ascontiguousarray(body1, np.uint32)
-        b2 = np.ascontiguousarray(body2, np.uint32)
         check(_abi.load().pehip_set_contact_order(self._ctx, len(b1), ptr(b1, _abi._U32), ptr(b2, _abi._U32)), self._ctx)
 
     def setForces(self, force, torque):
+        f = _per_body(force, self.size, 3, "force")
+        t = _per_body(torque, self.size, 3, "torque")
         self._ensure_device()
-        f = np.ascontiguousarray(force, float)
-        t = np.ascontiguousarray(torque, float)
         check(_abi.load().pehip_set_forces(self._ctx, self.size, ptr(f), ptr(t)), self._ctx)
 
     # ------------------------------------------------------------------ state
@@ -646,9 +658,12 @@ class World:
         return st
 
     def setState(self, st):
-        self._ensure_device()
         n = self.size
-        c = {k: np.ascontiguousarray(st[k], float) for k in ("pos", "q", "v", "w", "normal", "d")}
+        c = {k: _per_body(st[k], n, w, k) for k, w in (("pos", 3), ("q", 4), ("v", 3), ("w", 3), ("normal", 3),
+                                                       ("d", 0))}
+        if "mgpos" in st and self._m_owner:
+            _per_body(st["mgpos"], len(self._m_owner), 3, "mgpos")
+        self._ensure_device()
         check(_abi.load().pehip_upload_state(self._ctx, n, ptr(c["pos"]), ptr(c["q"]), ptr(c["v"]), ptr(c["w"]),
                                              ptr(c["normal"]), ptr(c["d"])), self._ctx)
         if self._arrays is not None:

@@ -84,6 +84,32 @@ def test_world_api_errors():
     assert w._csf[0, 0] == 0.2
 
 
+def test_world_refuses_arrays_of_the_wrong_size_before_the_library():
+    # The C-ABI reads exactly 3n (4n for q, n for d) values per state array, 3n per force array
+    # and as many second bodies as first bodies: a shorter array would be read past its end, so
+    # the host refuses it (before touching a device, so no GPU is needed here).
+    w = World()
+    m = w.createMaterial("m", 1000.0, 0, 0.1, 0.1)
+    for k in range(3):
+        w.createSphere(k, (2.0 * k, 0, 1), 0.5, m)
+    w.createPlane(9, (0, 0, 1), 0.0, m)
+    n = w.size
+    good = {"pos": np.zeros((n, 3)), "q": np.tile([1.0, 0, 0, 0], (n, 1)), "v": np.zeros((n, 3)),
+            "w": np.zeros((n, 3)), "normal": np.zeros((n, 3)), "d": np.zeros(n)}
+    for key in good:
+        bad = dict(good)
+        bad[key] = good[key][:-1]
+        with pytest.raises(ValueError, match=key):
+            w.setState(bad)
+    with pytest.raises(ValueError, match="force"):
+        w.setForces(np.zeros((n - 1, 3)), np.zeros((n, 3)))
+    with pytest.raises(ValueError, match="torque"):
+        w.setForces(np.zeros((n, 3)), np.zeros(3 * n + 3))
+    with pytest.raises(ValueError, match="contact order"):
+        w.setContactOrder([0, 1], [3])
+    assert w._ctx is None   # nothing reached the library
+
+
 def test_mat3_inverse_host():
     I = np.zeros(9)
     I[0], I[4], I[8] = 2.0, 3.0, 5.0
