@@ -475,6 +475,13 @@ class World:
         n = len(arrays["shape"])
         if base:
             raise ValueError("bulk creation only into an empty world")
+        # the device upload reads a fixed number of values per body from each array
+        for key, k in (("mat", 1), ("fixed", 1), ("param", 3), ("pos", 3), ("q", 4), ("v", 3), ("w", 3),
+                       ("inv_mass", 1), ("I", 9), ("Iinv", 9), ("normal", 3), ("d", 1)):
+            if key not in arrays or np.asarray(arrays[key]).size != k * n:
+                raise ValueError("body array %s: expected %d values (%d bodies)" % (key, k * n, n))
+        if sids is not None and np.asarray(sids).size != n:
+            raise ValueError("sids: expected %d values" % n)
         shape = np.asarray(arrays["shape"])
         unions = np.flatnonzero(shape == UNION)
         arrays = dict(arrays)
@@ -489,6 +496,9 @@ class World:
             if np.any(np.asarray(arrays["fixed"])[unions]):
                 raise ValueError("fixed unions are not supported")
             k = len(owner)
+            for key in ("sid", "mat", "radius"):
+                if np.asarray(members[key]).size != k:
+                    raise ValueError("member array %s: expected %d values (%d members)" % (key, k, k))
             rad = np.asarray(members["radius"], float)
             rpos = np.asarray(members["rpos"], float).reshape(k, 3)
             arrays["param"] = np.array(arrays["param"], float)

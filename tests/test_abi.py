@@ -108,6 +108,15 @@ def test_world_refuses_arrays_of_the_wrong_size_before_the_library():
     with pytest.raises(ValueError, match="contact order"):
         w.setContactOrder([0, 1], [3])
     assert w._ctx is None   # nothing reached the library
+    # bulk creation: every body array holds its fixed count per body
+    from pe_amd import scenes
+    a = scenes.drop_scene(2, 2, 1, r=0.5, spacing=1.2, jitter=0.0, box=(4, 4, 8), seed=1).arrays()
+    for key in ("pos", "q", "I", "d"):
+        bad = dict(a)
+        bad[key] = a[key][:-1]
+        with pytest.raises(ValueError, match=key):
+            World().add_bodies(bad)
+    assert World().add_bodies(a).size == len(a["shape"])
 
 
 def test_mat3_inverse_host():
