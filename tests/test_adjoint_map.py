@@ -25,26 +25,36 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "..", "pe_amd", "csrc")
 
 
-@pytest.fixture(scope="module")
-def host_map(tmp_path_factory):
+def build_host_map(tmp_path_factory, extra=(), opt="-O2"):
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(hipcc):
         pytest.skip("hipcc is not available")
     exe = str(tmp_path_factory.mktemp("adjmap") / "adjoint_map_host")
-    cmd = [hipcc, "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-x", "hip",
+    cmd = [hipcc, "--offload-arch=gfx950", opt, "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", *extra, "-I", CSRC, "-x", "hip",
            os.path.join(HERE, "native", "adjoint_map_host.hip"), "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr
+    return exe
+
+
+def case_line(model, K, z):
+    vals = [model, K["rad1"], K["rad2"], K["im1"], K["im2"], *K["I1"].ravel(), *K["I2"].ravel(),
+            int(K["plane"]), *K["pn"], K["pd"], *K["xp"], K["erp"], K["dtinv"], *z]
+    return " ".join(repr(float(v)) if not isinstance(v, int) else str(v) for v in vals)
+
+
+@pytest.fixture(scope="module")
+def host_map(tmp_path_factory):
+    exe = build_host_map(tmp_path_factory)
 
     def run(model, K, z):
-        vals = [model, K["rad1"], K["rad2"], K["im1"], K["im2"], *K["I1"].ravel(), *K["I2"].ravel(),
-                int(K["plane"]), *K["pn"], K["pd"], *K["xp"], K["erp"], K["dtinv"], *z]
-        line = " ".join(repr(float(v)) if not isinstance(v, int) else str(v) for v in vals)
-        out = subprocess.run([exe], input=line + "\n", capture_output=True, text=True, timeout=60)
+        out = subprocess.run([exe], input=case_line(model, K, z) + "\n", capture_output=True, text=True,
+                             timeout=60)
         assert out.returncode == 0, out.stderr
         rows = np.array([[float(x) for x in l.split()] for l in out.stdout.strip().splitlines()])
         return rows[0], rows[1:]          # outputs (15), Jacobian (22 x 15)
+    run.exe = exe
     return run
 
 
@@ -266,3 +276,22 @@ def test_planted_mistakes_are_caught(host_map, planted):
     out, _ = host_map(3, K, z)
     bad = contact_outputs(3, K, z, planted=planted)
     assert np.abs(out[:3] - bad[:3]).max() > 1e-6 * np.abs(z[18:21]).max()
+
+
+def test_map_clean_under_sanitizers(tmp_path_factory, host_map):
+    # The same driver built with AddressSanitizer and UndefinedBehaviorSanitizer on its host code
+    # (no recovery, leak detection on) runs every case above without a report and prints the same
+    # bits as the plain build: no out-of-bounds or freed-memory access, no leak, and none of the
+    # undefined behaviour UBSan checks (signed overflow, bad shifts, null or misaligned accesses).
+    san = build_host_map(tmp_path_factory, opt="-O1",
+                         extra=("-g", "-Xarch_host", "-fsanitize=address,undefined",
+                                "-Xarch_host", "-fno-sanitize-recover=undefined"))
+    lines = "\n".join(case_line(m, *converged_case(m, s_, pl, sliding=s_ != 8)) for m, s_, pl in CASES) + "\n"
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:exitcode=99",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    got = subprocess.run([san], input=lines, capture_output=True, text=True, timeout=300, env=env)
+    assert got.returncode == 0 and "runtime error" not in got.stderr and "Sanitizer" not in got.stderr, got.stderr
+    plain = subprocess.run([host_map.exe], input=lines, capture_output=True, text=True, timeout=60)
+    assert plain.returncode == 0
+    assert got.stdout == plain.stdout
+    assert len(got.stdout.splitlines()) == len(CASES) * 23
