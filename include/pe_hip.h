@@ -104,7 +104,10 @@ void pehip_default_params(pehip_params* p);
 /* Replaces the CollisionSystem<HCSITS> singleton constructor (HCSITS.h:408-431) and
  * World (src/core/World.cpp:65).  device = HIP device ordinal. */
 pehip_status pehip_create(const pehip_params* params, int device, pehip_ctx** out);
+/* Ends the world (World::~World, src/core/World.cpp:88) and frees the context's device memory. */
 pehip_status pehip_destroy(pehip_ctx* ctx);
+/* All settings of pehip_params at once: the CollisionSystem and World setters each field cites
+ * (HCSITS.h:770-840, World.h:736-767), validated as they validate; the next step uses them. */
 pehip_status pehip_set_params(pehip_ctx* ctx, const pehip_params* params);
 pehip_status pehip_get_params(const pehip_ctx* ctx, pehip_params* params);
 
@@ -172,7 +175,8 @@ pehip_status pehip_upload_state(pehip_ctx* ctx, uint32_t n, const double* pos, c
 pehip_status pehip_set_forces(pehip_ctx* ctx, uint32_t n, const double* force,
                               const double* torque);
 
-/* Reference-order mode: the order in which the next step relaxes its contacts, as
+/* No counterpart in pe (a test hook: pe relaxes its contacts in HashGrids' emission order).
+ * Reference-order mode: the order in which the next step relaxes its contacts, as
  * body-index pairs (body1, body2) in pe's contact emission order.  Contacts of one
  * pair keep their narrowphase order.  Used only with PEHIP_SWEEP_REFERENCE_ORDER;
  * consumed by the next pehip_step. */
@@ -201,23 +205,30 @@ pehip_status pehip_step_noalloc(pehip_ctx* ctx, double dt, uint32_t nsteps);
  * without resolving or integrating; the contacts are then read with pehip_get_contacts. */
 pehip_status pehip_detect(pehip_ctx* ctx);
 
+/* Every body's dynamic state as the RigidBody getters read it (getPosition, getLinearVel,
+ * getAngularVel, getQuaternion, pe/core/rigidbody/RigidBody.h:146-151) and the planes' normal and
+ * displacement (Plane::getNormal, getDisplacement), in the layout of pehip_upload_state.  Any
+ * output may be NULL. */
 pehip_status pehip_download_state(pehip_ctx* ctx, uint32_t n, double* pos, double* q,
                                   double* v, double* w, double* plane_normal, double* plane_d);
 
-/* Contacts of the last step or detection in the library's order (deterministic).
+/* The collision system's active contacts (contacts_, HCSITS.h:291) of the last step or
+ * detection, in the library's order (deterministic; pe's is HashGrids' emission order).
  * cap = capacity of the arrays; *n receives the number of contacts (if *n > cap only
  * cap entries are written).  Any output pointer may be NULL. */
 pehip_status pehip_get_contacts(pehip_ctx* ctx, size_t cap, size_t* n, uint32_t* body1,
                                 uint32_t* body2, double* pos, double* normal, double* dist,
                                 double* mu);
-/* Sweep order of the last step: slot k of the sweep relaxes contact contact_index[k]
+/* No counterpart in pe (its sweeps follow the contacts' order).
+ * Sweep order of the last step: slot k of the sweep relaxes contact contact_index[k]
  * (pehip_get_contacts numbering).  Coloured mode: colour c occupies slots
  * [colour_offsets[c], colour_offsets[c+1]); *ncolours receives the colour count and
  * colour_offsets needs room for it + 1 entries (at most 129).  Any pointer may be NULL. */
 pehip_status pehip_get_sweep_order(pehip_ctx* ctx, size_t cap, size_t* n, uint32_t* contact_index,
                                    uint32_t* ncolours, uint32_t* colour_offsets);
 
-/* Solved world-frame impulses of the last step, in pehip_get_contacts order. */
+/* Solved impulses of the last step (p_, HCSITS.h:327, world frame as pe keeps them,
+ * HCSITS.h:2083), in pehip_get_contacts order. */
 pehip_status pehip_get_impulses(pehip_ctx* ctx, size_t cap, size_t* n, double* p);
 
 typedef struct {
@@ -250,24 +261,27 @@ pehip_status pehip_get_stats(pehip_ctx* ctx, pehip_stats* out);
 pehip_status pehip_get_step_stats(pehip_ctx* ctx, pehip_stats* out);
 
 /* Phase timing and per-sweep-launch timing with HIP events on the context's stream
- * (off by default); enabling resets the profiling totals. */
+ * (off by default); enabling resets the profiling totals.  pe's counterpart is its per-phase
+ * wall-clock timers (HCSITS.h:351), run inside pe_PROFILING_SECTION blocks (HCSITS.h:1479-1531). */
 pehip_status pehip_set_profiling(pehip_ctx* ctx, int enable);
 
-/* Evaluates the device trig of the solver on n arguments (checks against a host libm):
+/* No counterpart in pe (a test hook).  Evaluates the device trig of the solver on n arguments
+ * (checks against a host libm):
  * fn 0 = sin(x), 1 = cos(x), 2 = atan2(y, x), 3 = acos(x), 4 / 5 = sin(x) / cos(x) as the
  * golden-section search evaluates them (one sincos); exact selects the correctly
  * rounded implementation (see pehip_params.exact_trig).  y may be NULL unless fn = 2. */
 pehip_status pehip_eval_trig(pehip_ctx* ctx, int fn, int exact, size_t n, const double* x,
                              const double* y, double* out);
 
-/* The context's HIP stream (hipStream_t), for callers that time or order work on it. */
+/* No counterpart in pe.  The context's HIP stream (hipStream_t), for callers that time or
+ * order work on it. */
 void* pehip_stream(pehip_ctx* ctx);
 
 /* Human-readable message of the last failing call on this context (or of the last
  * failing pehip_create when ctx is NULL). */
 const char* pehip_last_error(const pehip_ctx* ctx);
 
-/* Library version string. */
+/* Library version string (no counterpart in pe). */
 const char* pehip_version(void);
 
 /* ------------------------------------------------------------------------------------------
