@@ -7,7 +7,7 @@
 //   theCollisionSystem() and its HCSITS setters/getters            (HCSITS.h:198-260, 715-840)
 //   createMaterial                                                  (src/core/Materials.cpp:175-247)
 //   createSphere / createBox / createCapsule / createPlane          (Sphere.cpp:1187, Box.cpp:1312,
-//                                                                    Capsule.cpp:1077, Plane.cpp:1202)
+//                                                                    Capsule.cpp:1219, Plane.cpp:1202)
 //   createUnion / createGranularParticle                            (Union.cpp:1960, 2376)
 // Body handles (SphereID, BoxID, PlaneID) read and write their state through the device;
 // the state is downloaded lazily after a step and uploaded before the next one.
@@ -500,7 +500,7 @@ inline BoxID createBox(id_t uid, const Vec3& gpos, const Vec3& lengths, Material
   return e.add<Box>(boxType, uid, material, false, prm, gpos, Quat(), m, real(1) / m, I, Vec3(), 0);
 }
 
-// createCapsule (Capsule.cpp:1077) with CapsuleBase's mass properties (CapsuleBase.cpp:60-92,
+// createCapsule (Capsule.cpp:1219) with CapsuleBase's mass properties (CapsuleBase.cpp:60-92,
 // 147-168): the axis is the body x-axis, the moments combine the cylinder and the end caps.
 inline CapsuleID createCapsule(id_t uid, const Vec3& gpos, real radius, real length, MaterialID material,
                                bool visible = true) {

@@ -60,15 +60,15 @@ typedef enum {
 } pehip_sweep_mode;
 
 typedef struct {
-  uint32_t max_iterations;     /* setMaxIterations, HCSITS.h:793 (default 100, HCSITS.h:420) */
+  uint32_t max_iterations;     /* setMaxIterations, HCSITS.h:793 (default 100, HCSITS.h:421) */
   int32_t relaxation_model;    /* setRelaxationModel, HCSITS.h:814 */
-  double erp;                  /* setErrorReductionParameter, HCSITS.h:840 (default 0.7) */
+  double erp;                  /* setErrorReductionParameter, HCSITS.h:840 (default 0.7, :420) */
   double abort_threshold;      /* <= 0: always max_iterations sweeps, as pe (HCSITS.h:1935-1946
                                   is compiled out); > 0: stop when the L-inf impulse change of
                                   a sweep drops below it (the reference's 1e-7, HCSITS.h:424) */
-  double gravity[3];           /* World::setGravity, World.h:715 */
-  double damping;              /* World::setDamping, World.h:767 (1 = off) */
-  double relaxation_param;     /* setRelaxationParameter, HCSITS.h:779 (MPI-only, stored) */
+  double gravity[3];           /* World::setGravity, World.h:754, 767 */
+  double damping;              /* World::setDamping, World.h:736 (1 = off) */
+  double relaxation_param;     /* setRelaxationParameter, HCSITS.h:770 (MPI-only, stored) */
   uint32_t max_sub_iterations; /* golden-section steps, HCSITS.h:423 (default 20) */
   int32_t sweep_mode;          /* pehip_sweep_mode */
   int32_t exact_trig;          /* 1: correctly rounded sin/cos/atan2/acos in the golden-section

@@ -165,7 +165,7 @@ class World:
         self._damping = float(damping)
         self._params_dirty = True
 
-    # collision-system setters (HCSITS.h:779-840)
+    # collision-system setters (HCSITS.h:770-840)
     def setMaxIterations(self, n):
         self._max_iterations = int(n)
         self._params_dirty = True
@@ -321,7 +321,7 @@ class World:
                          mat3_inverse(I), (0.0, 0.0, 0.0), 0.0, uid)
 
     def createCapsule(self, uid, gpos, radius, length, material):
-        """createCapsule (Capsule.cpp:1077) with CapsuleBase's mass properties (CapsuleBase.cpp:60-92,
+        """createCapsule (Capsule.cpp:1219) with CapsuleBase's mass properties (CapsuleBase.cpp:60-92,
         147-168): the axis is the body x-axis; the moments combine the cylinder and the end caps,
         with the density recomputed from the mass as the reference does."""
         r, L = float(radius), float(length)
