@@ -1,0 +1,53 @@
+// Host side of the C++ drop-in (include/pe_amd/pe.hpp) for the CPU tests: creates a fixed set of
+// bodies through pe's factory functions and prints the body and union-member arrays the library
+// would receive, one "name value value ..." line each (%.17g).  Nothing is stepped, so no device
+// is needed.
+#include <cstdio>
+#include <vector>
+
+#include <pe_amd/pe.hpp>
+
+using namespace pe;
+
+template <class T>
+static void put(const char* name, const std::vector<T>& a) {
+  std::printf("%s", name);
+  for (const T& x : a) std::printf(" %.17g", static_cast<double>(x));
+  std::printf("\n");
+}
+
+int main() {
+  MaterialID gran = createMaterial("granular", 2500, 0, 0.2, 0.2, 0.5, 1, 1, 0, 0);
+  MaterialID wood = createMaterial("wood", 700, 0.1, 0.35, 0.3, 0.5, 1, 1, 0, 0);
+  createPlane(100, Vec3(0, 0, 1), 0, gran);
+  createPlane(101, Vec3(1, -2, 0.5), -3.25, wood);
+  createSphere(1, Vec3(1.5, -0.25, 2), 0.375, gran);
+  BoxID b = createBox(2, Vec3(-1, 2, 0.75), Vec3(0.5, 1.25, 0.8), wood);
+  createCapsule(3, Vec3(0.3, 0.1, 4), 0.2, 1.1, gran);
+  createUnion(4, Vec3(2, 2, 3),
+              {UnionSphere{Vec3(0.3, 0, 0), 0.25, gran}, UnionSphere{Vec3(-0.1, 0.2, 0.05), 0.3, wood},
+               UnionSphere{Vec3(0, -0.15, 0.2), 0.2, gran}});
+  createUnion(5, Vec3(-2, -1, 1.5), {UnionSphere{Vec3(0.1, 0.1, 0.1), 0.4, wood}});
+  createSphere(6, Vec3(0, 0, 7), 0.5, wood);
+  b->setFixed(true);
+  const detail::Engine& e = detail::E();
+  put("shape", e.shape);
+  put("sid", e.sid);
+  put("mat", e.mat);
+  put("fixed", e.fixed);
+  put("param", e.param);
+  put("pos", e.pos);
+  put("q", e.q);
+  put("inv_mass", e.inv_mass);
+  put("I", e.I);
+  put("Iinv", e.Iinv);
+  put("normal", e.normal);
+  put("d", e.d);
+  put("m_owner", e.m_owner);
+  put("m_sid", e.m_sid);
+  put("m_mat", e.m_mat);
+  put("m_rad", e.m_rad);
+  put("m_rpos", e.m_rpos);
+  put("m_gpos", e.m_gpos);
+  return 0;
+}

@@ -151,3 +151,44 @@ def test_ctypes_structs_match_header_layout(tmp_path):
         assert int(got[cname + " size"]) == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
+
+
+def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
+    # The C++ drop-in (include/pe_amd/pe.hpp) and the Python World build the arrays the library
+    # receives on the host: the same factory calls (planes, a sphere, a fixed box, a capsule, a
+    # three-sphere union of two materials, a one-sphere union) must give the same bits in every
+    # body and member array, system IDs included.  Host only: nothing is stepped.
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None or not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("g++ or the library not available")
+    exe = str(tmp_path / "dropin_host_arrays")
+    lib = os.path.dirname(_abi.LIB_PATH)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "dropin_host_arrays.cpp"), "-o", exe, "-L", lib, "-lpehip",
+                    "-Wl,-rpath," + lib], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True, timeout=60).stdout
+    cpp = {l.split()[0]: np.array([float(x) for x in l.split()[1:]]) for l in out.splitlines()}
+
+    w = World()
+    gran = w.createMaterial("granular", 2500, 0, 0.2, 0.2)
+    wood = w.createMaterial("wood", 700, 0.1, 0.35, 0.3)
+    w.createPlane(100, (0, 0, 1), 0, gran)
+    w.createPlane(101, (1, -2, 0.5), -3.25, wood)
+    w.createSphere(1, (1.5, -0.25, 2), 0.375, gran)
+    box = w.createBox(2, (-1, 2, 0.75), (0.5, 1.25, 0.8), wood)
+    w.createCapsule(3, (0.3, 0.1, 4), 0.2, 1.1, gran)
+    w.createUnion(4, (2, 2, 3), [((0.3, 0, 0), 0.25, gran), ((-0.1, 0.2, 0.05), 0.3, wood),
+                                 ((0, -0.15, 0.2), 0.2, gran)])
+    w.createUnion(5, (-2, -1, 1.5), [((0.1, 0.1, 0.1), 0.4, wood)])
+    w.createSphere(6, (0, 0, 7), 0.5, wood)
+    w.setFixed(box)
+    a, m = w.host_arrays(), w.members()
+    py = {k: a[k] for k in ("shape", "sid", "mat", "fixed", "param", "pos", "q", "inv_mass", "I", "Iinv", "normal",
+                            "d")}
+    py.update({"m_owner": m["owner"], "m_sid": m["sid"], "m_mat": m["mat"], "m_rad": m["radius"],
+               "m_rpos": m["rpos"], "m_gpos": m["gpos"]})
+    assert set(cpp) == set(py)
+    for k in py:
+        assert np.array_equal(cpp[k], np.asarray(py[k], float).ravel()), k
+    assert int(cpp["m_owner"][-1]) == 6 and len(cpp["m_sid"]) == 4   # the unions' members follow them
