@@ -1,7 +1,7 @@
 // Host side of the C++ drop-in (include/pe_amd/pe.hpp) for the CPU tests: creates a fixed set of
 // bodies through pe's factory functions and prints the body and union-member arrays the library
-// would receive, one "name value value ..." line each (%.17g).  Nothing is stepped, so no device
-// is needed.
+// would receive, one "name value value ..." line each (%.17g), then moves and turns a union and
+// prints its members again.  Nothing is stepped, so no device is needed.
 #include <cstdio>
 #include <vector>
 
@@ -24,7 +24,7 @@ int main() {
   createSphere(1, Vec3(1.5, -0.25, 2), 0.375, gran);
   BoxID b = createBox(2, Vec3(-1, 2, 0.75), Vec3(0.5, 1.25, 0.8), wood);
   createCapsule(3, Vec3(0.3, 0.1, 4), 0.2, 1.1, gran);
-  createUnion(4, Vec3(2, 2, 3),
+  UnionID u = createUnion(4, Vec3(2, 2, 3),
               {UnionSphere{Vec3(0.3, 0, 0), 0.25, gran}, UnionSphere{Vec3(-0.1, 0.2, 0.05), 0.3, wood},
                UnionSphere{Vec3(0, -0.15, 0.2), 0.2, gran}});
   createUnion(5, Vec3(-2, -1, 1.5), {UnionSphere{Vec3(0.1, 0.1, 0.1), 0.4, wood}});
@@ -49,5 +49,12 @@ int main() {
   put("m_rad", e.m_rad);
   put("m_rpos", e.m_rpos);
   put("m_gpos", e.m_gpos);
+  // a union moved and then turned (Union::setPosition / setOrientation): its members follow
+  u->setPosition(Vec3(2.5, 1.75, 3.25));
+  put("moved_m_gpos", e.m_gpos);
+  u->setOrientation(Quat(Vec3(0.3, -0.5, 0.8), 0.7));
+  put("turned_q", e.q);
+  put("turned_m_gpos", e.m_gpos);
+  put("turned_pos", e.pos);
   return 0;
 }

@@ -157,7 +157,8 @@ def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
     # The C++ drop-in (include/pe_amd/pe.hpp) and the Python World build the arrays the library
     # receives on the host: the same factory calls (planes, a sphere, a fixed box, a capsule, a
     # three-sphere union of two materials, a one-sphere union) must give the same bits in every
-    # body and member array, system IDs included.  Host only: nothing is stepped.
+    # body and member array, system IDs included; the drop-in's union setters then move the
+    # members rigidly.  Host only: nothing is stepped.
     import shutil
     import subprocess
     if shutil.which("g++") is None or not os.path.exists(_abi.LIB_PATH):
@@ -188,7 +189,30 @@ def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
                             "d")}
     py.update({"m_owner": m["owner"], "m_sid": m["sid"], "m_mat": m["mat"], "m_rad": m["radius"],
                "m_rpos": m["rpos"], "m_gpos": m["gpos"]})
-    assert set(cpp) == set(py)
+    assert set(py) <= set(cpp)
     for k in py:
         assert np.array_equal(cpp[k], np.asarray(py[k], float).ravel()), k
     assert int(cpp["m_owner"][-1]) == 6 and len(cpp["m_sid"]) == 4   # the unions' members follow them
+
+    # Union::setPosition translates the members by the union's displacement; setOrientation puts
+    # each at pos + R(q) rpos (Union.cpp:251-272, 306-322).  Checked against a rigid motion built
+    # independently: the displacement, and the textbook rotation matrix of the unit quaternion.
+    g0, rp = cpp["m_gpos"].reshape(-1, 3), cpp["m_rpos"].reshape(-1, 3)
+    mine = cpp["m_owner"] == 5
+    moved = cpp["moved_m_gpos"].reshape(-1, 3)
+    dp = np.array([2.5, 1.75, 3.25]) - np.array([2.0, 2.0, 3.0])
+    assert np.allclose(moved[mine], g0[mine] + dp, rtol=0, atol=1e-15)
+    assert np.array_equal(moved[~mine], g0[~mine])
+    qw, qx, qy, qz = cpp["turned_q"].reshape(-1, 4)[5]
+    assert abs(qw * qw + qx * qx + qy * qy + qz * qz - 1) < 1e-15 and abs(qw - np.cos(0.35)) < 1e-15
+    R = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy)],
+                  [2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx)],
+                  [2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)]])
+    x = cpp["turned_pos"].reshape(-1, 3)[5]
+    turned = cpp["turned_m_gpos"].reshape(-1, 3)
+    assert np.allclose(turned[mine], x + rp[mine] @ R.T, rtol=0, atol=1e-14)
+    assert np.array_equal(turned[~mine], g0[~mine])
+    # the members keep their distances to each other (a rigid motion)
+    d0 = np.linalg.norm(g0[mine][:, None] - g0[mine][None], axis=-1)
+    d1 = np.linalg.norm(turned[mine][:, None] - turned[mine][None], axis=-1)
+    assert np.allclose(d0, d1, rtol=0, atol=1e-14)
