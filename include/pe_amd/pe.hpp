@@ -287,6 +287,50 @@ class Engine {
     return h;
   }
 
+  // World::remove -> HCSITS::remove -> BodyStorage::remove (World.cpp:298-306, HCSITS.h:1346-1373,
+  // BodyStorage.h:563-573): the last body moves into the removed body's place in the body order
+  // and the removed body's handle is deleted; a union leaves with its members.  The next step
+  // uploads the remaining bodies.
+  void remove(size_t i) {
+    sync_host();
+    const size_t last = shape.size() - 1;
+    auto move_last = [&](auto& a, size_t k) {  // k values per body
+      for (size_t j = 0; j < k; ++j) a[k * i + j] = a[k * last + j];
+      a.resize(k * last);
+    };
+    move_last(shape, 1); move_last(fixed, 1); move_last(sid, 1); move_last(uid, 1); move_last(mat, 1);
+    move_last(param, 3); move_last(pos, 3); move_last(q, 4); move_last(v, 3); move_last(w, 3);
+    move_last(inv_mass, 1); move_last(mass, 1); move_last(I, 9); move_last(Iinv, 9); move_last(normal, 3);
+    move_last(d, 1); move_last(force, 3); move_last(torque, 3);
+    move_last(vpos, 1); move_last(vv, 1); move_last(vw, 1); move_last(vq, 1);
+    if (i != last) {
+      handles[i] = std::move(handles[last]);
+      handles[i]->index_ = i;
+    }
+    handles.pop_back();
+    if (!m_owner.empty()) {
+      // the removed union's members leave; the moved body's follow it; members stay grouped by
+      // union in body order, each union's in its own order
+      const auto owner_of = [&](size_t k) { return m_owner[k] == last ? (uint32_t)i : m_owner[k]; };
+      std::vector<size_t> keep;
+      for (size_t k = 0; k < m_owner.size(); ++k)
+        if (m_owner[k] != i) keep.push_back(k);
+      std::stable_sort(keep.begin(), keep.end(), [&](size_t a, size_t b) { return owner_of(a) < owner_of(b); });
+      std::vector<uint32_t> o, mm;
+      std::vector<uint64_t> ms;
+      std::vector<real> mr, mrp, mg;
+      for (size_t k : keep) {
+        o.push_back(owner_of(k));
+        mm.push_back(m_mat[k]);
+        ms.push_back(m_sid[k]);
+        mr.push_back(m_rad[k]);
+        for (int j = 0; j < 3; ++j) { mrp.push_back(m_rpos[3 * k + j]); mg.push_back(m_gpos[3 * k + j]); }
+      }
+      m_owner.swap(o); m_mat.swap(mm); m_sid.swap(ms); m_rad.swap(mr); m_rpos.swap(mrp); m_gpos.swap(mg);
+    }
+    bodies_dirty = true;
+  }
+
   // download the device state after a step (lazy)
   void sync_host() {
     if (!state_stale || !ctx_) return;
@@ -624,6 +668,14 @@ inline PlaneID createPlane(id_t uid, Vec3 normal, real d, MaterialID material, b
                : Quat(Vec3(-normal[1], normal[0], real(0)), std::acos(normal[2]));
   const real zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   return detail::E().add<Plane>(planeType, uid, material, true, zero, normal * d, q, 0, 0, zero, normal, d);
+}
+
+// destroy (src/core/rigidbody/RigidBody.cpp:766-775): the body leaves the world and its handle is
+// deleted (the BodyID is invalid afterwards, as in pe); the last body takes its place in the body
+// order that World::operator[] walks.
+inline void destroy(BodyID body) {
+  if (!body) throw std::invalid_argument("Invalid rigid body");
+  detail::E().remove(body->index());
 }
 
 // ---------------------------------------------------------------- World / CollisionSystem

@@ -457,6 +457,36 @@ class World:
         self._w[body] = np.zeros(3)
         self._bodies_dirty = True
 
+    def destroy(self, body):
+        """destroy(BodyID) (RigidBody.cpp:766-775) -> World::remove -> HCSITS::remove (World.cpp:298-306,
+        HCSITS.h:1346-1373): the body leaves the world.  pe's BodyStorage::remove
+        (BodyStorage.h:563-573) moves the last body into the removed body's place, so the last body
+        takes index ``body``.  A union leaves with its members.  The next step uploads the
+        remaining bodies."""
+        self._materialise_lists()
+        n = len(self._shape)
+        body = int(body)
+        if not 0 <= body < n:
+            raise IndexError("invalid body index %d (%d bodies)" % (body, n))
+        last = n - 1
+        for lst in (self._shape, self._sid, self._mat, self._fixed, self._param, self._pos, self._q, self._v,
+                    self._w, self._inv_mass, self._I, self._Iinv, self._normal, self._d, self._uid):
+            lst[body] = lst[last]
+            lst.pop()
+        if self._m_owner:
+            # the removed union's members leave; the moved body's follow it; members stay grouped
+            # by union in body order, each union's in its own order (a stable sort)
+            owner = [body if o == last else o for o in self._m_owner]
+            keep = sorted((k for k, o in enumerate(self._m_owner) if o != body), key=lambda k: owner[k])
+            self._m_owner = [owner[k] for k in keep]
+            for name in ("_m_sid", "_m_uid", "_m_mat", "_m_rad", "_m_rpos", "_m_gpos"):
+                src = getattr(self, name)
+                setattr(self, name, [src[k] for k in keep])
+        self._u_mass.pop(body, None)
+        if last != body and last in self._u_mass:
+            self._u_mass[body] = self._u_mass.pop(last)
+        self._bodies_dirty = True
+
     def unionMass(self, body):
         """Total mass of a union (Union::getMass), as created or as read from a body file."""
         return self._u_mass[body]

@@ -1,7 +1,8 @@
 // Host side of the C++ drop-in (include/pe_amd/pe.hpp) for the CPU tests: creates a fixed set of
 // bodies through pe's factory functions and prints the body and union-member arrays the library
 // would receive, one "name value value ..." line each (%.17g), then moves and turns a union and
-// prints its members again.  Nothing is stepped, so no device is needed.
+// prints its members again, then destroys two bodies and prints every array again.  Nothing is
+// stepped, so no device is needed.
 #include <cstdio>
 #include <vector>
 
@@ -21,7 +22,7 @@ int main() {
   MaterialID wood = createMaterial("wood", 700, 0.1, 0.35, 0.3, 0.5, 1, 1, 0, 0);
   createPlane(100, Vec3(0, 0, 1), 0, gran);
   createPlane(101, Vec3(1, -2, 0.5), -3.25, wood);
-  createSphere(1, Vec3(1.5, -0.25, 2), 0.375, gran);
+  SphereID s1 = createSphere(1, Vec3(1.5, -0.25, 2), 0.375, gran);
   BoxID b = createBox(2, Vec3(-1, 2, 0.75), Vec3(0.5, 1.25, 0.8), wood);
   createCapsule(3, Vec3(0.3, 0.1, 4), 0.2, 1.1, gran);
   UnionID u = createUnion(4, Vec3(2, 2, 3),
@@ -56,5 +57,34 @@ int main() {
   put("turned_q", e.q);
   put("turned_m_gpos", e.m_gpos);
   put("turned_pos", e.pos);
+  // destroy: a sphere from the middle (the last body, a sphere, takes its place), then the turned
+  // union (the other union, now last, takes its place with its member)
+  destroy(s1);
+  destroy(u);
+  put("after_shape", e.shape);
+  put("after_sid", e.sid);
+  put("after_mat", e.mat);
+  put("after_fixed", e.fixed);
+  put("after_param", e.param);
+  put("after_pos", e.pos);
+  put("after_q", e.q);
+  put("after_inv_mass", e.inv_mass);
+  put("after_I", e.I);
+  put("after_Iinv", e.Iinv);
+  put("after_normal", e.normal);
+  put("after_d", e.d);
+  put("after_m_owner", e.m_owner);
+  put("after_m_sid", e.m_sid);
+  put("after_m_mat", e.m_mat);
+  put("after_m_rad", e.m_rad);
+  put("after_m_rpos", e.m_rpos);
+  put("after_m_gpos", e.m_gpos);
+  std::vector<double> idx, uids;
+  for (size_t k = 0; k < theWorld()->size(); ++k) {
+    idx.push_back((double)(*theWorld())[k]->index());
+    uids.push_back((double)(*theWorld())[k]->getID());
+  }
+  put("after_handle_index", idx);
+  put("after_handle_uid", uids);
   return 0;
 }

@@ -158,7 +158,8 @@ def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
     # receives on the host: the same factory calls (planes, a sphere, a fixed box, a capsule, a
     # three-sphere union of two materials, a one-sphere union) must give the same bits in every
     # body and member array, system IDs included; the drop-in's union setters then move the
-    # members rigidly.  Host only: nothing is stepped.
+    # members rigidly, and destroying bodies leaves pe's body order on both.  Host only: nothing is
+    # stepped.
     import shutil
     import subprocess
     if shutil.which("g++") is None or not os.path.exists(_abi.LIB_PATH):
@@ -216,3 +217,27 @@ def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
     d0 = np.linalg.norm(g0[mine][:, None] - g0[mine][None], axis=-1)
     d1 = np.linalg.norm(turned[mine][:, None] - turned[mine][None], axis=-1)
     assert np.allclose(d0, d1, rtol=0, atol=1e-14)
+
+    # destroy (RigidBody.cpp:766-775): pe's BodyStorage::remove (BodyStorage.h:563-573) moves the
+    # last body into the removed one's place.  Removing body 2 (a sphere) then body 5 (the
+    # three-sphere union) leaves the bodies [0, 1, 7, 3, 4, 6] of the original order, and the
+    # one-sphere union's member now belongs to body 5.
+    w.destroy(2)
+    w.destroy(5)
+    perm = [0, 1, 7, 3, 4, 6]
+    a2, m2 = w.host_arrays(), w.members()
+    for k in py:
+        if not k.startswith("m_"):
+            assert np.array_equal(np.asarray(a2[k]), np.asarray(py[k])[perm]), k
+            assert np.array_equal(cpp["after_" + k], np.asarray(a2[k], float).ravel()), k
+    assert m2["owner"].tolist() == [5] and m2["sid"].tolist() == [int(py["m_sid"][-1])]
+    assert np.array_equal(m2["gpos"], np.asarray(py["m_gpos"]).reshape(-1, 3)[-1:])
+    for k, mk in (("m_owner", "owner"), ("m_sid", "sid"), ("m_mat", "mat"), ("m_rad", "radius"),
+                  ("m_rpos", "rpos"), ("m_gpos", "gpos")):
+        assert np.array_equal(cpp["after_" + k], np.asarray(m2[mk], float).ravel()), k
+    assert w.unionMass(5) > 0 and 6 not in w._u_mass
+    # the C++ handles follow their bodies: World::operator[] walks the new order
+    assert cpp["after_handle_index"].tolist() == list(range(6))
+    assert cpp["after_handle_uid"].tolist() == [100, 101, 6, 2, 3, 5]
+    with pytest.raises(IndexError):
+        w.destroy(6)
