@@ -61,7 +61,50 @@ struct Quat {
     axis.normalize();
     v[0] = c; v[1] = s * axis[0]; v[2] = s * axis[1]; v[3] = s * axis[2];
   }
+  // Quaternion(xangle, yangle, zangle) (Quaternion.h:309-315): rotations about x, then y, then z
+  Quat(real xangle, real yangle, real zangle) : v{1, 0, 0, 0} {
+    rotateX(xangle);
+    rotateY(yangle);
+    rotateZ(zangle);
+  }
   real operator[](size_t i) const { return v[i]; }
+  // Quaternion::rotateX/Y/Z (Quaternion.h:645-697): the quaternion turned about a global axis
+  void rotateX(real angle) {
+    const real s = std::sin(angle * real(0.5)), c = std::cos(angle * real(0.5));
+    const Quat r(c * v[0] - s * v[1], c * v[1] + s * v[0], c * v[2] - s * v[3], c * v[3] + s * v[2]);
+    *this = r;
+  }
+  void rotateY(real angle) {
+    const real s = std::sin(angle * real(0.5)), c = std::cos(angle * real(0.5));
+    const Quat r(c * v[0] - s * v[2], c * v[1] + s * v[3], c * v[2] + s * v[0], c * v[3] - s * v[1]);
+    *this = r;
+  }
+  void rotateZ(real angle) {
+    const real s = std::sin(angle * real(0.5)), c = std::cos(angle * real(0.5));
+    const Quat r(c * v[0] - s * v[3], c * v[1] - s * v[2], c * v[2] + s * v[1], c * v[3] + s * v[0]);
+    *this = r;
+  }
+  // operator*(Quaternion, Quaternion) (Quaternion.h:1146-1163): renormalised when the product's
+  // squared length is 1e-8 or more away from one
+  Quat operator*(const Quat& r) const {
+    const real a = v[0] * r.v[0] - v[1] * r.v[1] - v[2] * r.v[2] - v[3] * r.v[3];
+    const real i = v[0] * r.v[1] + v[1] * r.v[0] + v[2] * r.v[3] - v[3] * r.v[2];
+    const real j = v[0] * r.v[2] + v[2] * r.v[0] + v[3] * r.v[1] - v[1] * r.v[3];
+    const real k = v[0] * r.v[3] + v[3] * r.v[0] + v[1] * r.v[2] - v[2] * r.v[1];
+    const real len2 = a * a + i * i + j * j + k * k;
+    if (std::fabs(len2 - real(1)) < real(1e-8)) return Quat(a, i, j, k);
+    const real il = real(1) / std::sqrt(len2);
+    return Quat(a * il, i * il, j * il, k * il);
+  }
+  // Quaternion::rotate (Quaternion.h:744-756): the vector turned by the quaternion, in two steps
+  Vec3 rotate(const Vec3& x) const {
+    const real w = v[1] * x[0] + v[2] * x[1] + v[3] * x[2];
+    const real a = v[0] * x[0] - v[3] * x[1] + v[2] * x[2];
+    const real b = v[0] * x[1] - v[1] * x[2] + v[3] * x[0];
+    const real c = v[0] * x[2] - v[2] * x[0] + v[1] * x[1];
+    return Vec3(v[0] * a + v[1] * w + v[2] * c - v[3] * b, v[0] * b + v[2] * w + v[3] * a - v[1] * c,
+                v[0] * c + v[3] * w + v[1] * b - v[2] * a);
+  }
   // Quaternion::toRotationMatrix (Quaternion.h:591-602), row-major
   void toRotationMatrix(real* m) const {
     m[0] = real(1) - real(2) * v[2] * v[2] - real(2) * v[3] * v[3];
@@ -127,6 +170,23 @@ class RigidBody {
   void setLinearVel(const Vec3& v);
   void setAngularVel(const Vec3& w);
   void setOrientation(const Quat& q);
+  // pe's pose utilities (RigidBody.h:163-229): translations, rotations about the body's centre,
+  // the global origin or a point, and body-frame points in the world frame
+  void translate(real dx, real dy, real dz);
+  void translate(const Vec3& dp);
+  void rotate(real x, real y, real z, real angle);
+  void rotate(const Vec3& axis, real angle);
+  void rotate(real xangle, real yangle, real zangle);
+  void rotate(const Vec3& euler);
+  void rotate(const Quat& dq);
+  void rotateAroundOrigin(real x, real y, real z, real angle);
+  void rotateAroundOrigin(const Vec3& axis, real angle);
+  void rotateAroundOrigin(real xangle, real yangle, real zangle);
+  void rotateAroundOrigin(const Vec3& euler);
+  void rotateAroundPoint(const Vec3& point, const Vec3& axis, real angle);
+  void rotateAroundPoint(const Vec3& point, const Vec3& euler);
+  Vec3 pointFromBFtoWF(real px, real py, real pz) const;
+  Vec3 pointFromBFtoWF(const Vec3& rpos) const;
   void setFixed(bool fixed);
   void addForce(const Vec3& f);
   void addTorque(const Vec3& t);
@@ -287,6 +347,67 @@ class Engine {
     return h;
   }
 
+  // pose access of body i (host arrays; callers synchronise first)
+  Quat quat(size_t i) const { return Quat(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]); }
+  void set_quat(size_t i, const Quat& qq) { for (int k = 0; k < 4; ++k) q[4 * i + k] = qq[k]; }
+  Vec3 position(size_t i) const { return Vec3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]); }
+  void set_position(size_t i, const Vec3& p) { for (int k = 0; k < 3; ++k) pos[3 * i + k] = p[k]; }
+  // A plane's normal from its orientation, the third column of R (Plane.cpp:355-358), and its
+  // displacement from the origin, trans(normal) * gpos (Plane.cpp:226, 264, 361, 433).
+  void plane_normal_from_quat(size_t i) {
+    real R[9];
+    quat(i).toRotationMatrix(R);
+    normal[3 * i] = R[2];
+    normal[3 * i + 1] = R[5];
+    normal[3 * i + 2] = R[8];
+  }
+  void plane_displacement(size_t i) {
+    d[i] = normal[3 * i] * pos[3 * i] + normal[3 * i + 1] * pos[3 * i + 1] + normal[3 * i + 2] * pos[3 * i + 2];
+  }
+  // A union's members after the union turned: each at gpos + R rpos (Sphere::update(dq),
+  // Sphere.cpp:1052-1070); after it moved by dp: each moved by dp (Sphere::update(dp),
+  // Sphere.cpp:1031).
+  void members_follow_rotation(size_t i) {
+    real R[9];
+    quat(i).toRotationMatrix(R);
+    const real* x = &pos[3 * i];
+    for (size_t m = 0; m < m_owner.size(); ++m)
+      if (m_owner[m] == i) {
+        const real* r = &m_rpos[3 * m];
+        for (int k = 0; k < 3; ++k) m_gpos[3 * m + k] = x[k] + (R[3 * k] * r[0] + R[3 * k + 1] * r[1] + R[3 * k + 2] * r[2]);
+      }
+  }
+  void members_translate(size_t i, const real* dp) {
+    for (size_t m = 0; m < m_owner.size(); ++m)
+      if (m_owner[m] == i)
+        for (int k = 0; k < 3; ++k) m_gpos[3 * m + k] = m_gpos[3 * m + k] + dp[k];
+  }
+  // The pose change shared by pe's rotate functions (Sphere/Box/Capsule/Plane/Union.cpp): q <- dq q,
+  // and for rotations about the origin or a point the position turned with it (gpos <- dq.rotate(gpos),
+  // or point + dq.rotate(gpos - point)); then a plane's normal (and its displacement, except after a
+  // rotation about the origin, Plane.cpp:693-790) and a union's members.
+  void turn(size_t i, const Quat& dq, bool about_origin, const Vec3* point) {
+    sync_host();
+    if (about_origin) {
+      set_position(i, dq.rotate(position(i)));
+    } else if (point) {
+      const Vec3 g = position(i), p = *point;
+      const Vec3 r = dq.rotate(Vec3(g[0] - p[0], g[1] - p[1], g[2] - p[2]));
+      set_position(i, Vec3(p[0] + r[0], p[1] + r[1], p[2] + r[2]));
+    }
+    set_quat(i, dq * quat(i));
+    after_turn(i, !about_origin);
+  }
+  void after_turn(size_t i, bool plane_d) {
+    if (shape[i] == planeType) {
+      plane_normal_from_quat(i);
+      if (plane_d) plane_displacement(i);
+    } else if (shape[i] == unionType) {
+      members_follow_rotation(i);
+    }
+    state_dirty = true;
+  }
+
   // World::remove -> HCSITS::remove -> BodyStorage::remove (World.cpp:298-306, HCSITS.h:1346-1373,
   // BodyStorage.h:563-573): the last body moves into the removed body's place in the body order
   // and the removed body's handle is deleted; a union leaves with its members.  The next step
@@ -421,18 +542,18 @@ inline const Quat& RigidBody::getQuaternion() const {
 }
 inline real RigidBody::getMass() const { return detail::E().mass[index_]; }
 inline real RigidBody::getInvMass() const { return detail::E().inv_mass[index_]; }
-// Union::setPosition (Union.cpp:251-272): every member moves by the union's displacement
-// (Sphere::update(dp), Sphere.cpp:1031).
+// setPosition: Union::setPosition (Union.cpp:251-272) moves every member by the union's
+// displacement (Sphere::update(dp), Sphere.cpp:1031); Plane::setPosition (Plane.cpp:255-268)
+// recomputes the plane's displacement from the origin.
 inline void RigidBody::setPosition(const Vec3& p) {
   detail::Engine& e = detail::E();
   e.sync_host();
   if (e.shape[index_] == unionType) {
     const real dp[3] = {p[0] - e.pos[3 * index_], p[1] - e.pos[3 * index_ + 1], p[2] - e.pos[3 * index_ + 2]};
-    for (size_t m = 0; m < e.m_owner.size(); ++m)
-      if (e.m_owner[m] == index_)
-        for (int k = 0; k < 3; ++k) e.m_gpos[3 * m + k] = e.m_gpos[3 * m + k] + dp[k];
+    e.members_translate(index_, dp);
   }
   for (int k = 0; k < 3; ++k) e.pos[3 * index_ + k] = p[k];
+  if (e.shape[index_] == planeType) e.plane_displacement(index_);
   e.state_dirty = true;
 }
 inline void RigidBody::setLinearVel(const Vec3& vel) {
@@ -449,23 +570,78 @@ inline void RigidBody::setAngularVel(const Vec3& om) {
   for (int k = 0; k < 3; ++k) e.w[3 * index_ + k] = om[k];
   e.state_dirty = true;
 }
-// Union::setOrientation (Union.cpp:306-322): every member goes to pos + R rpos
-// (Sphere::update(dq), Sphere.cpp:1059).
+// setOrientation: Union::setOrientation (Union.cpp:306-322) puts every member at pos + R rpos
+// (Sphere::update(dq), Sphere.cpp:1059); Plane::setOrientation (Plane.cpp:340-355) recomputes the
+// plane's normal and displacement.
 inline void RigidBody::setOrientation(const Quat& qq) {
   detail::Engine& e = detail::E();
   e.sync_host();
-  for (int k = 0; k < 4; ++k) e.q[4 * index_ + k] = qq[k];
-  if (e.shape[index_] == unionType) {
-    real R[9];
-    qq.toRotationMatrix(R);
-    const real* x = &e.pos[3 * index_];
-    for (size_t m = 0; m < e.m_owner.size(); ++m)
-      if (e.m_owner[m] == index_) {
-        const real* r = &e.m_rpos[3 * m];
-        for (int k = 0; k < 3; ++k) e.m_gpos[3 * m + k] = x[k] + (R[3 * k] * r[0] + R[3 * k + 1] * r[1] + R[3 * k + 2] * r[2]);
-      }
-  }
+  e.set_quat(index_, qq);
+  e.after_turn(index_, true);
+}
+// translate (Box.cpp, Union.cpp:563-580, Plane.cpp:425-435): gpos += dp; a union's members move
+// by dp, a plane's displacement is recomputed
+inline void RigidBody::translate(real dx, real dy, real dz) { translate(Vec3(dx, dy, dz)); }
+inline void RigidBody::translate(const Vec3& dp) {
+  detail::Engine& e = detail::E();
+  e.sync_host();
+  for (int k = 0; k < 3; ++k) e.pos[3 * index_ + k] = e.pos[3 * index_ + k] + dp[k];
+  if (e.shape[index_] == unionType) e.members_translate(index_, dp.v);
+  if (e.shape[index_] == planeType) e.plane_displacement(index_);
   e.state_dirty = true;
+}
+// rotate (Box.cpp:456-630 and the same in Sphere.cpp, Capsule.cpp, Plane.cpp:474-640,
+// Union.cpp:624-830): q <- Quat(axis, angle) q, or q <- dq q; with Euler angles a sphere, box,
+// capsule or plane turns its own quaternion about x, then y, then z, and a union multiplies by
+// Quat(xangle, yangle, zangle)
+inline void RigidBody::rotate(real x, real y, real z, real angle) { rotate(Vec3(x, y, z), angle); }
+inline void RigidBody::rotate(const Vec3& axis, real angle) {
+  detail::E().turn(index_, Quat(axis, angle), false, nullptr);
+}
+inline void RigidBody::rotate(real xangle, real yangle, real zangle) {
+  detail::Engine& e = detail::E();
+  if (e.shape[index_] == unionType) {
+    e.turn(index_, Quat(xangle, yangle, zangle), false, nullptr);
+    return;
+  }
+  e.sync_host();
+  Quat qq = e.quat(index_);
+  qq.rotateX(xangle);
+  qq.rotateY(yangle);
+  qq.rotateZ(zangle);
+  e.set_quat(index_, qq);
+  e.after_turn(index_, true);
+}
+inline void RigidBody::rotate(const Vec3& euler) { rotate(euler[0], euler[1], euler[2]); }
+inline void RigidBody::rotate(const Quat& dq) { detail::E().turn(index_, dq, false, nullptr); }
+// rotateAroundOrigin (Box.cpp:649-780, Plane.cpp:666-790, Union.cpp:850-1000): the position turns
+// about the global origin with the body; a plane keeps its displacement
+inline void RigidBody::rotateAroundOrigin(real x, real y, real z, real angle) {
+  rotateAroundOrigin(Vec3(x, y, z), angle);
+}
+inline void RigidBody::rotateAroundOrigin(const Vec3& axis, real angle) {
+  detail::E().turn(index_, Quat(axis, angle), true, nullptr);
+}
+inline void RigidBody::rotateAroundOrigin(real xangle, real yangle, real zangle) {
+  detail::E().turn(index_, Quat(xangle, yangle, zangle), true, nullptr);
+}
+inline void RigidBody::rotateAroundOrigin(const Vec3& euler) { rotateAroundOrigin(euler[0], euler[1], euler[2]); }
+// rotateAroundPoint (Box.cpp:806-870, Plane.cpp:811-880, Union.cpp:1027-1100)
+inline void RigidBody::rotateAroundPoint(const Vec3& point, const Vec3& axis, real angle) {
+  detail::E().turn(index_, Quat(axis, angle), false, &point);
+}
+inline void RigidBody::rotateAroundPoint(const Vec3& point, const Vec3& euler) {
+  detail::E().turn(index_, Quat(euler[0], euler[1], euler[2]), false, &point);
+}
+// pointFromBFtoWF (RigidBody.h:930-949): gpos + R rpos
+inline Vec3 RigidBody::pointFromBFtoWF(real px, real py, real pz) const { return pointFromBFtoWF(Vec3(px, py, pz)); }
+inline Vec3 RigidBody::pointFromBFtoWF(const Vec3& rpos) const {
+  const Vec3 g = getPosition();
+  real R[9];
+  getQuaternion().toRotationMatrix(R);
+  return Vec3(g[0] + (R[0] * rpos[0] + R[1] * rpos[1] + R[2] * rpos[2]),
+              g[1] + (R[3] * rpos[0] + R[4] * rpos[1] + R[5] * rpos[2]),
+              g[2] + (R[6] * rpos[0] + R[7] * rpos[1] + R[8] * rpos[2]));
 }
 // RigidBody::fix (src/core/rigidbody/RigidBody.cpp:402-418)
 inline void RigidBody::setFixed(bool fx) {

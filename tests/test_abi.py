@@ -241,3 +241,95 @@ def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
     assert cpp["after_handle_uid"].tolist() == [100, 101, 6, 2, 3, 5]
     with pytest.raises(IndexError):
         w.destroy(6)
+
+
+def test_cpp_dropin_pose_utilities_follow_pe(tmp_path, oracle_mod):
+    # pe's translate / rotate / rotateAroundOrigin / rotateAroundPoint / setPosition / setOrientation
+    # and pointFromBFtoWF in the C++ drop-in (tests/native/dropin_rotations.cpp), replayed here
+    # with the oracle's restatements of pe's quaternion product, Quaternion(axis, angle), rotate and
+    # toRotationMatrix (pinned by closed forms in test_oracle.py), step by step in pe's order
+    # (Box.cpp:456-870, Plane.cpp:216-880, Union.cpp:563-1100, Quaternion.h:309-315, 645-697):
+    # every pose, plane normal and displacement and union member position bit for bit.
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None or not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("g++ or the library not available")
+    O = oracle_mod
+    exe = str(tmp_path / "dropin_rotations")
+    lib = os.path.dirname(_abi.LIB_PATH)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "dropin_rotations.cpp"), "-o", exe, "-L", lib, "-lpehip",
+                    "-Wl,-rpath," + lib], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True, timeout=60).stdout
+    got = {l.split()[0]: [float(x) for x in l.split()[1:]] for l in out.splitlines()}
+    rpos = np.array(got["m_rpos"]).reshape(-1, 3)
+
+    Q = O.quat_from_axis_angle
+    mul = O.quat_mul
+
+    def euler(x, y, z):          # Quaternion(xangle, yangle, zangle): identity turned about x, y, z
+        return mul(Q((0, 0, 1), z), mul(Q((0, 1, 0), y), mul(Q((1, 0, 0), x), np.array([1.0, 0, 0, 0]))))
+
+    def turned_about(pos, dq, point):
+        dp = [pos[k] - point[k] for k in range(3)]
+        r = O.quat_rotate(dq, dp)
+        return np.array([point[k] + r[k] for k in range(3)])
+
+    def members(pos, q):
+        R = O.quat_to_rotation(q)
+        return np.array([[pos[k] + (R[k, 0] * r[0] + R[k, 1] * r[1] + R[k, 2] * r[2]) for k in range(3)] for r in rpos])
+
+    def check(step, pos, q, normal=(0.0, 0.0, 0.0), d=0.0, mem=None):
+        row = np.array(got[step])
+        assert np.array_equal(row[:3], pos), (step, "pos")
+        assert np.array_equal(row[3:7], q), (step, "q")
+        assert np.array_equal(row[7:10], normal), (step, "normal")
+        assert row[10] == d, (step, "d")
+        if mem is not None:
+            assert np.array_equal(row[11:].reshape(-1, 3), mem), (step, "members")
+
+    # box
+    pos, q = np.array([1.25, -0.5, 2.0]), np.array([1.0, 0, 0, 0])
+    check("box0", pos, q)
+    q = mul(Q((1, 2, 3), 0.4), q); check("box1", pos, q)
+    q = mul(Q((0, 0, 1), 0.5), mul(Q((0, 1, 0), -0.2), mul(Q((1, 0, 0), 0.3), q))); check("box2", pos, q)
+    q = mul(Q((-1, 0.5, 2), 1.1), q); check("box3", pos, q)
+    dq = Q((0, 1, 1), 0.7); pos = O.quat_rotate(dq, pos); q = mul(dq, q); check("box4", pos, q)
+    dq = euler(0.2, 0.1, -0.3); pos = O.quat_rotate(dq, pos); q = mul(dq, q); check("box5", pos, q)
+    dq = Q((0.3, 0.3, 1), 0.9); pos = turned_about(pos, dq, (1, -1, 0.5)); q = mul(dq, q); check("box6", pos, q)
+    dq = euler(0.4, -0.6, 0.2); pos = turned_about(pos, dq, (-0.5, 0.25, 1)); q = mul(dq, q); check("box7", pos, q)
+    pos = np.array([pos[0] + 0.1, pos[1] + 0.2, pos[2] + -0.3]); check("box8", pos, q)
+    R = O.quat_to_rotation(q)
+    bp = (0.25, -0.5, 0.125)
+    assert got["box_point"] == [pos[k] + (R[k, 0] * bp[0] + R[k, 1] * bp[1] + R[k, 2] * bp[2]) for k in range(3)]
+    # sphere, capsule (Euler angles turn the body's own quaternion)
+    check("sph1", np.array([-1, 0.75, 1.5]), mul(Q((1, 0, 1), 0.6), np.array([1.0, 0, 0, 0])))
+    qc = mul(Q((0, 0, 1), 0.25), mul(Q((0, 1, 0), -0.1), mul(Q((1, 0, 0), 0.7), np.array([1.0, 0, 0, 0]))))
+    check("cap1", np.array([0.5, 2, 3]), qc)
+    # plane: the normal is R's third column; its displacement n.gpos, kept by rotateAroundOrigin
+    prow = np.array(got["pl0"])
+    pos, q, d = prow[:3], prow[3:7], -0.5
+
+    def nrm(q):
+        R = O.quat_to_rotation(q)
+        return np.array([R[0, 2], R[1, 2], R[2, 2]])
+
+    def disp(n, p):
+        return n[0] * p[0] + n[1] * p[1] + n[2] * p[2]
+    check("pl0", pos, q, (0, 0, 1), d)
+    q = mul(Q((1, 0.5, 0), 0.3), q); n = nrm(q); d = disp(n, pos); check("pl1", pos, q, n, d)
+    dq = Q((0, 1, 0.2), 0.25); pos = O.quat_rotate(dq, pos); q = mul(dq, q); n = nrm(q); check("pl2", pos, q, n, d)
+    pos = np.array([0.2, 0.3, 1.5]); d = disp(n, pos); check("pl3", pos, q, n, d)
+    q = Q((1, 1, 0), 0.3); n = nrm(q); d = disp(n, pos); check("pl4", pos, q, n, d)
+    pos = np.array([pos[0] + 0.05, pos[1] + -0.1, pos[2] + 0.2]); d = disp(n, pos); check("pl5", pos, q, n, d)
+    # union: its members follow (gpos + R rpos after a turn, + dp after a translation)
+    pos, q = np.array([2.0, 2.0, 3.0]), np.array([1.0, 0, 0, 0])
+    mem = np.array(got["un0"][11:]).reshape(-1, 3)
+    check("un0", pos, q, mem=mem)
+    q = mul(euler(0.3, 0.6, -0.4), q); mem = members(pos, q); check("un1", pos, q, mem=mem)
+    dq = Q((0.5, -1, 0.3), 0.45); q = mul(dq, q); pos = O.quat_rotate(dq, pos); mem = members(pos, q)
+    check("un2", pos, q, mem=mem)
+    dp = (-0.2, 0.1, 0.35)
+    pos = np.array([pos[k] + dp[k] for k in range(3)]); mem = mem + np.array(dp); check("un3", pos, q, mem=mem)
+    dq = Q((0, 0, 1), -0.8); q = mul(dq, q); pos = turned_about(pos, dq, (1, 1, 1)); mem = members(pos, q)
+    check("un4", pos, q, mem=mem)
