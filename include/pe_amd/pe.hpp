@@ -18,10 +18,12 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <limits>
 #include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../pe_hip.h"
@@ -279,7 +281,8 @@ class Engine {
   std::vector<uint32_t> m_owner, m_mat;
   std::vector<uint64_t> m_sid;
   std::vector<real> m_rad, m_rpos, m_gpos;
-  std::mt19937_64 rng{12345};  // draws of createGranularParticle
+  std::mt19937_64 rng{12345};  // pe's global generator: rand, and createGranularParticle's draws
+  uint32_t seed = 12345;       // its seed (setSeed / getSeed)
   bool forces_pending = false;
   bool bodies_dirty = true, state_dirty = false, state_stale = false, params_dirty = true;
   uint64_t next_sid = 1;
@@ -816,9 +819,31 @@ inline UnionID createUnion(id_t uid, const Vec3& gpos, const std::vector<UnionSp
   return u;
 }
 
+// pe's random numbers (pe/util/Random.h:123-330): one global generator, seed 12345 unless
+// setSeed says otherwise; rand<T>() is uniform on [0, 1) for floating-point T and on [0, max] for
+// integers, rand<T>(min, max) on [min, max) or [min, max].  The generator is a std::mt19937_64,
+// not pe's Boost mt19937, so a seed gives another (equally uniform) sequence than pe's.
+inline uint32_t getSeed() { return detail::E().seed; }
+inline void setSeed(uint32_t seed) {
+  detail::E().seed = seed;
+  detail::E().rng.seed(seed);
+}
+template <class T>
+inline T rand() {
+  if constexpr (std::is_floating_point<T>::value) return std::uniform_real_distribution<T>(T(0), T(1))(detail::E().rng);
+  else return std::uniform_int_distribution<T>(T(0), std::numeric_limits<T>::max())(detail::E().rng);
+}
+template <class T>
+inline T rand(T min, T max) {
+  if (min > max) throw std::invalid_argument("Invalid min/max value pair");
+  if constexpr (std::is_floating_point<T>::value) return std::uniform_real_distribution<T>(min, max)(detail::E().rng);
+  else return std::uniform_int_distribution<T>(min, max)(detail::E().rng);
+}
+
 // createGranularParticle (Union.cpp:2376-2402): two to four spheres of radius U(0.6, 0.8) * radius,
 // each offset from the centre along a normalised U(-1, 1)^3 direction by radius minus its own
-// radius.  The draws come from a std::mt19937_64 (seed 12345), not pe's Boost generator.
+// radius.  The draws come from the generator of rand (seed 12345 unless setSeed), a
+// std::mt19937_64, not pe's Boost generator.
 inline UnionID createGranularParticle(id_t uid, const Vec3& gpos, real radius, MaterialID material, bool visible = true) {
   if (radius <= real(0)) throw std::invalid_argument("Invalid sphere radius");
   std::mt19937_64& g = detail::E().rng;

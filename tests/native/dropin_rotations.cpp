@@ -3,8 +3,8 @@
 // setOrientation calls to a box, a sphere, a capsule, a plane and a union of three spheres, and
 // prints each body's pose after every call, one line each (%.17g):
 //   <step> pos[3] q[4] normal[3] d member-positions[3 per member of that body]
-// followed by the members' body-frame positions and one pointFromBFtoWF result.  Nothing is
-// stepped, so no device is needed.
+// followed by the members' body-frame positions, one pointFromBFtoWF result and draws of pe's
+// random-number functions.  Nothing is stepped, so no device is needed.
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -67,5 +67,22 @@ int main() {
   std::printf("\n");
   const Vec3 w = box->pointFromBFtoWF(Vec3(0.25, -0.5, 0.125));
   std::printf("box_point %.17g %.17g %.17g\n", w[0], w[1], w[2]);
+
+  // pe's random numbers: the default seed, a seed that repeats its sequence, the ranges, and
+  // granular particles that follow the seed
+  std::printf("seed0 %u\n", getSeed());
+  for (int pass = 0; pass < 2; ++pass) {
+    setSeed(7);
+    std::printf("draws%d %u", pass, getSeed());
+    for (int k = 0; k < 4; ++k) std::printf(" %.17g", rand<real>());
+    for (int k = 0; k < 4; ++k) std::printf(" %.17g", rand<real>(-2, 3));
+    for (int k = 0; k < 4; ++k) std::printf(" %d", rand<int>(1, 6));
+    UnionID g = createGranularParticle(100 + pass, Vec3(0, 0, 10 + 5 * pass), 0.5, m);
+    std::printf(" %zu", g->size());
+    const size_t gi = g->index();
+    for (size_t k = 0; k < e.m_owner.size(); ++k)
+      if (e.m_owner[k] == gi) std::printf(" %.17g", e.m_rad[k]);
+    std::printf("\n");
+  }
   return 0;
 }

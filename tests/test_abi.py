@@ -333,3 +333,14 @@ def test_cpp_dropin_pose_utilities_follow_pe(tmp_path, oracle_mod):
     pos = np.array([pos[k] + dp[k] for k in range(3)]); mem = mem + np.array(dp); check("un3", pos, q, mem=mem)
     dq = Q((0, 0, 1), -0.8); q = mul(dq, q); pos = turned_about(pos, dq, (1, 1, 1)); mem = members(pos, q)
     check("un4", pos, q, mem=mem)
+
+    # pe's random numbers (Random.h): default seed 12345; setSeed repeats the sequence, and the
+    # granular particle drawn after it; rand<real>() in [0, 1), rand<real>(-2, 3) in [-2, 3),
+    # rand<int>(1, 6) in [1, 6]
+    assert got["seed0"] == [12345]
+    a, b = got["draws0"], got["draws1"]
+    assert a == b and a[0] == 7
+    assert all(0 <= x < 1 for x in a[1:5]) and all(-2 <= x < 3 for x in a[5:9])
+    assert all(x in (1, 2, 3, 4, 5, 6) for x in a[9:13])
+    n = int(a[13])
+    assert 2 <= n <= 4 and len(a) == 14 + n and all(0.3 <= r <= 0.4 for r in a[14:])
