@@ -15,6 +15,7 @@
 #define PE_AMD_PE_HPP
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -840,6 +841,48 @@ inline T rand(T min, T max) {
   else return std::uniform_int_distribution<T>(min, max)(detail::E().rng);
 }
 
+namespace timing {
+// pe's wall-clock timer, timing::WcTimer = Timer<WcPolicy> (pe/util/timing/Timer.h:104-330,
+// WcPolicy.h:78-81): start/end measure one interval in seconds; lap ends one and starts the next;
+// total, average, min, max and last summarise the intervals so far.
+class WcTimer {
+ public:
+  WcTimer() { reset(); }
+  void start() { start_ = now(); }
+  void end() {
+    end_ = now();
+    ++counter_;
+    const double diff = end_ - start_;
+    time_ += diff;
+    if (diff < min_) min_ = diff;
+    if (diff > max_) max_ = diff;
+    last_ = diff;
+  }
+  void lap() {
+    end();
+    start_ = end_;
+  }
+  void reset() {
+    counter_ = 0;
+    start_ = end_ = time_ = last_ = max_ = 0.0;
+    min_ = std::numeric_limits<double>::max();
+  }
+  size_t getCounter() const { return counter_; }
+  double total() const { return time_; }
+  double average() const { return time_ / double(counter_); }
+  double min() const { return min_; }
+  double max() const { return max_; }
+  double last() const { return last_; }
+
+ private:
+  static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  }
+  size_t counter_;
+  double start_, end_, time_, min_, max_, last_;
+};
+}  // namespace timing
+
 // createGranularParticle (Union.cpp:2376-2402): two to four spheres of radius U(0.6, 0.8) * radius,
 // each offset from the centre along a normalised U(-1, 1)^3 direction by radius minus its own
 // radius.  The draws come from the generator of rand (seed 12345 unless setSeed), a
@@ -880,8 +923,64 @@ inline void destroy(BodyID body) {
 }
 
 // ---------------------------------------------------------------- World / CollisionSystem
+namespace detail {
+// Whether body i is of handle type T (RigidBody: every body).
+template <class T>
+inline bool body_is(size_t i) {
+  if constexpr (std::is_same<T, RigidBody>::value) return true;
+  else {
+    const uint8_t t = E().shape[i];
+    if constexpr (std::is_same<T, Sphere>::value) return t == sphereType;
+    else if constexpr (std::is_same<T, Box>::value) return t == boxType;
+    else if constexpr (std::is_same<T, Capsule>::value) return t == capsuleType;
+    else if constexpr (std::is_same<T, Plane>::value) return t == planeType;
+    else return t == unionType;
+  }
+}
+// World::Iterator and World::Bodies::CastIterator<T> (pe/util/PtrVector.h:1990-2230): the bodies
+// in body order, * and -> give the handle; a cast iterator skips the bodies of other types.
+template <class T>
+class BodyIterator {
+ public:
+  explicit BodyIterator(size_t i = 0) : i_(i) { skip(); }
+  T* operator*() const { return static_cast<T*>(E().handles[i_].get()); }
+  T* operator->() const { return **this; }
+  BodyIterator& operator++() {
+    ++i_;
+    skip();
+    return *this;
+  }
+  BodyIterator operator++(int) {
+    BodyIterator old = *this;
+    ++*this;
+    return old;
+  }
+  bool operator==(const BodyIterator& o) const { return i_ == o.i_; }
+  bool operator!=(const BodyIterator& o) const { return i_ != o.i_; }
+
+ private:
+  void skip() {
+    const size_t n = E().handles.size();
+    while (i_ < n && !body_is<T>(i_)) ++i_;
+    if (i_ > n) i_ = n;
+  }
+  size_t i_;
+};
+}  // namespace detail
+
 class World {
  public:
+  typedef detail::BodyIterator<RigidBody> Iterator;
+  struct Bodies {
+    template <class T>
+    using CastIterator = detail::BodyIterator<T>;
+  };
+  Iterator begin() { return Iterator(0); }
+  Iterator end() { return Iterator(size()); }
+  template <class T>
+  detail::BodyIterator<T> begin() { return detail::BodyIterator<T>(0); }
+  template <class T>
+  detail::BodyIterator<T> end() { return detail::BodyIterator<T>(size()); }
   void setGravity(real gx, real gy, real gz) {
     detail::Engine& e = detail::E();
     e.params.gravity[0] = gx; e.params.gravity[1] = gy; e.params.gravity[2] = gz;

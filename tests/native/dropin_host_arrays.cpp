@@ -31,6 +31,25 @@ int main() {
   createUnion(5, Vec3(-2, -1, 1.5), {UnionSphere{Vec3(0.1, 0.1, 0.1), 0.4, wood}});
   createSphere(6, Vec3(0, 0, 7), 0.5, wood);
   b->setFixed(true);
+  // World::Iterator walks every body in body order; a cast iterator only the bodies of its type
+  std::vector<double> all, boxes, spheres, capsules, planes, unions, union_sizes;
+  for (World::Iterator it = theWorld()->begin(); it != theWorld()->end(); ++it) all.push_back((double)it->getID());
+  for (World::Bodies::CastIterator<Box> it = theWorld()->begin<Box>(); it != theWorld()->end<Box>(); ++it)
+    boxes.push_back((double)(*it)->getID());
+  for (auto it = theWorld()->begin<Sphere>(); it != theWorld()->end<Sphere>(); it++) spheres.push_back((double)it->getID());
+  for (auto it = theWorld()->begin<Capsule>(); it != theWorld()->end<Capsule>(); ++it) capsules.push_back((double)it->getID());
+  for (auto it = theWorld()->begin<Plane>(); it != theWorld()->end<Plane>(); ++it) planes.push_back((double)it->getID());
+  for (auto it = theWorld()->begin<Union>(); it != theWorld()->end<Union>(); ++it) {
+    unions.push_back((double)it->getID());
+    union_sizes.push_back((double)it->size());
+  }
+  put("iter_all", all);
+  put("iter_box", boxes);
+  put("iter_sphere", spheres);
+  put("iter_capsule", capsules);
+  put("iter_plane", planes);
+  put("iter_union", unions);
+  put("iter_union_size", union_sizes);
   const detail::Engine& e = detail::E();
   put("shape", e.shape);
   put("sid", e.sid);

@@ -84,5 +84,15 @@ int main() {
       if (e.m_owner[k] == gi) std::printf(" %.17g", e.m_rad[k]);
     std::printf("\n");
   }
+  // pe's wall-clock timer: three intervals, then their summary
+  timing::WcTimer t;
+  for (int k = 0; k < 3; ++k) {
+    t.start();
+    volatile double x = 0;
+    for (int j = 0; j < 200000 * (k + 1); ++j) x = x + 1e-9 * j;
+    t.end();
+  }
+  std::printf("timer %zu %.17g %.17g %.17g %.17g %.17g\n", t.getCounter(), t.total(), t.average(), t.min(), t.max(),
+              t.last());
   return 0;
 }

@@ -194,6 +194,11 @@ def test_cpp_dropin_host_arrays_equal_python_world(tmp_path):
     for k in py:
         assert np.array_equal(cpp[k], np.asarray(py[k], float).ravel()), k
     assert int(cpp["m_owner"][-1]) == 6 and len(cpp["m_sid"]) == 4   # the unions' members follow them
+    # World::Iterator and World::Bodies::CastIterator<T>: body order, one type at a time
+    assert cpp["iter_all"].tolist() == [100, 101, 1, 2, 3, 4, 5, 6]
+    assert cpp["iter_box"].tolist() == [2] and cpp["iter_sphere"].tolist() == [1, 6]
+    assert cpp["iter_capsule"].tolist() == [3] and cpp["iter_plane"].tolist() == [100, 101]
+    assert cpp["iter_union"].tolist() == [4, 5] and cpp["iter_union_size"].tolist() == [3, 1]
 
     # Union::setPosition translates the members by the union's displacement; setOrientation puts
     # each at pos + R(q) rpos (Union.cpp:251-272, 306-322).  Checked against a rigid motion built
@@ -344,3 +349,7 @@ def test_cpp_dropin_pose_utilities_follow_pe(tmp_path, oracle_mod):
     assert all(x in (1, 2, 3, 4, 5, 6) for x in a[9:13])
     n = int(a[13])
     assert 2 <= n <= 4 and len(a) == 14 + n and all(0.3 <= r <= 0.4 for r in a[14:])
+    # timing::WcTimer: count, total = the sum, average, min <= last <= max, all positive
+    cnt, total, avg, tmin, tmax, last = got["timer"]
+    assert cnt == 3 and 0 < tmin <= last <= tmax and tmin <= avg <= tmax
+    assert abs(total - 3 * avg) <= 1e-12 * total
