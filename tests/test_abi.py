@@ -353,3 +353,27 @@ def test_cpp_dropin_pose_utilities_follow_pe(tmp_path, oracle_mod):
     cnt, total, avg, tmin, tmax, last = got["timer"]
     assert cnt == 3 and 0 < tmin <= last <= tmax and tmin <= avg <= tmax
     assert abs(total - 3 * avg) <= 1e-12 * total
+
+
+def test_cpp_dropin_host_code_clean_under_sanitizers(tmp_path):
+    # The drop-in's host code (body factories, union setters, destroy, iterators, pose utilities,
+    # random numbers) built with AddressSanitizer and UndefinedBehaviorSanitizer by g++ (no
+    # recovery, leak detection on): both test programs run without a report and print what the
+    # plain builds print (the wall-clock timer line aside).
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None or not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("g++ or the library not available")
+    lib = os.path.dirname(_abi.LIB_PATH)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:exitcode=99", UBSAN_OPTIONS="print_stacktrace=1")
+    for name in ("dropin_host_arrays", "dropin_rotations"):
+        src = os.path.join(ROOT, "tests", "native", name + ".cpp")
+        outs = []
+        for flags in ([], ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]):
+            exe = str(tmp_path / (name + ("_san" if flags else "")))
+            subprocess.run(["g++", "-O2", "-std=c++17", *flags, "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+                            "-L", lib, "-lpehip", "-Wl,-rpath," + lib], check=True)
+            r = subprocess.run([exe], capture_output=True, text=True, timeout=120, env=env)
+            assert r.returncode == 0 and r.stderr == "", (name, flags, r.stderr[-2000:])
+            outs.append([l for l in r.stdout.splitlines() if not l.startswith("timer")])
+        assert outs[0] == outs[1], name
