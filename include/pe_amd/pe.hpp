@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <cstring>
 #include <limits>
 #include <memory>
 #include <random>
@@ -280,13 +282,14 @@ class Engine {
   std::vector<Quat> vq;
   // union members (spheres), grouped by union in creation order
   std::vector<uint32_t> m_owner, m_mat;
-  std::vector<uint64_t> m_sid;
+  std::vector<uint64_t> m_sid, m_uid;
   std::vector<real> m_rad, m_rpos, m_gpos;
   std::mt19937_64 rng{12345};  // pe's global generator: rand, and createGranularParticle's draws
   uint32_t seed = 12345;       // its seed (setSeed / getSeed)
   bool forces_pending = false;
   bool bodies_dirty = true, state_dirty = false, state_stale = false, params_dirty = true;
   uint64_t next_sid = 1;
+  static constexpr id_t kNoSid = ~id_t(0);  // add(): take the next system ID
 
   pehip_ctx* ctx() {
     if (!ctx_) {
@@ -327,12 +330,12 @@ class Engine {
 
   template <class T>
   T* add(int type, id_t user_id, MaterialID m, bool fx, const real* prm, const Vec3& gpos, const Quat& qq,
-         real massv, real im, const real* Ib, const Vec3& nrm, real dd) {
+         real massv, real im, const real* Ib, const Vec3& nrm, real dd, id_t given_sid = kNoSid) {
     sync_host();
     const size_t i = shape.size();
     shape.push_back((uint8_t)type);
     fixed.push_back(fx ? 1 : 0);
-    sid.push_back(next_sid++);
+    sid.push_back(given_sid == kNoSid ? next_sid++ : given_sid);
     uid.push_back(user_id);
     mat.push_back((uint32_t)m);
     for (int k = 0; k < 3; ++k) { param.push_back(prm[k]); pos.push_back(gpos[k]); v.push_back(0); w.push_back(0);
@@ -442,16 +445,17 @@ class Engine {
         if (m_owner[k] != i) keep.push_back(k);
       std::stable_sort(keep.begin(), keep.end(), [&](size_t a, size_t b) { return owner_of(a) < owner_of(b); });
       std::vector<uint32_t> o, mm;
-      std::vector<uint64_t> ms;
+      std::vector<uint64_t> ms, mu;
       std::vector<real> mr, mrp, mg;
       for (size_t k : keep) {
         o.push_back(owner_of(k));
         mm.push_back(m_mat[k]);
         ms.push_back(m_sid[k]);
+        mu.push_back(m_uid[k]);
         mr.push_back(m_rad[k]);
         for (int j = 0; j < 3; ++j) { mrp.push_back(m_rpos[3 * k + j]); mg.push_back(m_gpos[3 * k + j]); }
       }
-      m_owner.swap(o); m_mat.swap(mm); m_sid.swap(ms); m_rad.swap(mr); m_rpos.swap(mrp); m_gpos.swap(mg);
+      m_owner.swap(o); m_mat.swap(mm); m_sid.swap(ms); m_uid.swap(mu); m_rad.swap(mr); m_rpos.swap(mrp); m_gpos.swap(mg);
     }
     bodies_dirty = true;
   }
@@ -810,6 +814,7 @@ inline UnionID createUnion(id_t uid, const Vec3& gpos, const std::vector<UnionSp
   for (size_t k = 0; k < n; ++k) {
     e.m_owner.push_back((uint32_t)(e.shape.size() - 1));
     e.m_sid.push_back(e.next_sid++);
+    e.m_uid.push_back(uid);  // a member's user ID is its union's
     e.m_mat.push_back((uint32_t)spheres[k].material);
     e.m_rad.push_back(spheres[k].radius);
     for (int j = 0; j < 3; ++j) {
@@ -923,6 +928,295 @@ inline void destroy(BodyID body) {
 }
 
 // ---------------------------------------------------------------- World / CollisionSystem
+// ---------------------------------------------------------------- pe's binary body files
+// BodyBinaryWriter / BodyBinaryReader (src/core/BodyBinaryWriter.cpp:46-175,
+// BodyBinaryReader.cpp:60-330, pe/core/Marshalling.h:55-570), for one process.  Every value is
+// big-endian.  The header is 'P' 'E' 0 1, the sizes of real, int, size_t, id_t and bool, the
+// process count and the chunk offsets.  The global chunk holds the planes, and the local chunk
+// the spheres, boxes, capsules and unions of spheres, each after its ID counter.  A body record
+// is sid, uid, visible, fixed, position, quaternion, linear and angular velocity, material and
+// the shape's parameters.  A union record has no material; it carries its total mass, body
+// inertia, bounding box and member spheres.
+namespace detail {
+enum : uint8_t { kFileSphere = 1, kFileBox = 2, kFileCapsule = 3, kFileCylinder = 4, kFilePlane = 5, kFileMesh = 6,
+                 kFileUnion = 7 };  // pe/core/GeomType.h:45-53
+struct FileOut {
+  std::vector<uint8_t> b;
+  bool single = false;
+  void u8(uint8_t x) { b.push_back(x); }
+  void u32(uint32_t x) { for (int k = 3; k >= 0; --k) b.push_back(uint8_t(x >> (8 * k))); }
+  void u64(uint64_t x) { for (int k = 7; k >= 0; --k) b.push_back(uint8_t(x >> (8 * k))); }
+  void r(double x) {
+    if (single) {
+      const float f = static_cast<float>(x);
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      u32(u);
+    } else {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      u64(u);
+    }
+  }
+  void rs(const real* x, size_t n) { for (size_t k = 0; k < n; ++k) r(x[k]); }
+};
+struct FileIn {
+  const std::vector<uint8_t>& d;
+  size_t pos, end;
+  bool single = false;
+  FileIn(const std::vector<uint8_t>& data, size_t lo, size_t hi) : d(data), pos(lo), end(hi) {}
+  void need(size_t n) const { if (pos + n > end) throw std::runtime_error("truncated body file"); }
+  uint64_t be(int bytes) {
+    need((size_t)bytes);
+    uint64_t x = 0;
+    for (int k = 0; k < bytes; ++k) x = (x << 8) | d[pos++];
+    return x;
+  }
+  uint8_t u8() { return (uint8_t)be(1); }
+  uint32_t u32() { return (uint32_t)be(4); }
+  uint64_t u64() { return be(8); }
+  double r() {
+    if (single) {
+      const uint32_t u = u32();
+      float f;
+      std::memcpy(&f, &u, 4);
+      return f;
+    }
+    const uint64_t u = u64();
+    double x;
+    std::memcpy(&x, &u, 8);
+    return x;
+  }
+  void rs(real* x, size_t n) { for (size_t k = 0; k < n; ++k) x[k] = r(); }
+};
+inline size_t file_params(uint8_t t) { return t == kFileSphere ? 1 : t == kFileBox ? 3 : t == kFileCapsule ? 2 : 0; }
+inline uint8_t file_type(uint8_t shape) {
+  return shape == sphereType ? kFileSphere : shape == boxType ? kFileBox : shape == capsuleType ? kFileCapsule
+         : shape == planeType ? kFilePlane : kFileUnion;
+}
+}  // namespace detail
+
+class BodyBinaryWriter {
+ public:
+  // setFloatingPointSize (BodyBinaryWriter.h:89): 0 = real (double), 2 = single, 3 = double
+  void setFloatingPointSize(int fpSize) {
+    if (fpSize != 0 && fpSize != 2 && fpSize != 3) throw std::invalid_argument("Invalid floating point conversion size.");
+    fp_ = fpSize;
+  }
+  void writeFile(const char* filename) {
+    detail::Engine& e = detail::E();
+    e.sync_host();
+    const size_t n = e.shape.size();
+    auto bodies = [&](detail::FileOut& o, uint8_t t) {
+      std::vector<size_t> idx;
+      for (size_t i = 0; i < n; ++i)
+        if (detail::file_type(e.shape[i]) == t) idx.push_back(i);
+      o.u8(t);
+      o.u32((uint32_t)idx.size());
+      for (size_t i : idx) {
+        o.u64(e.sid[i]); o.u64(e.uid[i]); o.u8(1); o.u8(e.fixed[i]);
+        o.rs(&e.pos[3 * i], 3); o.rs(&e.q[4 * i], 4); o.rs(&e.v[3 * i], 3); o.rs(&e.w[3 * i], 3);
+        if (t == detail::kFileUnion) {
+          // Marshalling.h:523-570: total mass, body inertia, the merged sphere boxes, members
+          o.r(e.mass[i]);
+          o.rs(&e.I[9 * i], 9);
+          real lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+          uint64_t count = 0;
+          for (size_t m = 0; m < e.m_owner.size(); ++m)
+            if (e.m_owner[m] == i) {
+              ++count;
+              const real len = e.m_rad[m] + real(1e-8);  // SphereBase.h:208-218, contact threshold
+              for (int j = 0; j < 3; ++j) {
+                lo[j] = std::min(lo[j], e.m_gpos[3 * m + j] - len);
+                hi[j] = std::max(hi[j], e.m_gpos[3 * m + j] + len);
+              }
+            }
+          o.rs(lo, 3);
+          o.rs(hi, 3);
+          o.u64(count);
+          for (size_t m = 0; m < e.m_owner.size(); ++m)
+            if (e.m_owner[m] == i) {
+              o.u8(detail::kFileSphere);
+              o.u64(e.m_sid[m]); o.u64(e.m_uid[m]); o.u8(1); o.u8(e.fixed[i]);
+              o.rs(&e.m_gpos[3 * m], 3); o.rs(&e.m_rpos[3 * m], 3); o.rs(&e.q[4 * i], 4);
+              o.u64(e.m_mat[m]);
+              o.r(e.m_rad[m]);
+            }
+        } else {
+          o.u64(e.mat[i]);
+          o.rs(&e.param[3 * i], detail::file_params(t));
+        }
+      }
+    };
+    detail::FileOut glob, loc, head;
+    glob.single = loc.single = head.single = fp_ == 2;
+    size_t nplanes = 0;
+    for (size_t i = 0; i < n; ++i) nplanes += e.shape[i] == planeType;
+    glob.u64(nplanes);
+    for (uint8_t t : {detail::kFileSphere, detail::kFileBox, detail::kFileCapsule, detail::kFileCylinder,
+                      detail::kFilePlane, detail::kFileMesh, detail::kFileUnion}) {
+      if (t == detail::kFilePlane) bodies(glob, t);
+      else { glob.u8(t); glob.u32(0); }
+    }
+    loc.u64(e.next_sid);
+    for (uint8_t t : {detail::kFileSphere, detail::kFileBox, detail::kFileCapsule, detail::kFileCylinder,
+                      detail::kFileMesh, detail::kFileUnion})
+      bodies(loc, t);
+    for (uint8_t c : {uint8_t('P'), uint8_t('E'), uint8_t(0), uint8_t(1)}) head.u8(c);
+    head.u8(fp_ == 0 ? 8 : uint8_t(1u << fp_)); head.u8(4); head.u8(8); head.u8(8); head.u8(1);
+    head.u32(1);
+    const uint32_t head_size = (uint32_t)head.b.size() + 3 * 4;
+    head.u32(head_size);
+    head.u32(head_size + (uint32_t)glob.b.size());
+    head.u32(head_size + (uint32_t)(glob.b.size() + loc.b.size()));
+    std::FILE* f = std::fopen(filename, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot open ") + filename);
+    bool ok = std::fwrite(head.b.data(), 1, head.b.size(), f) == head.b.size();
+    ok = ok && std::fwrite(glob.b.data(), 1, glob.b.size(), f) == glob.b.size();
+    ok = ok && std::fwrite(loc.b.data(), 1, loc.b.size(), f) == loc.b.size();
+    if (std::fclose(f) != 0 || !ok) throw std::runtime_error(std::string("cannot write ") + filename);
+  }
+
+ private:
+  int fp_ = 0;
+};
+
+class BodyBinaryReader {
+ public:
+  // readFile (BodyBinaryReader.cpp:60-330) into an empty world whose materials exist: the local
+  // chunks' bodies first, then the global chunk's, with the file's system IDs, poses and
+  // velocities; masses from the shapes and materials, a union's from the file (Union.cpp:100-133).
+  void readFile(const char* filename) {
+    detail::Engine& e = detail::E();
+    if (!e.shape.empty()) throw std::invalid_argument("bulk creation only into an empty world");
+    std::vector<uint8_t> data;
+    {
+      std::FILE* f = std::fopen(filename, "rb");
+      if (!f) throw std::runtime_error(std::string("cannot open ") + filename);
+      uint8_t buf[65536];
+      size_t got;
+      while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) data.insert(data.end(), buf, buf + got);
+      std::fclose(f);
+    }
+    detail::FileIn h(data, 0, data.size());
+    if (h.u8() != 'P' || h.u8() != 'E') throw std::runtime_error("Invalid file format.");
+    if (h.u8() != 0 || h.u8() != 1) throw std::runtime_error("Unsupported file format version.");
+    const uint8_t rsize = h.u8();
+    if (rsize != 4 && rsize != 8) throw std::runtime_error("Invalid size of floating point data.");
+    if (h.u8() != 4 || h.u8() != 8 || h.u8() != 8 || h.u8() != 1)
+      throw std::runtime_error("Size of the int, size_t, id_t or bool data type does not match the file.");
+    const uint32_t p = h.u32(), off_global = h.u32();
+    std::vector<uint32_t> offs(p + 1);
+    for (uint32_t& o : offs) o = h.u32();
+    bool bad = offs[0] < off_global || offs[p] > data.size();
+    for (uint32_t k = 0; k < p; ++k) bad = bad || offs[k + 1] < offs[k];
+    if (bad) throw std::runtime_error("Invalid body data offset.");
+    uint64_t counter = 0, top = 0;
+    bool any = false;
+    auto chunk = [&](size_t lo, size_t hi) {
+      detail::FileIn r(data, lo, hi);
+      r.single = rsize == 4;
+      const uint64_t c = r.u64();
+      while (r.pos < hi) {
+        const uint8_t t = r.u8();
+        const uint32_t cnt = r.u32();
+        if (cnt && t != detail::kFileUnion && t != detail::kFilePlane && !detail::file_params(t))
+          throw std::runtime_error("Unsupported geometry type " + std::to_string(t) + " in body file");
+        for (uint32_t k = 0; k < cnt; ++k) {
+          const id_t sid = r.u64(), uid = r.u64();
+          (void)r.u8();
+          const bool fx = r.u8() != 0;
+          real pos[3], q[4], v[3], w[3];
+          r.rs(pos, 3); r.rs(q, 4); r.rs(v, 3); r.rs(w, 3);
+          top = any ? std::max<uint64_t>(top, sid) : sid;
+          any = true;
+          if (t == detail::kFileUnion) {
+            read_union(e, r, sid, uid, fx, pos, q, v, w, top);
+            continue;
+          }
+          const uint64_t mat = r.u64();
+          real prm[3] = {0, 0, 0};
+          r.rs(prm, detail::file_params(t));
+          if (mat >= e.materials.size())
+            throw std::invalid_argument("body " + std::to_string(sid) + " uses material " + std::to_string(mat) +
+                                        ", which the world does not have");
+          BodyID b;
+          const Vec3 g(pos[0], pos[1], pos[2]);
+          if (t == detail::kFileSphere) b = createSphere(uid, g, prm[0], mat);
+          else if (t == detail::kFileBox) b = createBox(uid, g, Vec3(prm[0], prm[1], prm[2]), mat);
+          else if (t == detail::kFileCapsule) b = createCapsule(uid, g, prm[0], prm[1], mat);
+          else b = createPlane(uid, Vec3(0, 0, 1), 0, mat);
+          const size_t i = b->index();
+          e.sid[i] = sid;
+          for (int j = 0; j < 4; ++j) e.q[4 * i + j] = q[j];
+          for (int j = 0; j < 3; ++j) { e.pos[3 * i + j] = pos[j]; e.v[3 * i + j] = v[j]; e.w[3 * i + j] = w[j]; }
+          if (t == detail::kFilePlane) {
+            // Plane.cpp:121-142: the normal is the orientation applied to +z, d = n . gpos
+            const Vec3 nrm = e.quat(i).rotate(Vec3(0, 0, 1));
+            for (int j = 0; j < 3; ++j) e.normal[3 * i + j] = nrm[j];
+            e.plane_displacement(i);
+          } else if (fx) {
+            b->setFixed(true);
+          }
+        }
+      }
+      return c;
+    };
+    for (uint32_t k = 0; k < p; ++k) counter = std::max<uint64_t>(counter, chunk(offs[k], offs[k + 1]));
+    chunk(off_global, offs[0]);
+    e.next_sid = std::max<uint64_t>(counter, any ? top + 1 : 0);
+    e.bodies_dirty = true;
+  }
+
+ private:
+  // Marshalling.h:560-620 with spheres for members; the union keeps the file's mass, inertia
+  // and orientation, and its members their positions
+  static void read_union(detail::Engine& e, detail::FileIn& r, id_t sid, id_t uid, bool fx, const real* pos,
+                         const real* q, const real* v, const real* w, uint64_t& top) {
+    if (fx) throw std::invalid_argument("union " + std::to_string(sid) + " is fixed: fixed unions are not supported");
+    const real mass = r.r();
+    real I[9], aabb[6];
+    r.rs(I, 9);
+    r.rs(aabb, 6);
+    const uint64_t size = r.u64();
+    if (!size) throw std::invalid_argument("union " + std::to_string(sid) + " has no members");
+    struct Member { id_t sid, uid; real gpos[3], rpos[3]; uint64_t mat; real rad; };
+    std::vector<Member> ms(size);
+    real bound = 0, rmin = 0;
+    for (uint64_t k = 0; k < size; ++k) {
+      const uint8_t mt = r.u8();
+      if (mt != detail::kFileSphere)
+        throw std::invalid_argument("Unsupported geometry type " + std::to_string(mt) + " in a union");
+      Member& m = ms[k];
+      m.sid = r.u64(); m.uid = r.u64();
+      (void)r.u8(); (void)r.u8();
+      real mq[4];
+      r.rs(m.gpos, 3); r.rs(m.rpos, 3); r.rs(mq, 4);
+      m.mat = r.u64();
+      m.rad = r.r();
+      if (m.mat >= e.materials.size())
+        throw std::invalid_argument("body " + std::to_string(m.sid) + " uses material " + std::to_string(m.mat) +
+                                    ", which the world does not have");
+      top = std::max<uint64_t>(top, m.sid);
+      bound = std::max(bound, std::sqrt(m.rpos[0] * m.rpos[0] + m.rpos[1] * m.rpos[1] + m.rpos[2] * m.rpos[2]) + m.rad);
+      rmin = k ? std::min(rmin, m.rad) : m.rad;
+    }
+    const real prm[3] = {bound * (real(1) + real(1e-12)), rmin, 0};
+    Union* u = e.add<Union>(unionType, uid, ms[0].mat, false, prm, Vec3(pos[0], pos[1], pos[2]),
+                            Quat(q[0], q[1], q[2], q[3]), mass, real(1) / mass, I, Vec3(), 0, sid);
+    const size_t i = u->index();
+    for (int j = 0; j < 3; ++j) { e.v[3 * i + j] = v[j]; e.w[3 * i + j] = w[j]; }
+    for (const Member& m : ms) {
+      e.m_owner.push_back((uint32_t)i);
+      e.m_sid.push_back(m.sid);
+      e.m_uid.push_back(m.uid);
+      e.m_mat.push_back((uint32_t)m.mat);
+      e.m_rad.push_back(m.rad);
+      for (int j = 0; j < 3; ++j) { e.m_rpos.push_back(m.rpos[j]); e.m_gpos.push_back(m.gpos[j]); }
+    }
+  }
+};
+
 namespace detail {
 // Whether body i is of handle type T (RigidBody: every body).
 template <class T>

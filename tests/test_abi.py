@@ -357,7 +357,7 @@ def test_cpp_dropin_pose_utilities_follow_pe(tmp_path, oracle_mod):
 
 def test_cpp_dropin_host_code_clean_under_sanitizers(tmp_path):
     # The drop-in's host code (body factories, union setters, destroy, iterators, pose utilities,
-    # random numbers) built with AddressSanitizer and UndefinedBehaviorSanitizer by g++ (no
+    # random numbers, body files) built with AddressSanitizer and UndefinedBehaviorSanitizer by g++ (no
     # recovery, leak detection on): both test programs run without a report and print what the
     # plain builds print (the wall-clock timer line aside).
     import shutil
@@ -366,14 +366,83 @@ def test_cpp_dropin_host_code_clean_under_sanitizers(tmp_path):
         pytest.skip("g++ or the library not available")
     lib = os.path.dirname(_abi.LIB_PATH)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:exitcode=99", UBSAN_OPTIONS="print_stacktrace=1")
-    for name in ("dropin_host_arrays", "dropin_rotations"):
+    for name in ("dropin_host_arrays", "dropin_rotations", "dropin_body_file"):
         src = os.path.join(ROOT, "tests", "native", name + ".cpp")
         outs = []
         for flags in ([], ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]):
             exe = str(tmp_path / (name + ("_san" if flags else "")))
             subprocess.run(["g++", "-O2", "-std=c++17", *flags, "-I", os.path.join(ROOT, "include"), src, "-o", exe,
                             "-L", lib, "-lpehip", "-Wl,-rpath," + lib], check=True)
-            r = subprocess.run([exe], capture_output=True, text=True, timeout=120, env=env)
-            assert r.returncode == 0 and r.stderr == "", (name, flags, r.stderr[-2000:])
-            outs.append([l for l in r.stdout.splitlines() if not l.startswith("timer")])
-        assert outs[0] == outs[1], name
+            runs = [[exe]]
+            if name == "dropin_body_file":   # write a file, then read it back
+                f = str(tmp_path / ("bf_san.pe" if flags else "bf.pe"))
+                runs = [[exe, "write", f, "0"], [exe, "read", f]]
+            lines = []
+            for cmd in runs:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, env=env)
+                assert r.returncode == 0 and r.stderr == "", (name, flags, r.stderr[-2000:])
+                lines += [l for l in r.stdout.splitlines() if not l.startswith("timer")]
+            outs.append(lines)
+        assert outs[0] == outs[1] and outs[0], name
+
+
+def _mixed_world(World, quat_axis_angle):
+    # the scene of tests/native/dropin_body_file.cpp (write mode), built with the Python World
+    w = World()
+    gran = w.createMaterial("granular", 2500, 0, 0.2, 0.2)
+    wood = w.createMaterial("wood", 700, 0.1, 0.35, 0.3)
+    w.createPlane(100, (0, 0, 1), 0, gran)
+    w.createPlane(101, (1, -2, 0.5), -3.25, wood)
+    s1 = w.createSphere(1, (1.5, -0.25, 2), 0.375, gran)
+    box = w.createBox(2, (-1, 2, 0.75), (0.5, 1.25, 0.8), wood)
+    w.createCapsule(3, (0.3, 0.1, 4), 0.2, 1.1, gran)
+    u = w.createUnion(4, (2, 2, 3), [((0.3, 0, 0), 0.25, gran), ((-0.1, 0.2, 0.05), 0.3, wood),
+                                     ((0, -0.15, 0.2), 0.2, gran)])
+    w.createUnion(5, (-2, -1, 1.5), [((0.1, 0.1, 0.1), 0.4, wood)])
+    w.createSphere(6, (0, 0, 7), 0.5, wood)
+    w.setFixed(box)
+    a = w.host_arrays()
+    a["v"][s1] = (0.5, -0.25, 1.0)
+    a["w"][s1] = (2, 0, -1)
+    a["q"][s1] = quat_axis_angle((1, 2, 3), 0.4)
+    a["v"][u] = (0, 0, -2)
+    return w
+
+
+def test_cpp_dropin_body_files_equal_python_ones(tmp_path):
+    # BodyBinaryWriter / BodyBinaryReader in the C++ drop-in against pe_amd.checkpoint (pe's format,
+    # pinned by tests/test_checkpoint.py): the same scene written by both is the same bytes, in
+    # double and in single precision, and a file read by both gives the same body and member
+    # arrays, system IDs and next system ID (pe's reader order: local chunk, then global).
+    import shutil
+    import subprocess
+    from pe_amd import checkpoint
+    from pe_amd.world import quat_axis_angle
+    if shutil.which("g++") is None or not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("g++ or the library not available")
+    exe = str(tmp_path / "dropin_body_file")
+    lib = os.path.dirname(_abi.LIB_PATH)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "dropin_body_file.cpp"), "-o", exe, "-L", lib, "-lpehip",
+                    "-Wl,-rpath," + lib], check=True)
+    for fp in (0, 2):
+        fc, fpy = str(tmp_path / ("c%d.pe" % fp)), str(tmp_path / ("py%d.pe" % fp))
+        subprocess.run([exe, "write", fc, str(fp)], check=True, timeout=60)
+        checkpoint.write_body_file(_mixed_world(World, quat_axis_angle), fpy, fp_size=fp)
+        assert open(fc, "rb").read() == open(fpy, "rb").read(), fp
+
+        out = subprocess.run([exe, "read", fpy], capture_output=True, text=True, check=True, timeout=60).stdout
+        cpp = {l.split()[0]: np.array([float(x) for x in l.split()[1:]]) for l in out.splitlines()}
+        w = World()
+        w.createMaterial("granular", 2500, 0, 0.2, 0.2)
+        w.createMaterial("wood", 700, 0.1, 0.35, 0.3)
+        checkpoint.load_body_file(w, fpy)
+        a, m = w.host_arrays(), w.members()
+        for k in ("shape", "sid", "uid", "mat", "fixed", "param", "pos", "q", "v", "w", "inv_mass", "I", "Iinv",
+                  "normal", "d"):
+            assert np.array_equal(cpp[k], np.asarray(a[k], float).ravel()), (fp, k)
+        for k, mk in (("m_owner", "owner"), ("m_sid", "sid"), ("m_uid", "uid"), ("m_mat", "mat"),
+                      ("m_rad", "radius"), ("m_rpos", "rpos"), ("m_gpos", "gpos")):
+            assert np.array_equal(cpp[k], np.asarray(m[mk], float).ravel()), (fp, k)
+        assert int(cpp["next_sid"][0]) == w._next_sid
+        assert cpp["sid"].tolist() == [3, 12, 4, 5, 6, 10, 1, 2]   # local chunk by type, then the planes
