@@ -55,6 +55,34 @@ struct Vec3 {
   }
 };
 
+// A 3x3 matrix, row-major (pe/math/Matrix3x3.h); Rot3 is pe's rotation-matrix type.
+struct Mat3 {
+  real v[9];
+  Mat3() : v{0, 0, 0, 0, 0, 0, 0, 0, 0} {}
+  real& operator[](size_t i) { return v[i]; }
+  real operator[](size_t i) const { return v[i]; }
+  real operator()(size_t i, size_t j) const { return v[3 * i + j]; }
+  // Matrix3x3 products (Matrix3x3.h:2172-2178, 2241-2253)
+  Mat3 operator*(const Mat3& r) const {
+    Mat3 m;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) m.v[3 * i + j] = v[3 * i] * r.v[j] + v[3 * i + 1] * r.v[3 + j] + v[3 * i + 2] * r.v[6 + j];
+    return m;
+  }
+  Vec3 operator*(const Vec3& x) const {
+    return Vec3(v[0] * x[0] + v[1] * x[1] + v[2] * x[2], v[3] * x[0] + v[4] * x[1] + v[5] * x[2],
+                v[6] * x[0] + v[7] * x[1] + v[8] * x[2]);
+  }
+};
+// trans (Matrix3x3.h:1857-1860)
+inline Mat3 trans(const Mat3& m) {
+  Mat3 t;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t.v[3 * i + j] = m.v[3 * j + i];
+  return t;
+}
+typedef Mat3 Rot3;
+
 struct Quat {
   real v[4];  // (w, x, y, z)
   Quat() : v{1, 0, 0, 0} {}
@@ -109,6 +137,11 @@ struct Quat {
     const real c = v[0] * x[2] - v[2] * x[0] + v[1] * x[1];
     return Vec3(v[0] * a + v[1] * w + v[2] * c - v[3] * b, v[0] * b + v[2] * w + v[3] * a - v[1] * c,
                 v[0] * c + v[3] * w + v[1] * b - v[2] * a);
+  }
+  Mat3 toRotationMatrix() const {
+    Mat3 m;
+    toRotationMatrix(m.v);
+    return m;
   }
   // Quaternion::toRotationMatrix (Quaternion.h:591-602), row-major
   void toRotationMatrix(real* m) const {
@@ -171,6 +204,13 @@ class RigidBody {
   const Quat& getQuaternion() const;
   real getMass() const;
   real getInvMass() const;
+  // RigidBody.h:152-158: the rotation matrix, the body-frame inertia and its inverse, and both in
+  // the world frame (R I trans(R), R I^-1 trans(R), RigidBody.h:842-869)
+  Rot3 getRotation() const;
+  Mat3 getBodyInertia() const;
+  Mat3 getInvBodyInertia() const;
+  Mat3 getInertia() const;
+  Mat3 getInvInertia() const;
   void setPosition(const Vec3& p);
   void setLinearVel(const Vec3& v);
   void setAngularVel(const Vec3& w);
@@ -550,6 +590,25 @@ inline const Quat& RigidBody::getQuaternion() const {
 }
 inline real RigidBody::getMass() const { return detail::E().mass[index_]; }
 inline real RigidBody::getInvMass() const { return detail::E().inv_mass[index_]; }
+inline Rot3 RigidBody::getRotation() const { return getQuaternion().toRotationMatrix(); }
+inline Mat3 RigidBody::getBodyInertia() const {
+  Mat3 m;
+  for (int k = 0; k < 9; ++k) m.v[k] = detail::E().I[9 * index_ + k];
+  return m;
+}
+inline Mat3 RigidBody::getInvBodyInertia() const {
+  Mat3 m;
+  for (int k = 0; k < 9; ++k) m.v[k] = detail::E().Iinv[9 * index_ + k];
+  return m;
+}
+inline Mat3 RigidBody::getInertia() const {
+  const Rot3 R = getRotation();
+  return R * getBodyInertia() * trans(R);
+}
+inline Mat3 RigidBody::getInvInertia() const {
+  const Rot3 R = getRotation();
+  return R * getInvBodyInertia() * trans(R);
+}
 // setPosition: Union::setPosition (Union.cpp:251-272) moves every member by the union's
 // displacement (Sphere::update(dp), Sphere.cpp:1031); Plane::setPosition (Plane.cpp:255-268)
 // recomputes the plane's displacement from the origin.
@@ -1286,6 +1345,13 @@ class World {
     detail::E().params.damping = damping;
     detail::E().params_dirty = true;
   }
+  // World::getGravity, getDamping, getBody (World.h:358-444)
+  Vec3 getGravity() const {
+    const pehip_params& p = detail::E().params;
+    return Vec3(p.gravity[0], p.gravity[1], p.gravity[2]);
+  }
+  real getDamping() const { return detail::E().params.damping; }
+  BodyID getBody(unsigned int index) const { return detail::E().handles.at(index).get(); }
   size_t size() const { return detail::E().shape.size(); }
   BodyID operator[](size_t i) { return detail::E().handles.at(i).get(); }
   void simulationStep(real timestep) { run(1, timestep); }

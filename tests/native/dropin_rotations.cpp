@@ -68,6 +68,23 @@ int main() {
   const Vec3 w = box->pointFromBFtoWF(Vec3(0.25, -0.5, 0.125));
   std::printf("box_point %.17g %.17g %.17g\n", w[0], w[1], w[2]);
 
+  // the turned box's rotation and inertias (RigidBody.h:152-158, 842-869) and World's getters
+  auto mat = [](const char* name, const Mat3& M) {
+    std::printf("%s", name);
+    for (int k = 0; k < 9; ++k) std::printf(" %.17g", M[k]);
+    std::printf("\n");
+  };
+  mat("box_R", box->getRotation());
+  mat("box_Ib", box->getBodyInertia());
+  mat("box_Ibinv", box->getInvBodyInertia());
+  mat("box_Iw", box->getInertia());
+  mat("box_Iwinv", box->getInvInertia());
+  theWorld()->setGravity(0, 0, -9.81);
+  theWorld()->setDamping(0.875);
+  const Vec3 g = theWorld()->getGravity();
+  std::printf("world %.17g %.17g %.17g %.17g %llu\n", g[0], g[1], g[2], theWorld()->getDamping(),
+              (unsigned long long)theWorld()->getBody(1)->getID());
+
   // pe's random numbers: the default seed, a seed that repeats its sequence, the ranges, and
   // granular particles that follow the seed
   std::printf("seed0 %u\n", getSeed());

@@ -305,6 +305,19 @@ def test_cpp_dropin_pose_utilities_follow_pe(tmp_path, oracle_mod):
     dq = euler(0.4, -0.6, 0.2); pos = turned_about(pos, dq, (-0.5, 0.25, 1)); q = mul(dq, q); check("box7", pos, q)
     pos = np.array([pos[0] + 0.1, pos[1] + 0.2, pos[2] + -0.3]); check("box8", pos, q)
     R = O.quat_to_rotation(q)
+    # getRotation is toRotationMatrix of the quaternion; getInertia / getInvInertia are
+    # R I trans(R) in pe's product order (RigidBody.h:842-869, Matrix3x3.h:2241-2253)
+    assert got["box_R"] == R.ravel().tolist()
+    Ib, Ibinv = np.array(got["box_Ib"]).reshape(3, 3), np.array(got["box_Ibinv"]).reshape(3, 3)
+    assert Ib[0, 0] > 0 and np.allclose(Ib @ Ibinv, np.eye(3), rtol=0, atol=1e-15)
+
+    def rirt(M):
+        t = [[R[i, 0] * M[0, j] + R[i, 1] * M[1, j] + R[i, 2] * M[2, j] for j in range(3)] for i in range(3)]
+        return [t[i][0] * R[j, 0] + t[i][1] * R[j, 1] + t[i][2] * R[j, 2] for i in range(3) for j in range(3)]
+    assert got["box_Iw"] == rirt(Ib) and got["box_Iwinv"] == rirt(Ibinv)
+    Iw = np.array(got["box_Iw"]).reshape(3, 3)
+    assert np.allclose(Iw, Iw.T, rtol=0, atol=1e-12) and abs(np.trace(Iw) - np.trace(Ib)) < 1e-12 * np.trace(Ib)
+    assert got["world"] == [0.0, 0.0, -9.81, 0.875, 2.0]   # getBody(1) is the sphere of uid 2
     bp = (0.25, -0.5, 0.125)
     assert got["box_point"] == [pos[k] + (R[k, 0] * bp[0] + R[k, 1] * bp[1] + R[k, 2] * bp[2]) for k in range(3)]
     # sphere, capsule (Euler angles turn the body's own quaternion)
