@@ -94,9 +94,9 @@ int main() {
     for (int k = 0; k < 4; ++k) std::printf(" %.17g", rand<real>());
     for (int k = 0; k < 4; ++k) std::printf(" %.17g", rand<real>(-2, 3));
     for (int k = 0; k < 4; ++k) std::printf(" %d", rand<int>(1, 6));
-    UnionID g = createGranularParticle(100 + pass, Vec3(0, 0, 10 + 5 * pass), 0.5, m);
-    std::printf(" %zu", g->size());
-    const size_t gi = g->index();
+    UnionID gp = createGranularParticle(100 + pass, Vec3(0, 0, 10 + 5 * pass), 0.5, m);
+    std::printf(" %zu", gp->size());
+    const size_t gi = gp->index();
     for (size_t k = 0; k < e.m_owner.size(); ++k)
       if (e.m_owner[k] == gi) std::printf(" %.17g", e.m_rad[k]);
     std::printf("\n");
