@@ -1145,6 +1145,8 @@ class BodyBinaryReader {
   // readFile (BodyBinaryReader.cpp:60-330) into an empty world whose materials exist: the local
   // chunks' bodies first, then the global chunk's, with the file's system IDs, poses and
   // velocities; masses from the shapes and materials, a union's from the file (Union.cpp:100-133).
+  // The whole file is parsed and checked before the first body is created, so a file that is
+  // refused leaves the world as it was.
   void readFile(const char* filename) {
     detail::Engine& e = detail::E();
     if (!e.shape.empty()) throw std::invalid_argument("bulk creation only into an empty world");
@@ -1165,113 +1167,142 @@ class BodyBinaryReader {
     if (h.u8() != 4 || h.u8() != 8 || h.u8() != 8 || h.u8() != 1)
       throw std::runtime_error("Size of the int, size_t, id_t or bool data type does not match the file.");
     const uint32_t p = h.u32(), off_global = h.u32();
-    std::vector<uint32_t> offs(p + 1);
-    for (uint32_t& o : offs) o = h.u32();
+    std::vector<uint32_t> offs;
+    for (uint32_t k = 0; k <= p; ++k) offs.push_back(h.u32());
     bool bad = offs[0] < off_global || offs[p] > data.size();
     for (uint32_t k = 0; k < p; ++k) bad = bad || offs[k + 1] < offs[k];
     if (bad) throw std::runtime_error("Invalid body data offset.");
-    uint64_t counter = 0, top = 0;
-    bool any = false;
-    auto chunk = [&](size_t lo, size_t hi) {
-      detail::FileIn r(data, lo, hi);
-      r.single = rsize == 4;
-      const uint64_t c = r.u64();
-      while (r.pos < hi) {
-        const uint8_t t = r.u8();
-        const uint32_t cnt = r.u32();
-        if (cnt && t != detail::kFileUnion && t != detail::kFilePlane && !detail::file_params(t))
-          throw std::runtime_error("Unsupported geometry type " + std::to_string(t) + " in body file");
-        for (uint32_t k = 0; k < cnt; ++k) {
-          const id_t sid = r.u64(), uid = r.u64();
-          (void)r.u8();
-          const bool fx = r.u8() != 0;
-          real pos[3], q[4], v[3], w[3];
-          r.rs(pos, 3); r.rs(q, 4); r.rs(v, 3); r.rs(w, 3);
-          top = any ? std::max<uint64_t>(top, sid) : sid;
-          any = true;
-          if (t == detail::kFileUnion) {
-            read_union(e, r, sid, uid, fx, pos, q, v, w, top);
-            continue;
-          }
-          const uint64_t mat = r.u64();
-          real prm[3] = {0, 0, 0};
-          r.rs(prm, detail::file_params(t));
-          if (mat >= e.materials.size())
-            throw std::invalid_argument("body " + std::to_string(sid) + " uses material " + std::to_string(mat) +
-                                        ", which the world does not have");
-          BodyID b;
-          const Vec3 g(pos[0], pos[1], pos[2]);
-          if (t == detail::kFileSphere) b = createSphere(uid, g, prm[0], mat);
-          else if (t == detail::kFileBox) b = createBox(uid, g, Vec3(prm[0], prm[1], prm[2]), mat);
-          else if (t == detail::kFileCapsule) b = createCapsule(uid, g, prm[0], prm[1], mat);
-          else b = createPlane(uid, Vec3(0, 0, 1), 0, mat);
-          const size_t i = b->index();
-          e.sid[i] = sid;
-          for (int j = 0; j < 4; ++j) e.q[4 * i + j] = q[j];
-          for (int j = 0; j < 3; ++j) { e.pos[3 * i + j] = pos[j]; e.v[3 * i + j] = v[j]; e.w[3 * i + j] = w[j]; }
-          if (t == detail::kFilePlane) {
-            // Plane.cpp:121-142: the normal is the orientation applied to +z, d = n . gpos
-            const Vec3 nrm = e.quat(i).rotate(Vec3(0, 0, 1));
-            for (int j = 0; j < 3; ++j) e.normal[3 * i + j] = nrm[j];
-            e.plane_displacement(i);
-          } else if (fx) {
-            b->setFixed(true);
-          }
+    std::vector<Record> recs;
+    uint64_t counter = 0;
+    for (uint32_t k = 0; k < p; ++k) counter = std::max<uint64_t>(counter, parse(data, offs[k], offs[k + 1], rsize, recs));
+    parse(data, off_global, offs[0], rsize, recs);
+    uint64_t top = 0;
+    for (const Record& r : recs) {
+      top = std::max<uint64_t>(top, r.sid);
+      const size_t nm = e.materials.size();
+      if (r.t != detail::kFileUnion && r.mat >= nm) throw bad_material(r.sid, r.mat);
+      if (r.t == detail::kFileUnion) {
+        if (r.fixed) throw std::invalid_argument("union " + std::to_string(r.sid) + " is fixed: fixed unions are not supported");
+        if (r.members.empty()) throw std::invalid_argument("union " + std::to_string(r.sid) + " has no members");
+        for (const Member& m : r.members) {
+          if (m.mat >= nm) throw bad_material(m.sid, m.mat);
+          top = std::max<uint64_t>(top, m.sid);
         }
       }
-      return c;
-    };
-    for (uint32_t k = 0; k < p; ++k) counter = std::max<uint64_t>(counter, chunk(offs[k], offs[k + 1]));
-    chunk(off_global, offs[0]);
-    e.next_sid = std::max<uint64_t>(counter, any ? top + 1 : 0);
+    }
+    for (const Record& r : recs) create(e, r);
+    e.next_sid = std::max<uint64_t>(counter, recs.empty() ? 0 : top + 1);
     e.bodies_dirty = true;
   }
 
  private:
-  // Marshalling.h:560-620 with spheres for members; the union keeps the file's mass, inertia
-  // and orientation, and its members their positions
-  static void read_union(detail::Engine& e, detail::FileIn& r, id_t sid, id_t uid, bool fx, const real* pos,
-                         const real* q, const real* v, const real* w, uint64_t& top) {
-    if (fx) throw std::invalid_argument("union " + std::to_string(sid) + " is fixed: fixed unions are not supported");
-    const real mass = r.r();
-    real I[9], aabb[6];
-    r.rs(I, 9);
-    r.rs(aabb, 6);
-    const uint64_t size = r.u64();
-    if (!size) throw std::invalid_argument("union " + std::to_string(sid) + " has no members");
-    struct Member { id_t sid, uid; real gpos[3], rpos[3]; uint64_t mat; real rad; };
-    std::vector<Member> ms(size);
-    real bound = 0, rmin = 0;
-    for (uint64_t k = 0; k < size; ++k) {
-      const uint8_t mt = r.u8();
-      if (mt != detail::kFileSphere)
-        throw std::invalid_argument("Unsupported geometry type " + std::to_string(mt) + " in a union");
-      Member& m = ms[k];
-      m.sid = r.u64(); m.uid = r.u64();
-      (void)r.u8(); (void)r.u8();
-      real mq[4];
-      r.rs(m.gpos, 3); r.rs(m.rpos, 3); r.rs(mq, 4);
-      m.mat = r.u64();
-      m.rad = r.r();
-      if (m.mat >= e.materials.size())
-        throw std::invalid_argument("body " + std::to_string(m.sid) + " uses material " + std::to_string(m.mat) +
-                                    ", which the world does not have");
-      top = std::max<uint64_t>(top, m.sid);
-      bound = std::max(bound, std::sqrt(m.rpos[0] * m.rpos[0] + m.rpos[1] * m.rpos[1] + m.rpos[2] * m.rpos[2]) + m.rad);
-      rmin = k ? std::min(rmin, m.rad) : m.rad;
+  struct Member {
+    id_t sid, uid;
+    real gpos[3], rpos[3];
+    uint64_t mat;
+    real rad;
+  };
+  struct Record {
+    uint8_t t;
+    id_t sid, uid;
+    bool fixed;
+    real pos[3], q[4], v[3], w[3];
+    uint64_t mat = 0;
+    real prm[3] = {0, 0, 0};
+    real mass = 0, I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<Member> members;
+  };
+  static std::invalid_argument bad_material(id_t sid, uint64_t mat) {
+    return std::invalid_argument("body " + std::to_string(sid) + " uses material " + std::to_string(mat) +
+                                 ", which the world does not have");
+  }
+  // One chunk's records (Marshalling.h:184-570; a union's members restricted to spheres);
+  // returns the chunk's ID counter.
+  static uint64_t parse(const std::vector<uint8_t>& data, size_t lo, size_t hi, uint8_t rsize, std::vector<Record>& out) {
+    detail::FileIn r(data, lo, hi);
+    r.single = rsize == 4;
+    const uint64_t counter = r.u64();
+    while (r.pos < hi) {
+      const uint8_t t = r.u8();
+      const uint32_t cnt = r.u32();
+      if (cnt && t != detail::kFileUnion && t != detail::kFilePlane && !detail::file_params(t))
+        throw std::runtime_error("Unsupported geometry type " + std::to_string(t) + " in body file");
+      for (uint32_t k = 0; k < cnt; ++k) {
+        Record rec;
+        rec.t = t;
+        rec.sid = r.u64();
+        rec.uid = r.u64();
+        (void)r.u8();
+        rec.fixed = r.u8() != 0;
+        r.rs(rec.pos, 3); r.rs(rec.q, 4); r.rs(rec.v, 3); r.rs(rec.w, 3);
+        if (t == detail::kFileUnion) {
+          real aabb[6];
+          rec.mass = r.r();
+          r.rs(rec.I, 9);
+          r.rs(aabb, 6);
+          const uint64_t size = r.u64();
+          for (uint64_t j = 0; j < size; ++j) {
+            const uint8_t mt = r.u8();
+            if (mt != detail::kFileSphere)
+              throw std::invalid_argument("Unsupported geometry type " + std::to_string(mt) + " in a union");
+            Member m;
+            m.sid = r.u64();
+            m.uid = r.u64();
+            (void)r.u8(); (void)r.u8();
+            real mq[4];
+            r.rs(m.gpos, 3); r.rs(m.rpos, 3); r.rs(mq, 4);
+            m.mat = r.u64();
+            m.rad = r.r();
+            rec.members.push_back(m);
+          }
+        } else {
+          rec.mat = r.u64();
+          r.rs(rec.prm, detail::file_params(t));
+        }
+        out.push_back(std::move(rec));
+      }
     }
-    const real prm[3] = {bound * (real(1) + real(1e-12)), rmin, 0};
-    Union* u = e.add<Union>(unionType, uid, ms[0].mat, false, prm, Vec3(pos[0], pos[1], pos[2]),
-                            Quat(q[0], q[1], q[2], q[3]), mass, real(1) / mass, I, Vec3(), 0, sid);
-    const size_t i = u->index();
-    for (int j = 0; j < 3; ++j) { e.v[3 * i + j] = v[j]; e.w[3 * i + j] = w[j]; }
-    for (const Member& m : ms) {
-      e.m_owner.push_back((uint32_t)i);
-      e.m_sid.push_back(m.sid);
-      e.m_uid.push_back(m.uid);
-      e.m_mat.push_back((uint32_t)m.mat);
-      e.m_rad.push_back(m.rad);
-      for (int j = 0; j < 3; ++j) { e.m_rpos.push_back(m.rpos[j]); e.m_gpos.push_back(m.gpos[j]); }
+    return counter;
+  }
+  static void create(detail::Engine& e, const Record& r) {
+    if (r.t == detail::kFileUnion) {
+      real bound = 0, rmin = r.members[0].rad;
+      for (const Member& m : r.members) {
+        bound = std::max(bound, std::sqrt(m.rpos[0] * m.rpos[0] + m.rpos[1] * m.rpos[1] + m.rpos[2] * m.rpos[2]) + m.rad);
+        rmin = std::min(rmin, m.rad);
+      }
+      const real prm[3] = {bound * (real(1) + real(1e-12)), rmin, 0};
+      Union* u = e.add<Union>(unionType, r.uid, r.members[0].mat, false, prm, Vec3(r.pos[0], r.pos[1], r.pos[2]),
+                              Quat(r.q[0], r.q[1], r.q[2], r.q[3]), r.mass, real(1) / r.mass, r.I, Vec3(), 0, r.sid);
+      const size_t i = u->index();
+      for (int j = 0; j < 3; ++j) { e.v[3 * i + j] = r.v[j]; e.w[3 * i + j] = r.w[j]; }
+      for (const Member& m : r.members) {
+        e.m_owner.push_back((uint32_t)i);
+        e.m_sid.push_back(m.sid);
+        e.m_uid.push_back(m.uid);
+        e.m_mat.push_back((uint32_t)m.mat);
+        e.m_rad.push_back(m.rad);
+        for (int j = 0; j < 3; ++j) { e.m_rpos.push_back(m.rpos[j]); e.m_gpos.push_back(m.gpos[j]); }
+      }
+      return;
+    }
+    BodyID b;
+    const Vec3 g(r.pos[0], r.pos[1], r.pos[2]);
+    if (r.t == detail::kFileSphere) b = createSphere(r.uid, g, r.prm[0], r.mat);
+    else if (r.t == detail::kFileBox) b = createBox(r.uid, g, Vec3(r.prm[0], r.prm[1], r.prm[2]), r.mat);
+    else if (r.t == detail::kFileCapsule) b = createCapsule(r.uid, g, r.prm[0], r.prm[1], r.mat);
+    else b = createPlane(r.uid, Vec3(0, 0, 1), 0, r.mat);
+    const size_t i = b->index();
+    e.sid[i] = r.sid;
+    for (int j = 0; j < 4; ++j) e.q[4 * i + j] = r.q[j];
+    for (int j = 0; j < 3; ++j) { e.pos[3 * i + j] = r.pos[j]; e.v[3 * i + j] = r.v[j]; e.w[3 * i + j] = r.w[j]; }
+    if (r.t == detail::kFilePlane) {
+      // Plane.cpp:121-142: the normal is the orientation applied to +z, d = n . gpos
+      const Vec3 nrm = e.quat(i).rotate(Vec3(0, 0, 1));
+      for (int j = 0; j < 3; ++j) e.normal[3 * i + j] = nrm[j];
+      e.plane_displacement(i);
+    } else if (r.fixed) {
+      b->setFixed(true);
     }
   }
 };

@@ -4,6 +4,8 @@
 //   dropin_body_file read <file>              reads the file with BodyBinaryReader into a world
 //                                             holding the scene's two materials and prints the
 //                                             body and member arrays, one "name values" line each
+//   dropin_body_file tryread <file> <nmat>    reads it into a world of nmat (1 or 2) materials and
+//                                             prints whether it was refused and the body count
 // Nothing is stepped, so no device is needed.
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +26,20 @@ static void put(const char* name, const std::vector<T>& a) {
 int main(int argc, char** argv) {
   if (argc < 3) return 2;
   MaterialID gran = createMaterial("granular", 2500, 0, 0.2, 0.2, 0.5, 1, 1, 0, 0);
+  if (std::strcmp(argv[1], "tryread") == 0) {
+    // a world holding only the first material: a file whose bodies use the second, or a file cut
+    // short, is refused, and the world keeps no body of it
+    if (argc > 3 && std::strcmp(argv[3], "2") == 0) createMaterial("wood", 700, 0.1, 0.35, 0.3, 0.5, 1, 1, 0, 0);
+    try {
+      BodyBinaryReader r;
+      r.readFile(argv[2]);
+      std::printf("read\n");
+    } catch (const std::exception& ex) {
+      std::printf("refused %s\n", ex.what());
+    }
+    std::printf("bodies %zu members %zu\n", theWorld()->size(), detail::E().m_owner.size());
+    return 0;
+  }
   MaterialID wood = createMaterial("wood", 700, 0.1, 0.35, 0.3, 0.5, 1, 1, 0, 0);
   if (std::strcmp(argv[1], "write") == 0) {
     createPlane(100, Vec3(0, 0, 1), 0, gran);

@@ -459,3 +459,18 @@ def test_cpp_dropin_body_files_equal_python_ones(tmp_path):
             assert np.array_equal(cpp[k], np.asarray(m[mk], float).ravel()), (fp, k)
         assert int(cpp["next_sid"][0]) == w._next_sid
         assert cpp["sid"].tolist() == [3, 12, 4, 5, 6, 10, 1, 2]   # local chunk by type, then the planes
+    # a file that is refused leaves the world empty: bodies of a material the world lacks, or the
+    # file cut short (the whole file is checked before the first body is created)
+    import struct
+    fpy = str(tmp_path / "py0.pe")
+    cut, short = str(tmp_path / "cut.pe"), str(tmp_path / "short.pe")
+    raw = open(fpy, "rb").read()[:-20]
+    open(cut, "wb").write(raw)                       # offsets past the end of the file
+    open(short, "wb").write(raw[:21] + struct.pack(">I", len(raw)) + raw[25:])   # a record cut short
+    for f, nmat, why in ((fpy, "1", "which the world does not have"), (cut, "2", "Invalid body data offset"),
+                         (short, "2", "truncated"), (fpy, "2", None)):
+        r = subprocess.run([exe, "tryread", f, nmat], capture_output=True, text=True, check=True, timeout=60).stdout
+        if why:
+            assert r.startswith("refused") and why in r and "bodies 0 members 0" in r, r
+        else:
+            assert r.startswith("read") and "bodies 8 members 4" in r, r
