@@ -219,6 +219,8 @@ def read_body_file(filename):
     real = "d" if sizes[0] == 8 else "f"
     p, = h.get("I")
     off_global, = h.get("I")
+    if h.pos + 4 * (p + 1) > len(data):             # before building a format of p + 1 fields
+        raise ValueError("truncated body file")
     offs = h.get("I" * (p + 1))
     if offs[0] < off_global or any(b < a for a, b in zip(offs, offs[1:])) or offs[-1] > len(data):
         raise ValueError("Invalid body data offset.")

@@ -145,6 +145,12 @@ def test_rejects_bad_files(tmp_path):
     bad.write_bytes(bytes(data[:-7]))
     with pytest.raises(ValueError):
         checkpoint.read_body_file(str(bad))
+    # a process count far beyond the file: refused before its offset table is read
+    v = bytearray(data)
+    v[9:13] = struct.pack(">I", 0xFFFFFFFF)
+    bad.write_bytes(bytes(v))
+    with pytest.raises(ValueError, match="truncated body file"):
+        checkpoint.read_body_file(str(bad))
     # a cylinder (type 4) in the local chunk: not a shape of this library
     off_l = struct.unpack(">I", bytes(data[17:21]))[0]
     v = bytearray(data)
